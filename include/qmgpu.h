@@ -1,0 +1,154 @@
+/*
+ * qmgpu.h — C-ABI of the MI355X (gfx950) columnar operator library.
+ *
+ * This is the drop-in boundary for QueryMode's local operator hot path:
+ * predicate filter -> hash group-by aggregate -> hash-join probe -> top-K.
+ * Table columns are uploaded once and stay resident in HBM; a query plan is a
+ * compact binary encoding of the QueryDescriptor subset the path needs; a
+ * result is a device-resident columnar batch that is read out in the QMCB
+ * wire layout.
+ *
+ * Conventions (the same ones the reference's native C-ABI uses,
+ * wasm/src/nodejs.zig:101-742, and its WASM FFI, src/wasm-engine.ts:21-188):
+ *   - opaque handles, explicit *_close / *_drop, a global qm_shutdown;
+ *   - every call returns an int status (QM_OK == 0, negative on failure);
+ *     no C++ exception crosses the ABI;
+ *   - output buffers are caller-owned with an explicit capacity;
+ *   - a failing call leaves a message readable through qm_last_error().
+ * Handles may be used from several host threads; device submission is
+ * serialised per context on one HIP stream.
+ */
+#ifndef QMGPU_H
+#define QMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: mirror QueryModeError codes (src/errors.ts:7-17) ---- */
+#define QM_OK                    0
+#define QM_ERR_TABLE_NOT_FOUND  -1  /* TABLE_NOT_FOUND  */
+#define QM_ERR_COLUMN_NOT_FOUND -2  /* COLUMN_NOT_FOUND */
+#define QM_ERR_INVALID_FORMAT   -3  /* INVALID_FORMAT (bad plan / QMCB bytes) */
+#define QM_ERR_SCHEMA_MISMATCH  -4  /* SCHEMA_MISMATCH  */
+#define QM_ERR_INVALID_FILTER   -5  /* INVALID_FILTER   */
+#define QM_ERR_INVALID_AGGREGATE -6 /* INVALID_AGGREGATE */
+#define QM_ERR_MEMORY_EXCEEDED  -7  /* MEMORY_EXCEEDED (HBM budget) */
+#define QM_ERR_QUERY_FAILED     -8  /* QUERY_FAILED (device / runtime error) */
+#define QM_ERR_UNSUPPORTED      -9  /* valid QueryMode query outside the GPU path */
+#define QM_ERR_INVALID_ARG     -10  /* null handle, bad size, ... */
+#define QM_ERR_OVERFLOW        -11  /* exact integer result does not fit its type */
+
+/* ---- column dtypes: the QMCB tags (src/columnar.ts:22-32) plus the
+ * narrow integer widths the QMCB tag set lacks (int8 columns are filtered by
+ * the reference's JS path, src/operators.ts:531-543; they are widened to I32
+ * when a result is written as QMCB). ---- */
+#define QM_F64     0
+#define QM_I64     1
+#define QM_UTF8    2   /* not accepted on the device path */
+#define QM_BOOL    3   /* stored one byte per row on the device */
+#define QM_F32VEC  4   /* not accepted on the device path */
+#define QM_NULLT   5
+#define QM_I32     6
+#define QM_F32     7
+#define QM_I8     16
+#define QM_I16    17
+
+typedef struct qm_ctx    qm_ctx;
+typedef struct qm_table  qm_table;
+typedef struct qm_plan   qm_plan;
+typedef struct qm_result qm_result;
+typedef struct qm_batch  qm_batch;
+
+/* One host column handed to qm_table_create. `null_bitmap` follows QMCB:
+ * ceil(nrows/8) bytes, bit (row & 7) of byte (row >> 3) set = row is null;
+ * NULL means the column has no nulls. */
+typedef struct qm_column_desc {
+  const char    *name;
+  int32_t        dtype;
+  const void    *data;        /* nrows values, little-endian, host memory */
+  const uint8_t *null_bitmap; /* optional */
+} qm_column_desc;
+
+/* Per-result execution statistics (the GPU analogue of QueryResult's
+ * scan/pipeline timing fields, src/types.ts:262-295). */
+typedef struct qm_stats {
+  uint64_t rows_scanned;      /* logical table rows consumed (probe + build for joins) */
+  uint64_t rows_selected;     /* rows that passed the filter */
+  uint64_t groups;            /* output rows */
+  uint64_t bytes_algorithmic; /* bytes of every referenced column, read once */
+  double   device_ms;         /* device time of the query's kernels (HIP events) */
+  double   scan_ms;           /* device time of the dominant scan kernel alone */
+  double   wall_ms;           /* host wall time of qm_exec */
+  uint32_t kernel_launches;
+  uint32_t retries;           /* hash-table regrowths */
+  uint32_t strategy;          /* group-by strategy picked by the tuner (QM_STRAT_*) */
+  uint32_t _pad;
+} qm_stats;
+
+#define QM_STRAT_NONE        0
+#define QM_STRAT_DIRECT_LDS  1  /* dense small key domain, per-workgroup LDS table */
+#define QM_STRAT_HASH_LDS    2  /* global open addressing + per-workgroup LDS cache */
+#define QM_STRAT_DIRECT_GLB  3  /* dense large key domain, direct global table */
+#define QM_STRAT_HASH_GLB    4  /* global open addressing, no LDS cache */
+
+/* ---- lifecycle ----
+ * Replaces the executor construction QueryMode.local() does
+ * (src/local.ts:31-34, src/local-executor.ts:36-40 memoryBudgetBytes):
+ * `hbm_budget_bytes` bounds resident tables + per-query device state. */
+int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out);
+int qm_shutdown(qm_ctx *ctx);
+/* Copies the calling thread's last error message (NUL-terminated, truncated
+ * to cap). Returns the full message length. ctx may be NULL. */
+int qm_last_error(qm_ctx *ctx, char *buf, size_t cap);
+/* Device name / CU count / budget bookkeeping. */
+int qm_device_info(qm_ctx *ctx, char *name, size_t cap, uint32_t *num_cus,
+                   uint64_t *budget, uint64_t *used);
+
+/* ---- tables ----
+ * Replaces FragmentSource + page decode (src/operators.ts:96-103,
+ * 282-404): decoded columns are uploaded once into resident HBM instead of
+ * being decoded into JS arrays page by page. */
+int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows,
+                    uint32_t ncols, const qm_column_desc *cols, qm_table **out);
+int qm_table_drop(qm_table *t);
+uint64_t qm_table_rows(const qm_table *t);
+
+/* ---- plans ----
+ * `plan` is the binary QueryDescriptor subset (src/client.ts:1254-1277):
+ * filters / filterGroups (FilterOp, src/types.ts:130-134), groupBy,
+ * aggregates (AggregateOp, src/types.ts:137-144), product computed columns,
+ * join {leftKey,rightKey,type}, sortColumn/sortDirection, limit/offset,
+ * projections. Its layout is given in the package's plan encoder. */
+int qm_plan_compile(qm_ctx *ctx, const uint8_t *plan, size_t len, qm_plan **out);
+int qm_plan_close(qm_plan *p);
+
+/* ---- execution ----
+ * Replaces LocalExecutor.execute / executeJoin (src/local-executor.ts:386-508,
+ * 597-667) driving ScanOperator -> AggregateOperator / HashJoinOperator ->
+ * TopKOperator / InMemorySortOperator -> ProjectOperator
+ * (src/operators.ts:3073-3150). `right` is the join build side (the
+ * reference builds the RIGHT side, src/operators.ts:2349-2436) or NULL. */
+int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out);
+
+/* ---- batched execution: many queries over one resident table in fused
+ * persistent launches (no reference counterpart; the reference runs one
+ * pipeline per query). Results come back in submission order. */
+int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
+                  qm_table *table, qm_result **results_out);
+
+/* ---- results (QueryResult.columnarData, src/types.ts:262-295) ---- */
+int qm_result_rows(qm_result *r, uint64_t *rows);
+int qm_result_qmcb_size(qm_result *r, uint64_t *size);
+/* Writes the QMCB encoding (src/columnar.ts:523-613) into out[0..cap). */
+int qm_result_qmcb(qm_result *r, uint8_t *out, uint64_t cap, uint64_t *written);
+int qm_result_stats(qm_result *r, qm_stats *out);
+int qm_result_close(qm_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMGPU_H */

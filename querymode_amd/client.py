@@ -1,0 +1,398 @@
+"""DataFrame API, the GPU QueryExecutor and QueryResult.
+
+Mirrors the reference's builder (src/client.ts:176-371, 491-546, 876-901),
+QueryExecutor (src/client.ts:1280-1304) and QueryMode.local()/table()/sql()
+(src/local.ts:23-70); the executor drives the device library through its
+C-ABI instead of LocalExecutor's page-by-page JS/WASM pipeline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field, replace
+from typing import Any, Iterable
+
+import numpy as np
+
+from . import _native as N
+from . import qmcb
+from .errors import QueryModeError
+from .operators import GpuQueryOperator, drain_pipeline
+from .plan import AggregateOp, FilterOp, JoinDescriptor, QueryDescriptor, encode_plan
+
+_NP_TO_QM = {
+    np.dtype(np.float64): N.QM_F64, np.dtype(np.int64): N.QM_I64, np.dtype(np.int32): N.QM_I32,
+    np.dtype(np.float32): N.QM_F32, np.dtype(np.int8): N.QM_I8, np.dtype(np.int16): N.QM_I16,
+    np.dtype(np.bool_): N.QM_BOOL,
+}
+
+
+@dataclass
+class QueryResult:
+    """src/types.ts:262-295 (the fields this path fills)."""
+    rows: list[dict]
+    rowCount: int
+    columns: list[str]
+    columnarData: bytes | None = None
+    bytesRead: int = 0
+    pagesSkipped: int = 0
+    durationMs: float = 0.0
+    stats: dict = field(default_factory=dict)
+
+    def batch(self) -> qmcb.ColumnarBatch:
+        return qmcb.decode(self.columnarData)
+
+
+class Device:
+    """One GPU context (qm_init): resident tables + an HBM budget."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0):
+        self.lib = N.load()
+        h = C.c_void_p()
+        N.check(self.lib.qm_init(device, hbm_budget_bytes, C.byref(h)))
+        self.handle = h
+        self.tables: dict[str, GpuTable] = {}
+
+    def info(self) -> dict:
+        name = C.create_string_buffer(128)
+        cus, budget, used = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        N.check(self.lib.qm_device_info(self.handle, name, 128, C.byref(cus), C.byref(budget), C.byref(used)))
+        return {"arch": name.value.decode(), "cus": cus.value, "budget": budget.value, "used": used.value}
+
+    def register(self, name: str, columns: dict[str, np.ndarray],
+                 nulls: dict[str, np.ndarray] | None = None) -> "GpuTable":
+        """Upload decoded columns once; they stay resident in HBM."""
+        nulls = nulls or {}
+        arrays, descs, keep = [], [], []
+        nrows = None
+        for cname, arr in columns.items():
+            a = np.ascontiguousarray(arr)
+            if a.dtype not in _NP_TO_QM:
+                raise QueryModeError("UNSUPPORTED", f"column {cname}: dtype {a.dtype} is not on the device path")
+            if nrows is None:
+                nrows = len(a)
+            elif len(a) != nrows:
+                raise QueryModeError("SCHEMA_MISMATCH", f"column {cname} has {len(a)} rows, expected {nrows}")
+            if a.dtype == np.bool_:
+                a = a.astype(np.uint8)
+            arrays.append(a)
+            nb = None
+            if cname in nulls and nulls[cname] is not None:
+                nb = qmcb.pack_bits(np.asarray(nulls[cname], dtype=bool))
+                keep.append(nb)
+            d = N.ColumnDesc(cname.encode(), _NP_TO_QM[np.dtype(arr.dtype)], a.ctypes.data_as(C.c_void_p),
+                             nb.ctypes.data_as(C.c_void_p) if nb is not None else None)
+            descs.append(d)
+        arr_t = (N.ColumnDesc * len(descs))(*descs)
+        h = C.c_void_p()
+        N.check(self.lib.qm_table_create(self.handle, name.encode(), nrows or 0, len(descs), arr_t, C.byref(h)))
+        t = GpuTable(self, name, h, nrows or 0, list(columns.keys()))
+        if name in self.tables:
+            self.tables[name].drop()
+        self.tables[name] = t
+        return t
+
+    def register_qmcb(self, name: str, data: bytes) -> "GpuTable":
+        b = qmcb.decode(data)
+        if b is None:
+            raise QueryModeError("INVALID_FORMAT", "not a QMCB buffer")
+        cols, nulls = {}, {}
+        for c in b.columns:
+            if c.dtype not in (qmcb.DTYPE_F64, qmcb.DTYPE_I64, qmcb.DTYPE_I32, qmcb.DTYPE_F32, qmcb.DTYPE_BOOL):
+                raise QueryModeError("UNSUPPORTED", f"column {c.name}: QMCB dtype {c.dtype} not on the device path")
+            cols[c.name] = c.data
+            if c.null_bitmap is not None:
+                nulls[c.name] = qmcb.unpack_bits(c.null_bitmap, b.row_count)
+        return self.register(name, cols, nulls)
+
+    def register_parquet(self, name: str, path: str) -> "GpuTable":
+        """Host-side Parquet decode (pyarrow), then one upload."""
+        import pyarrow.parquet as pq
+        tbl = pq.read_table(path)
+        cols, nulls = {}, {}
+        for cname in tbl.column_names:
+            col = tbl.column(cname).combine_chunks()
+            mask = np.asarray(col.is_null(), dtype=bool) if col.null_count else None
+            arr = col.fill_null(0).to_numpy(zero_copy_only=False) if col.null_count else col.to_numpy()
+            cols[cname] = np.asarray(arr)
+            if mask is not None:
+                nulls[cname] = mask
+        return self.register(name, cols, nulls)
+
+    def close(self) -> None:
+        for t in list(self.tables.values()):
+            t.drop()
+        if self.handle:
+            self.lib.qm_shutdown(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class GpuTable:
+    def __init__(self, dev: Device, name: str, handle, rows: int, columns: list[str]):
+        self.dev, self.name, self.handle, self.rows, self.columns = dev, name, handle, rows, columns
+
+    def drop(self) -> None:
+        if self.handle:
+            self.dev.lib.qm_table_drop(self.handle)
+            self.handle = None
+            self.dev.tables.pop(self.name, None)
+
+
+class _Plan:
+    def __init__(self, dev: Device, desc: QueryDescriptor):
+        self.dev = dev
+        blob = encode_plan(desc)
+        h = C.c_void_p()
+        N.check(dev.lib.qm_plan_compile(dev.handle, blob, len(blob), C.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self.dev.lib.qm_plan_close(self.handle)
+            self.handle = None
+
+
+def _read_result(lib, rh) -> tuple[bytes, dict]:
+    try:
+        size = C.c_uint64()
+        N.check(lib.qm_result_qmcb_size(rh, C.byref(size)))
+        buf = (C.c_uint8 * max(1, size.value))()
+        wr = C.c_uint64()
+        N.check(lib.qm_result_qmcb(rh, buf, size.value, C.byref(wr)))
+        st = N.Stats()
+        N.check(lib.qm_result_stats(rh, C.byref(st)))
+        return bytes(buf)[: wr.value], st.as_dict()
+    finally:
+        lib.qm_result_close(rh)
+
+
+class GpuExecutor:
+    """QueryExecutor (src/client.ts:1280-1304) backed by the device library."""
+
+    def __init__(self, dev: Device):
+        self.dev = dev
+
+    def _table(self, name: str) -> GpuTable:
+        t = self.dev.tables.get(name)
+        if t is None or not t.handle:
+            raise QueryModeError("TABLE_NOT_FOUND", f"Table not found: {name}")
+        return t
+
+    def run_qmcb(self, desc: QueryDescriptor) -> tuple[bytes, dict]:
+        left = self._table(desc.table)
+        right = self._table(desc.join.right.table) if desc.join is not None else None
+        plan = _Plan(self.dev, desc)
+        try:
+            rh = C.c_void_p()
+            N.check(self.dev.lib.qm_exec(plan.handle, left.handle, right.handle if right else None, C.byref(rh)))
+        finally:
+            plan.close()
+        return _read_result(self.dev.lib, rh)
+
+    def execute(self, desc: QueryDescriptor) -> QueryResult:
+        t0 = time.perf_counter()
+        data, st = self.run_qmcb(desc)
+        b = qmcb.decode(data)
+        rows = qmcb.to_rows(b)
+        return QueryResult(rows=rows, rowCount=b.row_count, columns=[c.name for c in b.columns], columnarData=data,
+                           durationMs=(time.perf_counter() - t0) * 1e3, stats=st)
+
+    def execute_batch(self, descs: list[QueryDescriptor]) -> list[QueryResult]:
+        """Many queries over one resident table in fused launches (qm_batch_exec)."""
+        if not descs:
+            return []
+        tname = descs[0].table
+        if any(d.table != tname for d in descs):
+            raise QueryModeError("INVALID_ARGUMENT", "a batch runs over one table")
+        table = self._table(tname)
+        plans = [_Plan(self.dev, d) for d in descs]
+        try:
+            arr = (C.c_void_p * len(plans))(*[p.handle for p in plans])
+            outs = (C.c_void_p * len(plans))()
+            t0 = time.perf_counter()
+            N.check(self.dev.lib.qm_batch_exec(self.dev.handle, arr, len(plans), table.handle, outs))
+        finally:
+            for p in plans:
+                p.close()
+        res = []
+        for i in range(len(descs)):
+            data, st = _read_result(self.dev.lib, C.c_void_p(outs[i]))
+            b = qmcb.decode(data)
+            res.append(QueryResult(rows=qmcb.to_rows(b), rowCount=b.row_count, columns=[c.name for c in b.columns],
+                                   columnarData=data, durationMs=(time.perf_counter() - t0) * 1e3, stats=st))
+        return res
+
+    def explain(self, desc: QueryDescriptor) -> dict:
+        ops = ["GpuScan(filter)" if desc.filters or desc.filterGroups else "GpuScan"]
+        if desc.join is not None:
+            ops.append(f"GpuHashJoin(build={desc.join.right.table}, {desc.join.leftKey}={desc.join.rightKey})")
+        if desc.aggregates:
+            ops.append(f"GpuHashAggregate(groupBy={desc.groupBy})")
+        if desc.sortColumn:
+            ops.append("GpuTopK" if desc.limit is not None else "GpuSort")
+        return {"table": desc.table, "operators": ops, "device": self.dev.info()}
+
+    def to_operator(self, desc: QueryDescriptor) -> GpuQueryOperator:
+        return GpuQueryOperator(lambda: self.run_qmcb(desc)[0])
+
+    # reference-style camelCase alias
+    toOperator = to_operator
+
+    def count(self, desc: QueryDescriptor) -> int:
+        d = replace(desc, aggregates=[AggregateOp("count", "*", "count")], groupBy=[], sortColumn=None,
+                    limit=None, offset=None, projections=[])
+        return int(self.execute(d).rows[0]["count"])
+
+
+def _parse_product(expr) -> tuple[str, str]:
+    if isinstance(expr, (tuple, list)) and len(expr) == 3 and expr[0] in ("*", "multiply"):
+        return expr[1], expr[2]
+    if isinstance(expr, str):
+        s = expr.strip()
+        if s.startswith("multiply(") and s.endswith(")"):
+            a, b = s[9:-1].split(",")
+            return a.strip(), b.strip()
+        if "*" in s:
+            a, b = s.split("*")
+            return a.strip(), b.strip()
+    raise QueryModeError("UNSUPPORTED", "computed columns on the device are products of two columns: 'a * b'")
+
+
+class DataFrame:
+    """Immutable lazy builder (src/client.ts:88-921)."""
+
+    def __init__(self, table: str, executor: GpuExecutor, desc: QueryDescriptor | None = None):
+        self._executor = executor
+        self._desc = desc or QueryDescriptor(table=table)
+
+    def _derive(self, **kw) -> "DataFrame":
+        return DataFrame(self._desc.table, self._executor, replace(self._desc, **kw))
+
+    # ---- filters (src/client.ts:176-230) ----
+    def filter(self, column: str, op: str, value: Any = None) -> "DataFrame":
+        return self._derive(filters=self._desc.filters + [FilterOp(column, op, value)])
+
+    where = filter
+
+    def whereBetween(self, column: str, low, high) -> "DataFrame":
+        return self.filter(column, "between", [low, high])
+
+    def whereNotBetween(self, column: str, low, high) -> "DataFrame":
+        return self.filter(column, "not_between", [low, high])
+
+    def whereIn(self, column: str, values: Iterable) -> "DataFrame":
+        return self.filter(column, "in", list(values))
+
+    def whereNotIn(self, column: str, values: Iterable) -> "DataFrame":
+        return self.filter(column, "not_in", list(values))
+
+    def whereNull(self, column: str) -> "DataFrame":
+        return self.filter(column, "is_null")
+
+    def whereNotNull(self, column: str) -> "DataFrame":
+        return self.filter(column, "is_not_null")
+
+    def whereOr(self, *groups) -> "DataFrame":
+        gs = [[f if isinstance(f, FilterOp) else FilterOp(*f) for f in g] for g in groups]
+        return self._derive(filterGroups=self._desc.filterGroups + gs)
+
+    # ---- shape ----
+    def select(self, *columns: str) -> "DataFrame":
+        return self._derive(projections=list(columns))
+
+    def sort(self, column: str, direction: str = "asc") -> "DataFrame":
+        return self._derive(sortColumn=column, sortDirection=direction)
+
+    orderBy = sort
+
+    def limit(self, n: int) -> "DataFrame":
+        return self._derive(limit=n)
+
+    def offset(self, n: int) -> "DataFrame":
+        return self._derive(offset=n)
+
+    def aggregate(self, fn: str, column: str, alias: str | None = None) -> "DataFrame":
+        return self._derive(aggregates=self._desc.aggregates + [AggregateOp(fn, column, alias)])
+
+    def groupBy(self, *columns: str) -> "DataFrame":
+        return self._derive(groupBy=list(columns))
+
+    def join(self, other: "DataFrame", keys: dict, type: str = "inner") -> "DataFrame":
+        return self._derive(join=JoinDescriptor(other.toDescriptor(), keys["left"], keys["right"], type))
+
+    def computed(self, alias: str, expr) -> "DataFrame":
+        a, b = _parse_product(expr)
+        return self._derive(computedColumns=self._desc.computedColumns + [(alias, a, b)])
+
+    # ---- terminal ----
+    def toDescriptor(self) -> QueryDescriptor:
+        return self._desc
+
+    def collect(self) -> QueryResult:
+        return self._executor.execute(self._desc)
+
+    exec = collect
+
+    def explain(self) -> dict:
+        return self._executor.explain(self._desc)
+
+    def toOperator(self) -> GpuQueryOperator:
+        return self._executor.to_operator(self._desc)
+
+    def count(self) -> int:
+        return self._executor.count(self._desc)
+
+    def first(self) -> dict | None:
+        r = self.limit(1).collect()
+        return r.rows[0] if r.rows else None
+
+    def cursor(self, batch_size: int = 4096):
+        op = GpuQueryOperator(lambda: self._executor.run_qmcb(self._desc)[0], batch_size)
+        try:
+            while True:
+                b = op.next()
+                if b is None:
+                    break
+                yield b
+        finally:
+            op.close()
+
+
+class QueryMode:
+    """Entry point (src/local.ts:23-70): QueryMode.gpu() instead of QueryMode.local()."""
+
+    def __init__(self, dev: Device):
+        self.device = dev
+        self.executor = GpuExecutor(dev)
+
+    @staticmethod
+    def gpu(device: int = 0, hbm_budget_bytes: int = 0) -> "QueryMode":
+        return QueryMode(Device(device, hbm_budget_bytes))
+
+    def register_table(self, name: str, columns: dict, nulls: dict | None = None) -> GpuTable:
+        return self.device.register(name, columns, nulls)
+
+    def register_qmcb(self, name: str, data: bytes) -> GpuTable:
+        return self.device.register_qmcb(name, data)
+
+    def register_parquet(self, name: str, path: str) -> GpuTable:
+        return self.device.register_parquet(name, path)
+
+    def table(self, name: str) -> DataFrame:
+        return DataFrame(name, self.executor)
+
+    def sql(self, query: str) -> DataFrame:
+        from .sql import sql_to_descriptor
+        return DataFrame("", self.executor, sql_to_descriptor(query))
+
+    def close(self) -> None:
+        self.device.close()
+
+
+__all__ = ["QueryMode", "DataFrame", "GpuExecutor", "Device", "GpuTable", "QueryResult", "drain_pipeline"]

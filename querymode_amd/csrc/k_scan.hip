@@ -1,0 +1,488 @@
+// k_scan.hip — fused scan: predicate filter -> (optional hash-join probe) ->
+// hash group-by aggregate, for one or several queries over one resident
+// table in a single persistent launch.
+//
+// Replaces, for the local executor's hot path:
+//   ScanOperator + scanFilterIndices/applyAndFilters (src/operators.ts:282-547)
+//   the WASM filters + intersect (wasm/src/wasm/aggregates.zig:605-1065)
+//   AggregateOperator + computePartialAggColumnar + mergeStates
+//     (src/operators.ts:1646-1710, src/partial-agg.ts:141-267)
+//   HashJoinOperator's in-memory probe when it feeds an aggregate
+//     (src/operators.ts:2673-2713)
+//
+// Layout: each thread owns 8 consecutive rows of a tile (blockDim*8 rows);
+// column loads are 64/128-bit vector loads; predicates AND into an 8-bit
+// mask in registers (no index lists, no intersections). Selected rows update
+// a per-workgroup LDS table (dense replicated array for small key domains,
+// or an open-addressing cache for large ones) that is merged into the global
+// table with atomics when the workgroup finishes; rows whose key misses the
+// LDS cache go straight to the global table.
+#include "k_common.hip.h"
+
+namespace qm {
+
+static constexpr int LDS_HASH_PROBES = 8;
+static constexpr uint32_t GLB_PROBE_LIMIT = 4096;
+
+// Stored (salted) 64-bit key of a row for GM_HASH. Returns false when the
+// row's key must go to a reserved side entry (*side = 0: unrepresentable
+// key, 1: NULL key).
+__device__ __forceinline__ uint64_t key_code_hash(const DKey &k, const int64_t kv[MAX_KEYS],
+                                                  const uint32_t knull[MAX_KEYS], int *side) {
+  *side = -1;
+  if (k.ncols == 1) {
+    if (knull[0]) { *side = 1; return 0; }
+    uint64_t raw = (uint64_t)kv[0];
+    uint64_t s = raw + KEY_SALT;
+    if (s == 0) { *side = 0; return 0; }
+    return s;
+  }
+  // packed codes: code_i = v - kmin (null -> dom - 1)
+  uint64_t packed = 0;
+  int shift = 0;
+  for (int i = 0; i < k.ncols; ++i) {
+    uint64_t code = knull[i] ? (k.dom[i] - 1) : (uint64_t)(kv[i] - k.kmin[i]);
+    packed |= code << shift;
+    shift += k.bits[i];
+  }
+  return packed + KEY_SALT;  // packed < 2^63 so never 0
+}
+
+__device__ __forceinline__ uint64_t key_index_direct(const DKey &k, const int64_t kv[MAX_KEYS],
+                                                     const uint32_t knull[MAX_KEYS]) {
+  uint64_t idx = 0;
+  for (int i = 0; i < k.ncols; ++i) {
+    uint64_t code = knull[i] ? (k.dom[i] - 1) : (uint64_t)(kv[i] - k.kmin[i]);
+    idx += code * k.stride[i];
+  }
+  return idx;
+}
+
+// Global hash-table lookup/insert. Returns the entry index or UINT64_MAX on
+// overflow (probe limit or fill limit exceeded; the host regrows and reruns).
+__device__ __forceinline__ uint64_t glb_find(const DQuery &q, uint64_t stored) {
+  const uint64_t mask = q.gcap - 1;
+  uint64_t h = mix64(stored) & mask;
+  const int W = q.entry_words;
+  for (uint32_t p = 0; p < GLB_PROBE_LIMIT; ++p) {
+    uint64_t slot = (h + p) & mask;
+    uint64_t *e = q.table + slot * W;
+    uint64_t k = g_load(e);
+    if (k == stored) return slot;
+    if (k == 0) {
+      uint64_t old = g_atomic_cas(e, 0, stored);
+      if (old == 0) {
+        // new group: bound the fill so probe sequences stay short
+        uint64_t n = g_atomic_add(q.selected + 1, 1);
+        if (n + 1 > (q.gcap >> 1) + (q.gcap >> 2)) q.status[0] = 1;
+        return slot;
+      }
+      if (old == stored) return slot;
+    }
+  }
+  q.status[0] = 1;
+  return ~0ull;
+}
+
+// ---- state updates ----
+// value fetch for a state: integer or f64, with nulls
+struct RowVals {
+  int64_t iv[RPT];
+  double fv[RPT];
+  uint32_t nm;
+};
+
+__device__ __forceinline__ void fetch_state_vals(const DState &s, const DCol *cols, uint64_t row0,
+                                                 RowVals &rv) {
+  const DCol &a = cols[s.col_a];
+  rv.nm = load8_nulls(a, row0);
+  if (s.product) {
+    const DCol &b = cols[s.col_b];
+    double x[RPT], y[RPT];
+    if (dt_is_float(a.dtype)) load8_f(a, row0, x);
+    else { int64_t t[RPT]; load8_i(a, row0, t); for (int r = 0; r < RPT; ++r) x[r] = (double)t[r]; }
+    if (dt_is_float(b.dtype)) load8_f(b, row0, y);
+    else { int64_t t[RPT]; load8_i(b, row0, t); for (int r = 0; r < RPT; ++r) y[r] = (double)t[r]; }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) rv.fv[r] = x[r] * y[r];
+    rv.nm |= load8_nulls(b, row0);
+  } else if (dt_is_float(a.dtype)) {
+    load8_f(a, row0, rv.fv);
+  } else {
+    load8_i(a, row0, rv.iv);
+  }
+}
+
+// Per-value contribution of a state, as the words to atomically combine.
+struct Contrib {
+  uint64_t a, b, c;  // meaning depends on kind
+  bool skip;
+};
+
+__device__ __forceinline__ Contrib contrib_of(const DState &s, bool is_float, int64_t iv, double fv) {
+  Contrib k; k.a = k.b = k.c = 0; k.skip = false;
+  switch (s.kind) {
+    case SK_COUNT: k.a = 1; break;
+    case SK_SUM_I64: k.a = (uint64_t)iv; break;
+    case SK_SUM_128:
+      if (s.fixed) {
+        uint64_t bits = f64_bits(fv);
+        if (((bits >> 52) & 0x7FF) == 0x7FF) {
+          k.c = f64_is_nan_bits(bits) ? SUMF_NAN : ((bits >> 63) ? SUMF_NINF : SUMF_PINF);
+        } else {
+          u128 q = f64_to_fixed(fv, s.scale);
+          k.a = q.lo; k.b = q.hi;
+        }
+      } else {
+        k.a = (uint64_t)iv; k.b = (iv < 0) ? ~0ull : 0ull;
+      }
+      break;
+    default: {  // SK_MAX_CODE
+      uint64_t code;
+      if (is_float) {
+        if (fv != fv) { k.skip = true; break; }  // NaN never updates MIN/MAX
+        code = code_f64_bits(f64_bits(fv));
+      } else {
+        code = code_i64(iv);
+      }
+      k.a = s.is_min ? ~code : code;
+      break;
+    }
+  }
+  return k;
+}
+
+__device__ __forceinline__ void apply_glb(const DState &s, uint64_t *e, const Contrib &k) {
+  uint64_t *w = e + s.word;
+  switch (s.kind) {
+    case SK_COUNT: case SK_SUM_I64: g_atomic_add(w, k.a); break;
+    case SK_SUM_128:
+      g_add128(w, k.a, k.b);
+      if (s.fixed && k.c) g_atomic_or(w + 2, k.c);
+      break;
+    default: g_atomic_max(w, k.a); break;
+  }
+}
+
+__device__ __forceinline__ void apply_lds(const DState &s, lds_u64 *e, const Contrib &k) {
+  lds_u64 *w = e + s.word;
+  switch (s.kind) {
+    case SK_COUNT: case SK_SUM_I64: l_atomic_add(w, k.a); break;
+    case SK_SUM_128:
+      l_add128(w, k.a, k.b);
+      if (s.fixed && k.c) l_atomic_or(w + 2, k.c);
+      break;
+    default: l_atomic_max(w, k.a); break;
+  }
+}
+
+// merge one entry's states (non-atomic source) into a global entry
+__device__ __forceinline__ void merge_entry_glb(const DQuery &q, uint64_t *dst, const uint64_t *src) {
+  for (int i = 0; i < q.nstates; ++i) {
+    const DState &s = q.states[i];
+    const uint64_t *w = src + s.word;
+    uint64_t *d = dst + s.word;
+    switch (s.kind) {
+      case SK_COUNT: case SK_SUM_I64: if (w[0]) g_atomic_add(d, w[0]); break;
+      case SK_SUM_128:
+        g_add128(d, w[0], w[1]);
+        if (s.fixed && w[2]) g_atomic_or(d + 2, w[2]);
+        break;
+      default: if (w[0]) g_atomic_max(d, w[0]); break;
+    }
+  }
+}
+
+// ---- one query over the thread's 8 rows ----
+__device__ void agg_rows(const DScan &sc, const DQuery &q, const DCol *cols, uint64_t row0,
+                         uint32_t mask, lds_u64 *lds) {
+  const int W = q.entry_words;
+  // Resolve each selected row to an entry: bit 63 set = global index.
+  uint64_t ent[RPT];
+  if (q.key.ncols == 0) {
+    for (int r = 0; r < RPT; ++r) ent[r] = 0;  // LDS replica slot 0 / global entry 0
+    if (q.lmode == LM_NONE) for (int r = 0; r < RPT; ++r) ent[r] = (1ull << 63);
+  } else {
+    int64_t kv[MAX_KEYS][RPT];
+    uint32_t kn[MAX_KEYS] = {0, 0};
+#pragma unroll
+    for (int i = 0; i < MAX_KEYS; ++i) {
+      if (i >= q.key.ncols) break;
+      const DCol &c = cols[q.key.col[i]];
+      kn[i] = load8_nulls(c, row0);
+      if (q.key.is_float[i]) {
+        double f[RPT];
+        load8_f(c, row0, f);
+        for (int r = 0; r < RPT; ++r) kv[i][r] = (int64_t)canon_f64_key_bits(f64_bits(f[r]));
+      } else {
+        load8_i(c, row0, kv[i]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      ent[r] = ~0ull;
+      if (!((mask >> r) & 1)) continue;
+      int64_t v[MAX_KEYS] = {0, 0};
+      uint32_t n[MAX_KEYS] = {0, 0};
+#pragma unroll
+      for (int i = 0; i < MAX_KEYS; ++i) {
+        if (i < q.key.ncols) { v[i] = kv[i][r]; n[i] = (kn[i] >> r) & 1; }
+      }
+      if (q.gmode == GM_DIRECT) {
+        uint64_t idx = key_index_direct(q.key, v, n);
+        ent[r] = (q.lmode == LM_DIRECT) ? idx : (idx | (1ull << 63));
+      } else {
+        int side;
+        uint64_t stored = key_code_hash(q.key, v, n, &side);
+        if (side >= 0) { ent[r] = (q.gcap + (uint64_t)side) | (1ull << 63); continue; }
+        if (q.lmode == LM_HASH) {
+          const uint32_t lmask = (uint32_t)q.lds_entries - 1;
+          uint32_t h = (uint32_t)mix64(stored) & lmask;
+          for (int p = 0; p < LDS_HASH_PROBES; ++p) {
+            uint32_t slot = (h + p) & lmask;
+            lds_u64 *e = lds + (uint64_t)slot * W;
+            uint64_t k = *e;
+            if (k == 0) k = l_atomic_cas(e, 0, stored);
+            if (k == 0 || k == stored) { ent[r] = slot; break; }
+          }
+          if (ent[r] != ~0ull) continue;
+        }
+        uint64_t g = glb_find(q, stored);
+        ent[r] = (g == ~0ull) ? ~0ull : (g | (1ull << 63));
+      }
+    }
+  }
+  // LDS replica for the direct LDS table
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t rep_base = (q.lmode == LM_DIRECT)
+                                ? (uint64_t)(lane & (uint32_t)(q.lds_replicas - 1)) * q.lds_stride
+                                : 0;
+  for (int si = 0; si < q.nstates; ++si) {
+    const DState &s = q.states[si];
+    RowVals rv;
+    bool is_float = false;
+    if (s.col_a >= 0) {
+      fetch_state_vals(s, cols, row0, rv);
+      is_float = s.product || dt_is_float(cols[s.col_a].dtype);
+    } else {
+      rv.nm = 0;
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      if (!((mask >> r) & 1)) continue;
+      if ((rv.nm >> r) & 1) continue;  // null values do not feed aggregates
+      uint64_t e = ent[r];
+      if (e == ~0ull) continue;        // dropped on overflow (rerun follows)
+      Contrib k = contrib_of(s, is_float, (s.col_a >= 0 && !is_float) ? rv.iv[r] : 0,
+                             is_float ? rv.fv[r] : 0.0);
+      if (k.skip) continue;
+      if (e >> 63) {
+        apply_glb(s, q.table + (e & ~(1ull << 63)) * W, k);
+      } else {
+        apply_lds(s, lds + rep_base + e * W, k);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t lds_words_total(const DQuery *qs, int nq) {
+  uint64_t t = 0;
+  for (int i = 0; i < nq; ++i) {
+    uint64_t e = (uint64_t)qs[i].lds_offset + qs[i].lds_words;
+    if (e > t) t = e;
+  }
+  return t;
+}
+
+// Merge a workgroup's LDS table into the global table.
+__device__ void flush_lds(const DQuery &q, lds_u64 *lds) {
+  const int W = q.entry_words;
+  if (q.lmode == LM_DIRECT) {
+    const uint64_t nslots = (uint64_t)q.lds_entries;
+    for (uint64_t slot = threadIdx.x; slot < nslots; slot += blockDim.x) {
+      uint64_t acc[24];
+      for (int w = 0; w < W; ++w) acc[w] = 0;
+      for (int rep = 0; rep < q.lds_replicas; ++rep) {
+        lds_u64 *e = lds + (uint64_t)rep * q.lds_stride + slot * W;
+        for (int i = 0; i < q.nstates; ++i) {
+          const DState &s = q.states[i];
+          int w = s.word;
+          switch (s.kind) {
+            case SK_COUNT: case SK_SUM_I64: acc[w] += e[w]; break;
+            case SK_SUM_128: {
+              uint64_t lo = acc[w] + e[w];
+              acc[w + 1] += e[w + 1] + (lo < acc[w] ? 1 : 0);
+              acc[w] = lo;
+              if (s.fixed) acc[w + 2] |= e[w + 2];
+              break;
+            }
+            default: if (e[w] > acc[w]) acc[w] = e[w]; break;
+          }
+        }
+      }
+      if (acc[q.count_word] == 0) continue;  // no row reached this slot
+      merge_entry_glb(q, q.table + slot * W, acc);
+    }
+  } else if (q.lmode == LM_HASH) {
+    for (uint64_t slot = threadIdx.x; slot < (uint64_t)q.lds_entries; slot += blockDim.x) {
+      lds_u64 *e = lds + slot * W;
+      uint64_t stored = e[0];
+      if (stored == 0) continue;
+      uint64_t acc[24];
+      for (int w = 0; w < W; ++w) acc[w] = e[w];
+      if (acc[q.count_word] == 0) continue;
+      uint64_t g = glb_find(q, stored);
+      if (g == ~0ull) continue;
+      merge_entry_glb(q, q.table + g * W, acc);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *__restrict__ qs) {
+  extern __shared__ uint64_t lds_raw[];
+  // LDS: [0, MAX_QUERIES) selected-row counters, then the query tables.
+  lds_u64 *lds_cnt = (lds_u64 *)lds_raw;
+  lds_u64 *lds = lds_cnt + MAX_QUERIES;
+  const int nq = sc.nqueries;
+  const uint64_t lw = MAX_QUERIES + lds_words_total(qs, nq);
+  for (uint64_t i = threadIdx.x; i < lw; i += blockDim.x) lds_cnt[i] = 0;
+  __syncthreads();
+
+  const uint64_t tile_rows = (uint64_t)blockDim.x * RPT;
+  const uint64_t ntiles = (sc.nrows + tile_rows - 1) / tile_rows;
+
+  for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t row0 = tile * tile_rows + (uint64_t)threadIdx.x * RPT;
+    uint32_t valid = 0;
+    if (row0 < sc.nrows) {
+      uint64_t left = sc.nrows - row0;
+      valid = left >= RPT ? 0xFFu : ((1u << left) - 1);
+    }
+    if (sc.join_mode == 0) {
+      for (int qi = 0; qi < nq; ++qi) {
+        const DQuery &q = qs[qi];
+        uint32_t m = eval_query8(q, sc.cols, row0, valid);
+        if (m) l_atomic_add(lds_cnt + qi, (uint64_t)__builtin_popcount(m));
+        if (m && q.gmode != GM_NONE) agg_rows(sc, q, sc.cols, row0, m, lds + q.lds_offset);
+      }
+    } else {
+      // Fused probe: filter the probe rows, then per match aggregate with the
+      // build row's columns visible as side-1 columns (single query).
+      const DQuery &q = qs[0];
+      uint32_t m = eval_query8(q, sc.cols, row0, valid);
+      if (!m) continue;
+      const DCol &kc = sc.cols[sc.join_col];
+      int64_t kv[RPT];
+      if (sc.join_key_float) {
+        double f[RPT]; load8_f(kc, row0, f);
+        for (int r = 0; r < RPT; ++r) kv[r] = (int64_t)canon_f64_key_bits(f64_bits(f[r]));
+      } else {
+        load8_i(kc, row0, kv);
+      }
+      m &= ~load8_nulls(kc, row0);  // NULL never matches in joins
+      for (int r = 0; r < RPT; ++r) {
+        if (!((m >> r) & 1)) continue;
+        uint32_t start, count;
+        if (!join_lookup(sc, kv[r], &start, &count)) continue;
+        for (uint32_t j = 0; j < count; ++j) {
+          uint32_t brow = sc.join_rows[start + j];
+          l_atomic_add(lds_cnt, 1);
+          // One joined row: key and state values through single-row gathers.
+          const int W = q.entry_words;
+          uint64_t ent = ~0ull;
+          {
+            int64_t v[MAX_KEYS]; uint32_t n[MAX_KEYS];
+            for (int i = 0; i < q.key.ncols; ++i) {
+              const DCol &c = sc.cols[q.key.col[i]];
+              uint64_t row = c.side ? brow : (row0 + r);
+              n[i] = load1_null(c, row);
+              v[i] = q.key.is_float[i] ? (int64_t)canon_f64_key_bits(f64_bits(load1_f(c, row)))
+                                       : load1_i(c, row);
+            }
+            if (q.key.ncols == 0) {
+              ent = (q.lmode == LM_NONE) ? (1ull << 63) : 0;
+            } else if (q.gmode == GM_DIRECT) {
+              uint64_t idx = key_index_direct(q.key, v, n);
+              ent = (q.lmode == LM_DIRECT) ? idx : (idx | (1ull << 63));
+            } else {
+              int side;
+              uint64_t stored = key_code_hash(q.key, v, n, &side);
+              if (side >= 0) ent = (q.gcap + (uint64_t)side) | (1ull << 63);
+              else {
+                uint64_t g = glb_find(q, stored);
+                ent = (g == ~0ull) ? ~0ull : (g | (1ull << 63));
+              }
+            }
+          }
+          if (ent == ~0ull) continue;
+          const uint32_t lane = threadIdx.x & 63;
+          const uint64_t rep_base = (q.lmode == LM_DIRECT)
+              ? (uint64_t)(lane & (uint32_t)(q.lds_replicas - 1)) * q.lds_stride : 0;
+          for (int si = 0; si < q.nstates; ++si) {
+            const DState &s = q.states[si];
+            int64_t iv = 0; double fv = 0.0; bool isf = false; bool isnull = false;
+            if (s.col_a >= 0) {
+              const DCol &a = sc.cols[s.col_a];
+              uint64_t ra = a.side ? brow : (row0 + r);
+              isnull = load1_null(a, ra);
+              if (s.product) {
+                const DCol &b = sc.cols[s.col_b];
+                uint64_t rb = b.side ? brow : (row0 + r);
+                isnull = isnull || load1_null(b, rb);
+                double x = dt_is_float(a.dtype) ? load1_f(a, ra) : (double)load1_i(a, ra);
+                double y = dt_is_float(b.dtype) ? load1_f(b, rb) : (double)load1_i(b, rb);
+                fv = x * y; isf = true;
+              } else if (dt_is_float(a.dtype)) {
+                fv = load1_f(a, ra); isf = true;
+              } else {
+                iv = load1_i(a, ra);
+              }
+            }
+            if (isnull) continue;
+            Contrib k = contrib_of(s, isf, iv, fv);
+            if (k.skip) continue;
+            if (ent >> 63) apply_glb(s, q.table + (ent & ~(1ull << 63)) * W, k);
+            else apply_lds(s, lds + q.lds_offset + rep_base + ent * W, k);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // selected-row counters: one global atomic per workgroup per query
+  if (threadIdx.x < (unsigned)nq && lds_cnt[threadIdx.x])
+    g_atomic_add(qs[threadIdx.x].selected, lds_cnt[threadIdx.x]);
+  for (int qi = 0; qi < nq; ++qi) {
+    const DQuery &q = qs[qi];
+    if (q.lmode != LM_NONE) flush_lds(q, lds + q.lds_offset);
+  }
+}
+
+// Zero-fill (graph-safe replacement for a memset node).
+__global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t v) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace qm
+
+// ---- host launchers ----
+extern "C" hipError_t qmk_scan_agg(const qm::DScan *sc, const qm::DQuery *d_queries, int grid,
+                                   int block, size_t lds_bytes, hipStream_t st) {
+  if (lds_bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void *)qm::scan_agg_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(qm::scan_agg_kernel, dim3(grid), dim3(block), lds_bytes, st, *sc, d_queries);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(qm::fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}

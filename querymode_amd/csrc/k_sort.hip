@@ -1,0 +1,77 @@
+// k_sort.hip — full ORDER BY without LIMIT (InMemorySortOperator,
+// src/operators.ts:1847-1889): a stable device radix sort of order codes
+// (rocPRIM), NULL/NaN last, ties in input order — the order V8's stable
+// Array.prototype.sort gives with rowComparator (src/types.ts:26-38).
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include "k_common.hip.h"
+#include "qm_kernels.h"
+
+namespace qm {
+
+__global__ void sort_keys_kernel(DCol c, uint64_t n, int desc, uint64_t *keys, uint64_t *vals, uint8_t *cls) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    bool null = load1_null(c, i);
+    uint64_t code;
+    if (dt_is_float(c.dtype)) {
+      double v = load1_f(c, i);
+      if (v != v) null = true;
+      code = code_f64_bits(f64_bits(v));
+    } else {
+      code = code_i64(load1_i(c, i));
+    }
+    keys[i] = null ? 0 : (desc ? ~code : code);
+    cls[i] = null ? 1 : 0;
+    vals[i] = i;
+  }
+}
+
+__global__ void gather_cls_kernel(const uint8_t *cls, const uint64_t *idx, uint64_t n, uint8_t *out) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = cls[idx[i]];
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static size_t rocprim_bytes(uint64_t n) {
+  size_t a = 0, b = 0;
+  rocprim::radix_sort_pairs(nullptr, a, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                            (uint64_t *)nullptr, (size_t)n, 0, 64);
+  rocprim::radix_sort_pairs(nullptr, b, (uint8_t *)nullptr, (uint8_t *)nullptr, (uint64_t *)nullptr,
+                            (uint64_t *)nullptr, (size_t)n, 0, 1);
+  return a > b ? a : b;
+}
+
+extern "C" size_t qmk_sort_tmp_bytes(uint64_t n) {
+  return 4 * align256(n * 8) + 2 * align256(n) + align256(rocprim_bytes(n));
+}
+
+extern "C" hipError_t qmk_sort_keys(const DCol *c, uint64_t n, int desc, uint64_t *rows_out, void *tmp,
+                                    size_t tmp_bytes, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (tmp_bytes < qmk_sort_tmp_bytes(n)) return hipErrorInvalidValue;
+  char *p = static_cast<char *>(tmp);
+  uint64_t *k0 = (uint64_t *)p; p += align256(n * 8);
+  uint64_t *k1 = (uint64_t *)p; p += align256(n * 8);
+  uint64_t *v0 = (uint64_t *)p; p += align256(n * 8);
+  uint64_t *v1 = (uint64_t *)p; p += align256(n * 8);
+  uint8_t *c0 = (uint8_t *)p; p += align256(n);
+  uint8_t *c1 = (uint8_t *)p; p += align256(n);
+  size_t rb = rocprim_bytes(n);
+  void *rtmp = p;
+  unsigned grid = (unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256);
+  hipLaunchKernelGGL(sort_keys_kernel, dim3(grid), dim3(256), 0, st, *c, n, desc, k0, v0, c0);
+  hipError_t e = rocprim::radix_sort_pairs(rtmp, rb, k0, k1, v0, v1, (size_t)n, 0, 64, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(gather_cls_kernel, dim3(grid), dim3(256), 0, st, c0, v1, n, c1);
+  e = rocprim::radix_sort_pairs(rtmp, rb, c1, c0, v1, rows_out, (size_t)n, 0, 1, st);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}

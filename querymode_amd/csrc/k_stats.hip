@@ -1,0 +1,84 @@
+// k_stats.hip — per-column statistics computed once at upload: min/max
+// (the zone-map values the reference keeps per page, src/decode.ts:9-100),
+// the largest finite |x| (fixed-point scale of exact FP64 sums), NaN and
+// NULL counts, and a 4096-register HyperLogLog sketch of the distinct
+// values (hash-table sizing for GROUP BY and joins).
+#include "k_common.hip.h"
+#include "qm_kernels.h"
+
+namespace qm {
+
+// stats words: [0] max(~code) (MIN), [1] max(code) (MAX), [2] max finite |x| bits,
+// [3] NaN count, [4] NULL count, [5] non-null count
+__global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *hll) {
+  __shared__ uint32_t sh_hll[4096];
+  for (int i = threadIdx.x; i < 4096; i += blockDim.x) sh_hll[i] = 0;
+  __syncthreads();
+  uint64_t mn = 0, mx = 0, mabs = 0, nan = 0, nul = 0, nn = 0;
+  const bool isf = dt_is_float(c.dtype);
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    if (load1_null(c, i)) { ++nul; continue; }
+    ++nn;
+    uint64_t keybits;
+    if (isf) {
+      double v = load1_f(c, i);
+      uint64_t b = f64_bits(v);
+      keybits = canon_f64_key_bits(b);
+      if (v != v) { ++nan; }
+      else {
+        uint64_t code = code_f64_bits(b);
+        if (~code > mn) mn = ~code;
+        if (code > mx) mx = code;
+        if (!f64_is_inf_bits(b)) {
+          uint64_t a = b & 0x7FFFFFFFFFFFFFFFull;
+          if (a > mabs) mabs = a;
+        }
+      }
+    } else {
+      int64_t v = load1_i(c, i);
+      keybits = (uint64_t)v;
+      uint64_t code = code_i64(v);
+      if (~code > mn) mn = ~code;
+      if (code > mx) mx = code;
+    }
+    uint64_t h = mix64(keybits ^ 0x5bd1e9955bd1e995ull);
+    uint32_t reg = (uint32_t)(h >> 52);
+    uint64_t w = (h << 12) | (1ull << 11);
+    uint32_t rank = (uint32_t)__clzll((long long)w) + 1;
+    atomicMax(&sh_hll[reg], rank);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t t;
+    t = __shfl_xor(mn, off); if (t > mn) mn = t;
+    t = __shfl_xor(mx, off); if (t > mx) mx = t;
+    t = __shfl_xor(mabs, off); if (t > mabs) mabs = t;
+    nan += __shfl_xor(nan, off);
+    nul += __shfl_xor(nul, off);
+    nn += __shfl_xor(nn, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (mn) g_atomic_max(&stats[0], mn);
+    if (mx) g_atomic_max(&stats[1], mx);
+    if (mabs) g_atomic_max(&stats[2], mabs);
+    if (nan) g_atomic_add(&stats[3], nan);
+    if (nul) g_atomic_add(&stats[4], nul);
+    if (nn) g_atomic_add(&stats[5], nn);
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < 4096; r += blockDim.x)
+    if (sh_hll[r]) atomicMax(&hll[r], sh_hll[r]);
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" hipError_t qmk_col_stats(const DCol *c, uint64_t n, uint64_t *stats, uint32_t *hll, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t g = (n + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(col_stats_kernel, dim3((unsigned)g), dim3(256), 0, st, *c, n, stats, hll);
+  return hipGetLastError();
+}

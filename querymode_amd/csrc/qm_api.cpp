@@ -1,0 +1,1601 @@
+// qm_api.cpp — the C-ABI: contexts, resident tables, plan binding, the
+// executor that drives the gfx950 kernels, and QMCB result encoding.
+//
+// Execution mirrors the reference's pipeline assembly (src/operators.ts:
+// 3073-3150, src/local-executor.ts:386-508 and 597-667):
+//   filters -> [join probe] -> aggregate -> (TopK | sort | limit) -> project
+// with every stage on the device; the host only binds names, picks the
+// group-by strategy from column statistics and sizes the device tables.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <unordered_map>
+#include "qm_host.h"
+
+using namespace qm;
+
+// ===========================================================================
+// errors
+// ===========================================================================
+static thread_local std::string g_last_error;
+
+static int set_error(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define API_BEGIN try {
+#define API_END                                                         \
+  }                                                                     \
+  catch (const qm::Error &e) { return set_error(e.code, e.msg); }       \
+  catch (const std::bad_alloc &) { return set_error(QM_ERR_MEMORY_EXCEEDED, "host out of memory"); } \
+  catch (const std::exception &e) { return set_error(QM_ERR_QUERY_FAILED, e.what()); } \
+  catch (...) { return set_error(QM_ERR_QUERY_FAILED, "unknown failure"); }  \
+  return QM_OK;
+
+// ===========================================================================
+// device memory pool with the HBM budget
+// ===========================================================================
+namespace qm {
+size_t pool_round(size_t bytes) {
+  if (bytes < 256) bytes = 256;
+  const size_t big = 64ull << 20;
+  if (bytes <= big) {
+    size_t r = 256;
+    while (r < bytes) r <<= 1;
+    return r;
+  }
+  return (bytes + big - 1) / big * big;
+}
+}  // namespace qm
+
+void *qm_ctx::alloc(size_t bytes) {
+  {
+    std::lock_guard<std::mutex> g(pool_mu);
+    auto it = pool.find(bytes);
+    if (it != pool.end()) {
+      void *p = it->second;
+      pool.erase(it);
+      pooled -= bytes;
+      return p;
+    }
+  }
+  if (used.load() + bytes > budget) trim_pool();
+  if (used.load() + bytes > budget)
+    fail(QM_ERR_MEMORY_EXCEEDED, "HBM budget exceeded: %llu + %llu > %llu bytes", (unsigned long long)used.load(),
+         (unsigned long long)bytes, (unsigned long long)budget);
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    trim_pool();
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(QM_ERR_MEMORY_EXCEEDED, "device allocation of %llu bytes failed", (unsigned long long)bytes);
+    }
+  }
+  used += bytes;
+  return p;
+}
+
+void qm_ctx::release(void *p, size_t bytes) {
+  std::lock_guard<std::mutex> g(pool_mu);
+  pool.emplace(bytes, p);
+  pooled += bytes;
+}
+
+void qm_ctx::trim_pool() {
+  std::lock_guard<std::mutex> g(pool_mu);
+  for (auto &kv : pool) {
+    (void)hipFree(kv.second);
+    used -= kv.first;
+  }
+  pool.clear();
+  pooled = 0;
+}
+
+// ===========================================================================
+// helpers
+// ===========================================================================
+namespace {
+
+int dt_size(int dt) {
+  switch (dt) {
+    case DT_F64: case DT_I64: return 8;
+    case DT_I32: case DT_F32: return 4;
+    case DT_I16: return 2;
+    case DT_I8: case DT_BOOL: return 1;
+  }
+  return 0;
+}
+bool dt_int(int dt) { return dt == DT_I8 || dt == DT_I16 || dt == DT_I32 || dt == DT_I64 || dt == DT_BOOL; }
+bool dt_float(int dt) { return dt == DT_F64 || dt == DT_F32; }
+
+uint64_t pad_rows(uint64_t n) { return (n + ROW_PAD - 1) / ROW_PAD * ROW_PAD + ROW_PAD; }
+uint64_t pow2ceil(uint64_t x) { uint64_t r = 1; while (r < x) r <<= 1; return r; }
+
+double hll_estimate(const uint32_t *reg) {
+  const int m = 4096;
+  double sum = 0;
+  int zeros = 0;
+  for (int i = 0; i < m; ++i) { sum += std::ldexp(1.0, -(int)reg[i]); if (!reg[i]) ++zeros; }
+  double alpha = 0.7213 / (1.0 + 1.079 / m);
+  double e = alpha * m * (double)m / sum;
+  if (e <= 2.5 * m && zeros > 0) e = m * std::log((double)m / zeros);
+  return e;
+}
+
+// Device-time bracket on the context stream.
+struct Timer {
+  qm_ctx *c;
+  explicit Timer(qm_ctx *ctx) : c(ctx) { HIPX(hipEventRecord(c->ev0, c->stream)); }
+  double stop() {
+    HIPX(hipEventRecord(c->ev1, c->stream));
+    HIPX(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    return ms;
+  }
+};
+
+template <class T> void h2d(qm_ctx *c, void *dst, const T *src, size_t n) {
+  if (n) HIPX(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+}
+template <class T> void d2h(qm_ctx *c, T *dst, const void *src, size_t n) {
+  if (n) {
+    HIPX(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    HIPX(hipStreamSynchronize(c->stream));
+  }
+}
+void zero(qm_ctx *c, void *p, size_t bytes) {
+  HIPX(qmk_fill_u64(static_cast<uint64_t *>(p), (bytes + 7) / 8, 0, c->stream));
+}
+
+int qmcb_tag(int dt) {
+  switch (dt) {
+    case DT_F64: return QM_F64;
+    case DT_I64: return QM_I64;
+    case DT_I32: return QM_I32;
+    case DT_F32: return QM_F32;
+    case DT_BOOL: return QM_BOOL;
+  }
+  return QM_I32;
+}
+// result dtype of a gathered column (QMCB has no I8/I16 tag)
+int out_dt_of(int dt) { return (dt == DT_I8 || dt == DT_I16) ? DT_I32 : dt; }
+
+// ===========================================================================
+// column space: names -> DScan.cols slots (left = probe/scanned, right = build)
+// ===========================================================================
+struct Ref {
+  int slot_a = -1, slot_b = -1;  // product when slot_b >= 0
+  bool product() const { return slot_b >= 0; }
+};
+
+struct ColSpace {
+  const qm_table *L = nullptr, *R = nullptr;
+  std::string rkey;
+  const Plan *plan = nullptr;
+  std::vector<DCol> cols;
+  std::vector<const TCol *> tcols;
+
+  int slot_of(const TCol *t, int side) {
+    for (size_t i = 0; i < tcols.size(); ++i)
+      if (tcols[i] == t && cols[i].side == side) return (int)i;
+    if ((int)cols.size() >= MAX_COLS) fail(QM_ERR_UNSUPPORTED, "query references more than %d columns", MAX_COLS);
+    DCol c;
+    c.data = t->d;
+    c.nulls = t->dnull;
+    c.dtype = t->dtype;
+    c.side = side;
+    cols.push_back(c);
+    tcols.push_back(t);
+    return (int)cols.size() - 1;
+  }
+  // plain column of the (joined) row, src/operators.ts:2278-2293 naming
+  int plain(const std::string &name) {
+    int i = L->find(name);
+    if (i >= 0) return slot_of(&L->cols[i], 0);
+    if (R) {
+      if (name != rkey) {
+        int j = R->find(name);
+        if (j >= 0) return slot_of(&R->cols[j], 1);
+      }
+      if (name.rfind("right_", 0) == 0) {
+        std::string base = name.substr(6);
+        int j = R->find(base);
+        if (j >= 0 && base != rkey && L->find(base) >= 0) return slot_of(&R->cols[j], 1);
+      }
+    }
+    fail(QM_ERR_COLUMN_NOT_FOUND, "column not found: %s", name.c_str());
+  }
+  // aggregate input: plain column, computed product alias, or multiply(a,b)
+  Ref value(const std::string &name) {
+    Ref r;
+    for (const Computed &c : plan->computed) {
+      if (c.alias == name) { r.slot_a = plain(c.a); r.slot_b = plain(c.b); return r; }
+    }
+    if (name.rfind("multiply(", 0) == 0 && name.back() == ')') {
+      std::string in = name.substr(9, name.size() - 10);
+      size_t comma = in.find(',');
+      if (comma != std::string::npos) {
+        auto trim = [](std::string s) {
+          while (!s.empty() && s.front() == ' ') s.erase(s.begin());
+          while (!s.empty() && s.back() == ' ') s.pop_back();
+          return s;
+        };
+        r.slot_a = plain(trim(in.substr(0, comma)));
+        r.slot_b = plain(trim(in.substr(comma + 1)));
+        return r;
+      }
+    }
+    r.slot_a = plain(name);
+    return r;
+  }
+  const ColStats &st(int slot) const { return tcols[slot]->st; }
+  bool nullable(int slot) const { return tcols[slot]->st.nulls > 0; }
+};
+
+// ===========================================================================
+// binding filters into a DQuery
+// ===========================================================================
+struct ListStore {
+  std::vector<DevBuf> bufs;
+};
+
+void bind_filters(qm_ctx *ctx, ColSpace &cs, const std::vector<Filter> &ands,
+                  const std::vector<std::vector<Filter>> &groups, DQuery &q, ListStore &ls, bool right_side) {
+  q.npreds = 0;
+  q.ngroups = 0;
+  auto add = [&](const Filter &f, int group) {
+    if (q.npreds >= MAX_PREDS) fail(QM_ERR_UNSUPPORTED, "more than %d filter predicates", MAX_PREDS);
+    int slot;
+    if (right_side) {
+      int j = cs.R->find(f.col);
+      if (j < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "column not found: %s", f.col.c_str());
+      slot = cs.slot_of(&cs.R->cols[j], 0);
+    } else {
+      int j = cs.L->find(f.col);
+      if (j < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "column not found: %s", f.col.c_str());
+      slot = cs.slot_of(&cs.L->cols[j], 0);
+    }
+    BoundPred bp = coerce_filter(f, cs.cols[slot].dtype);
+    bp.p.col = (int8_t)slot;
+    bp.p.group = (uint8_t)group;
+    if (!bp.ilist.empty()) {
+      DevBuf b(ctx, bp.ilist.size() * 8);
+      h2d(ctx, b.p, bp.ilist.data(), bp.ilist.size());
+      bp.p.list = b.p;
+      bp.p.list_n = (int)bp.ilist.size();
+      ls.bufs.push_back(std::move(b));
+    } else if (!bp.flist.empty() || bp.p.kind == PK_F64_IN) {
+      size_t n = bp.flist.size();
+      DevBuf b(ctx, (n ? n : 1) * 8);
+      h2d(ctx, b.p, bp.flist.data(), n);
+      bp.p.list = b.p;
+      bp.p.list_n = (int)n;
+      ls.bufs.push_back(std::move(b));
+    }
+    q.preds[q.npreds++] = bp.p;
+  };
+  for (const Filter &f : ands) add(f, 0);
+  // An empty OR group is always true (Array.prototype.every on []).
+  bool trivially_true = false;
+  for (const auto &g : groups) if (g.empty()) trivially_true = true;
+  if (!groups.empty() && !trivially_true) {
+    if (groups.size() > 250) fail(QM_ERR_UNSUPPORTED, "too many OR groups");
+    int gi = 1;
+    for (const auto &g : groups) {
+      for (const Filter &f : g) add(f, gi);
+      ++gi;
+    }
+    q.ngroups = (int)groups.size();
+  }
+}
+
+// ===========================================================================
+// aggregate binding and strategy
+// ===========================================================================
+struct AggOut {
+  std::string name;
+  DOut o;
+};
+
+struct AggBinding {
+  DQuery q;
+  std::vector<AggOut> outs;
+  std::vector<int> key_slots;
+  bool always_one_row = false;
+  uint64_t nslots = 0;
+  uint64_t table_words = 0;
+};
+
+const int LDS_BLOCK_BUDGET_WORDS = 48 * 1024 / 8;  // 48 KiB of LDS per workgroup for tables
+
+struct StateReq {
+  uint8_t kind, is_min, fixed, product;
+  int8_t a, b;
+  int scale;
+};
+
+void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, bool fused_join, AggBinding &ab,
+                    uint32_t regrow) {
+  DQuery &q = ab.q;
+  std::vector<StateReq> reqs;
+  auto want = [&](StateReq r) -> int {
+    for (size_t i = 0; i < reqs.size(); ++i) {
+      const StateReq &x = reqs[i];
+      if (x.kind == r.kind && x.is_min == r.is_min && x.a == r.a && x.b == r.b && x.product == r.product)
+        return (int)i;
+    }
+    reqs.push_back(r);
+    return (int)reqs.size() - 1;
+  };
+  // state 0: row count (COUNT(*), occupancy)
+  want(StateReq{SK_COUNT, 0, 0, 0, -1, -1, 0});
+
+  // ---- keys ----
+  const int nk = (int)pl.groupby.size();
+  if (nk > MAX_KEYS) fail(QM_ERR_UNSUPPORTED, "GROUP BY over more than %d columns", MAX_KEYS);
+  DKey &key = q.key;
+  memset(&key, 0, sizeof key);
+  key.ncols = (int8_t)nk;
+  bool all_int = true;
+  long double dom_prod = 1;
+  double distinct_prod = 1;
+  for (int i = 0; i < nk; ++i) {
+    int s = cs.plain(pl.groupby[i]);
+    ab.key_slots.push_back(s);
+    key.col[i] = (int8_t)s;
+    const TCol *t = cs.tcols[s];
+    key.is_float[i] = dt_float(t->dtype) ? 1 : 0;
+    key.nullable[i] = t->st.nulls > 0 ? 1 : 0;
+    if (dt_float(t->dtype)) {
+      all_int = false;
+    } else {
+      uint64_t d = t->st.has_int ? (uint64_t)(t->st.imax - t->st.imin) + 1 : 0;
+      if (t->st.has_int && d == 0) d = ~0ull;  // full 64-bit range
+      key.kmin[i] = t->st.has_int ? t->st.imin : 0;
+      key.dom[i] = d + (key.nullable[i] ? 1 : 0);
+      if (key.dom[i] == 0) key.dom[i] = 1;
+      dom_prod *= (long double)key.dom[i];
+    }
+    distinct_prod *= std::max(1.0, t->st.distinct * 1.05 + (key.nullable[i] ? 1 : 0));
+  }
+
+  // ---- states and outputs ----
+  struct Pending { int req; int cnt_req; uint8_t kind; int out_dt; bool src_float; int scale; std::string name; };
+  std::vector<Pending> pend;
+  auto nonnull_count = [&](const Ref &r) -> int {
+    bool nullable = cs.nullable(r.slot_a) || (r.product() && cs.nullable(r.slot_b));
+    if (!nullable) return 0;
+    return want(StateReq{SK_COUNT, 0, 0, (uint8_t)(r.product() ? 1 : 0), (int8_t)r.slot_a, (int8_t)r.slot_b, 0});
+  };
+  for (const Agg &a : pl.aggs) {
+    if (a.fn >= AGG_COUNT_DISTINCT)
+      fail(QM_ERR_UNSUPPORTED, "aggregate %s: only sum/avg/min/max/count run on the device", a.alias.c_str());
+    if (a.col == "*") {
+      if (a.fn != AGG_COUNT) fail(QM_ERR_INVALID_AGGREGATE, "aggregate %s over '*'", a.alias.c_str());
+      pend.push_back(Pending{-1, 0, OK_COUNT, DT_I64, false, 0, a.alias});
+      continue;
+    }
+    Ref r = cs.value(a.col);
+    const int dta = cs.cols[r.slot_a].dtype;
+    const bool isf = r.product() || dt_float(dta);
+    if (dta == DT_BOOL && a.fn != AGG_COUNT)
+      fail(QM_ERR_UNSUPPORTED, "aggregate %s over a boolean column", a.alias.c_str());
+    int cnt = nonnull_count(r);
+    if (a.fn == AGG_COUNT) { pend.push_back(Pending{-1, cnt, OK_COUNT, DT_I64, false, 0, a.alias}); continue; }
+    if (a.fn == AGG_SUM || a.fn == AGG_AVG) {
+      int scale = 0;
+      StateReq sr{0, 0, 0, (uint8_t)(r.product() ? 1 : 0), (int8_t)r.slot_a, (int8_t)r.slot_b, 0};
+      uint8_t ok;
+      int odt;
+      if (isf) {
+        double bound = cs.st(r.slot_a).max_abs;
+        if (r.product()) {
+          const int dtb = cs.cols[r.slot_b].dtype;
+          double ba = dt_float(dta) ? cs.st(r.slot_a).max_abs
+                                    : std::max(std::fabs((double)cs.st(r.slot_a).imin), std::fabs((double)cs.st(r.slot_a).imax));
+          double bb = dt_float(dtb) ? cs.st(r.slot_b).max_abs
+                                    : std::max(std::fabs((double)cs.st(r.slot_b).imin), std::fabs((double)cs.st(r.slot_b).imax));
+          bound = ba * bb * 1.0000001;
+          if (!std::isfinite(bound)) bound = 1.7976931348623157e308;
+        }
+        scale = fixed_scale_for(bound);
+        sr.kind = SK_SUM_128; sr.fixed = 1; sr.scale = scale;
+        ok = (a.fn == AGG_SUM) ? OK_SUM_F64 : OK_AVG_F64;
+        odt = DT_F64;
+      } else if (dta == DT_I64) {
+        sr.kind = SK_SUM_128;
+        ok = (a.fn == AGG_SUM) ? OK_SUM_I128 : OK_AVG_BIG;
+        odt = (a.fn == AGG_SUM) ? DT_I64 : DT_F64;
+      } else {
+        sr.kind = SK_SUM_I64;
+        ok = (a.fn == AGG_SUM) ? OK_SUM_I64 : OK_AVG_I64;
+        odt = (a.fn == AGG_SUM) ? DT_I64 : DT_F64;
+      }
+      int ri = want(sr);
+      reqs[ri].scale = scale;
+      pend.push_back(Pending{ri, cnt, ok, odt, isf, scale, a.alias});
+      continue;
+    }
+    // MIN / MAX
+    StateReq sr{SK_MAX_CODE, (uint8_t)(a.fn == AGG_MIN ? 1 : 0), 0, (uint8_t)(r.product() ? 1 : 0),
+                (int8_t)r.slot_a, (int8_t)r.slot_b, 0};
+    int ri = want(sr);
+    int odt = r.product() ? DT_F64 : out_dt_of(dta);
+    pend.push_back(Pending{ri, cnt, (uint8_t)(a.fn == AGG_MIN ? OK_MIN : OK_MAX), odt, isf, 0, a.alias});
+  }
+  if (reqs.size() > (size_t)MAX_STATES) fail(QM_ERR_UNSUPPORTED, "too many aggregate states");
+
+  // ---- strategy ----
+  const bool direct_ok = nk == 0 || (all_int && dom_prod <= (long double)std::max<uint64_t>(1ull << 16, 2 * nrows) &&
+                                     dom_prod <= (long double)(1ull << 30));
+  q.gmode = direct_ok ? GM_DIRECT : GM_HASH;
+  const int key_words = (q.gmode == GM_HASH) ? 1 : 0;
+  int w = key_words;
+  q.nstates = (int)reqs.size();
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    DState &s = q.states[i];
+    memset(&s, 0, sizeof s);
+    s.kind = reqs[i].kind;
+    s.is_min = reqs[i].is_min;
+    s.fixed = reqs[i].fixed;
+    s.product = reqs[i].product;
+    s.col_a = reqs[i].a;
+    s.col_b = reqs[i].b;
+    s.scale = reqs[i].scale;
+    s.word = (int16_t)w;
+    w += (s.kind == SK_SUM_128) ? (s.fixed ? 3 : 2) : 1;
+  }
+  if (w > 24) fail(QM_ERR_UNSUPPORTED, "aggregate state too wide (%d words)", w);
+  q.entry_words = w;
+  q.count_word = q.states[0].word;
+
+  if (q.gmode == GM_DIRECT) {
+    uint64_t dom = 1;
+    for (int i = nk - 1; i >= 0; --i) { key.stride[i] = dom; dom *= key.dom[i]; }
+    q.gcap = dom;
+    ab.nslots = dom;
+    ab.always_one_row = (nk == 0);
+    // LDS: replicated dense table when the domain fits
+    uint64_t ent_words = dom * (uint64_t)w;
+    if (ent_words <= (uint64_t)LDS_BLOCK_BUDGET_WORDS) {
+      uint64_t stride = ent_words;
+      if (stride % 32 != 1) stride += (33 - stride % 32) % 32;  // bank spread between replicas
+      int reps = 1;
+      while (reps < 64 && (uint64_t)(reps * 2) * stride <= (uint64_t)LDS_BLOCK_BUDGET_WORDS) reps *= 2;
+      q.lmode = LM_DIRECT;
+      q.lds_entries = (int)dom;
+      q.lds_replicas = reps;
+      q.lds_stride = (int)stride;
+      q.lds_words = (int)(stride * reps);
+    } else {
+      q.lmode = LM_NONE;
+    }
+    ab.table_words = dom * w;
+  } else {
+    // HASH: size from the HLL distinct estimate (src/partial-agg.ts keeps one
+    // Map entry per distinct key; here the table is sized once up front).
+    if (!all_int && nk > 1) fail(QM_ERR_UNSUPPORTED, "multi-column GROUP BY with a float key");
+    if (nk == 2) {
+      for (int i = 0; i < 2; ++i) {
+        int bits = 1;
+        while (bits < 63 && (bits >= 64 ? false : ((1ull << bits) < key.dom[i]))) ++bits;
+        key.bits[i] = bits;
+      }
+      if (key.bits[0] + key.bits[1] > 63) fail(QM_ERR_UNSUPPORTED, "GROUP BY key domain exceeds 63 bits");
+    }
+    double est = std::min((double)nrows, distinct_prod);
+    uint64_t cap = pow2ceil((uint64_t)std::max(2048.0, est * 2.0));
+    cap <<= (2 * regrow);
+    q.gcap = cap;
+    ab.nslots = cap + 2;
+    ab.table_words = (cap + 2) * (uint64_t)w;
+    // LDS cache for skewed keys
+    int lw = LDS_BLOCK_BUDGET_WORDS / w;
+    int le = 1;
+    while (le * 2 <= lw) le *= 2;
+    bool skewed = false;
+    for (int s : ab.key_slots) if (cs.st(s).dup_frac >= 0.05) skewed = true;
+    if (est <= (double)le * 8) skewed = true;
+    if (skewed && !fused_join && le >= 64) {
+      q.lmode = LM_HASH;
+      q.lds_entries = le;
+      q.lds_words = le * w;
+    } else {
+      q.lmode = LM_NONE;
+    }
+  }
+  if (fused_join && q.lmode == LM_HASH) q.lmode = LM_NONE;
+
+  // ---- outputs ----
+  for (int i = 0; i < nk; ++i) {
+    AggOut ao;
+    ao.name = pl.groupby[i];
+    memset(&ao.o, 0, sizeof ao.o);
+    ao.o.kind = OK_KEY;
+    ao.o.key_idx = (uint8_t)i;
+    const int dt = cs.cols[ab.key_slots[i]].dtype;
+    ao.o.out_dtype = dt_float(dt) ? DT_F64 : out_dt_of(dt);
+    ao.o.cnt_word = -1;
+    ab.outs.push_back(ao);
+  }
+  for (const Pending &p : pend) {
+    AggOut ao;
+    ao.name = p.name;
+    memset(&ao.o, 0, sizeof ao.o);
+    ao.o.kind = p.kind;
+    ao.o.out_dtype = p.out_dt;
+    ao.o.src_is_float = p.src_float ? 1 : 0;
+    ao.o.scale = p.scale;
+    ao.o.word = (p.req >= 0) ? q.states[p.req].word : 0;
+    ao.o.cnt_word = q.states[p.cnt_req].word;
+    ab.outs.push_back(ao);
+  }
+  if ((int)ab.outs.size() > MAX_OUTS) fail(QM_ERR_UNSUPPORTED, "too many output columns");
+  (void)ctx;
+}
+
+// ===========================================================================
+// result assembly helpers
+// ===========================================================================
+OutCol make_out(qm_ctx *ctx, const std::string &name, int dt, uint64_t rows, bool with_nulls) {
+  OutCol oc;
+  oc.name = name;
+  oc.dtype = dt;
+  uint64_t pr = pad_rows(rows);
+  oc.data = DevBuf(ctx, pr * dt_size(dt));
+  if (with_nulls) {
+    oc.nulls = DevBuf(ctx, pr / 8 + 64);
+    zero(ctx, oc.nulls.p, pr / 8 + 64);
+  }
+  return oc;
+}
+
+DCol dcol_of(const OutCol &o) {
+  DCol c;
+  c.data = o.data.p;
+  c.nulls = o.nulls.as<uint8_t>();
+  c.dtype = o.dtype;
+  c.side = 0;
+  return c;
+}
+
+// Gather every column of `res` through `rows` (n entries) into a new result.
+void permute_result(qm_result *res, const uint64_t *d_rows, uint64_t n) {
+  qm_ctx *ctx = res->ctx;
+  std::vector<OutCol> nc;
+  for (OutCol &o : res->cols) {
+    OutCol x = make_out(ctx, o.name, o.dtype, n, o.nulls.p != nullptr);
+    DCol c = dcol_of(o);
+    HIPX(qmk_gather(&c, d_rows, nullptr, n, o.dtype, x.data.p, x.nulls.as<uint32_t>(), ctx->stream));
+    nc.push_back(std::move(x));
+  }
+  res->cols = std::move(nc);
+  res->rows = n;
+}
+
+// rows of a sorted/limited order over one column `c` of length n:
+// returns device row ids (ascending order positions) for [offset, offset+limit)
+DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t limit, int64_t offset, uint64_t *out_n,
+                  const DQuery *d_noq, uint32_t *launches) {
+  uint64_t start = (uint64_t)offset;
+  uint64_t want = (limit >= 0) ? (uint64_t)limit : (n > start ? n - start : 0);
+  uint64_t avail = n > start ? n - start : 0;
+  uint64_t take = std::min(want, avail);
+  *out_n = take;
+  DevBuf rows(ctx, std::max<uint64_t>(take, 1) * 8);
+  if (take == 0) return rows;
+  uint64_t k = start + take;
+  if (limit >= 0 && qmk_topk_cap((uint32_t)std::min<uint64_t>(k, 1u << 30)) && k < (1u << 30)) {
+    DScan sc;
+    memset(&sc, 0, sizeof sc);
+    sc.nrows = n;
+    sc.ncols = 1;
+    sc.nqueries = 1;
+    sc.cols[0] = c;
+    const uint64_t tiles = (n + 2047) / 2048;
+    int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * 2);
+    if (grid < 1) grid = 1;
+    DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
+    DevBuf counts(ctx, (size_t)grid * 4 + 64);
+    HIPX(qmk_topk_scan(&sc, d_noq, 0, desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(), counts.as<uint32_t>(),
+                       ctx->stream));
+    ++*launches;
+    const TopKey *lists = cand.as<TopKey>();
+    const uint32_t *lcounts = counts.as<uint32_t>();
+    uint32_t nlists = (uint32_t)grid;
+    DevBuf m1, m1c;
+    if (nlists > 16) {
+      uint32_t g2 = (nlists + 15) / 16;
+      m1 = DevBuf(ctx, (size_t)g2 * k * sizeof(TopKey));
+      m1c = DevBuf(ctx, (size_t)g2 * 4 + 64);
+      HIPX(qmk_topk_merge(lists, lcounts, nlists, 16, (uint32_t)k, m1.as<TopKey>(), m1c.as<uint32_t>(), ctx->stream));
+      ++*launches;
+      lists = m1.as<TopKey>();
+      lcounts = m1c.as<uint32_t>();
+      nlists = g2;
+    }
+    DevBuf fin(ctx, (size_t)k * sizeof(TopKey));
+    DevBuf finc(ctx, 64);
+    HIPX(qmk_topk_merge(lists, lcounts, nlists, nlists, (uint32_t)k, fin.as<TopKey>(), finc.as<uint32_t>(),
+                        ctx->stream));
+    HIPX(qmk_topk_rows(fin.as<TopKey>(), finc.as<uint32_t>(), (uint32_t)start, rows.as<uint64_t>(), (uint32_t)take,
+                       ctx->stream));
+    *launches += 2;
+    return rows;
+  }
+  // full stable sort, then slice
+  size_t tb = qmk_sort_tmp_bytes(n);
+  DevBuf tmp(ctx, tb);
+  DevBuf all(ctx, n * 8);
+  HIPX(qmk_sort_keys(&c, n, desc ? 1 : 0, all.as<uint64_t>(), tmp.p, tmp.n, ctx->stream));
+  *launches += 4;
+  HIPX(hipMemcpyAsync(rows.p, all.as<uint64_t>() + start, take * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  return rows;
+}
+
+int find_out(const qm_result *r, const std::string &name) {
+  for (size_t i = 0; i < r->cols.size(); ++i) if (r->cols[i].name == name) return (int)i;
+  return -1;
+}
+
+// ORDER BY / LIMIT / OFFSET over a materialised result
+void sort_limit_result(qm_result *res, const Plan &pl, const DQuery *d_noq, uint32_t *launches) {
+  qm_ctx *ctx = res->ctx;
+  if (pl.has_sort) {
+    int ci = find_out(res, pl.sort_col);
+    if (ci < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "sort column not found: %s", pl.sort_col.c_str());
+    uint64_t n_out = 0;
+    DCol c = dcol_of(res->cols[ci]);
+    DevBuf rows = order_rows(ctx, c, res->rows, pl.desc, pl.limit, pl.offset, &n_out, d_noq, launches);
+    permute_result(res, rows.as<uint64_t>(), n_out);
+    return;
+  }
+  if (pl.limit >= 0 || pl.offset > 0) {
+    uint64_t start = std::min<uint64_t>((uint64_t)pl.offset, res->rows);
+    uint64_t take = res->rows - start;
+    if (pl.limit >= 0) take = std::min<uint64_t>(take, (uint64_t)pl.limit);
+    std::vector<uint64_t> idx(take);
+    for (uint64_t i = 0; i < take; ++i) idx[i] = start + i;
+    DevBuf rows(ctx, std::max<uint64_t>(take, 1) * 8);
+    h2d(ctx, rows.p, idx.data(), take);
+    permute_result(res, rows.as<uint64_t>(), take);
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// join index
+// ===========================================================================
+namespace {
+struct JoinIndex {
+  DevBuf table, rows, cursor, tmp;
+  int mode = 0;      // 1 DIRECT, 2 HASH
+  uint64_t cap = 0;
+  int64_t kmin = 0;
+  int key_float = 0;
+  uint64_t build_rows = 0;
+};
+
+DevBuf select_rows(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, uint64_t limit_cap, uint64_t *n_out,
+                   uint32_t *launches);
+
+void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32_t *launches, DevBuf *keep) {
+  const qm_table *R = cs.R;
+  int rk = R->find(pl.rkey);
+  if (rk < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "join key not found: %s", pl.rkey.c_str());
+  int lk = cs.L->find(pl.lkey);
+  if (lk < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "join key not found: %s", pl.lkey.c_str());
+  const TCol &kc = R->cols[rk];
+  const TCol &lc = cs.L->cols[lk];
+  if (dt_float(kc.dtype) != dt_float(lc.dtype))
+    fail(QM_ERR_UNSUPPORTED, "join keys of mixed integer/float type");
+  ji.key_float = dt_float(kc.dtype) ? 1 : 0;
+  // right-side filters (src/local-executor.ts:631-635)
+  const uint64_t *sel = nullptr;
+  uint64_t nbuild = R->rows;
+  if (!pl.rfilters.empty() || !pl.rgroups.empty()) {
+    ColSpace rcs;
+    rcs.L = R;
+    rcs.plan = &pl;
+    DQuery q;
+    memset(&q, 0, sizeof q);
+    ListStore ls;
+    bind_filters(ctx, rcs, pl.rfilters, pl.rgroups, q, ls, false);
+    DevBuf dq(ctx, sizeof(DQuery));
+    h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
+    DScan sc;
+    memset(&sc, 0, sizeof sc);
+    sc.nrows = R->rows;
+    sc.ncols = (int)rcs.cols.size();
+    sc.nqueries = 1;
+    for (size_t i = 0; i < rcs.cols.size(); ++i) sc.cols[i] = rcs.cols[i];
+    *keep = select_rows(ctx, sc, dq.as<DQuery>(), ~0ull, &nbuild, launches);
+    sel = keep->as<uint64_t>();
+    HIPX(hipStreamSynchronize(ctx->stream));
+  }
+  ji.build_rows = nbuild;
+  DCol key;
+  key.data = kc.d;
+  key.nulls = kc.dnull;
+  key.dtype = kc.dtype;
+  key.side = 1;
+  uint64_t dom = 0;
+  if (!ji.key_float && kc.st.has_int) dom = (uint64_t)(kc.st.imax - kc.st.imin) + 1;
+  if (!ji.key_float && kc.st.has_int && dom != 0 && dom <= std::max<uint64_t>(4 * nbuild, 1u << 16) &&
+      dom <= (1ull << 31)) {
+    ji.mode = 1;
+    ji.cap = dom;
+    ji.kmin = kc.st.imin;
+  } else {
+    ji.mode = 2;
+    ji.cap = pow2ceil(std::max<uint64_t>(1024, 2 * nbuild));
+  }
+  uint64_t nent = (ji.mode == 1) ? ji.cap : ji.cap + 1;
+  uint64_t twords = (ji.mode == 1) ? ji.cap : 2 * (ji.cap + 1);
+  ji.table = DevBuf(ctx, twords * 8);
+  zero(ctx, ji.table.p, twords * 8);
+  ji.rows = DevBuf(ctx, std::max<uint64_t>(nbuild, 1) * 4);
+  ji.cursor = DevBuf(ctx, nent * 4 + 64);
+  zero(ctx, ji.cursor.p, nent * 4 + 64);
+  ji.tmp = DevBuf(ctx, (2 * nent + qmk_exscan_tmp_words(nent)) * 8);
+  DevBuf status(ctx, 64);
+  zero(ctx, status.p, 64);
+  HIPX(qmk_join_build(&key, nbuild, sel, ji.mode, ji.key_float, ji.kmin, ji.cap, ji.table.as<uint64_t>(),
+                      ji.rows.as<uint32_t>(), ji.cursor.as<uint32_t>(), ji.tmp.as<uint64_t>(), status.as<uint32_t>(),
+                      ctx->stream));
+  *launches += 8;
+  uint32_t st[2];
+  d2h(ctx, st, status.p, 2);
+  if (st[0]) fail(QM_ERR_QUERY_FAILED, "join hash table overflow");
+}
+
+void attach_join(DScan &sc, const JoinIndex &ji, int probe_slot) {
+  sc.join_table = ji.table.as<uint64_t>();
+  sc.join_rows = ji.rows.as<uint32_t>();
+  sc.join_cap = ji.cap;
+  sc.join_min = ji.kmin;
+  sc.join_col = probe_slot;
+  sc.join_mode = ji.mode;
+  sc.join_key_float = ji.key_float;
+}
+
+// stream compaction: ascending row ids of rows passing the query
+DevBuf select_rows(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, uint64_t limit_cap, uint64_t *n_out,
+                   uint32_t *launches) {
+  const uint64_t tile_rows = 256 * RPT;
+  const uint64_t ntiles = std::max<uint64_t>(1, (sc.nrows + tile_rows - 1) / tile_rows);
+  uint64_t nblocks = std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 8);
+  uint64_t tpb = (ntiles + nblocks - 1) / nblocks;
+  nblocks = (ntiles + tpb - 1) / tpb;
+  DevBuf counts(ctx, nblocks * 8), offs(ctx, nblocks * 8), stmp(ctx, qmk_exscan_tmp_words(nblocks) * 8 + 64);
+  HIPX(qmk_select_count(&sc, d_q, nblocks, tpb, counts.as<uint64_t>(), ctx->stream));
+  uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nblocks);
+  HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(), ctx->stream));
+  uint64_t total = 0;
+  d2h(ctx, &total, total_d, 1);
+  uint64_t n = std::min(total, limit_cap);
+  DevBuf rows(ctx, std::max<uint64_t>(n, 1) * 8 + 64);
+  if (n) HIPX(qmk_select_write(&sc, d_q, nblocks, tpb, offs.as<uint64_t>(), n, rows.as<uint64_t>(), ctx->stream));
+  *launches += 5;
+  *n_out = n;
+  return rows;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
+  API_BEGIN
+  if (!out) fail(QM_ERR_INVALID_ARG, "null out");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    fail(QM_ERR_QUERY_FAILED, "no HIP device available");
+  }
+  if (device < 0 || device >= n) fail(QM_ERR_INVALID_ARG, "device %d out of range (%d devices)", device, n);
+  HIPX(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  HIPX(hipGetDeviceProperties(&prop, device));
+  std::unique_ptr<qm_ctx> c(new qm_ctx());
+  c->device = device;
+  size_t fr = 0, tot = 0;
+  HIPX(hipMemGetInfo(&fr, &tot));
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(tot * 0.9);
+  c->num_cus = prop.multiProcessorCount;
+  c->name = prop.gcnArchName;
+  HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPX(hipEventCreate(&c->ev0));
+  HIPX(hipEventCreate(&c->ev1));
+  HIPX(hipEventCreate(&c->evk0));
+  HIPX(hipEventCreate(&c->evk1));
+  *out = c.release();
+  API_END
+}
+
+extern "C" int qm_shutdown(qm_ctx *ctx) {
+  API_BEGIN
+  if (!ctx) return QM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->trim_pool();
+  (void)hipEventDestroy(ctx->ev0);
+  (void)hipEventDestroy(ctx->ev1);
+  (void)hipEventDestroy(ctx->evk0);
+  (void)hipEventDestroy(ctx->evk1);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  API_END
+}
+
+extern "C" int qm_last_error(qm_ctx *ctx, char *buf, size_t cap) {
+  (void)ctx;
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, g_last_error.size());
+    memcpy(buf, g_last_error.data(), n);
+    buf[n] = 0;
+  }
+  return (int)g_last_error.size();
+}
+
+extern "C" int qm_device_info(qm_ctx *ctx, char *name, size_t cap, uint32_t *num_cus, uint64_t *budget,
+                              uint64_t *used) {
+  API_BEGIN
+  if (!ctx) fail(QM_ERR_INVALID_ARG, "null context");
+  if (name && cap) {
+    size_t n = std::min(cap - 1, ctx->name.size());
+    memcpy(name, ctx->name.data(), n);
+    name[n] = 0;
+  }
+  if (num_cus) *num_cus = (uint32_t)ctx->num_cus;
+  if (budget) *budget = ctx->budget;
+  if (used) *used = ctx->used.load() - ctx->pooled;
+  API_END
+}
+
+extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, uint32_t ncols,
+                               const qm_column_desc *cols, qm_table **out) {
+  API_BEGIN
+  if (!ctx || !out || (ncols && !cols)) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (nrows >= (1ull << 40)) fail(QM_ERR_INVALID_ARG, "table too large");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  std::unique_ptr<qm_table> t(new qm_table());
+  t->ctx = ctx;
+  t->name = name ? name : "";
+  t->rows = nrows;
+  struct Cleanup {
+    qm_table *t;
+    bool armed = true;
+    ~Cleanup() {
+      if (!armed) return;
+      for (auto &c : t->cols) {
+        if (c.d) t->ctx->release(c.d, c.bytes);
+        if (c.dnull) t->ctx->release(c.dnull, c.null_bytes);
+      }
+    }
+  } cleanup{t.get()};
+  const uint64_t pr = pad_rows(nrows);
+  DevBuf stats(ctx, 8 * 8), hll(ctx, 4096 * 4);
+  for (uint32_t i = 0; i < ncols; ++i) {
+    const qm_column_desc &cd = cols[i];
+    int dt = cd.dtype;
+    if (dt != QM_F64 && dt != QM_I64 && dt != QM_I32 && dt != QM_F32 && dt != QM_I8 && dt != QM_I16 && dt != QM_BOOL)
+      fail(QM_ERR_UNSUPPORTED, "column %s: dtype %d is not on the device path", cd.name ? cd.name : "?", dt);
+    if (!cd.name) fail(QM_ERR_INVALID_ARG, "column %u has no name", i);
+    if (nrows && !cd.data) fail(QM_ERR_INVALID_ARG, "column %s has no data", cd.name);
+    for (auto &c : t->cols) if (c.name == cd.name) fail(QM_ERR_SCHEMA_MISMATCH, "duplicate column %s", cd.name);
+    TCol c;
+    c.name = cd.name;
+    c.dtype = dt;
+    const int es = dt_size(dt);
+    c.bytes = pool_round(pr * es);
+    c.d = ctx->alloc(c.bytes);
+    t->cols.push_back(c);
+    TCol &tc = t->cols.back();
+    zero(ctx, tc.d, pr * es);
+    if (nrows) HIPX(hipMemcpyAsync(tc.d, cd.data, nrows * es, hipMemcpyHostToDevice, ctx->stream));
+    bool any_null = false;
+    if (cd.null_bitmap) {
+      for (uint64_t b = 0; b < (nrows + 7) / 8 && !any_null; ++b) {
+        uint8_t m = cd.null_bitmap[b];
+        if (b == nrows / 8 && (nrows & 7)) m &= (uint8_t)((1u << (nrows & 7)) - 1);
+        if (m) any_null = true;
+      }
+    }
+    if (any_null) {
+      tc.null_bytes = pool_round(pr / 8 + 64);
+      tc.dnull = static_cast<uint8_t *>(ctx->alloc(tc.null_bytes));
+      zero(ctx, tc.dnull, pr / 8 + 64);
+      std::vector<uint8_t> nb(cd.null_bitmap, cd.null_bitmap + (nrows + 7) / 8);
+      if (nrows & 7) nb.back() &= (uint8_t)((1u << (nrows & 7)) - 1);
+      HIPX(hipMemcpyAsync(tc.dnull, nb.data(), nb.size(), hipMemcpyHostToDevice, ctx->stream));
+      HIPX(hipStreamSynchronize(ctx->stream));
+    }
+    // statistics (min/max, max |x|, NaN/NULL counts, HLL)
+    zero(ctx, stats.p, 64);
+    zero(ctx, hll.p, 4096 * 4);
+    DCol dc;
+    dc.data = tc.d;
+    dc.nulls = tc.dnull;
+    dc.dtype = dt;
+    dc.side = 0;
+    HIPX(qmk_col_stats(&dc, nrows, stats.as<uint64_t>(), hll.as<uint32_t>(), ctx->stream));
+    uint64_t sw[8];
+    std::vector<uint32_t> regs(4096);
+    d2h(ctx, sw, stats.p, 8);
+    d2h(ctx, regs.data(), hll.p, 4096);
+    ColStats &s = tc.st;
+    s.nan = sw[3];
+    s.nulls = sw[4];
+    s.nonnull = sw[5];
+    if (dt_float(dt)) {
+      if (sw[1]) {
+        s.has_f = true;
+        s.fmin = bits_f64(decode_f64_code(~sw[0]));
+        s.fmax = bits_f64(decode_f64_code(sw[1]));
+      }
+      s.max_abs = bits_f64(sw[2]);
+    } else if (s.nonnull) {
+      s.has_int = true;
+      s.imin = decode_i64(~sw[0]);
+      s.imax = decode_i64(sw[1]);
+      s.max_abs = std::max(std::fabs((double)s.imin), std::fabs((double)s.imax));
+    }
+    s.distinct = s.nonnull ? std::max(1.0, hll_estimate(regs.data())) : 0.0;
+    // skew probe on a strided host sample
+    if (nrows >= 2) {
+      const uint64_t ns = std::min<uint64_t>(nrows, 16384);
+      std::vector<uint64_t> smp(ns);
+      const uint8_t *src = static_cast<const uint8_t *>(cd.data);
+      for (uint64_t k = 0; k < ns; ++k) {
+        uint64_t r = (ns == nrows) ? k : (k * (nrows / ns));
+        uint64_t v = 0;
+        memcpy(&v, src + r * es, es);
+        smp[k] = v;
+      }
+      std::sort(smp.begin(), smp.end());
+      uint64_t dup = 0;
+      for (uint64_t k = 0; k < ns; ++k) {
+        bool same = (k > 0 && smp[k] == smp[k - 1]) || (k + 1 < ns && smp[k] == smp[k + 1]);
+        if (same) ++dup;
+      }
+      s.dup_frac = (double)dup / (double)ns;
+    }
+  }
+  HIPX(hipStreamSynchronize(ctx->stream));
+  cleanup.armed = false;
+  *out = t.release();
+  API_END
+}
+
+extern "C" int qm_table_drop(qm_table *t) {
+  API_BEGIN
+  if (!t) return QM_OK;
+  qm_ctx *ctx = t->ctx;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipStreamSynchronize(ctx->stream));
+  for (auto &c : t->cols) {
+    if (c.d) ctx->release(c.d, c.bytes);
+    if (c.dnull) ctx->release(c.dnull, c.null_bytes);
+  }
+  delete t;
+  API_END
+}
+
+extern "C" uint64_t qm_table_rows(const qm_table *t) { return t ? t->rows : 0; }
+
+extern "C" int qm_plan_compile(qm_ctx *ctx, const uint8_t *plan, size_t len, qm_plan **out) {
+  API_BEGIN
+  if (!ctx || !out || (!plan && len)) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  std::unique_ptr<qm_plan> p(new qm_plan());
+  p->ctx = ctx;
+  p->plan = parse_plan(plan, len);
+  if (p->plan.has_join && p->plan.jtype != 0)
+    fail(QM_ERR_UNSUPPORTED, "only inner joins run on the device");
+  *out = p.release();
+  API_END
+}
+
+extern "C" int qm_plan_close(qm_plan *p) {
+  delete p;
+  return QM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// execution
+// ---------------------------------------------------------------------------
+namespace {
+
+uint64_t referenced_bytes(const ColSpace &cs, uint64_t lrows, uint64_t rrows) {
+  uint64_t b = 0;
+  for (size_t i = 0; i < cs.cols.size(); ++i)
+    b += (uint64_t)dt_size(cs.cols[i].dtype) * (cs.cols[i].side ? rrows : lrows);
+  return b;
+}
+
+void scan_launch_shape(qm_ctx *ctx, uint64_t nrows, uint64_t lds_words, int *grid, int *block, size_t *lds_bytes) {
+  *block = 256;
+  const uint64_t tile = 256 * RPT;
+  uint64_t tiles = std::max<uint64_t>(1, (nrows + tile - 1) / tile);
+  *lds_bytes = (size_t)(lds_words + MAX_QUERIES) * 8;
+  int per_cu = 4;
+  if (*lds_bytes > 0) per_cu = std::max(1, std::min(4, (int)((160 * 1024) / std::max<size_t>(*lds_bytes, 1))));
+  *grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * per_cu);
+}
+
+// Run one or more aggregate queries (same table) through the fused scan and
+// finalise each into a result.
+void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std::vector<const Plan *> &plans,
+                    std::vector<qm_result *> &results, bool fused_join, JoinIndex *ji) {
+  const size_t nq = plans.size();
+  std::vector<ColSpace> spaces(1);
+  ColSpace &cs = spaces[0];
+  cs.L = T;
+  cs.R = fused_join ? R : nullptr;
+  cs.rkey = fused_join ? plans[0]->rkey : "";
+  std::vector<AggBinding> ab(nq);
+  std::vector<ListStore> ls(nq);
+  std::vector<uint32_t> regrow(nq, 0);
+  for (uint32_t attempt = 0;; ++attempt) {
+    Timer tm(ctx);
+    uint32_t launches = 0;
+    std::vector<DevBuf> tables(nq), status(nq), selected(nq);
+    uint64_t lds_off = 0;
+    for (size_t i = 0; i < nq; ++i) {
+      cs.plan = plans[i];
+      memset(&ab[i].q, 0, sizeof(DQuery));
+      ab[i].outs.clear();
+      ab[i].key_slots.clear();
+      ls[i].bufs.clear();
+      bind_filters(ctx, cs, plans[i]->filters, plans[i]->groups, ab[i].q, ls[i], false);
+      bind_aggregate(ctx, cs, *plans[i], T->rows, fused_join, ab[i], regrow[i]);
+      DQuery &q = ab[i].q;
+      tables[i] = DevBuf(ctx, ab[i].table_words * 8);
+      zero(ctx, tables[i].p, ab[i].table_words * 8);
+      ++launches;
+      status[i] = DevBuf(ctx, 64);
+      selected[i] = DevBuf(ctx, 64);
+      zero(ctx, status[i].p, 64);
+      zero(ctx, selected[i].p, 64);
+      q.table = tables[i].as<uint64_t>();
+      q.status = status[i].as<uint32_t>();
+      q.selected = selected[i].as<uint64_t>();
+      q.lds_offset = (int)lds_off;
+      lds_off += (uint64_t)q.lds_words;
+    }
+    // launch
+    DScan sc;
+    memset(&sc, 0, sizeof sc);
+    sc.nrows = T->rows;
+    sc.ncols = (int)cs.cols.size();
+    sc.nqueries = (int)nq;
+    for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+    if (fused_join) {
+      int ps = cs.plain(plans[0]->lkey);
+      sc.ncols = (int)cs.cols.size();
+      for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+      attach_join(sc, *ji, ps);
+    }
+    std::vector<DQuery> hq(nq);
+    for (size_t i = 0; i < nq; ++i) hq[i] = ab[i].q;
+    DevBuf dq(ctx, nq * sizeof(DQuery));
+    h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(hq.data()), nq * sizeof(DQuery));
+    int grid, block;
+    size_t lds_bytes;
+    scan_launch_shape(ctx, T->rows, lds_off, &grid, &block, &lds_bytes);
+    if (lds_bytes > 160 * 1024) fail(QM_ERR_UNSUPPORTED, "fused queries exceed the workgroup LDS");
+    if (T->rows) {
+      HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+      HIPX(qmk_scan_agg(&sc, dq.as<DQuery>(), grid, block, lds_bytes, ctx->stream));
+      HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+      ++launches;
+    }
+    // overflow check
+    bool retry = false;
+    for (size_t i = 0; i < nq; ++i) {
+      uint32_t st[2];
+      d2h(ctx, st, status[i].p, 2);
+      if (st[0]) { regrow[i] += 1; retry = true; }
+    }
+    if (retry) {
+      if (attempt >= 4) fail(QM_ERR_QUERY_FAILED, "group-by hash table kept overflowing");
+      continue;
+    }
+    // finalise each query
+    for (size_t i = 0; i < nq; ++i) {
+      AggBinding &b = ab[i];
+      DFinal f;
+      memset(&f, 0, sizeof f);
+      f.table = tables[i].as<uint64_t>();
+      f.nslots = b.nslots;
+      f.entry_words = b.q.entry_words;
+      f.count_word = b.q.count_word;
+      f.gmode = b.q.gmode;
+      f.always_one_row = b.always_one_row ? 1 : 0;
+      f.key = b.q.key;
+      f.status = status[i].as<uint32_t>();
+      uint64_t nb = qmk_compact_blocks(f.nslots);
+      DevBuf cnt(ctx, nb * 8), offs(ctx, nb * 8), stmp(ctx, qmk_exscan_tmp_words(nb) * 8 + 64);
+      HIPX(qmk_compact_count(&f, cnt.as<uint64_t>(), ctx->stream));
+      uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nb);
+      HIPX(qmk_exscan_u64(cnt.as<uint64_t>(), nb, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(), ctx->stream));
+      uint64_t total = 0;
+      d2h(ctx, &total, total_d, 1);
+      qm_result *res = new qm_result();
+      results[i] = res;
+      res->ctx = ctx;
+      res->rows = total;
+      f.nouts = (int)b.outs.size();
+      for (size_t k = 0; k < b.outs.size(); ++k) {
+        const DOut &o = b.outs[k].o;
+        bool can_null = o.kind != OK_COUNT;
+        OutCol oc = make_out(ctx, b.outs[k].name, o.out_dtype, total, can_null);
+        f.outs[k] = o;
+        f.outs[k].data = oc.data.p;
+        f.outs[k].nulls = oc.nulls.as<uint32_t>();
+        res->cols.push_back(std::move(oc));
+      }
+      if (total) HIPX(qmk_compact_write(&f, offs.as<uint64_t>(), ctx->stream));
+      launches += 4;
+      uint32_t st[2];
+      d2h(ctx, st, status[i].p, 2);
+      if (st[1]) fail(QM_ERR_OVERFLOW, "SUM over an int64 column does not fit int64");
+      uint64_t sel = 0;
+      d2h(ctx, &sel, selected[i].p, 1);
+      res->stats.rows_selected = sel;
+      res->stats.strategy = b.q.gmode == GM_DIRECT ? (b.q.lmode == LM_DIRECT ? QM_STRAT_DIRECT_LDS : QM_STRAT_DIRECT_GLB)
+                                                   : (b.q.lmode == LM_HASH ? QM_STRAT_HASH_LDS : QM_STRAT_HASH_GLB);
+      res->stats.retries = attempt;
+      DQuery noq;
+      memset(&noq, 0, sizeof noq);
+      DevBuf dnoq(ctx, sizeof(DQuery));
+      h2d(ctx, dnoq.p, reinterpret_cast<const uint8_t *>(&noq), sizeof noq);
+      sort_limit_result(res, *plans[i], dnoq.as<DQuery>(), &launches);
+    }
+    double ms = tm.stop();
+    float scan_ms = 0;
+    if (T->rows) HIPX(hipEventElapsedTime(&scan_ms, ctx->evk0, ctx->evk1));
+    uint64_t rrows = fused_join ? ji->build_rows : 0;
+    for (size_t i = 0; i < nq; ++i) {
+      qm_result *res = results[i];
+      res->stats.rows_scanned = T->rows + (fused_join ? R->rows : 0);
+      res->stats.groups = res->rows;
+      res->stats.bytes_algorithmic = referenced_bytes(cs, T->rows, rrows);
+      res->stats.device_ms = ms;
+      res->stats.scan_ms = scan_ms;
+      res->stats.kernel_launches = launches;
+    }
+    return;
+  }
+}
+
+// projection list for row-returning queries
+std::vector<std::pair<std::string, int>> projection(ColSpace &cs, const Plan &pl, bool join) {
+  std::vector<std::pair<std::string, int>> out;
+  if (!pl.proj.empty()) {
+    for (const std::string &n : pl.proj) out.push_back({n, cs.plain(n)});
+    return out;
+  }
+  for (const TCol &c : cs.L->cols) out.push_back({c.name, cs.slot_of(&c, 0)});
+  if (join) {
+    // right columns minus the right key, colliding names prefixed right_
+    for (const TCol &c : cs.R->cols) {
+      if (c.name == cs.rkey) continue;
+      std::string n = (cs.L->find(c.name) >= 0) ? "right_" + c.name : c.name;
+      out.push_back({n, cs.slot_of(&c, 1)});
+    }
+  }
+  return out;
+}
+
+void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Plan &pl, qm_result *res) {
+  Timer tm(ctx);
+  uint32_t launches = 0;
+  ColSpace cs;
+  cs.L = T;
+  cs.R = pl.has_join ? R : nullptr;
+  cs.rkey = pl.rkey;
+  cs.plan = &pl;
+  DQuery q;
+  memset(&q, 0, sizeof q);
+  ListStore ls;
+  bind_filters(ctx, cs, pl.filters, pl.groups, q, ls, false);
+  auto proj = projection(cs, pl, pl.has_join);
+  res->ctx = ctx;
+  JoinIndex ji;
+  DevBuf keep;
+  DevBuf left_rows, right_rows;
+  uint64_t n = 0;
+  DScan sc;
+  memset(&sc, 0, sizeof sc);
+  sc.nrows = T->rows;
+  sc.nqueries = 1;
+  DevBuf dq(ctx, sizeof(DQuery));
+  h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
+  if (pl.has_join) {
+    build_join(ctx, cs, pl, ji, &launches, &keep);
+    int ps = cs.plain(pl.lkey);
+    sc.ncols = (int)cs.cols.size();
+    for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+    attach_join(sc, ji, ps);
+    const uint64_t tile_rows = 256 * RPT;
+    const uint64_t ntiles = std::max<uint64_t>(1, (sc.nrows + tile_rows - 1) / tile_rows);
+    uint64_t nblocks = std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 8);
+    uint64_t tpb = (ntiles + nblocks - 1) / nblocks;
+    nblocks = (ntiles + tpb - 1) / tpb;
+    DevBuf counts(ctx, nblocks * 8), offs(ctx, nblocks * 8), stmp(ctx, qmk_exscan_tmp_words(nblocks) * 8 + 64);
+    HIPX(qmk_join_count(&sc, dq.as<DQuery>(), nblocks, tpb, counts.as<uint64_t>(), ctx->stream));
+    uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nblocks);
+    HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
+                        ctx->stream));
+    uint64_t total = 0;
+    d2h(ctx, &total, total_d, 1);
+    n = total;
+    if (!pl.has_sort && pl.limit >= 0) n = std::min<uint64_t>(n, (uint64_t)pl.limit + (uint64_t)pl.offset);
+    left_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
+    right_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
+    if (n)
+      HIPX(qmk_join_write(&sc, dq.as<DQuery>(), nblocks, tpb, offs.as<uint64_t>(), n, left_rows.as<uint64_t>(),
+                          right_rows.as<uint64_t>(), ctx->stream));
+    launches += 5;
+    res->stats.rows_scanned = T->rows + R->rows;
+  } else {
+    sc.ncols = (int)cs.cols.size();
+    for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+    const bool topk = pl.has_sort && pl.limit >= 0 && qmk_topk_cap((uint32_t)std::min<int64_t>(pl.limit + pl.offset, 1 << 30)) &&
+                      pl.limit + pl.offset < (1 << 30);
+    res->stats.rows_scanned = T->rows;
+    if (topk) {
+      // fused filter + top-K over the table (src/operators.ts:1716-1841)
+      int sslot = cs.plain(pl.sort_col);
+      sc.ncols = (int)cs.cols.size();
+      for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+      h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
+      uint64_t k = (uint64_t)(pl.limit + pl.offset);
+      const uint64_t tiles = std::max<uint64_t>(1, (T->rows + 2047) / 2048);
+      int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * 2);
+      uint64_t take = 0;
+      if (k > 0) {
+        DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
+        DevBuf ccount(ctx, (size_t)grid * 4 + 64);
+        HIPX(qmk_topk_scan(&sc, dq.as<DQuery>(), sslot, pl.desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(),
+                           ccount.as<uint32_t>(), ctx->stream));
+        ++launches;
+        const TopKey *lists = cand.as<TopKey>();
+        const uint32_t *lc = ccount.as<uint32_t>();
+        uint32_t nl = (uint32_t)grid;
+        DevBuf m1, m1c;
+        if (nl > 16) {
+          uint32_t g2 = (nl + 15) / 16;
+          m1 = DevBuf(ctx, (size_t)g2 * k * sizeof(TopKey));
+          m1c = DevBuf(ctx, (size_t)g2 * 4 + 64);
+          HIPX(qmk_topk_merge(lists, lc, nl, 16, (uint32_t)k, m1.as<TopKey>(), m1c.as<uint32_t>(), ctx->stream));
+          ++launches;
+          lists = m1.as<TopKey>();
+          lc = m1c.as<uint32_t>();
+          nl = g2;
+        }
+        DevBuf fin(ctx, (size_t)k * sizeof(TopKey)), finc(ctx, 64);
+        HIPX(qmk_topk_merge(lists, lc, nl, nl, (uint32_t)k, fin.as<TopKey>(), finc.as<uint32_t>(), ctx->stream));
+        uint32_t fc = 0;
+        d2h(ctx, &fc, finc.p, 1);
+        take = fc > (uint64_t)pl.offset ? fc - (uint64_t)pl.offset : 0;
+        left_rows = DevBuf(ctx, std::max<uint64_t>(take, 1) * 8);
+        HIPX(qmk_topk_rows(fin.as<TopKey>(), finc.as<uint32_t>(), (uint32_t)pl.offset, left_rows.as<uint64_t>(),
+                           (uint32_t)take, ctx->stream));
+        launches += 2;
+      } else {
+        left_rows = DevBuf(ctx, 8);
+      }
+      n = take;
+    } else {
+      uint64_t cap = ~0ull;
+      if (!pl.has_sort && pl.limit >= 0) cap = (uint64_t)pl.limit + (uint64_t)pl.offset;
+      left_rows = select_rows(ctx, sc, dq.as<DQuery>(), cap, &n, &launches);
+    }
+  }
+  // gather the projected columns
+  res->rows = n;
+  for (auto &pc : proj) {
+    const DCol &c = cs.cols[pc.second];
+    int odt = out_dt_of(c.dtype);
+    OutCol oc = make_out(ctx, pc.first, odt, n, c.nulls != nullptr);
+    const uint64_t *rows = c.side ? right_rows.as<uint64_t>() : left_rows.as<uint64_t>();
+    HIPX(qmk_gather(&c, rows, nullptr, n, odt, oc.data.p, oc.nulls.as<uint32_t>(), ctx->stream));
+    ++launches;
+    res->cols.push_back(std::move(oc));
+  }
+  const bool topk_done = !pl.has_join && pl.has_sort && pl.limit >= 0 &&
+                         qmk_topk_cap((uint32_t)std::min<int64_t>(pl.limit + pl.offset, 1 << 30)) &&
+                         pl.limit + pl.offset < (1 << 30);
+  if (!topk_done) {
+    // sort / limit / offset over the materialised rows
+    Plan p2 = pl;
+    if (!pl.has_sort && !pl.has_join) p2.limit = -1, p2.offset = pl.offset;
+    if (!pl.has_sort && pl.has_join) { /* n already capped at limit+offset */ }
+    DQuery noq;
+    memset(&noq, 0, sizeof noq);
+    DevBuf dnoq(ctx, sizeof(DQuery));
+    h2d(ctx, dnoq.p, reinterpret_cast<const uint8_t *>(&noq), sizeof noq);
+    if (pl.has_sort && find_out(res, pl.sort_col) < 0) {
+      // sort column not projected: gather it as a hidden column
+      int s = cs.plain(pl.sort_col);
+      const DCol &c = cs.cols[s];
+      OutCol oc = make_out(ctx, "\x01sort", out_dt_of(c.dtype), n, c.nulls != nullptr);
+      const uint64_t *rows = c.side ? right_rows.as<uint64_t>() : left_rows.as<uint64_t>();
+      HIPX(qmk_gather(&c, rows, nullptr, n, oc.dtype, oc.data.p, oc.nulls.as<uint32_t>(), ctx->stream));
+      res->cols.push_back(std::move(oc));
+      p2.sort_col = "\x01sort";
+      sort_limit_result(res, p2, dnoq.as<DQuery>(), &launches);
+      res->cols.pop_back();
+    } else if (pl.has_sort || pl.offset > 0 || (pl.limit >= 0 && pl.has_join)) {
+      sort_limit_result(res, p2, dnoq.as<DQuery>(), &launches);
+    }
+  }
+  res->stats.device_ms = tm.stop();
+  res->stats.groups = res->rows;
+  res->stats.rows_selected = n;
+  res->stats.kernel_launches = launches;
+  res->stats.bytes_algorithmic = referenced_bytes(cs, T->rows, pl.has_join ? R->rows : 0);
+}
+
+}  // namespace
+
+extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out) {
+  API_BEGIN
+  if (!p || !left || !out) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  qm_ctx *ctx = p->ctx;
+  if (left->ctx != ctx || (right && right->ctx != ctx)) fail(QM_ERR_INVALID_ARG, "handles from different contexts");
+  const Plan &pl = p->plan;
+  if (pl.has_join && !right) fail(QM_ERR_TABLE_NOT_FOUND, "join needs the right (build) table");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  std::unique_ptr<qm_result> res;
+  const bool agg = !pl.aggs.empty() || !pl.groupby.empty();
+  if (agg) {
+    if (pl.aggs.empty()) fail(QM_ERR_INVALID_AGGREGATE, "GROUP BY without aggregates");
+    std::vector<const Plan *> plans{&pl};
+    std::vector<qm_result *> results(1, nullptr);
+    JoinIndex ji;
+    DevBuf keep;
+    if (pl.has_join) {
+      ColSpace cs;
+      cs.L = left;
+      cs.R = right;
+      cs.rkey = pl.rkey;
+      cs.plan = &pl;
+      uint32_t l = 0;
+      Timer tb(ctx);
+      build_join(ctx, cs, pl, ji, &l, &keep);
+      double bms = tb.stop();
+      try {
+        run_aggregates(ctx, left, right, plans, results, true, &ji);
+      } catch (...) {
+        delete results[0];
+        throw;
+      }
+      results[0]->stats.device_ms += bms;
+      results[0]->stats.kernel_launches += l;
+    } else {
+      try {
+        run_aggregates(ctx, left, nullptr, plans, results, false, nullptr);
+      } catch (...) {
+        delete results[0];
+        throw;
+      }
+    }
+    res.reset(results[0]);
+  } else {
+    res.reset(new qm_result());
+    run_rows_query(ctx, left, right, pl, res.get());
+  }
+  HIPX(hipStreamSynchronize(ctx->stream));
+  res->stats.wall_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = res.release();
+  API_END
+}
+
+extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_table *table,
+                             qm_result **results_out) {
+  API_BEGIN
+  if (!ctx || !table || (nplans && (!plans || !results_out))) fail(QM_ERR_INVALID_ARG, "null argument");
+  for (uint32_t i = 0; i < nplans; ++i) results_out[i] = nullptr;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<uint32_t> agg_idx, other_idx;
+  for (uint32_t i = 0; i < nplans; ++i) {
+    if (!plans[i]) fail(QM_ERR_INVALID_ARG, "null plan %u", i);
+    const Plan &pl = plans[i]->plan;
+    bool agg = !pl.aggs.empty();
+    if (agg && !pl.has_join) agg_idx.push_back(i); else other_idx.push_back(i);
+  }
+  std::vector<qm_result *> all(nplans, nullptr);
+  try {
+    // fused launches of up to MAX_QUERIES aggregate queries sharing one scan
+    size_t pos = 0;
+    while (pos < agg_idx.size()) {
+      size_t n = std::min<size_t>(MAX_QUERIES, agg_idx.size() - pos);
+      std::vector<const Plan *> ps;
+      for (size_t k = 0; k < n; ++k) ps.push_back(&plans[agg_idx[pos + k]]->plan);
+      std::vector<qm_result *> rs(n, nullptr);
+      try {
+        run_aggregates(ctx, table, nullptr, ps, rs, false, nullptr);
+      } catch (const Error &e) {
+        for (auto *r : rs) delete r;
+        if (e.code == QM_ERR_UNSUPPORTED && n > 1) {
+          // too much LDS / columns for one launch: run them one by one
+          for (size_t k = 0; k < n; ++k) {
+            std::vector<const Plan *> one{ps[k]};
+            std::vector<qm_result *> r1(1, nullptr);
+            run_aggregates(ctx, table, nullptr, one, r1, false, nullptr);
+            all[agg_idx[pos + k]] = r1[0];
+          }
+          pos += n;
+          continue;
+        }
+        throw;
+      }
+      for (size_t k = 0; k < n; ++k) all[agg_idx[pos + k]] = rs[k];
+      pos += n;
+    }
+    for (uint32_t i : other_idx) {
+      if (plans[i]->plan.has_join) fail(QM_ERR_UNSUPPORTED, "joins are not batched; run them with qm_exec");
+      qm_result *r = new qm_result();
+      all[i] = r;
+      run_rows_query(ctx, table, nullptr, plans[i]->plan, r);
+    }
+  } catch (...) {
+    for (auto *r : all) delete r;
+    throw;
+  }
+  HIPX(hipStreamSynchronize(ctx->stream));
+  double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (uint32_t i = 0; i < nplans; ++i) {
+    all[i]->stats.wall_ms = wall;
+    results_out[i] = all[i];
+  }
+  API_END
+}
+
+// ---------------------------------------------------------------------------
+// results
+// ---------------------------------------------------------------------------
+namespace {
+struct QmcbCol {
+  bool has_nulls;
+  std::vector<uint8_t> bitmap;
+};
+
+size_t qmcb_data_bytes(int dt, uint64_t rows) {
+  if (dt == DT_BOOL) return (rows + 7) / 8;
+  return rows * (uint64_t)dt_size(dt);
+}
+
+std::vector<QmcbCol> scan_nulls(qm_result *r) {
+  std::vector<QmcbCol> v(r->cols.size());
+  const uint64_t nb = (r->rows + 7) / 8;
+  for (size_t i = 0; i < r->cols.size(); ++i) {
+    v[i].has_nulls = false;
+    if (!r->cols[i].nulls.p || !nb) continue;
+    v[i].bitmap.resize(nb);
+    d2h(r->ctx, v[i].bitmap.data(), r->cols[i].nulls.p, nb);
+    for (uint8_t b : v[i].bitmap) if (b) { v[i].has_nulls = true; break; }
+  }
+  return v;
+}
+
+uint64_t qmcb_size(qm_result *r, const std::vector<QmcbCol> &nc) {
+  uint64_t s = 10;
+  for (auto &c : r->cols) s += 2 + c.name.size() + 2;
+  for (size_t i = 0; i < r->cols.size(); ++i) {
+    if (nc[i].has_nulls) s += (r->rows + 7) / 8;
+    s += qmcb_data_bytes(r->cols[i].dtype, r->rows);
+  }
+  return s;
+}
+}  // namespace
+
+extern "C" int qm_result_rows(qm_result *r, uint64_t *rows) {
+  API_BEGIN
+  if (!r || !rows) fail(QM_ERR_INVALID_ARG, "null argument");
+  *rows = r->rows;
+  API_END
+}
+
+extern "C" int qm_result_qmcb_size(qm_result *r, uint64_t *size) {
+  API_BEGIN
+  if (!r || !size) fail(QM_ERR_INVALID_ARG, "null argument");
+  if (r->rows > 0xFFFFFFFFull) fail(QM_ERR_UNSUPPORTED, "QMCB row count is 32-bit");
+  std::lock_guard<std::mutex> g(r->ctx->mu);
+  *size = qmcb_size(r, scan_nulls(r));
+  API_END
+}
+
+extern "C" int qm_result_qmcb(qm_result *r, uint8_t *out, uint64_t cap, uint64_t *written) {
+  API_BEGIN
+  if (!r || !out) fail(QM_ERR_INVALID_ARG, "null argument");
+  if (r->rows > 0xFFFFFFFFull) fail(QM_ERR_UNSUPPORTED, "QMCB row count is 32-bit");
+  if (r->cols.size() > 0xFFFF) fail(QM_ERR_UNSUPPORTED, "too many columns");
+  qm_ctx *ctx = r->ctx;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  auto nc = scan_nulls(r);
+  uint64_t need = qmcb_size(r, nc);
+  if (cap < need) fail(QM_ERR_INVALID_ARG, "QMCB buffer too small: need %llu bytes", (unsigned long long)need);
+  // header (src/columnar.ts:523-613)
+  uint8_t *w = out;
+  auto put32 = [&](uint32_t v) { memcpy(w, &v, 4); w += 4; };
+  auto put16 = [&](uint16_t v) { memcpy(w, &v, 2); w += 2; };
+  put32(0x42434D51u);
+  put32((uint32_t)r->rows);
+  put16((uint16_t)r->cols.size());
+  for (size_t i = 0; i < r->cols.size(); ++i) {
+    const OutCol &c = r->cols[i];
+    if (c.name.size() > 0xFFFF) fail(QM_ERR_UNSUPPORTED, "column name too long");
+    put16((uint16_t)c.name.size());
+    memcpy(w, c.name.data(), c.name.size());
+    w += c.name.size();
+    *w++ = (uint8_t)qmcb_tag(c.dtype);
+    *w++ = nc[i].has_nulls ? 1 : 0;
+  }
+  for (size_t i = 0; i < r->cols.size(); ++i) {
+    const OutCol &c = r->cols[i];
+    if (nc[i].has_nulls) {
+      memcpy(w, nc[i].bitmap.data(), nc[i].bitmap.size());
+      w += nc[i].bitmap.size();
+    }
+    if (c.dtype == DT_BOOL) {
+      std::vector<uint8_t> bytes(r->rows);
+      d2h(ctx, bytes.data(), c.data.p, r->rows);
+      uint64_t nb = (r->rows + 7) / 8;
+      memset(w, 0, nb);
+      for (uint64_t k = 0; k < r->rows; ++k) if (bytes[k]) w[k >> 3] |= (uint8_t)(1u << (k & 7));
+      w += nb;
+    } else {
+      size_t b = qmcb_data_bytes(c.dtype, r->rows);
+      if (b) {
+        HIPX(hipMemcpyAsync(w, c.data.p, b, hipMemcpyDeviceToHost, ctx->stream));
+        w += b;
+      }
+    }
+  }
+  HIPX(hipStreamSynchronize(ctx->stream));
+  if (written) *written = (uint64_t)(w - out);
+  API_END
+}
+
+extern "C" int qm_result_stats(qm_result *r, qm_stats *out) {
+  API_BEGIN
+  if (!r || !out) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = r->stats;
+  API_END
+}
+
+extern "C" int qm_result_close(qm_result *r) {
+  if (!r) return QM_OK;
+  qm_ctx *ctx = r->ctx;
+  if (ctx) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete r;
+  } else {
+    delete r;
+  }
+  return QM_OK;
+}

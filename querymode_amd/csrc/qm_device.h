@@ -1,0 +1,140 @@
+// qm_device.h — the device-side query program: the form a bound query takes
+// once the host planner has resolved column names, applied the reference's
+// literal coercions and picked a group-by strategy. Passed to kernels by
+// value (kernarg segment -> scalar loads), so every branch on it is uniform.
+#pragma once
+#include <stdint.h>
+#include "qm_common.h"
+
+namespace qm {
+
+// device column types (QMCB tags + narrow ints; BOOL is one byte per row)
+enum : int32_t {
+  DT_F64 = 0, DT_I64 = 1, DT_BOOL = 3, DT_I32 = 6, DT_F32 = 7, DT_I8 = 16, DT_I16 = 17,
+};
+
+static constexpr int MAX_COLS = 12;
+static constexpr int MAX_PREDS = 16;
+static constexpr int MAX_STATES = 16;
+static constexpr int MAX_KEYS = 2;
+static constexpr int MAX_QUERIES = 16;     // per fused launch of the scan kernel
+
+// Rows handled per thread per tile; tiles are blockDim.x * ROWS_PER_THREAD.
+static constexpr int RPT = 8;
+// Columns are allocated padded to this many rows so full-tile vector loads
+// never leave the allocation.
+static constexpr uint64_t ROW_PAD = 8192;
+
+struct DCol {
+  const void *data;
+  const uint8_t *nulls;  // QMCB bitmap: bit set = null; nullptr = no nulls
+  int32_t dtype;
+  int32_t side;          // 0 = scanned table row, 1 = join build row
+};
+
+// Predicate kinds after host coercion (src/decode.ts:485-578 and the WASM
+// numeric filters, wasm/src/wasm/aggregates.zig:605-911).
+enum : uint8_t {
+  PK_INT_RANGE = 1,   // lo <= v <= hi on the integer value, ^ negate
+  PK_F64_RANGE = 2,   // float interval with strictness flags, ^ negate; NaN -> nan_result
+  PK_INT_IN = 3,      // v in sorted int64 list, ^ negate
+  PK_F64_IN = 4,      // v in sorted f64 list (+NaN flag), ^ negate
+  PK_IS_NULL = 5,
+  PK_IS_NOT_NULL = 6,
+  PK_FALSE = 7,       // folded constant
+};
+
+struct DPred {
+  int8_t col;
+  uint8_t kind;
+  uint8_t group;       // 0 = top-level AND; g >= 1 = OR group g
+  uint8_t negate;
+  uint8_t nan_result;  // PK_F64_RANGE: result for NaN values (before negate is NOT applied)
+  uint8_t lo_strict, hi_strict;
+  uint8_t list_has_nan;
+  int64_t ilo, ihi;
+  double flo, fhi;
+  const void *list;
+  int32_t list_n;
+  int32_t _pad;
+};
+
+// aggregate state kinds; each owns `words` u64 words of a table entry
+enum : uint8_t {
+  SK_COUNT = 1,       // count of non-null values (column) or rows (col < 0)
+  SK_SUM_I64 = 2,     // exact int64 sum of an i8/i16/i32/bool column
+  SK_SUM_128 = 3,     // 128-bit sum: int64 column, or fixed-point f64 (+flags word)
+  SK_MAX_CODE = 4,    // unsigned max of an order code (MAX, or MIN via ~code)
+};
+
+struct DState {
+  uint8_t kind;
+  uint8_t is_min;      // SK_MAX_CODE: store ~code (MIN)
+  uint8_t fixed;       // SK_SUM_128: 1 = f64 fixed point (3 words: lo, hi, flags)
+  uint8_t product;     // value = col_a * col_b in f64
+  int8_t col_a, col_b;
+  int16_t word;        // first word of this state in a table entry
+  int32_t scale;       // fixed-point scale
+  int32_t _pad;
+};
+
+// Group-by addressing.
+enum : uint8_t { GM_NONE = 0, GM_DIRECT = 1, GM_HASH = 2 };
+enum : uint8_t { LM_NONE = 0, LM_DIRECT = 1, LM_HASH = 2 };
+
+struct DKey {
+  int8_t ncols;
+  int8_t col[MAX_KEYS];
+  uint8_t is_float[MAX_KEYS];
+  uint8_t nullable[MAX_KEYS];
+  // DIRECT: index = sum_i code_i * stride_i, code_i = v - min_i (or d_i - 1 for null)
+  // HASH with 2 columns: packed = code_0 | code_1 << bits_0 (codes as above)
+  int64_t kmin[MAX_KEYS];
+  uint64_t dom[MAX_KEYS];     // domain size per key column incl. null code
+  uint64_t stride[MAX_KEYS];
+  int32_t bits[MAX_KEYS];
+};
+
+// One query inside a scan launch.
+struct DQuery {
+  int32_t npreds;
+  int32_t ngroups;        // number of OR groups (0 = none)
+  int32_t nstates;
+  int32_t entry_words;    // u64 words per table entry (HASH: word 0 = key)
+  int32_t count_word;     // word of the row-count state (always present)
+  uint8_t gmode, lmode;
+  uint8_t _p0, _p1;
+  DPred preds[MAX_PREDS];
+  DState states[MAX_STATES];
+  DKey key;
+  uint64_t *table;        // global table: (gcap + 2) entries; slot gcap = unrepresentable key, gcap+1 = NULL key
+  uint64_t gcap;          // DIRECT: domain size; HASH: power of two
+  uint32_t *status;       // [0] overflow flag, [1] unused
+  uint64_t *selected;     // selected-row counter
+  int32_t lds_entries;    // LM_DIRECT: domain; LM_HASH: capacity (power of 2)
+  int32_t lds_replicas;   // LM_DIRECT
+  int32_t lds_stride;     // words between replicas (LM_DIRECT)
+  int32_t lds_words;      // total LDS words used by this query
+  int32_t lds_offset;     // word offset inside the block's LDS
+  int32_t _p2;
+};
+
+struct DScan {
+  uint64_t nrows;
+  int32_t ncols;
+  int32_t nqueries;
+  DCol cols[MAX_COLS];
+  // join build side (fused probe): when join_table != nullptr every scanned
+  // row is probed with key column `join_col`; build rows are located through
+  // the join index and exposed as side-1 columns.
+  const uint64_t *join_table;  // hash entries {stored key, start<<32|count} or DIRECT {start<<32|count}
+  const uint32_t *join_rows;   // build row ids grouped by key
+  uint64_t join_cap;           // HASH: power of 2; DIRECT: domain size
+  int64_t join_min;            // DIRECT: key offset
+  int32_t join_col;            // probe key column
+  int32_t join_mode;           // 0 none, 1 DIRECT, 2 HASH
+  int32_t join_key_float;      // probe key column is f64 (canonical bits)
+  int32_t _p;
+};
+
+}  // namespace qm

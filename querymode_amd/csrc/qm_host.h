@@ -1,0 +1,190 @@
+// qm_host.h — host-side structures of the library: errors, the device
+// memory pool, resident tables, parsed plans and results.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "../../include/qmgpu.h"
+#include "qm_kernels.h"
+
+namespace qm {
+
+struct Error {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const char *fmt, ...);
+void hip_check(hipError_t e, const char *what);
+#define HIPX(expr) ::qm::hip_check((expr), #expr)
+
+// ---- literal / plan model (the QueryDescriptor subset) ----
+enum : int { LIT_NONE = 0, LIT_F64 = 1, LIT_I64 = 2, LIT_LIST = 3, LIT_PAIR = 4 };
+enum : int {
+  OP_EQ = 0, OP_NEQ, OP_LT, OP_LTE, OP_GT, OP_GTE, OP_BETWEEN, OP_NOT_BETWEEN, OP_IN, OP_NOT_IN,
+  OP_IS_NULL, OP_IS_NOT_NULL, OP_LIKE, OP_NOT_LIKE,
+};
+enum : int { AGG_SUM = 0, AGG_AVG, AGG_MIN, AGG_MAX, AGG_COUNT, AGG_COUNT_DISTINCT, AGG_STDDEV, AGG_VARIANCE,
+             AGG_MEDIAN, AGG_PERCENTILE };
+
+struct Lit {
+  int kind = LIT_NONE;  // LIT_F64 (JS number) or LIT_I64 (JS bigint)
+  double f = 0;
+  int64_t i = 0;
+};
+
+struct Filter {
+  std::string col;
+  int op = 0;
+  int vkind = LIT_NONE;
+  Lit v;                 // scalar
+  std::vector<Lit> list; // in / not_in, between pair (2 entries)
+};
+
+struct Agg {
+  int fn = 0;
+  std::string col;
+  std::string alias;
+};
+
+struct Computed {
+  std::string alias;
+  int kind = 1;  // 1 = multiply
+  std::string a, b;
+};
+
+struct Plan {
+  std::vector<Filter> filters;
+  std::vector<std::vector<Filter>> groups;
+  std::vector<Computed> computed;
+  std::vector<std::string> groupby;
+  std::vector<Agg> aggs;
+  bool has_join = false;
+  std::string lkey, rkey;
+  int jtype = 0;
+  std::vector<Filter> rfilters;
+  std::vector<std::vector<Filter>> rgroups;
+  bool has_sort = false;
+  std::string sort_col;
+  bool desc = false;
+  int64_t limit = -1, offset = 0;
+  std::vector<std::string> proj;
+};
+
+Plan parse_plan(const uint8_t *p, size_t n);
+
+// predicate coercion: one reference FilterOp on a column of dtype `dt`
+// -> one device predicate (kind PK_*), following the reference's WASM and
+// JS filter paths. `list_storage` receives IN-list values to upload.
+struct BoundPred {
+  DPred p;
+  std::vector<int64_t> ilist;
+  std::vector<double> flist;
+};
+BoundPred coerce_filter(const Filter &f, int dt);
+
+// ---- context / memory ----
+struct qm_ctx_impl;
+}  // namespace qm
+
+struct qm_ctx {
+  int device = 0;
+  uint64_t budget = 0;
+  std::atomic<uint64_t> used{0};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around the dominant scan kernel
+  std::mutex mu;           // one query on the device at a time per context
+  std::mutex pool_mu;
+  std::multimap<size_t, void *> pool;  // free device buffers by size
+  uint64_t pooled = 0;
+  int num_cus = 256;
+  std::string name;
+
+  void *alloc(size_t bytes);
+  void release(void *p, size_t bytes);
+  void trim_pool();
+};
+
+namespace qm {
+
+size_t pool_round(size_t bytes);
+
+// RAII device buffer from the context pool
+struct DevBuf {
+  qm_ctx *ctx = nullptr;
+  void *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(qm_ctx *c, size_t bytes) : ctx(c), n(pool_round(bytes)) { p = c->alloc(n); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept { *this = std::move(o); }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { reset(); ctx = o.ctx; p = o.p; n = o.n; o.p = nullptr; o.n = 0; o.ctx = nullptr; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() { if (p && ctx) ctx->release(p, n); p = nullptr; n = 0; }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct ColStats {
+  bool has_int = false;      // imin/imax valid
+  int64_t imin = 0, imax = 0;
+  bool has_f = false;        // fmin/fmax valid (non-NaN values exist)
+  double fmin = 0, fmax = 0;
+  double max_abs = 0;        // largest finite |x|
+  uint64_t nan = 0, nulls = 0, nonnull = 0;
+  double distinct = 0;       // HyperLogLog estimate
+  double dup_frac = 0;       // share of a strided host sample whose value repeats in it
+};
+
+struct TCol {
+  std::string name;
+  int dtype = 0;
+  void *d = nullptr;
+  size_t bytes = 0;
+  uint8_t *dnull = nullptr;
+  size_t null_bytes = 0;
+  ColStats st;
+};
+
+}  // namespace qm
+
+struct qm_table {
+  qm_ctx *ctx = nullptr;
+  std::string name;
+  uint64_t rows = 0;
+  std::vector<qm::TCol> cols;
+  int find(const std::string &n) const {
+    for (size_t i = 0; i < cols.size(); ++i) if (cols[i].name == n) return (int)i;
+    return -1;
+  }
+};
+
+struct qm_plan {
+  qm_ctx *ctx = nullptr;
+  qm::Plan plan;
+};
+
+namespace qm {
+struct OutCol {
+  std::string name;
+  int dtype = 0;     // device dtype of `data`
+  DevBuf data;
+  DevBuf nulls;      // one byte per row, optional
+};
+}  // namespace qm
+
+struct qm_result {
+  qm_ctx *ctx = nullptr;
+  uint64_t rows = 0;
+  std::vector<qm::OutCol> cols;
+  qm_stats stats{};
+};

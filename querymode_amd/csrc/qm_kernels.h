@@ -1,0 +1,99 @@
+// qm_kernels.h — result descriptors and the host-side launcher entry points
+// of the gfx950 kernels (all launches go on the caller's stream).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "qm_device.h"
+
+namespace qm {
+
+enum : uint8_t {
+  OK_KEY = 1, OK_COUNT, OK_SUM_I64, OK_SUM_I128, OK_SUM_F64, OK_AVG_F64, OK_AVG_I64, OK_AVG_BIG,
+  OK_MIN, OK_MAX,
+};
+
+struct DOut {
+  uint8_t kind;
+  uint8_t key_idx;
+  uint8_t src_is_float;
+  uint8_t _p;
+  int32_t out_dtype;
+  int16_t word;       // state word
+  int16_t cnt_word;   // count word deciding null-ness / divisor (-1 = none)
+  int32_t scale;      // fixed-point scale
+  void *data;
+  uint32_t *nulls;    // output null bitmap (QMCB layout, bit set = null), pre-zeroed, or nullptr
+};
+
+static constexpr int MAX_OUTS = 20;
+
+struct DFinal {
+  const uint64_t *table;
+  uint64_t nslots;        // slots scanned (HASH: gcap + 2, DIRECT: domain)
+  int32_t entry_words;
+  int32_t count_word;     // row-count word (occupancy)
+  uint8_t gmode;
+  uint8_t always_one_row; // aggregate without GROUP BY: exactly one row
+  uint8_t _p[2];
+  int32_t nouts;
+  DKey key;
+  DOut outs[MAX_OUTS];
+  uint32_t *status;       // [1] <- int128 SUM did not fit int64
+};
+
+// top-K candidate (composite order key, lexicographic on (a, b))
+struct TopKey { uint64_t a, b; };
+
+}  // namespace qm
+
+extern "C" {
+hipError_t qmk_scan_agg(const qm::DScan *sc, const qm::DQuery *d_queries, int grid, int block,
+                        size_t lds_bytes, hipStream_t st);
+hipError_t qmk_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
+
+hipError_t qmk_exscan_u64(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *total,
+                          uint64_t *tmp_sums, hipStream_t st);
+uint64_t qmk_exscan_tmp_words(uint64_t n);
+
+uint64_t qmk_compact_blocks(uint64_t nslots);
+hipError_t qmk_compact_count(const qm::DFinal *f, uint64_t *block_counts, hipStream_t st);
+hipError_t qmk_compact_write(const qm::DFinal *f, const uint64_t *block_offsets, hipStream_t st);
+
+hipError_t qmk_select_count(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t nblocks, uint64_t tpb,
+                            uint64_t *block_counts, hipStream_t st);
+hipError_t qmk_select_write(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t nblocks, uint64_t tpb,
+                            const uint64_t *block_offsets, uint64_t out_cap, uint64_t *out_rows,
+                            hipStream_t st);
+hipError_t qmk_gather(const qm::DCol *c, const uint64_t *rows, const uint32_t *idx, uint64_t n,
+                      int32_t out_dt, void *out, uint32_t *nulls, hipStream_t st);
+
+// top-K: order keys from one sort column (optionally filtered by a query)
+hipError_t qmk_topk_scan(const qm::DScan *sc, const qm::DQuery *d_q, int sort_col, int desc,
+                         uint32_t k, int grid, qm::TopKey *cand, uint32_t *cand_counts, hipStream_t st);
+hipError_t qmk_topk_merge(const qm::TopKey *cand, const uint32_t *cand_counts, uint32_t nlists,
+                          uint32_t lists_per_block, uint32_t k, qm::TopKey *out, uint32_t *out_count,
+                          hipStream_t st);
+hipError_t qmk_topk_rows(const qm::TopKey *keys, const uint32_t *count, uint32_t offset, uint64_t *rows,
+                         uint32_t n, hipStream_t st);
+uint32_t qmk_topk_cap(uint32_t k);
+
+// full sort of n order keys (composite), returns permutation as row ids
+hipError_t qmk_sort_keys(const qm::DCol *c, uint64_t n, int desc, uint64_t *rows_out, void *tmp,
+                         size_t tmp_bytes, hipStream_t st);
+size_t qmk_sort_tmp_bytes(uint64_t n);
+
+// join build
+// sel: optional ascending build-row ids (rows passing the right side's filters)
+hipError_t qmk_join_build(const qm::DCol *key, uint64_t nrows, const uint64_t *sel, int mode, int key_float,
+                          int64_t kmin, uint64_t cap, uint64_t *table, uint32_t *rows, uint32_t *cursor,
+                          uint64_t *tmp, uint32_t *status, hipStream_t st);
+// materialised inner-join pairs (probe row, build row) in probe order
+hipError_t qmk_join_count(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t nblocks, uint64_t tpb,
+                          uint64_t *block_counts, hipStream_t st);
+hipError_t qmk_join_write(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t nblocks, uint64_t tpb,
+                          const uint64_t *block_offsets, uint64_t out_cap, uint64_t *out_left,
+                          uint64_t *out_right, hipStream_t st);
+
+// column statistics: min/max codes, finite max |x|, NaN count, HLL registers
+hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
+                         hipStream_t st);
+}

@@ -1,0 +1,119 @@
+"""Device path vs the CPU oracle on the same seeded inputs (sizes the oracle
+finishes in seconds). Every query runs through the C-ABI (libqmgpu.so)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from querymode_amd import datagen
+from querymode_amd.plan import AggregateOp, BigInt, FilterOp, JoinDescriptor, QueryDescriptor
+
+from ._compare import assert_groups_equal, assert_rows_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_both(qm, desc, tables, nulls=None):
+    nulls = nulls or {}
+    for name, cols in tables.items():
+        qm.register_table(name, cols, nulls.get(name))
+    got = qm.executor.execute(desc)
+    want = O.run(desc, {n: O.Table(c, nulls.get(n)) for n, c in tables.items()})
+    return got, want
+
+
+def test_filter_groupby_small(qm):
+    _say("case filter_groupby_small n=1M")
+    cols = datagen.filter_groupby_small(n=1_000_000, seed=0)
+    d = QueryDescriptor("t", filters=[FilterOp("status", "eq", 1)], groupBy=["category"],
+                        aggregates=[AggregateOp("sum", "amount", "total"), AggregateOp("count", "*")],
+                        sortColumn="total", sortDirection="desc")
+    got, want = run_both(qm, d, {"t": cols})
+    assert got.stats["strategy"] == "direct_lds"
+    assert_rows_equal(got.rows, want, float_cols={"total"}, ctx="c1")
+
+
+@pytest.mark.parametrize("skewed", [True, False])
+def test_filter_groupby_zipf(qm, skewed):
+    _say(f"case filter_groupby_zipf skewed={skewed} n=2M")
+    n = 2_000_000
+    cols = datagen.filter_groupby_zipf(n=n, seed=1)
+    if not skewed:
+        cols["key"] = datagen.uniform_int(1, 99, 0, n, -(2 ** 62), 2 ** 63, np.int64)
+    d = QueryDescriptor("t", filters=[FilterOp("a", "lt", 300000), FilterOp("b", "gt", 90), FilterOp("flag", "eq", 1)],
+                        groupBy=["key"],
+                        aggregates=[AggregateOp("sum", "b"), AggregateOp("count", "*"), AggregateOp("min", "a"),
+                                    AggregateOp("max", "a"), AggregateOp("avg", "b")])
+    got, want = run_both(qm, d, {"t": cols})
+    assert got.stats["strategy"] == ("hash_lds" if skewed else "hash_global")
+    assert_groups_equal(got.rows, want, ["key"], float_cols={"sum_b", "avg_b"}, ctx="c2")
+
+
+def test_global_aggregate_and_empty(qm):
+    _say("case global aggregate")
+    cols = datagen.filter_groupby_zipf(n=300_000, seed=2)
+    for flt in ([FilterOp("a", "lt", 500000)], [FilterOp("a", "lt", -5)]):
+        d = QueryDescriptor("t", filters=flt, aggregates=[
+            AggregateOp("sum", "b"), AggregateOp("count", "*"), AggregateOp("min", "a"), AggregateOp("max", "b"),
+            AggregateOp("avg", "a"), AggregateOp("sum", "key"), AggregateOp("avg", "key")])
+        got, want = run_both(qm, d, {"t": cols})
+        assert_rows_equal(got.rows, want, float_cols={"sum_b", "max_b", "avg_a", "avg_key"}, ctx=str(flt))
+
+
+def test_join_aggregate(qm):
+    _say("case join aggregate build=100K probe=2M")
+    build = datagen.join_build(n=100_000, seed=3)
+    probe = datagen.join_probe(n=2_000_000, nids=100_000, seed=3)
+    d = QueryDescriptor("p", join=JoinDescriptor(QueryDescriptor("b"), "fk", "id"), computedColumns=[("aw", "amount", "weight")],
+                        groupBy=["region"], aggregates=[AggregateOp("sum", "aw"), AggregateOp("count", "*")])
+    got, want = run_both(qm, d, {"p": probe, "b": build})
+    assert_groups_equal(got.rows, want, ["region"], float_cols={"sum_aw"}, ctx="join-agg")
+
+
+def test_join_materialised_order(qm):
+    _say("case join materialised")
+    rng = np.random.default_rng(7)
+    left = {"k": rng.integers(0, 300, 5000).astype(np.int64), "x": np.arange(5000, dtype=np.int32)}
+    right = {"k": rng.integers(0, 400, 700).astype(np.int64), "x": rng.random(700), "y": np.arange(700, dtype=np.int32)}
+    d = QueryDescriptor("l", join=JoinDescriptor(QueryDescriptor("r", filters=[FilterOp("y", "gte", 10)]), "k", "k"),
+                        filters=[FilterOp("x", "lt", 4000)])
+    got, want = run_both(qm, d, {"l": left, "r": right})
+    assert_rows_equal(got.rows, want, float_cols={"right_x"}, ctx="join rows")
+
+
+@pytest.mark.parametrize("k", [10, 100, 1000])
+def test_topk(qm, k):
+    _say(f"case topk k={k}")
+    cols = datagen.batch_table(n=1_000_000, seed=5)
+    d = QueryDescriptor("t", sortColumn="v2", sortDirection="desc", limit=k)
+    got, want = run_both(qm, d, {"t": cols})
+    assert_rows_equal(got.rows, want, ctx=f"topk {k}")
+
+
+def test_topk_filter_offset_asc(qm):
+    _say("case topk offset asc")
+    cols = datagen.batch_table(n=500_000, seed=6)
+    cols["k1"] = cols["k1"]  # ties: sort on a 4-valued column, tie-break by row id
+    d = QueryDescriptor("t", filters=[FilterOp("v2", "lt", 0.3)], sortColumn="k1", sortDirection="asc", limit=50,
+                        offset=20, projections=["k2", "k1"])
+    got, want = run_both(qm, d, {"t": cols})
+    assert_rows_equal(got.rows, want, ctx="topk asc offset ties")
+
+
+def test_select_limit_offset(qm):
+    _say("case select limit")
+    cols = datagen.filter_groupby_zipf(n=200_000, seed=8)
+    d = QueryDescriptor("t", filters=[FilterOp("b", "between", [95.5, 96.5])], limit=37, offset=5, projections=["a", "b"])
+    got, want = run_both(qm, d, {"t": cols})
+    assert_rows_equal(got.rows, want, ctx="select")
+    d2 = QueryDescriptor("t", filters=[FilterOp("a", "gte", 999000)], sortColumn="b")
+    got, want = run_both(qm, d2, {"t": cols})
+    assert_rows_equal(got.rows, want, ctx="select sort no limit")
