@@ -94,6 +94,8 @@ typedef struct qm_stats {
 #define QM_STRAT_HASH_LDS    2  /* global open addressing + per-workgroup LDS cache */
 #define QM_STRAT_DIRECT_GLB  3  /* dense large key domain, direct global table */
 #define QM_STRAT_HASH_GLB    4  /* global open addressing, no LDS cache */
+#define QM_STRAT_PART_HASH   5  /* partitioned: scatter by key hash, LDS table per partition */
+#define QM_STRAT_PART_DIRECT 6  /* partitioned: scatter by dense key slice, LDS slice per partition */
 
 /* ---- lifecycle ----
  * Replaces the executor construction QueryMode.local() does

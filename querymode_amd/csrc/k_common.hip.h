@@ -197,9 +197,61 @@ __device__ __forceinline__ uint32_t eval_pred8(const DPred &p, const DCol &c, ui
   return m & ~nm;  // null never matches (src/decode.ts:489-491)
 }
 
+// One predicate on 8 values already in registers (int64 values or f64 bits).
+__device__ __forceinline__ uint32_t eval_pred8_regs(const DPred &p, const int64_t v[RPT], uint32_t nm) {
+  if (p.kind == PK_FALSE) return 0;
+  if (p.kind == PK_IS_NULL) return nm;
+  if (p.kind == PK_IS_NOT_NULL) return (~nm) & 0xFFu;
+  uint32_t m = 0;
+  if (p.kind == PK_F64_RANGE || p.kind == PK_F64_IN) {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) m |= (pred_f64(p, bits_f64((uint64_t)v[r])) ? 1u : 0u) << r;
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) m |= (pred_int(p, v[r]) ? 1u : 0u) << r;
+  }
+  return m & ~nm;
+}
+
+// AND-only queries over <= MAX_PSLOTS columns: every predicate column is
+// loaded before any compare, so a tile costs one memory round trip.
+__device__ __forceinline__ uint32_t eval_and8_fast(const DQuery &q, const DCol *cols, uint64_t row0,
+                                                   uint32_t valid) {
+  if (!valid) return 0;
+  int64_t raw[MAX_PSLOTS][RPT];
+  uint32_t nm[MAX_PSLOTS];
+#pragma unroll
+  for (int s = 0; s < MAX_PSLOTS; ++s) {
+    nm[s] = 0;
+    if (s < q.nslots) {
+      const DCol &c = cols[q.slot_col[s]];
+      nm[s] = load8_nulls(c, row0);
+      if (dt_is_float(c.dtype)) {
+        double f[RPT];
+        load8_f(c, row0, f);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) raw[s][r] = (int64_t)f64_bits(f[r]);
+      } else {
+        load8_i(c, row0, raw[s]);
+      }
+    }
+  }
+  uint32_t m = valid;
+#pragma unroll
+  for (int s = 0; s < MAX_PSLOTS; ++s) {
+    if (s >= q.nslots) break;
+    for (int i = 0; i < q.npreds; ++i) {
+      const DPred &p = q.preds[i];
+      if (p.slot == s) m &= eval_pred8_regs(p, raw[s], nm[s]);
+    }
+  }
+  return m;
+}
+
 // AND of group-0 predicates, AND with OR over groups 1..ngroups.
 __device__ __forceinline__ uint32_t eval_query8(const DQuery &q, const DCol *cols, uint64_t row0,
                                                 uint32_t valid) {
+  if (q.and_only) return eval_and8_fast(q, cols, row0, valid);
   uint32_t m = valid;
   uint32_t orm = 0;
   int cur_group = -1;

@@ -58,13 +58,27 @@ __global__ void scan_reduce_kernel(const uint64_t *in, uint64_t n, uint64_t *sum
   }
 }
 
-// single block: in-place exclusive scan of `n` block sums, total -> *total
-__global__ void scan_sums_kernel(uint64_t *sums, uint64_t n, uint64_t *total) {
-  if (threadIdx.x == 0) {
-    uint64_t acc = 0;
-    for (uint64_t i = 0; i < n; ++i) { uint64_t t = sums[i]; sums[i] = acc; acc += t; }
-    if (total) *total = acc;
+// single block of 1024 threads: in-place exclusive scan of `n` block sums,
+// total -> *total
+__global__ void __launch_bounds__(1024) scan_sums_kernel(uint64_t *sums, uint64_t n, uint64_t *total) {
+  __shared__ uint64_t sh[1024];
+  const uint64_t per = (n + blockDim.x - 1) / blockDim.x;
+  const uint64_t b = (uint64_t)threadIdx.x * per;
+  const uint64_t e = (b + per < n) ? b + per : n;
+  uint64_t run = 0;
+  for (uint64_t i = b; i < e; ++i) run += sums[i];
+  sh[threadIdx.x] = run;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the thread totals
+  for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+    uint64_t v = (threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
   }
+  uint64_t acc = sh[threadIdx.x] - run;
+  for (uint64_t i = b; i < e; ++i) { uint64_t t = sums[i]; sums[i] = acc; acc += t; }
+  if (threadIdx.x == blockDim.x - 1 && total) *total = sh[threadIdx.x];
 }
 
 __global__ void scan_apply_kernel(const uint64_t *in, uint64_t n, const uint64_t *sums, uint64_t *out) {
@@ -341,7 +355,7 @@ extern "C" hipError_t qmk_exscan_u64(const uint64_t *in, uint64_t n, uint64_t *o
   if (n == 0) return hipSuccess;
   uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, st, in, n, tmp_sums);
-  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(64), 0, st, tmp_sums, nb, total);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, st, tmp_sums, nb, total);
   hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nb), dim3(SCAN_BLOCK), 0, st, in, n, tmp_sums, out);
   return hipGetLastError();
 }

@@ -150,7 +150,7 @@ template <class T> void d2h(qm_ctx *c, T *dst, const void *src, size_t n) {
   }
 }
 void zero(qm_ctx *c, void *p, size_t bytes) {
-  HIPX(qmk_fill_u64(static_cast<uint64_t *>(p), (bytes + 7) / 8, 0, c->stream));
+  if (bytes) HIPX(hipMemsetAsync(p, 0, bytes, c->stream));
 }
 
 int qmcb_tag(int dt) {
@@ -293,6 +293,79 @@ void bind_filters(qm_ctx *ctx, ColSpace &cs, const std::vector<Filter> &ands,
     }
     q.ngroups = (int)groups.size();
   }
+  // AND-only fast path: predicate columns held in registers
+  q.and_only = 0;
+  q.nslots = 0;
+  if (q.ngroups == 0) {
+    int ns = 0;
+    bool ok = true;
+    for (int i = 0; i < q.npreds && ok; ++i) {
+      int s = -1;
+      for (int k = 0; k < ns; ++k) if (q.slot_col[k] == q.preds[i].col) s = k;
+      if (s < 0) {
+        if (ns >= MAX_PSLOTS) { ok = false; break; }
+        q.slot_col[ns] = q.preds[i].col;
+        s = ns++;
+      }
+      q.preds[i].slot = (int8_t)s;
+    }
+    if (ok) {
+      q.and_only = 1;
+      q.nslots = (int8_t)ns;
+      // merge the predicates of a column into one range where they are plain
+      // (non-negated) ranges; IS NOT NULL is implied by any range
+      DPred merged[MAX_PREDS];
+      int nm = 0;
+      bool rewrite = true;
+      for (int s = 0; s < ns && rewrite; ++s) {
+        int idx[MAX_PREDS];
+        int nr = 0;
+        bool is_false = false;
+        for (int i = 0; i < q.npreds; ++i) {
+          const DPred &p = q.preds[i];
+          if (p.slot != s) continue;
+          if (p.kind == PK_FALSE) is_false = true;
+          else if (p.kind != PK_IS_NOT_NULL) idx[nr++] = i;
+        }
+        DPred out;
+        memset(&out, 0, sizeof out);
+        if (is_false) {
+          out.kind = PK_FALSE;
+        } else if (nr == 0) {
+          out.kind = PK_IS_NOT_NULL;
+        } else if (nr == 1) {
+          out = q.preds[idx[0]];
+        } else {
+          DPred acc = q.preds[idx[0]];
+          for (int k = 0; k < nr && rewrite; ++k) {
+            const DPred &p = q.preds[idx[k]];
+            const bool ir = p.kind == PK_INT_RANGE && !p.negate;
+            const bool fr = p.kind == PK_F64_RANGE && !p.negate && !p.nan_result;
+            if ((!ir && !fr) || p.kind != acc.kind) { rewrite = false; break; }
+            if (k == 0) continue;
+            if (ir) {
+              if (p.ilo > acc.ilo) acc.ilo = p.ilo;
+              if (p.ihi < acc.ihi) acc.ihi = p.ihi;
+            } else {
+              if (p.flo > acc.flo || (p.flo == acc.flo && p.lo_strict)) { acc.flo = p.flo; acc.lo_strict = p.lo_strict; }
+              if (p.fhi < acc.fhi || (p.fhi == acc.fhi && p.hi_strict)) { acc.fhi = p.fhi; acc.hi_strict = p.hi_strict; }
+            }
+          }
+          if (!rewrite) break;
+          out = acc;
+          if (out.kind == PK_INT_RANGE && out.ilo > out.ihi) { memset(&out, 0, sizeof out); out.kind = PK_FALSE; }
+        }
+        out.col = q.slot_col[s];
+        out.slot = (int8_t)s;
+        out.group = 0;
+        merged[nm++] = out;
+      }
+      if (rewrite) {
+        for (int i = 0; i < nm; ++i) q.preds[i] = merged[i];
+        q.npreds = nm;
+      }
+    }
+  }
 }
 
 // ===========================================================================
@@ -310,7 +383,72 @@ struct AggBinding {
   bool always_one_row = false;
   uint64_t nslots = 0;
   uint64_t table_words = 0;
+  // partitioned aggregation (high-cardinality keys)
+  bool part = false;
+  int pbits = 0, sbits = 0, nparts = 0, part_lc = 0;
+  bool has_mask = false;
+  std::vector<std::pair<int, int>> srcs;  // (col_a, col_b)
+  int8_t state_src[MAX_STATES] = {};
+  double est_groups = 0;
 };
+
+const int PART_LDS_WORDS = 128 * 1024 / 8;  // pass-3 LDS table budget (one 1024-thread workgroup per CU)
+
+// Decide the partitioned strategy and its record layout.
+void plan_partitioned(ColSpace &cs, uint64_t nrows, double est_groups, bool fused_join, AggBinding &ab) {
+  DQuery &q = ab.q;
+  ab.part = false;
+  if (fused_join || nrows < (1u << 20) || q.key.ncols == 0) return;
+  const int W = q.entry_words;
+  int lc = 1;
+  while (lc * 2 * W <= PART_LDS_WORDS) lc *= 2;
+  if (q.gmode == GM_HASH) {
+    if (est_groups <= 32768.0) return;
+    uint64_t parts = pow2ceil((uint64_t)std::ceil(est_groups * 2.0 / lc));
+    if (parts > 8192) return;
+    ab.nparts = (int)parts;
+    int pb = 0;
+    while ((1ull << pb) < parts) ++pb;
+    ab.pbits = pb;
+  } else {
+    if (q.lmode != LM_NONE || q.gcap < 32768) return;
+    uint64_t parts = (q.gcap + lc - 1) / lc;
+    if (parts > 8192) return;
+    int sb = 0;
+    while ((1 << sb) < lc) ++sb;
+    ab.sbits = sb;
+    ab.nparts = (int)parts;
+  }
+  ab.part_lc = lc;
+  // value sources of the states
+  ab.srcs.clear();
+  bool nullable_src = false;
+  for (int si = 0; si < q.nstates; ++si) {
+    const DState &s = q.states[si];
+    if (s.col_a < 0) { ab.state_src[si] = -1; continue; }
+    std::pair<int, int> key{s.col_a, s.product ? s.col_b : -1};
+    int idx = -1;
+    for (size_t j = 0; j < ab.srcs.size(); ++j) if (ab.srcs[j] == key) idx = (int)j;
+    if (idx < 0) { ab.srcs.push_back(key); idx = (int)ab.srcs.size() - 1; }
+    ab.state_src[si] = (int8_t)idx;
+    if (cs.nullable(s.col_a) || (s.product && cs.nullable(s.col_b))) nullable_src = true;
+  }
+  if ((int)ab.srcs.size() > MAX_SRC - 1) return;
+  bool side_keys = false;
+  if (q.gmode == GM_HASH) {
+    for (int i = 0; i < q.key.ncols; ++i) if (q.key.nullable[i]) side_keys = true;
+    if (q.key.ncols == 1) {
+      const ColStats &st = cs.st(q.key.col[0]);
+      if (q.key.is_float[0]) side_keys = true;
+      else if (st.has_int && (int64_t)KEY_UNREPRESENTABLE >= st.imin && (int64_t)KEY_UNREPRESENTABLE <= st.imax)
+        side_keys = true;
+    }
+  }
+  ab.has_mask = nullable_src || side_keys;
+  q.lmode = LM_NONE;
+  q.lds_words = 0;
+  ab.part = true;
+}
 
 const int LDS_BLOCK_BUDGET_WORDS = 48 * 1024 / 8;  // 48 KiB of LDS per workgroup for tables
 
@@ -490,6 +628,7 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
       if (key.bits[0] + key.bits[1] > 63) fail(QM_ERR_UNSUPPORTED, "GROUP BY key domain exceeds 63 bits");
     }
     double est = std::min((double)nrows, distinct_prod);
+    ab.est_groups = est;
     uint64_t cap = pow2ceil((uint64_t)std::max(2048.0, est * 2.0));
     cap <<= (2 * regrow);
     q.gcap = cap;
@@ -511,6 +650,8 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
     }
   }
   if (fused_join && q.lmode == LM_HASH) q.lmode = LM_NONE;
+
+  plan_partitioned(cs, nrows, ab.est_groups, fused_join, ab);
 
   // ---- outputs ----
   for (int i = 0; i < nk; ++i) {
@@ -1035,6 +1176,99 @@ void scan_launch_shape(qm_ctx *ctx, uint64_t nrows, uint64_t lds_words, int *gri
   *grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * per_cu);
 }
 
+// Partitioned aggregation: pass 1 (filter + compact + histogram), scan of
+// the partition histogram, pass 2 (scatter), pass 3 (per-partition LDS
+// aggregation into the global table). Returns the number of launches.
+uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const AggBinding &ab, uint64_t nrows,
+                         uint64_t *d_selected) {
+  const DQuery &q = ab.q;
+  const uint64_t tile_rows = 256 * RPT;
+  const uint64_t ntiles = std::max<uint64_t>(1, (nrows + tile_rows - 1) / tile_rows);
+  // register-descriptor pass 1 when the filter is one range per column
+  bool fast = q.and_only && ab.srcs.size() <= 4;
+  if (fast) {
+    int per_slot[MAX_PSLOTS] = {0, 0, 0, 0};
+    for (int i = 0; i < q.npreds; ++i) {
+      const DPred &p = q.preds[i];
+      if (p.kind == PK_INT_IN || p.kind == PK_F64_IN) fast = false;
+      if (p.slot >= 0 && p.slot < MAX_PSLOTS) per_slot[(int)p.slot]++;
+    }
+    for (int s2 = 0; s2 < q.nslots; ++s2) if (per_slot[s2] != 1) fast = false;
+    if (q.nslots < 1 || q.key.ncols < 1) fast = false;
+  }
+  const int fast_code = fast ? q.nslots * 4 + q.key.ncols : 0;
+  const size_t hist_lds = (size_t)ab.nparts * 4;
+  const int bpc = qmk_part_scan_blocks_per_cu(fast_code, hist_lds);
+  uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * bpc);
+  const uint64_t tpb = (ntiles + grid - 1) / grid;
+  grid = (ntiles + tpb - 1) / tpb;
+  const uint64_t P = (uint64_t)ab.nparts;
+  DPart pp;
+  memset(&pp, 0, sizeof pp);
+  pp.nsrc = (int)ab.srcs.size();
+  for (int j = 0; j < pp.nsrc; ++j) {
+    pp.src_a[j] = (int8_t)ab.srcs[j].first;
+    pp.src_b[j] = (int8_t)ab.srcs[j].second;
+    pp.src_float[j] = (ab.srcs[j].second >= 0 || dt_float(sc.cols[ab.srcs[j].first].dtype)) ? 1 : 0;
+  }
+  for (int si = 0; si < MAX_STATES; ++si) pp.state_src[si] = ab.state_src[si];
+  pp.has_mask = ab.has_mask ? 1 : 0;
+  pp.rec_words = 1 + pp.nsrc + pp.has_mask;
+  pp.pbits = ab.pbits;
+  pp.sbits = ab.sbits;
+  pp.nparts = (int)P;
+  pp.lds_entries = ab.part_lc;
+  const uint64_t R = (uint64_t)pp.rec_words;
+  const uint64_t blk_cap = tpb * tile_rows;
+  DevBuf recs(ctx, grid * blk_cap * R * 8), bcount(ctx, grid * 8 + 64), hist(ctx, P * grid * 8);
+  pp.recs = recs.as<uint64_t>();
+  pp.blk_cap = blk_cap;
+  pp.blk_count = bcount.as<uint64_t>();
+  pp.hist = hist.as<uint64_t>();
+  HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+  HIPX(qmk_part_scan(&sc, d_q, &pp, (uint32_t)grid, tpb, fast_code, 1, ctx->stream));
+  HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+  DevBuf poff(ctx, P * grid * 8), stmp(ctx, qmk_exscan_tmp_words(P * grid) * 8 + 64);
+  uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(P * grid);
+  HIPX(qmk_exscan_u64(hist.as<uint64_t>(), P * grid, poff.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
+                      ctx->stream));
+  uint64_t total = 0;
+  d2h(ctx, &total, total_d, 1);
+  h2d(ctx, d_selected, &total, 1);
+  DevBuf precs(ctx, std::max<uint64_t>(total, 1) * R * 8);
+  pp.part_off = poff.as<uint64_t>();
+  pp.precs = precs.as<uint64_t>();
+  const uint32_t rpb = (uint32_t)std::max<uint64_t>(1, grid / (uint64_t)ctx->num_cus);
+  HIPX(qmk_part_scatter(d_q, &pp, (uint32_t)grid, rpb, ctx->stream));
+  DevBuf starts(ctx, (P + 1) * 8);
+  HIPX(qmk_part_starts(poff.as<uint64_t>(), (uint32_t)P, (uint32_t)grid, total, starts.as<uint64_t>(), ctx->stream));
+  std::vector<uint64_t> st(P + 1);
+  d2h(ctx, st.data(), starts.p, P + 1);
+  // segments: split partitions larger than seg_max across workgroups
+  const uint64_t seg_max = std::max<uint64_t>(16384, total / ((uint64_t)ctx->num_cus * 2) + 1);
+  std::vector<uint64_t> seg;
+  for (uint64_t p = 0; p < P; ++p) {
+    uint64_t a = st[p], b = st[p + 1];
+    if (b <= a) continue;
+    uint64_t n = b - a, nsplit = (n + seg_max - 1) / seg_max;
+    for (uint64_t s = 0; s < nsplit; ++s) {
+      uint64_t s0 = a + s * seg_max, s1 = std::min(b, s0 + seg_max);
+      seg.push_back(p);
+      seg.push_back(s0);
+      seg.push_back(s1 - s0);
+      seg.push_back(nsplit == 1 ? 1 : 0);
+    }
+  }
+  const uint64_t nseg = seg.size() / 4;
+  DevBuf segbuf(ctx, std::max<uint64_t>(seg.size(), 4) * 8);
+  h2d(ctx, segbuf.p, seg.data(), seg.size());
+  pp.seg = segbuf.as<uint64_t>();
+  pp.inserted = q.selected + 1;
+  HIPX(qmk_part_agg(d_q, &pp, (uint32_t)nseg, (uint32_t)ctx->num_cus, q.entry_words, ctx->stream));
+  HIPX(hipStreamSynchronize(ctx->stream));
+  return 6;
+}
+
 // Run one or more aggregate queries (same table) through the fused scan and
 // finalise each into a result.
 void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std::vector<const Plan *> &plans,
@@ -1096,7 +1330,9 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
     size_t lds_bytes;
     scan_launch_shape(ctx, T->rows, lds_off, &grid, &block, &lds_bytes);
     if (lds_bytes > 160 * 1024) fail(QM_ERR_UNSUPPORTED, "fused queries exceed the workgroup LDS");
-    if (T->rows) {
+    if (T->rows && nq == 1 && ab[0].part) {
+      launches += run_partitioned(ctx, sc, dq.as<DQuery>(), ab[0], T->rows, selected[0].as<uint64_t>());
+    } else if (T->rows) {
       HIPX(hipEventRecord(ctx->evk0, ctx->stream));
       HIPX(qmk_scan_agg(&sc, dq.as<DQuery>(), grid, block, lds_bytes, ctx->stream));
       HIPX(hipEventRecord(ctx->evk1, ctx->stream));
@@ -1157,6 +1393,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       res->stats.rows_selected = sel;
       res->stats.strategy = b.q.gmode == GM_DIRECT ? (b.q.lmode == LM_DIRECT ? QM_STRAT_DIRECT_LDS : QM_STRAT_DIRECT_GLB)
                                                    : (b.q.lmode == LM_HASH ? QM_STRAT_HASH_LDS : QM_STRAT_HASH_GLB);
+      if (b.part) res->stats.strategy = b.q.gmode == GM_DIRECT ? QM_STRAT_PART_DIRECT : QM_STRAT_PART_HASH;
       res->stats.retries = attempt;
       DQuery noq;
       memset(&noq, 0, sizeof noq);

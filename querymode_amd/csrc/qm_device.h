@@ -18,6 +18,7 @@ static constexpr int MAX_PREDS = 16;
 static constexpr int MAX_STATES = 16;
 static constexpr int MAX_KEYS = 2;
 static constexpr int MAX_QUERIES = 16;     // per fused launch of the scan kernel
+static constexpr int MAX_PSLOTS = 4;       // predicate columns held in registers
 
 // Rows handled per thread per tile; tiles are blockDim.x * ROWS_PER_THREAD.
 static constexpr int RPT = 8;
@@ -56,7 +57,8 @@ struct DPred {
   double flo, fhi;
   const void *list;
   int32_t list_n;
-  int32_t _pad;
+  int8_t slot;         // register slot of the column in the AND-only fast path
+  int8_t _pad[3];
 };
 
 // aggregate state kinds; each owns `words` u64 words of a table entry
@@ -103,7 +105,9 @@ struct DQuery {
   int32_t entry_words;    // u64 words per table entry (HASH: word 0 = key)
   int32_t count_word;     // word of the row-count state (always present)
   uint8_t gmode, lmode;
-  uint8_t _p0, _p1;
+  uint8_t and_only;       // no OR groups and <= MAX_PSLOTS distinct predicate columns
+  int8_t nslots;          // distinct predicate columns (fast path)
+  int8_t slot_col[4];     // column of each slot
   DPred preds[MAX_PREDS];
   DState states[MAX_STATES];
   DKey key;
@@ -117,6 +121,32 @@ struct DQuery {
   int32_t lds_words;      // total LDS words used by this query
   int32_t lds_offset;     // word offset inside the block's LDS
   int32_t _p2;
+};
+
+// Partitioned aggregation (high-cardinality GROUP BY). Pass 1 compacts the
+// selected rows into records {key code, value words..., [null mask]};
+// pass 2 scatters them by key partition; pass 3 aggregates each partition
+// in an LDS table and writes its groups into the global table.
+static constexpr int MAX_SRC = 8;
+struct DPart {
+  int32_t nsrc;                 // distinct value sources of the states
+  int8_t src_a[MAX_SRC], src_b[MAX_SRC];
+  uint8_t src_float[MAX_SRC];   // value word holds f64 bits (else int64)
+  int8_t state_src[MAX_STATES]; // per state: source index (-1 = COUNT(*))
+  int32_t rec_words;            // 1 + nsrc (+1 mask word)
+  int32_t has_mask;             // mask word: bit j = source j null; bit 63 side key, bit 62 NULL key
+  int32_t pbits;                // HASH: partitions = 1 << pbits (top hash bits)
+  int32_t sbits;                // DIRECT: partition = idx >> sbits, LDS slot = idx & (2^sbits - 1)
+  int32_t nparts;
+  int32_t lds_entries;          // pass-3 LDS table entries
+  uint64_t *recs;               // pass-1 output: block b's records at recs + b * blk_cap * rec_words
+  uint64_t blk_cap;             // records per block region (rows the block scans)
+  uint64_t *blk_count;          // [grid] records written by each block
+  uint64_t *hist;               // [nparts][grid] counts
+  const uint64_t *part_off;     // exclusive scan of hist
+  uint64_t *precs;              // pass-2 output (partition order)
+  const uint64_t *seg;          // pass-3 segments: {part, start, count, exclusive} x nseg
+  uint64_t *inserted;           // groups inserted into the global table
 };
 
 struct DScan {

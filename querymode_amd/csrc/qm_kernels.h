@@ -93,6 +93,16 @@ hipError_t qmk_join_write(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t n
                           const uint64_t *block_offsets, uint64_t out_cap, uint64_t *out_left,
                           uint64_t *out_right, hipStream_t st);
 
+// partitioned aggregation
+hipError_t qmk_part_scan(const qm::DScan *sc, const qm::DQuery *d_q, const qm::DPart *pp, uint32_t grid,
+                         uint64_t tpb, int fast, int prefetch, hipStream_t st);
+int qmk_part_scan_blocks_per_cu(int fast, size_t lds);
+hipError_t qmk_part_scatter(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t g1, uint32_t rpb, hipStream_t st);
+hipError_t qmk_part_starts(const uint64_t *part_off, uint32_t nparts, uint32_t grid, uint64_t total,
+                           uint64_t *starts, hipStream_t st);
+hipError_t qmk_part_agg(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t nseg, uint32_t grid, int entry_words,
+                        hipStream_t st);
+
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);

@@ -53,13 +53,14 @@ def test_filter_groupby_zipf(qm, skewed):
                         aggregates=[AggregateOp("sum", "b"), AggregateOp("count", "*"), AggregateOp("min", "a"),
                                     AggregateOp("max", "a"), AggregateOp("avg", "b")])
     got, want = run_both(qm, d, {"t": cols})
-    assert got.stats["strategy"] == ("hash_lds" if skewed else "hash_global")
+    assert got.stats["strategy"] == "part_hash"
     assert_groups_equal(got.rows, want, ["key"], float_cols={"sum_b", "avg_b"}, ctx="c2")
 
 
 def test_global_aggregate_and_empty(qm):
     _say("case global aggregate")
     cols = datagen.filter_groupby_zipf(n=300_000, seed=2)
+    cols["key"] = cols["a"].astype(np.int64) * 1_000_003 - 7  # int64 sums stay inside int64
     for flt in ([FilterOp("a", "lt", 500000)], [FilterOp("a", "lt", -5)]):
         d = QueryDescriptor("t", filters=flt, aggregates=[
             AggregateOp("sum", "b"), AggregateOp("count", "*"), AggregateOp("min", "a"), AggregateOp("max", "b"),
@@ -117,3 +118,40 @@ def test_select_limit_offset(qm):
     d2 = QueryDescriptor("t", filters=[FilterOp("a", "gte", 999000)], sortColumn="b")
     got, want = run_both(qm, d2, {"t": cols})
     assert_rows_equal(got.rows, want, ctx="select sort no limit")
+
+
+@pytest.mark.parametrize("nullable", [False, True])
+def test_partitioned_direct_and_nulls(qm, nullable):
+    """Dense 1M-slot key domain (too large for LDS): partitioned by key slice."""
+    _say(f"case partitioned direct nullable={nullable}")
+    n = 2_000_000
+    cols = datagen.batch_table(n=n, seed=9)
+    cols["k2"] = cols["k2"] % 1_000_000
+    nulls = None
+    if nullable:
+        rng = np.random.default_rng(3)
+        nulls = {"t": {"k2": rng.random(n) < 0.01, "v2": rng.random(n) < 0.05, "v1": rng.random(n) < 0.02}}
+    d = QueryDescriptor("t", filters=[FilterOp("v2", "lt", 0.5)], groupBy=["k2"],
+                        aggregates=[AggregateOp("sum", "v2"), AggregateOp("count", "*"), AggregateOp("count", "v1"),
+                                    AggregateOp("min", "v1"), AggregateOp("max", "v1"), AggregateOp("sum", "v1"),
+                                    AggregateOp("avg", "v2")])
+    got, want = run_both(qm, d, {"t": cols}, nulls)
+    assert got.stats["strategy"] == "part_direct"
+    assert_groups_equal(got.rows, want, ["k2"], float_cols={"sum_v2", "avg_v2"}, ctx="part direct")
+
+
+def test_partitioned_hash_nullable_two_keys(qm):
+    _say("case partitioned hash nullable + two-column key")
+    n = 1_500_000
+    cols = datagen.filter_groupby_zipf(n=n, seed=10)
+    rng = np.random.default_rng(11)
+    nulls = {"t": {"key": rng.random(n) < 0.02, "b": rng.random(n) < 0.03}}
+    d = QueryDescriptor("t", filters=[FilterOp("flag", "eq", 1)], groupBy=["key"],
+                        aggregates=[AggregateOp("sum", "b"), AggregateOp("count", "b"), AggregateOp("max", "a")])
+    got, want = run_both(qm, d, {"t": cols}, nulls)
+    assert got.stats["strategy"] == "part_hash"
+    assert_groups_equal(got.rows, want, ["key"], float_cols={"sum_b"}, ctx="part hash nulls")
+    cols2 = {"x": (cols["a"] % 3000).astype(np.int32), "y": (cols["key"] % 700).astype(np.int64), "b": cols["b"]}
+    d2 = QueryDescriptor("u", groupBy=["x", "y"], aggregates=[AggregateOp("sum", "b"), AggregateOp("count", "*")])
+    got, want = run_both(qm, d2, {"u": cols2})
+    assert_groups_equal(got.rows, want, ["x", "y"], float_cols={"sum_b"}, ctx="two keys")
