@@ -20,21 +20,62 @@ namespace qm {
 
 static constexpr int PART_BLOCK = 256;
 static constexpr int UNROLL = 4;
-static constexpr int MAX_RW = 1 + MAX_SRC;  // record words (key + sources + mask) upper bound
+static constexpr int MAX_RW = 8;  // record words: 4 or 8 (32/64-byte records, copied as 16-byte vectors)
+
+// record copy / load in 16-byte pieces (records are 32-byte aligned)
+__device__ __forceinline__ void rec_load(const uint64_t *src, int R, uint64_t rw[MAX_RW]) {
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+  for (int q = 0; q < MAX_RW / 2; ++q) {
+    if (2 * q < R) {
+      uint4 v = s4[q];
+      rw[2 * q] = ((uint64_t)v.y << 32) | v.x;
+      rw[2 * q + 1] = ((uint64_t)v.w << 32) | v.z;
+    }
+  }
+}
+__device__ __forceinline__ void rec_copy(uint64_t *dst, const uint64_t *src, int R) {
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+  uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+  uint4 v[MAX_RW / 2];
+#pragma unroll
+  for (int q = 0; q < MAX_RW / 2; ++q) if (2 * q < R) v[q] = s4[q];
+#pragma unroll
+  for (int q = 0; q < MAX_RW / 2; ++q) if (2 * q < R) d4[q] = v[q];
+}
 
 __device__ __forceinline__ uint32_t part_of(const DQuery &q, const DPart &pp, uint64_t code) {
   if (q.gmode == GM_DIRECT) return (uint32_t)(code >> pp.sbits);
   return pp.pbits ? (uint32_t)(mix64(code) >> (64 - pp.pbits)) : 0u;
 }
 
+// heavy-hitter table lookup (HOT_SLOTS slots, linear probing)
+__device__ __forceinline__ int hot_slot(const uint64_t *hot, uint64_t hk) {
+  uint32_t h = (uint32_t)mix64(hk ^ 0xA5A5A5A5A5A5A5A5ull) & (HOT_SLOTS - 1);
+#pragma unroll
+  for (int p = 0; p < HOT_PROBES; ++p) {
+    uint32_t s = (h + p) & (HOT_SLOTS - 1);
+    uint64_t k = hot[s];
+    if (k == hk) return (int)s;
+    if (k == 0) return -1;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void load_hot(const DPart &pp, uint64_t *hot_lds) {
+  for (int i = threadIdx.x; i < HOT_SLOTS; i += blockDim.x) hot_lds[i] = pp.hot_n ? pp.hot_keys[i] : 0;
+}
+
 // ---- pass 1: filter + compact + partition histogram ----
 __global__ void __launch_bounds__(PART_BLOCK) part_scan_kernel(DScan sc, const DQuery *__restrict__ qs, DPart pp,
                                                                uint64_t tiles_per_block) {
-  extern __shared__ uint32_t hist_lds[];  // nparts counters
+  extern __shared__ uint32_t hist_lds[];  // nparts + 1 counters (the last: heavy hitters)
   __shared__ uint32_t s_fill;
+  __shared__ uint64_t hot_lds[HOT_SLOTS];
   const DQuery &q = qs[0];
-  for (int i = threadIdx.x; i < pp.nparts; i += blockDim.x) hist_lds[i] = 0;
+  for (int i = threadIdx.x; i <= pp.nparts; i += blockDim.x) hist_lds[i] = 0;
   if (threadIdx.x == 0) s_fill = 0;
+  load_hot(pp, hot_lds);
   __syncthreads();
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t tile_rows = (uint64_t)blockDim.x * RPT;
@@ -105,6 +146,8 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_kernel(DScan sc, const D
       rec_idx[r] = pos++;
       region[rec_idx[r] * R] = code;
       uint32_t p = (mask_w[r] >> 63) ? 0u : part_of(q, pp, code);
+      if (pp.hot_n && !(mask_w[r] >> 63) && hot_slot(hot_lds, q.gmode == GM_DIRECT ? code + 1 : code) >= 0)
+        p = pp.nparts;
       atomicAdd(&hist_lds[p], 1u);
     }
     for (int j = 0; j < pp.nsrc; ++j) {
@@ -146,7 +189,7 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_kernel(DScan sc, const D
     }
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < pp.nparts; p += blockDim.x)
+  for (int p = threadIdx.x; p <= pp.nparts; p += blockDim.x)
     pp.hist[(uint64_t)p * gridDim.x + blockIdx.x] = hist_lds[p];
   if (threadIdx.x == 0) pp.blk_count[blockIdx.x] = s_fill;
 }
@@ -160,9 +203,12 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
                                                                     int prefetch) {
   extern __shared__ uint32_t hist_lds[];
   __shared__ uint32_t s_fill;
+  __shared__ uint64_t hot_lds[HOT_SLOTS];
   const DQuery &q = qs[0];
-  for (int i = threadIdx.x; i < pp.nparts; i += blockDim.x) hist_lds[i] = 0;
+  for (int i = threadIdx.x; i <= pp.nparts; i += blockDim.x) hist_lds[i] = 0;
   if (threadIdx.x == 0) s_fill = 0;
+  load_hot(pp, hot_lds);
+  const int hot_n = pp.hot_n;
   const FastFilter F = fast_filter_of(q, sc.cols);
   const FastKey K = fast_key_of(q, sc.cols);
   const FastSrc S = fast_src_of(pp, q, sc.cols);
@@ -225,6 +271,7 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
         if (sd >= 0) { side_m |= 1u << r; if (sd) nullk_m |= 1u << r; }
         p = (sd >= 0 || !pp.pbits) ? 0u : (uint32_t)(mix64(code) >> (64 - pp.pbits));
       }
+      if (hot_n && sd < 0 && hot_slot(hot_lds, direct ? code + 1 : code) >= 0) p = pp.nparts;
       const uint64_t ri = pos0 + __builtin_popcount(m & ((1u << r) - 1));
       region[ri * R] = code;
       atomicAdd(&hist_lds[p], 1u);
@@ -280,61 +327,12 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
     }
   }
   __syncthreads();
-  for (int p = threadIdx.x; p < pp.nparts; p += blockDim.x)
+  for (int p = threadIdx.x; p <= pp.nparts; p += blockDim.x)
     pp.hist[(uint64_t)p * gridDim.x + blockIdx.x] = hist_lds[p];
   if (threadIdx.x == 0) pp.blk_count[blockIdx.x] = s_fill;
 }
 
-// ---- pass 2: scatter block regions into partition order. Workgroup b2
-// handles pass-1 regions [b2*rpb, (b2+1)*rpb): the partition ranges of those
-// regions are adjacent in the partition-major offsets, so each workgroup
-// writes one contiguous run per partition (few partially written lines). ----
-__global__ void __launch_bounds__(PART_BLOCK) part_scatter_kernel(const DQuery *__restrict__ qs, DPart pp,
-                                                                  uint32_t g1, uint32_t rpb) {
-  extern __shared__ uint64_t cur[];  // nparts cursors
-  const DQuery &q = qs[0];
-  const uint32_t b0 = blockIdx.x * rpb;
-  const uint32_t b1 = (b0 + rpb < g1) ? b0 + rpb : g1;
-  for (int p = threadIdx.x; p < pp.nparts; p += blockDim.x) cur[p] = pp.part_off[(uint64_t)p * g1 + b0];
-  __syncthreads();
-  const int R = pp.rec_words;
-  const int pbits = pp.pbits, sbits = pp.sbits;
-  const bool direct = q.gmode == GM_DIRECT;
-  const int has_mask = pp.has_mask;
-  for (uint32_t rb = b0; rb < b1; ++rb) {
-    const uint64_t n = pp.blk_count[rb];
-    const uint64_t *region = pp.recs + (uint64_t)rb * pp.blk_cap * R;
-    for (uint64_t i0 = 0; i0 < n; i0 += (uint64_t)UNROLL * blockDim.x) {
-      uint64_t idx[UNROLL], key[UNROLL], dst[UNROLL];
-      bool side[UNROLL];
-#pragma unroll
-      for (int u = 0; u < UNROLL; ++u) {
-        idx[u] = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
-        key[u] = 0;
-        side[u] = false;
-        if (idx[u] < n) {
-          key[u] = region[idx[u] * R];
-          if (has_mask) side[u] = (region[idx[u] * R + R - 1] >> 63) != 0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < UNROLL; ++u) {
-        if (idx[u] >= n) continue;
-        uint32_t p = side[u] ? 0u
-                             : (direct ? (uint32_t)(key[u] >> sbits)
-                                       : (pbits ? (uint32_t)(mix64(key[u]) >> (64 - pbits)) : 0u));
-        dst[u] = (uint64_t)atomicAdd((unsigned long long *)&cur[p], 1ull);
-      }
-      for (int w = 0; w < R; ++w) {
-#pragma unroll
-        for (int u = 0; u < UNROLL; ++u)
-          if (idx[u] < n) pp.precs[dst[u] * R + w] = region[idx[u] * R + w];
-      }
-    }
-  }
-}
-
-// ---- pass 3: aggregate each partition segment in LDS ----
+// apply one record's state contributions to an LDS or global entry
 __device__ __forceinline__ void rec_apply(const DQuery &q, const DPart &pp, const uint64_t *rw, int R,
                                           lds_u64 *lent, uint64_t *gent) {
   for (int si = 0; si < q.nstates; ++si) {
@@ -352,6 +350,91 @@ __device__ __forceinline__ void rec_apply(const DQuery &q, const DPart &pp, cons
     }
     if (lent) apply_lds(s, lent, k);
     else apply_glb(s, gent, k);
+  }
+}
+
+// ---- pass 2: scatter block regions into partition order. Workgroup b2
+// handles pass-1 regions [b2*rpb, (b2+1)*rpb): the partition ranges of those
+// regions are adjacent in the partition-major offsets, so each workgroup
+// writes one contiguous run per partition. Heavy-hitter records are
+// aggregated in LDS here instead of being written. ----
+__global__ void __launch_bounds__(1024) part_scatter_kernel(const DQuery *__restrict__ qs, DPart pp, uint32_t g1,
+                                                            uint32_t rpb) {
+  extern __shared__ uint64_t cur[];  // nparts + 1 cursors, then HOT_SLOTS hot entries
+  __shared__ uint64_t hot_lds[HOT_SLOTS];
+  const DQuery &q = qs[0];
+  const int W = q.entry_words;
+  const uint32_t b0 = blockIdx.x * rpb;
+  const uint32_t b1 = (b0 + rpb < g1) ? b0 + rpb : g1;
+  const int nparts = pp.nparts;
+  lds_u64 *hent = (lds_u64 *)(cur + nparts + 1);
+  for (int p = threadIdx.x; p <= nparts; p += blockDim.x) cur[p] = pp.part_off[(uint64_t)p * g1 + b0];
+  load_hot(pp, hot_lds);
+  const int hot_n = pp.hot_n;
+  if (hot_n)
+    for (int i = threadIdx.x; i < HOT_SLOTS * W; i += blockDim.x) hent[i] = 0;
+  __syncthreads();
+  const int R = pp.rec_words;
+  const int pbits = pp.pbits, sbits = pp.sbits;
+  const bool direct = q.gmode == GM_DIRECT;
+  const int has_mask = pp.has_mask;
+  for (uint32_t rb = b0; rb < b1; ++rb) {
+    const uint64_t n = pp.blk_count[rb];
+    const uint64_t *region = pp.recs + (uint64_t)rb * pp.blk_cap * R;
+    for (uint64_t i0 = 0; i0 < n; i0 += (uint64_t)UNROLL * blockDim.x) {
+      uint64_t idx[UNROLL], key[UNROLL], dst[UNROLL];
+      bool side[UNROLL];
+      int hs[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        idx[u] = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
+        key[u] = 0;
+        side[u] = false;
+        hs[u] = -1;
+        if (idx[u] < n) {
+          key[u] = region[idx[u] * R];
+          if (has_mask) side[u] = (region[idx[u] * R + R - 1] >> 63) != 0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        if (idx[u] >= n) continue;
+        if (hot_n && !side[u]) hs[u] = hot_slot(hot_lds, direct ? key[u] + 1 : key[u]);
+        if (hs[u] >= 0) {
+          uint64_t rw[MAX_RW];
+          rec_load(region + idx[u] * R, R, rw);
+          rec_apply(q, pp, rw, R, hent + (uint64_t)hs[u] * W, nullptr);
+          continue;
+        }
+        uint32_t p = side[u] ? 0u
+                             : (direct ? (uint32_t)(key[u] >> sbits)
+                                       : (pbits ? (uint32_t)(mix64(key[u]) >> (64 - pbits)) : 0u));
+        dst[u] = (uint64_t)atomicAdd((unsigned long long *)&cur[p], 1ull);
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)
+        if (idx[u] < n && hs[u] < 0) rec_copy(pp.precs + dst[u] * R, region + idx[u] * R, R);
+    }
+  }
+  if (!hot_n) return;
+  __syncthreads();
+  // merge the heavy hitters into the global table
+  for (int sl = threadIdx.x; sl < HOT_SLOTS; sl += blockDim.x) {
+    const uint64_t hk = hot_lds[sl];
+    lds_u64 *e = hent + (uint64_t)sl * W;
+    if (hk == 0 || e[q.count_word] == 0) continue;
+    uint64_t acc[24];
+    for (int w = 0; w < W; ++w) acc[w] = e[w];
+    uint64_t *dst;
+    if (direct) {
+      dst = q.table + (hk - 1) * W;
+    } else {
+      acc[0] = hk;
+      uint64_t g = glb_find(q, hk);
+      if (g == ~0ull) continue;
+      dst = q.table + g * W;
+    }
+    merge_entry_glb(q, dst, acc);
   }
 }
 
@@ -382,9 +465,8 @@ __global__ void __launch_bounds__(1024) part_agg_kernel(const DQuery *__restrict
       for (int u = 0; u < UNROLL; ++u) {
         uint64_t i = i0 + (uint64_t)u * blockDim.x + threadIdx.x;
         ok[u] = i < count;
-#pragma unroll
-        for (int w = 0; w < MAX_RW; ++w)
-          if (w < R) rw[u][w] = ok[u] ? base[i * R + w] : 0;
+        if (ok[u]) rec_load(base + i * R, R, rw[u]);
+        else rw[u][0] = 0;
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
@@ -479,7 +561,7 @@ static const void *fast_fn(int fast) {
 
 extern "C" hipError_t qmk_part_scan(const DScan *sc, const DQuery *d_q, const DPart *pp, uint32_t grid,
                                     uint64_t tpb, int fast, int prefetch, hipStream_t st) {
-  size_t lds = (size_t)pp->nparts * 4;
+  size_t lds = (size_t)(pp->nparts + 1) * 4;
   const void *fn = fast ? fast_fn(fast) : (const void *)part_scan_kernel;
   if (!fn) return hipErrorInvalidValue;
   if (lds > 65536) {
@@ -495,15 +577,15 @@ extern "C" hipError_t qmk_part_scan(const DScan *sc, const DQuery *d_q, const DP
 }
 
 extern "C" hipError_t qmk_part_scatter(const DQuery *d_q, const DPart *pp, uint32_t g1, uint32_t rpb,
-                                       hipStream_t st) {
-  size_t lds = (size_t)pp->nparts * 8;
+                                       int entry_words, hipStream_t st) {
+  size_t lds = (size_t)(pp->nparts + 1) * 8 + (pp->hot_n ? (size_t)HOT_SLOTS * entry_words * 8 : 0);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void *)part_scatter_kernel,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
   }
   uint32_t g2 = (g1 + rpb - 1) / rpb;
-  hipLaunchKernelGGL(part_scatter_kernel, dim3(g2), dim3(PART_BLOCK), lds, st, d_q, *pp, g1, rpb);
+  hipLaunchKernelGGL(part_scatter_kernel, dim3(g2), dim3(1024), lds, st, d_q, *pp, g1, rpb);
   return hipGetLastError();
 }
 

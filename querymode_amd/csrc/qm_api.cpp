@@ -390,6 +390,7 @@ struct AggBinding {
   std::vector<std::pair<int, int>> srcs;  // (col_a, col_b)
   int8_t state_src[MAX_STATES] = {};
   double est_groups = 0;
+  std::vector<uint64_t> hot_table;  // heavy-hitter slots (HOT_SLOTS), empty = none
 };
 
 const int PART_LDS_WORDS = 128 * 1024 / 8;  // pass-3 LDS table budget (one 1024-thread workgroup per CU)
@@ -433,7 +434,7 @@ void plan_partitioned(ColSpace &cs, uint64_t nrows, double est_groups, bool fuse
     ab.state_src[si] = (int8_t)idx;
     if (cs.nullable(s.col_a) || (s.product && cs.nullable(s.col_b))) nullable_src = true;
   }
-  if ((int)ab.srcs.size() > MAX_SRC - 1) return;
+  if ((int)ab.srcs.size() > 6) return;  // record = key + <= 6 sources + mask in 8 words
   bool side_keys = false;
   if (q.gmode == GM_HASH) {
     for (int i = 0; i < q.key.ncols; ++i) if (q.key.nullable[i]) side_keys = true;
@@ -445,6 +446,31 @@ void plan_partitioned(ColSpace &cs, uint64_t nrows, double est_groups, bool fuse
     }
   }
   ab.has_mask = nullable_src || side_keys;
+  // heavy hitters of a single integer key (from the upload sample)
+  ab.hot_table.clear();
+  if (q.key.ncols == 1 && !q.key.is_float[0]) {
+    const ColStats &st = cs.st(q.key.col[0]);
+    if (st.heavy.size() >= 2) {
+      std::vector<uint64_t> tab(HOT_SLOTS, 0);
+      int placed = 0;
+      for (int64_t v : st.heavy) {
+        uint64_t hk;
+        if (q.gmode == GM_DIRECT) {
+          if (v < q.key.kmin[0] || (uint64_t)(v - q.key.kmin[0]) >= q.key.dom[0]) continue;
+          hk = (uint64_t)(v - q.key.kmin[0]) + 1;
+        } else {
+          hk = (uint64_t)v + KEY_SALT;
+          if (hk == 0) continue;
+        }
+        uint32_t h = (uint32_t)mix64(hk ^ 0xA5A5A5A5A5A5A5A5ull) & (HOT_SLOTS - 1);
+        for (int pr = 0; pr < HOT_PROBES; ++pr) {
+          uint32_t sl = (h + pr) & (HOT_SLOTS - 1);
+          if (tab[sl] == 0) { tab[sl] = hk; ++placed; break; }
+        }
+      }
+      if (placed >= 2) ab.hot_table = tab;
+    }
+  }
   q.lmode = LM_NONE;
   q.lds_words = 0;
   ab.part = true;
@@ -1112,6 +1138,25 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
         if (same) ++dup;
       }
       s.dup_frac = (double)dup / (double)ns;
+      if (!dt_float(dt) && ns >= 4096) {
+        std::vector<std::pair<uint64_t, int64_t>> runs;  // (count, value)
+        for (uint64_t k = 0; k < ns;) {
+          uint64_t e = k;
+          while (e < ns && smp[e] == smp[k]) ++e;
+          if (e - k >= ns / 2048) {
+            int64_t v;
+            // sign-extend narrow integers
+            if (es == 1) v = (int8_t)smp[k];
+            else if (es == 2) v = (int16_t)smp[k];
+            else if (es == 4) v = (int32_t)smp[k];
+            else v = (int64_t)smp[k];
+            runs.push_back({e - k, v});
+          }
+          k = e;
+        }
+        std::sort(runs.begin(), runs.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+        for (size_t k = 0; k < runs.size() && k < 64; ++k) s.heavy.push_back(runs[k].second);
+      }
     }
   }
   HIPX(hipStreamSynchronize(ctx->stream));
@@ -1197,7 +1242,7 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
     if (q.nslots < 1 || q.key.ncols < 1) fast = false;
   }
   const int fast_code = fast ? q.nslots * 4 + q.key.ncols : 0;
-  const size_t hist_lds = (size_t)ab.nparts * 4;
+  const size_t hist_lds = (size_t)(ab.nparts + 1) * 4;
   const int bpc = qmk_part_scan_blocks_per_cu(fast_code, hist_lds);
   uint64_t grid = std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * bpc);
   const uint64_t tpb = (ntiles + grid - 1) / grid;
@@ -1213,14 +1258,23 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   }
   for (int si = 0; si < MAX_STATES; ++si) pp.state_src[si] = ab.state_src[si];
   pp.has_mask = ab.has_mask ? 1 : 0;
-  pp.rec_words = 1 + pp.nsrc + pp.has_mask;
+  // 32- or 64-byte records (whole-sector writes); the mask is the last word
+  pp.rec_words = (1 + pp.nsrc + pp.has_mask <= 4) ? 4 : 8;
   pp.pbits = ab.pbits;
   pp.sbits = ab.sbits;
   pp.nparts = (int)P;
   pp.lds_entries = ab.part_lc;
   const uint64_t R = (uint64_t)pp.rec_words;
   const uint64_t blk_cap = tpb * tile_rows;
-  DevBuf recs(ctx, grid * blk_cap * R * 8), bcount(ctx, grid * 8 + 64), hist(ctx, P * grid * 8);
+  // partition P (one past the last) collects the heavy-hitter records
+  const uint64_t P1 = P + 1;
+  DevBuf recs(ctx, grid * blk_cap * R * 8), bcount(ctx, grid * 8 + 64), hist(ctx, P1 * grid * 8);
+  DevBuf hot(ctx, HOT_SLOTS * 8);
+  if (!ab.hot_table.empty()) {
+    h2d(ctx, hot.p, ab.hot_table.data(), HOT_SLOTS);
+    pp.hot_keys = hot.as<uint64_t>();
+    pp.hot_n = HOT_SLOTS;
+  }
   pp.recs = recs.as<uint64_t>();
   pp.blk_cap = blk_cap;
   pp.blk_count = bcount.as<uint64_t>();
@@ -1228,9 +1282,9 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   HIPX(hipEventRecord(ctx->evk0, ctx->stream));
   HIPX(qmk_part_scan(&sc, d_q, &pp, (uint32_t)grid, tpb, fast_code, 1, ctx->stream));
   HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-  DevBuf poff(ctx, P * grid * 8), stmp(ctx, qmk_exscan_tmp_words(P * grid) * 8 + 64);
-  uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(P * grid);
-  HIPX(qmk_exscan_u64(hist.as<uint64_t>(), P * grid, poff.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
+  DevBuf poff(ctx, P1 * grid * 8), stmp(ctx, qmk_exscan_tmp_words(P1 * grid) * 8 + 64);
+  uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(P1 * grid);
+  HIPX(qmk_exscan_u64(hist.as<uint64_t>(), P1 * grid, poff.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
                       ctx->stream));
   uint64_t total = 0;
   d2h(ctx, &total, total_d, 1);
@@ -1238,12 +1292,13 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   DevBuf precs(ctx, std::max<uint64_t>(total, 1) * R * 8);
   pp.part_off = poff.as<uint64_t>();
   pp.precs = precs.as<uint64_t>();
-  const uint32_t rpb = (uint32_t)std::max<uint64_t>(1, grid / (uint64_t)ctx->num_cus);
-  HIPX(qmk_part_scatter(d_q, &pp, (uint32_t)grid, rpb, ctx->stream));
-  DevBuf starts(ctx, (P + 1) * 8);
-  HIPX(qmk_part_starts(poff.as<uint64_t>(), (uint32_t)P, (uint32_t)grid, total, starts.as<uint64_t>(), ctx->stream));
-  std::vector<uint64_t> st(P + 1);
-  d2h(ctx, st.data(), starts.p, P + 1);
+  // few scatter workgroups keep the number of partially written lines small
+  const uint32_t rpb = (uint32_t)std::max<uint64_t>(1, (grid + 127) / 128);
+  HIPX(qmk_part_scatter(d_q, &pp, (uint32_t)grid, rpb, q.entry_words, ctx->stream));
+  DevBuf starts(ctx, (P1 + 1) * 8);
+  HIPX(qmk_part_starts(poff.as<uint64_t>(), (uint32_t)P1, (uint32_t)grid, total, starts.as<uint64_t>(), ctx->stream));
+  std::vector<uint64_t> st(P1 + 1);
+  d2h(ctx, st.data(), starts.p, P1 + 1);
   // segments: split partitions larger than seg_max across workgroups
   const uint64_t seg_max = std::max<uint64_t>(16384, total / ((uint64_t)ctx->num_cus * 2) + 1);
   std::vector<uint64_t> seg;

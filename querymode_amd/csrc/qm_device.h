@@ -147,7 +147,14 @@ struct DPart {
   uint64_t *precs;              // pass-2 output (partition order)
   const uint64_t *seg;          // pass-3 segments: {part, start, count, exclusive} x nseg
   uint64_t *inserted;           // groups inserted into the global table
+  // heavy hitters: keys frequent enough to aggregate in LDS during pass 2
+  // instead of being scattered; their records count in partition `nparts`
+  const uint64_t *hot_keys;     // HOT_SLOTS open-addressing slots (HASH: key code, DIRECT: code + 1; 0 empty)
+  int32_t hot_n;                // 0 = no heavy hitters
+  int32_t _ph;
 };
+static constexpr int HOT_SLOTS = 128;
+static constexpr int HOT_PROBES = 8;
 
 struct DScan {
   uint64_t nrows;

@@ -143,6 +143,7 @@ struct ColStats {
   uint64_t nan = 0, nulls = 0, nonnull = 0;
   double distinct = 0;       // HyperLogLog estimate
   double dup_frac = 0;       // share of a strided host sample whose value repeats in it
+  std::vector<int64_t> heavy; // most frequent sample values (integer columns), >= 1/2048 of the sample
 };
 
 struct TCol {

@@ -97,7 +97,8 @@ hipError_t qmk_join_write(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t n
 hipError_t qmk_part_scan(const qm::DScan *sc, const qm::DQuery *d_q, const qm::DPart *pp, uint32_t grid,
                          uint64_t tpb, int fast, int prefetch, hipStream_t st);
 int qmk_part_scan_blocks_per_cu(int fast, size_t lds);
-hipError_t qmk_part_scatter(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t g1, uint32_t rpb, hipStream_t st);
+hipError_t qmk_part_scatter(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t g1, uint32_t rpb, int entry_words,
+                            hipStream_t st);
 hipError_t qmk_part_starts(const uint64_t *part_off, uint32_t nparts, uint32_t grid, uint64_t total,
                            uint64_t *starts, hipStream_t st);
 hipError_t qmk_part_agg(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t nseg, uint32_t grid, int entry_words,
