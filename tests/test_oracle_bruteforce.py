@@ -1,0 +1,243 @@
+"""The CPU oracle against independent brute force on random inputs.
+
+The expected values come from numpy's IEEE comparisons, Python integers
+(exact sums), math.fsum (exactly rounded FP64 sums), dictionaries (joins,
+groups) and numpy.lexsort (top-K) — not from the oracle's own code.
+"""
+from __future__ import annotations
+
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from querymode_amd.plan import AggregateOp, BigInt, FilterOp, JoinDescriptor, QueryDescriptor
+
+N = 40_000
+INT_DTYPES = [np.int8, np.int16, np.int32, np.int64]
+ALL_DTYPES = INT_DTYPES + [np.float32, np.float64]
+
+
+def _column(rng, dt, n):
+    if dt in (np.float32, np.float64):
+        x = rng.normal(0, 50, n).astype(dt)
+        x[rng.random(n) < 0.03] = np.nan
+        x[rng.random(n) < 0.01] = 0.0
+        x[rng.random(n) < 0.005] = -0.0
+        return x
+    info = np.iinfo(dt)
+    lo, hi = max(info.min, -200), min(info.max, 200)
+    x = rng.integers(lo, hi + 1, n).astype(dt)
+    if dt == np.int64:
+        big = rng.random(n) < 0.01
+        x[big] = rng.integers(-(2 ** 62), 2 ** 62, int(big.sum()))
+    return x
+
+
+def _expect(op, x, v, isnull):
+    """numpy restatement of one predicate; nulls never match (src/decode.ts:485-491)."""
+    x = x.astype(np.float64) if x.dtype.kind == "f" else x.astype(object)
+    if op == "eq":
+        m = x == v
+    elif op == "neq":
+        m = x != v
+    elif op == "lt":
+        m = x < v
+    elif op == "lte":
+        m = x <= v
+    elif op == "gt":
+        m = x > v
+    elif op == "gte":
+        m = x >= v
+    elif op == "between":
+        m = (x >= v[0]) & (x <= v[1])
+    elif op == "not_between":
+        m = (x < v[0]) | (x > v[1])
+    elif op == "in":
+        m = np.array([any(xi == e for e in v) for xi in x])
+    elif op == "not_in":
+        m = np.array([not any(xi == e for e in v) for xi in x])
+    elif op == "is_null":
+        return isnull.copy()
+    elif op == "is_not_null":
+        return ~isnull
+    m = np.asarray(m, dtype=bool)
+    return m & ~isnull
+
+
+CASES = [("eq", 7), ("neq", 7), ("lt", -3), ("lte", -3), ("gt", 11), ("gte", 11), ("between", [-20, 35]),
+         ("between", [35, -20]), ("not_between", [-20, 35]), ("in", [1, 2, 3, -100, 150]),
+         ("not_in", [1, 2, 3]), ("is_null", None), ("is_not_null", None), ("eq", 0), ("gt", 0)]
+
+
+@pytest.mark.parametrize("dt", ALL_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("op,lit", CASES)
+def test_filter_matches_numpy(dt, op, lit):
+    rng = np.random.default_rng(abs(hash((np.dtype(dt).name, op, str(lit)))) % 2 ** 32)
+    x = _column(rng, dt, N)
+    isnull = rng.random(N) < 0.05
+    t = O.Table({"x": x}, {"x": isnull})
+    got = O.filter_rows(t, [FilterOp("x", op, lit)])
+    want = np.nonzero(_expect(op, x, lit, isnull))[0]
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("lit", [BigInt(2 ** 62), BigInt(-(2 ** 61)), BigInt(5)])
+@pytest.mark.parametrize("op", ["lt", "gte", "eq", "neq"])
+def test_int64_bigint_literal_exact(op, lit):
+    # bigint literals compare exactly against int64 (src/operators.ts:487-547)
+    rng = np.random.default_rng(3)
+    x = rng.integers(-(2 ** 62), 2 ** 62, N).astype(np.int64)
+    x[:100] = int(lit)
+    x[100:200] = int(lit) + 1
+    t = O.Table({"x": x})
+    got = O.filter_rows(t, [FilterOp("x", op, lit)])
+    want = np.nonzero(_expect(op, x, int(lit), np.zeros(N, bool)))[0]
+    assert np.array_equal(got, want)
+
+
+def test_and_or_groups_match_numpy():
+    # filters are ANDed; each group is ANDed, groups ORed, then intersected (src/operators.ts:439-547)
+    rng = np.random.default_rng(11)
+    a = rng.integers(0, 100, N).astype(np.int32)
+    b = rng.normal(0, 1, N)
+    c = rng.integers(-5, 5, N).astype(np.int64)
+    t = O.Table({"a": a, "b": b, "c": c})
+    got = O.filter_rows(t, [FilterOp("a", "gte", 10)],
+                        [[FilterOp("b", "gt", 0.5), FilterOp("c", "lt", 0)], [FilterOp("a", "in", [11, 12, 13])],
+                         [FilterOp("c", "eq", 4)]])
+    m = (a >= 10) & (((b > 0.5) & (c < 0)) | np.isin(a, [11, 12, 13]) | (c == 4))
+    assert np.array_equal(got, np.nonzero(m)[0])
+    # page boundaries do not matter
+    got2 = O.filter_rows(t, [FilterOp("a", "gte", 10)],
+                         [[FilterOp("b", "gt", 0.5), FilterOp("c", "lt", 0)], [FilterOp("a", "in", [11, 12, 13])],
+                          [FilterOp("c", "eq", 4)]], page=777)
+    assert np.array_equal(got, got2)
+
+
+def _norm_key(v):
+    if v is None:
+        return ("null",)
+    if isinstance(v, float):
+        if math.isnan(v):
+            return ("nan",)
+        if v == 0.0:
+            return 0.0  # String(-0) === "0": -0 and +0 share a group
+    return v
+
+
+@pytest.mark.parametrize("kdt", [np.int32, np.int64, np.float64])
+@pytest.mark.parametrize("vdt", [np.int32, np.int64, np.float64, np.float32, np.int8])
+def test_groupby_matches_bruteforce(kdt, vdt):
+    rng = np.random.default_rng(hash((np.dtype(kdt).name, np.dtype(vdt).name)) % 2 ** 32)
+    n = 20_000
+    if kdt == np.float64:
+        k = rng.integers(-30, 30, n).astype(np.float64) / 4
+        k[rng.random(n) < 0.02] = np.nan
+        k[rng.random(n) < 0.02] = -0.0
+    else:
+        k = rng.integers(-300, 300, n).astype(kdt)
+    if vdt == np.float64 or vdt == np.float32:
+        v = (rng.normal(0, 1, n) * 10.0 ** rng.integers(-3, 6, n)).astype(vdt)
+    elif vdt == np.int64:
+        v = rng.integers(-(2 ** 50), 2 ** 50, n).astype(np.int64)
+    else:
+        info = np.iinfo(vdt)
+        v = rng.integers(info.min, info.max, n).astype(vdt)
+    knull = rng.random(n) < 0.03
+    vnull = rng.random(n) < 0.05
+    t = O.Table({"k": k, "v": v}, {"k": knull, "v": vnull})
+    got = O.run(QueryDescriptor(table="t", groupBy=["k"], aggregates=[
+        AggregateOp("sum", "v"), AggregateOp("count", "*"), AggregateOp("count", "v"), AggregateOp("min", "v"),
+        AggregateOp("max", "v"), AggregateOp("avg", "v")]), {"t": t})
+    groups: dict = {}
+    for i in range(n):
+        key = None if knull[i] else (float(k[i]) if kdt == np.float64 else int(k[i]))
+        g = groups.setdefault(_norm_key(key), {"rows": 0, "vals": []})
+        g["rows"] += 1
+        if not vnull[i]:
+            g["vals"].append(float(v[i]) if v.dtype.kind == "f" else int(v[i]))
+    assert len(got) == len(groups)
+    for row in got:
+        g = groups[_norm_key(row["k"])]
+        vals = g["vals"]
+        assert row["count_*"] == g["rows"]
+        assert row["count_v"] == len(vals)
+        if not vals:
+            assert row["sum_v"] is None and row["min_v"] is None and row["avg_v"] is None
+            continue
+        if v.dtype.kind == "f":
+            s = math.fsum(vals)
+            assert math.isclose(row["sum_v"], s, rel_tol=1e-12, abs_tol=1e-300)
+            assert math.isclose(row["avg_v"], s / len(vals), rel_tol=1e-12, abs_tol=1e-300)
+        else:
+            assert row["sum_v"] == sum(vals)  # exact integer sum
+            exact = Fraction(sum(vals), len(vals))
+            assert abs(Fraction(row["avg_v"]) - exact) <= abs(exact) * Fraction(1, 10 ** 12) + Fraction(1, 10 ** 300)
+        assert row["min_v"] == min(vals) and row["max_v"] == max(vals)
+
+
+def test_groupby_nan_poisons_sum_not_min_max():
+    # NaN poisons sum/avg but never updates min/max (src/partial-agg.ts:38-60)
+    t = O.Table({"k": np.array([1, 1, 1, 2], dtype=np.int32), "v": np.array([1.0, np.nan, 3.0, np.inf])})
+    got = {r["k"]: r for r in O.run(QueryDescriptor(table="t", groupBy=["k"], aggregates=[
+        AggregateOp("sum", "v"), AggregateOp("min", "v"), AggregateOp("max", "v"), AggregateOp("avg", "v")]),
+        {"t": t})}
+    assert math.isnan(got[1]["sum_v"]) and math.isnan(got[1]["avg_v"])
+    assert got[1]["min_v"] == 1.0 and got[1]["max_v"] == 3.0
+    assert got[2]["sum_v"] == math.inf and got[2]["min_v"] == math.inf
+
+
+def test_join_pairs_match_dict_probe():
+    # build = right side, probe order then build insertion order (src/operators.ts:2349-2713)
+    rng = np.random.default_rng(21)
+    nl, nr = 5000, 3000
+    lk = rng.integers(0, 2500, nl).astype(np.int64)
+    rk = rng.integers(0, 2500, nr).astype(np.int64)
+    lnull = rng.random(nl) < 0.02
+    rnull = rng.random(nr) < 0.02
+    left = O.Table({"fk": lk, "lid": np.arange(nl, dtype=np.int64)}, {"fk": lnull})
+    right = O.Table({"id": rk, "rid": np.arange(nr, dtype=np.int64)}, {"id": rnull})
+    build: dict = {}
+    for j in range(nr):
+        if not rnull[j]:
+            build.setdefault(int(rk[j]), []).append(j)
+    want = [(i, j) for i in range(nl) if not lnull[i] for j in build.get(int(lk[i]), [])]
+    got = O.run(QueryDescriptor(table="l", join=JoinDescriptor(QueryDescriptor(table="r"), "fk", "id"),
+                                projections=["lid", "rid"]), {"l": left, "r": right})
+    assert [(r["lid"], r["rid"]) for r in got] == want
+
+
+@pytest.mark.parametrize("desc", [True, False])
+@pytest.mark.parametrize("k", [1, 10, 100, 1000])
+def test_topk_matches_lexsort(desc, k):
+    rng = np.random.default_rng(k + 7 * desc)
+    n = 30_000
+    v = np.round(rng.normal(0, 10, n), 1)  # many ties
+    v[rng.random(n) < 0.01] = np.nan
+    isnull = rng.random(n) < 0.01
+    t = O.Table({"v": v, "rid": np.arange(n, dtype=np.int64)}, {"v": isnull})
+    got = O.run(QueryDescriptor(table="t", sortColumn="v", sortDirection="desc" if desc else "asc", limit=k,
+                                projections=["rid"]), {"t": t})
+    last = np.isnan(v) | isnull  # NULL and NaN sort last (src/types.ts:26-38)
+    key = np.where(last, 0.0, -v if desc else v)
+    order = np.lexsort((np.arange(n), key, last))
+    assert [r["rid"] for r in got] == order[:k].tolist()
+
+
+def test_product_aggregate_exact():
+    # SUM(amount * weight): FP64 product of f64 and f32-widened-to-f64 (src/sql/compiler.ts:287-310)
+    rng = np.random.default_rng(5)
+    n = 30_000
+    a = rng.lognormal(3, 1, n)
+    w = rng.random(n).astype(np.float32)
+    g = rng.integers(0, 8, n).astype(np.int32)
+    t = O.Table({"a": a, "w": w, "g": g})
+    got = O.run(QueryDescriptor(table="t", groupBy=["g"], aggregates=[AggregateOp("sum", "multiply(a,w)", "aw")]),
+                {"t": t})
+    for row in got:
+        sel = g == row["g"]
+        want = math.fsum((a[sel] * w[sel].astype(np.float64)).tolist())
+        assert math.isclose(row["aw"], want, rel_tol=1e-12)

@@ -1,0 +1,83 @@
+"""SQL front end: lowering to QueryDescriptor (src/sql/compiler.ts:39-140,
+287-310, 365-615)."""
+from __future__ import annotations
+
+import pytest
+
+from querymode_amd.errors import QueryModeError
+from querymode_amd.plan import AggregateOp, FilterOp
+from querymode_amd.sql import sql_to_descriptor as S
+
+
+def test_c2_query():
+    d = S("SELECT key, SUM(b), COUNT(*), MIN(a), MAX(a), AVG(b) FROM t "
+          "WHERE a < 300000 AND b > 90 AND flag = 1 GROUP BY key")
+    assert d.table == "t"
+    assert d.filters == [FilterOp("a", "lt", 300000), FilterOp("b", "gt", 90), FilterOp("flag", "eq", 1)]
+    assert [a.out_name() for a in d.aggregates] == ["sum_b", "count_*", "min_a", "max_a", "avg_b"]
+    assert d.groupBy == ["key"] and d.projections == ["key", "sum_b", "count_*", "min_a", "max_a", "avg_b"]
+
+
+def test_join_product_aggregate():
+    d = S("SELECT region, SUM(amount * weight) AS aw, COUNT(*) FROM probe p JOIN build b ON p.fk = b.id "
+          "GROUP BY region")
+    assert d.join.leftKey == "fk" and d.join.rightKey == "id" and d.join.right.table == "build"
+    assert d.aggregates[0] == AggregateOp("sum", "multiply(amount,weight)", "aw")
+    # ON written right-side first swaps the keys (compiler.ts:574-586)
+    d2 = S("SELECT COUNT(*) FROM probe JOIN build ON build.id = probe.fk")
+    assert (d2.join.leftKey, d2.join.rightKey) == ("fk", "id")
+    d3 = S("SELECT COUNT(*) FROM probe JOIN build USING (id)")
+    assert (d3.join.leftKey, d3.join.rightKey) == ("id", "id")
+
+
+def test_order_limit_offset():
+    d = S("SELECT * FROM t ORDER BY v2 DESC LIMIT 10 OFFSET 3")
+    assert (d.sortColumn, d.sortDirection, d.limit, d.offset) == ("v2", "desc", 10, 3)
+    assert S("SELECT a FROM t ORDER BY a").sortDirection == "asc"
+
+
+def test_or_groups_and_precedence():
+    d = S("SELECT * FROM t WHERE a > 2 AND b < 50 OR a = 1")
+    assert d.filters == [] and d.filterGroups == [[FilterOp("a", "gt", 2), FilterOp("b", "lt", 50)],
+                                                  [FilterOp("a", "eq", 1)]]
+    d = S("SELECT * FROM t WHERE a > 2 AND (b < 50 OR a = 1)")
+    assert d.filters == [FilterOp("a", "gt", 2)]
+    assert d.filterGroups == [[FilterOp("b", "lt", 50)], [FilterOp("a", "eq", 1)]]
+
+
+def test_not_pushdown():
+    assert S("SELECT * FROM t WHERE NOT (NOT (a > 5))").filters == [FilterOp("a", "gt", 5)]
+    assert S("SELECT * FROM t WHERE NOT a > 5").filters == [FilterOp("a", "lte", 5)]
+    d = S("SELECT * FROM t WHERE NOT (a > 5 AND b < 50)")
+    assert d.filterGroups == [[FilterOp("a", "lte", 5)], [FilterOp("b", "gte", 50)]]
+    assert S("SELECT * FROM t WHERE a NOT BETWEEN 2 AND 5").filters == [FilterOp("a", "not_between", [2, 5])]
+    assert S("SELECT * FROM t WHERE NOT a IN (1, 2)").filters == [FilterOp("a", "not_in", [1, 2])]
+    assert S("SELECT * FROM t WHERE NOT a IS NULL").filters == [FilterOp("a", "is_not_null", 0)]
+
+
+def test_literals():
+    assert S("SELECT * FROM t WHERE 5 < a").filters == [FilterOp("a", "gt", 5)]
+    assert S("SELECT * FROM t WHERE a >= -2.5").filters == [FilterOp("a", "gte", -2.5)]
+    assert S("SELECT * FROM t WHERE a <> 1e3").filters == [FilterOp("a", "neq", 1000.0)]
+
+
+@pytest.mark.parametrize("sql", [
+    "SELECT * FROM t WHERE c LIKE 'f%'",
+    "SELECT * FROM t WHERE a = NULL",
+    "SELECT * FROM t WHERE a > b",
+    "SELECT * FROM t WHERE c = 'x'",
+    "SELECT * FROM l LEFT JOIN r ON l.k = r.k",
+    "SELECT * FROM t ORDER BY a, b",
+])
+def test_off_path_is_unsupported(sql):
+    with pytest.raises(QueryModeError) as e:
+        S(sql)
+    assert e.value.code == "UNSUPPORTED"
+
+
+@pytest.mark.parametrize("sql", ["SELEC * FROM t", "SELECT * FROM t WHERE", "SELECT * FROM t LIMIT x",
+                                 "SELECT * FROM t WHERE a >"])
+def test_malformed_is_invalid_format(sql):
+    with pytest.raises(QueryModeError) as e:
+        S(sql)
+    assert e.value.code == "INVALID_FORMAT"
