@@ -19,6 +19,7 @@
 namespace qm {
 
 static constexpr int PART_BLOCK = 256;
+static constexpr int STAGE_WORDS = 512;  // per-wave LDS record stage (4 KB): 128 records of 4 words
 static constexpr int UNROLL = 4;
 static constexpr int MAX_RW = 8;  // record words: 4 or 8 (32/64-byte records, copied as 16-byte vectors)
 
@@ -198,17 +199,19 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_kernel(DScan sc, const D
 // sources. All predicate columns (and, with `prefetch`, the key columns) of a
 // tile are loaded before the first compare. ----
 template <int NS, int NK>
-__global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, const DQuery *__restrict__ qs,
+__global__ void __launch_bounds__(PART_BLOCK) __attribute__((amdgpu_waves_per_eu(3))) part_scan_fast_kernel(DScan sc, const DQuery *__restrict__ qs,
                                                                     DPart pp, uint64_t tiles_per_block,
                                                                     int prefetch) {
   extern __shared__ uint32_t hist_lds[];
   __shared__ uint32_t s_fill;
   __shared__ uint64_t hot_lds[HOT_SLOTS];
+  __shared__ __align__(16) uint64_t stage[PART_BLOCK / 64][STAGE_WORDS];
   const DQuery &q = qs[0];
   for (int i = threadIdx.x; i <= pp.nparts; i += blockDim.x) hist_lds[i] = 0;
   if (threadIdx.x == 0) s_fill = 0;
   load_hot(pp, hot_lds);
   const int hot_n = pp.hot_n;
+  uint64_t *stage_w = stage[threadIdx.x >> 6];
   const FastFilter F = fast_filter_of(q, sc.cols);
   const FastKey K = fast_key_of(q, sc.cols);
   const FastSrc S = fast_src_of(pp, q, sc.cols);
@@ -251,9 +254,14 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
     uint32_t base = 0;
     if (lane == 63) base = atomicAdd(&s_fill, wave_total);
     base = __shfl(base, 63, 64);
-    if (!m) continue;
+    // The wave's records go to region[base, base + wave_total). When they fit
+    // in the wave's LDS stage they are assembled there and copied out as
+    // coalesced 16-byte stores; otherwise each lane stores its own words.
+    const bool staged = wave_total * (uint32_t)R <= (uint32_t)STAGE_WORDS;  // uniform per wave
+    uint64_t *dst = staged ? stage_w : region;
+    const uint64_t pos0 = staged ? (uint64_t)(x - cnt) : (uint64_t)base + x - cnt;
+    if (m) {
     if (!prefetch) fast_key_load<NK>(K, row0, kv, kn);
-    const uint64_t pos0 = (uint64_t)base + x - cnt;
     uint32_t side_m = 0, nullk_m = 0;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
@@ -273,7 +281,7 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
       }
       if (hot_n && sd < 0 && hot_slot(hot_lds, direct ? code + 1 : code) >= 0) p = pp.nparts;
       const uint64_t ri = pos0 + __builtin_popcount(m & ((1u << r) - 1));
-      region[ri * R] = code;
+      dst[ri * R] = code;
       atomicAdd(&hist_lds[p], 1u);
     }
     uint32_t null_src[MAX_FSRC] = {0, 0, 0, 0};
@@ -289,7 +297,7 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
           if (!((m >> r) & 1)) continue;
           double fa = dt_is_float(S.dta[j]) ? bits_f64((uint64_t)xa[r]) : (double)xa[r];
           double fb = dt_is_float(S.dtb[j]) ? bits_f64((uint64_t)yb[r]) : (double)yb[r];
-          region[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + 1 + j] = f64_bits(fa * fb);
+          dst[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + 1 + j] = f64_bits(fa * fb);
         }
         null_src[j] = nulls8(S.na[j], row0) | nulls8(S.nb[j], row0);
       } else if (S.slot[j] >= 0) {
@@ -299,7 +307,7 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
 #pragma unroll
           for (int r = 0; r < RPT; ++r) {
             if (!((m >> r) & 1)) continue;
-            region[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + 1 + j] = (uint64_t)pv[s][r];
+            dst[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + 1 + j] = (uint64_t)pv[s][r];
           }
         }
         null_src[j] = nulls8(S.na[j], row0);
@@ -309,7 +317,7 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
 #pragma unroll
         for (int r = 0; r < RPT; ++r) {
           if (!((m >> r) & 1)) continue;
-          region[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + 1 + j] = (uint64_t)xa[r];
+          dst[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + 1 + j] = (uint64_t)xa[r];
         }
         null_src[j] = nulls8(S.na[j], row0);
       }
@@ -322,8 +330,21 @@ __global__ void __launch_bounds__(PART_BLOCK) part_scan_fast_kernel(DScan sc, co
 #pragma unroll
         for (int j = 0; j < MAX_FSRC; ++j) mw |= (uint64_t)((null_src[j] >> r) & 1) << j;
         if ((side_m >> r) & 1) mw |= (1ull << 63) | ((uint64_t)((nullk_m >> r) & 1) << 62);
-        region[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + R - 1] = mw;
+        dst[(pos0 + __builtin_popcount(m & ((1u << r) - 1))) * R + R - 1] = mw;
       }
+    }
+    }  // m
+    if (staged) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint32_t n16 = wave_total * (uint32_t)R / 2;
+      const uint4 *src4 = reinterpret_cast<const uint4 *>(stage_w);
+      uint4 *dst4 = reinterpret_cast<uint4 *>(region + (uint64_t)base * R);
+      for (uint32_t u = lane; u < n16; u += 64) dst4[u] = src4[u];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
   __syncthreads();
