@@ -45,9 +45,16 @@ __device__ __forceinline__ void rec_copy(uint64_t *dst, const uint64_t *src, int
   for (int q = 0; q < MAX_RW / 2; ++q) if (2 * q < R) d4[q] = v[q];
 }
 
+// Hash partition of a key code: the high 32 bits of the mixed code scaled to
+// [0, nparts) (any partition count; the LDS tables of pass 3 index by the low
+// bits, which stay independent).
+__device__ __forceinline__ uint32_t hash_part(uint64_t code, uint32_t nparts) {
+  return (uint32_t)(((mix64(code) >> 32) * (uint64_t)nparts) >> 32);
+}
+
 __device__ __forceinline__ uint32_t part_of(const DQuery &q, const DPart &pp, uint64_t code) {
   if (q.gmode == GM_DIRECT) return (uint32_t)(code >> pp.sbits);
-  return pp.pbits ? (uint32_t)(mix64(code) >> (64 - pp.pbits)) : 0u;
+  return pp.pbits ? hash_part(code, pp.nparts) : 0u;
 }
 
 // heavy-hitter table lookup (HOT_SLOTS slots, linear probing)
@@ -277,7 +284,7 @@ __global__ void __launch_bounds__(PART_BLOCK) __attribute__((amdgpu_waves_per_eu
         p = (uint32_t)(code >> pp.sbits);
       } else {
         if (sd >= 0) { side_m |= 1u << r; if (sd) nullk_m |= 1u << r; }
-        p = (sd >= 0 || !pp.pbits) ? 0u : (uint32_t)(mix64(code) >> (64 - pp.pbits));
+        p = (sd >= 0 || !pp.pbits) ? 0u : hash_part(code, pp.nparts);
       }
       if (hot_n && sd < 0 && hot_slot(hot_lds, direct ? code + 1 : code) >= 0) p = pp.nparts;
       const uint64_t ri = pos0 + __builtin_popcount(m & ((1u << r) - 1));
@@ -429,7 +436,7 @@ __global__ void __launch_bounds__(1024) part_scatter_kernel(const DQuery *__rest
         }
         uint32_t p = side[u] ? 0u
                              : (direct ? (uint32_t)(key[u] >> sbits)
-                                       : (pbits ? (uint32_t)(mix64(key[u]) >> (64 - pbits)) : 0u));
+                                       : (pbits ? hash_part(key[u], (uint32_t)nparts) : 0u));
         dst[u] = (uint64_t)atomicAdd((unsigned long long *)&cur[p], 1ull);
       }
 #pragma unroll

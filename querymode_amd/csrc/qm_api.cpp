@@ -396,7 +396,8 @@ struct AggBinding {
 const int PART_LDS_WORDS = 128 * 1024 / 8;  // pass-3 LDS table budget (one 1024-thread workgroup per CU)
 
 // Decide the partitioned strategy and its record layout.
-void plan_partitioned(ColSpace &cs, uint64_t nrows, double est_groups, bool fused_join, AggBinding &ab) {
+void plan_partitioned(const qm_ctx *ctx, ColSpace &cs, uint64_t nrows, double est_groups, bool fused_join,
+                      AggBinding &ab) {
   DQuery &q = ab.q;
   ab.part = false;
   if (fused_join || nrows < (1u << 20) || q.key.ncols == 0) return;
@@ -405,12 +406,14 @@ void plan_partitioned(ColSpace &cs, uint64_t nrows, double est_groups, bool fuse
   while (lc * 2 * W <= PART_LDS_WORDS) lc *= 2;
   if (q.gmode == GM_HASH) {
     if (est_groups <= 32768.0) return;
-    uint64_t parts = pow2ceil((uint64_t)std::ceil(est_groups * 2.0 / lc));
+    // pass-3 tables use the whole LDS budget at the target fill; the count
+    // is a multiple of the CU count so the persistent pass 3 ends in step
+    uint64_t parts = (uint64_t)std::ceil(est_groups / (lc * ctx->tune.part_load));
+    const uint64_t cus = (uint64_t)std::max(1, ctx->num_cus);
+    if (parts > cus) parts = (parts + cus - 1) / cus * cus;
     if (parts > 8192) return;
-    ab.nparts = (int)parts;
-    int pb = 0;
-    while ((1ull << pb) < parts) ++pb;
-    ab.pbits = pb;
+    ab.nparts = (int)std::max<uint64_t>(parts, 2);
+    ab.pbits = 1;  // hash-partitioned
   } else {
     if (q.lmode != LM_NONE || q.gcap < 32768) return;
     uint64_t parts = (q.gcap + lc - 1) / lc;
@@ -448,7 +451,7 @@ void plan_partitioned(ColSpace &cs, uint64_t nrows, double est_groups, bool fuse
   ab.has_mask = nullable_src || side_keys;
   // heavy hitters of a single integer key (from the upload sample)
   ab.hot_table.clear();
-  if (q.key.ncols == 1 && !q.key.is_float[0]) {
+  if (ctx->tune.hot && q.key.ncols == 1 && !q.key.is_float[0]) {
     const ColStats &st = cs.st(q.key.col[0]);
     if (st.heavy.size() >= 2) {
       std::vector<uint64_t> tab(HOT_SLOTS, 0);
@@ -677,7 +680,7 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
   }
   if (fused_join && q.lmode == LM_HASH) q.lmode = LM_NONE;
 
-  plan_partitioned(cs, nrows, ab.est_groups, fused_join, ab);
+  plan_partitioned(ctx, cs, nrows, ab.est_groups, fused_join, ab);
 
   // ---- outputs ----
   for (int i = 0; i < nk; ++i) {
@@ -980,6 +983,9 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(tot * 0.9);
   c->num_cus = prop.multiProcessorCount;
   c->name = prop.gcnArchName;
+  if (const char *e = getenv("QM_TUNE_SCATTER_RPB")) c->tune.scatter_rpb = std::max(0, atoi(e));
+  if (const char *e = getenv("QM_TUNE_PART_LOAD")) c->tune.part_load = std::min(0.9, std::max(0.1, atof(e)));
+  if (const char *e = getenv("QM_TUNE_HOT")) c->tune.hot = atoi(e) != 0;
   HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&c->ev0));
   HIPX(hipEventCreate(&c->ev1));
@@ -1292,8 +1298,8 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   DevBuf precs(ctx, std::max<uint64_t>(total, 1) * R * 8);
   pp.part_off = poff.as<uint64_t>();
   pp.precs = precs.as<uint64_t>();
-  // few scatter workgroups keep the number of partially written lines small
-  const uint32_t rpb = (uint32_t)std::max<uint64_t>(1, (grid + 127) / 128);
+  // one pass-1 region per scatter workgroup by default (every CU busy)
+  const uint32_t rpb = ctx->tune.scatter_rpb ? (uint32_t)ctx->tune.scatter_rpb : 1u;
   HIPX(qmk_part_scatter(d_q, &pp, (uint32_t)grid, rpb, q.entry_words, ctx->stream));
   DevBuf starts(ctx, (P1 + 1) * 8);
   HIPX(qmk_part_starts(poff.as<uint64_t>(), (uint32_t)P1, (uint32_t)grid, total, starts.as<uint64_t>(), ctx->stream));

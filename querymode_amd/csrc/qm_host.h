@@ -105,6 +105,13 @@ struct qm_ctx {
   uint64_t pooled = 0;
   int num_cus = 256;
   std::string name;
+  // tuning parameters of the partitioned group-by (results do not depend on
+  // them); defaults, overridable with QM_TUNE_* environment variables
+  struct Tune {
+    int scatter_rpb = 0;       // pass-1 regions per scatter workgroup (0 = auto)
+    double part_load = 0.5;    // target fill of a pass-3 LDS table
+    int hot = 1;               // aggregate heavy-hitter keys during the scatter
+  } tune;
 
   void *alloc(size_t bytes);
   void release(void *p, size_t bytes);
