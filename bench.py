@@ -26,14 +26,33 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One plain device-to-device copy of `nbytes` (the runtime's copy kernel)
+    through the HIP runtime the library is linked against."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=100_000_000)
-    ap.add_argument("--cpu-sample-rows", type=int, default=20_000_000)
+    ap.add_argument("--cpu-sample-rows", type=int, default=100_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, run one 1 GiB device-to-device copy (the known-size dispatch the "
+                         "FETCH_SIZE / WRITE_SIZE counters are scaled by)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md for how it would shard)",
@@ -81,13 +100,16 @@ def main() -> int:
     dev_ms = float(np.mean([s.device_ms for s in stats]))
     alg_bytes = stats[-1].bytes_algorithmic
     achieved = alg_bytes / (scan_ms * 1e-3) / 1e9
+    strategy = N.STRATEGIES.get(stats[-1].strategy)
+    kname = "part_scan_fast_kernel<3, 1>" if strategy in ("part_hash", "part_direct") else "scan_agg_kernel"
     traffic = None
     tfile = ROOT / "profiles" / "traffic_filter_groupby_zipf.json"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+        rec = json.loads(tfile.read_text())
+        if rec.get("kernel") == kname and rec.get("rows") == args.rows:
+            traffic = rec.get("hbm_bytes_per_launch")
+    if args.copy_calibration:
+        copy_calibration(1 << 30)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -100,8 +122,9 @@ def main() -> int:
         O.run(d2, {"t": t})
         cpu_s = time.perf_counter() - t1
         cpu = {"value": n / cpu_s, "unit": "rows/s", "cores": 1, "kind": "port",
-               "sample": f"first {n} rows of the same table, same query, CPU oracle (per-page index-list filter + "
-                         f"hash aggregate, single thread), {cpu_s:.2f} s"}
+               "sample": f"{'all' if n == args.rows else 'first'} {n} rows of the same table, same query: C "
+                         f"restatement of the reference CPU path (per-page index-list filter + sorted "
+                         f"intersection + hash aggregate), one thread, {cpu_s:.2f} s"}
 
     out = {
         "metric": "rows/s, single-query drop-in filter->group-by (100M rows, Zipf keys); HBM % of peak",
@@ -117,11 +140,11 @@ def main() -> int:
         "dtype": "int64/f64",
         "data": "synthetic (deterministic splitmix64 generator, seed 0)",
         "config": {"workload": "filter_groupby_zipf", "rows": args.rows, "groups": int(stats[-1].groups),
-                   "selected": int(stats[-1].rows_selected), "strategy": N.STRATEGIES.get(stats[-1].strategy),
+                   "selected": int(stats[-1].rows_selected), "strategy": strategy,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "kernel": {"name": "scan_agg_kernel", "avg_ms": scan_ms, "algorithmic_bytes": alg_bytes,
+        "kernel": {"name": kname, "avg_ms": scan_ms, "algorithmic_bytes": alg_bytes,
                    "query_device_ms": dev_ms},
         "cpu_baseline": cpu,
         "setup": {"generate_s": gen_s, "upload_s": upload_s},

@@ -2,6 +2,8 @@
 
 * ``libqmgpu.so``     the gfx950 HIP kernels + the C-ABI (hipcc, in-tree)
 * ``libqmdatagen.so`` the deterministic synthetic-column generator (gcc)
+* ``qm_napi.node``    the Node-API addon over the C-ABI (gcc), when the Node
+  headers are installed
 
 Everything lands in ``querymode_amd/lib`` so the built files travel with the
 source tree.
@@ -67,9 +69,25 @@ def build_datagen() -> Path:
     return target
 
 
+NODE_INCLUDE = Path(os.environ.get("QM_NODE_INCLUDE", "/usr/include/node"))
+
+
+def build_napi() -> Path | None:
+    """Node-API addon (C, node_api.h) linked against libqmgpu.so."""
+    if not (NODE_INCLUDE / "node_api.h").exists():
+        return None
+    target = LIB / "qm_napi.node"
+    tmp = LIB / "qm_napi.node.tmp"
+    _run(["gcc", "-O2", "-Wall", "-shared", "-fPIC", "-DNODE_GYP_MODULE_NAME=qm_napi", f"-I{NODE_INCLUDE}",
+          str(PKG / "napi" / "qm_napi.c"), "-o", str(tmp), f"-L{LIB}", "-lqmgpu", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, target)
+    return target
+
+
 def build_all() -> None:
     build_datagen()
     build_gpu_lib()
+    build_napi()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,449 @@
+/*
+ * qm_napi.c — Node-API addon over the C-ABI in include/qmgpu.h.
+ *
+ * This is the binding a TypeScript host loads to reach the device path: the
+ * reference's WASM engine is reached the same way through exported functions
+ * over linear memory (src/wasm-engine.ts:21-188), and its native build wraps
+ * opaque handles for JS (wasm/src/nodejs.zig:101-742). Handles cross as
+ * napi_external values; a handle closed explicitly is not closed again by its
+ * finaliser. Failures throw an Error whose `code` is the QueryModeError code
+ * (src/errors.ts:7-32). Results cross as QMCB ArrayBuffers (src/columnar.ts).
+ *
+ * Exports: init, shutdown, lastError, deviceInfo, tableCreate, tableDrop,
+ * tableRows, planCompile, planClose, exec, batchExec, resultRows,
+ * resultQmcb, resultStats, resultClose.
+ */
+#define NAPI_VERSION 6
+#include <node_api.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/qmgpu.h"
+
+enum { H_CTX = 1, H_TABLE, H_PLAN, H_RESULT };
+
+typedef struct handle {
+  int kind;
+  void *p;  /* NULL once closed */
+} handle;
+
+static const char *code_name(int rc) {
+  switch (rc) {
+    case QM_ERR_TABLE_NOT_FOUND: return "TABLE_NOT_FOUND";
+    case QM_ERR_COLUMN_NOT_FOUND: return "COLUMN_NOT_FOUND";
+    case QM_ERR_INVALID_FORMAT: return "INVALID_FORMAT";
+    case QM_ERR_SCHEMA_MISMATCH: return "SCHEMA_MISMATCH";
+    case QM_ERR_INVALID_FILTER: return "INVALID_FILTER";
+    case QM_ERR_INVALID_AGGREGATE: return "INVALID_AGGREGATE";
+    case QM_ERR_MEMORY_EXCEEDED: return "MEMORY_EXCEEDED";
+    case QM_ERR_UNSUPPORTED: return "UNSUPPORTED";
+    case QM_ERR_INVALID_ARG: return "INVALID_ARGUMENT";
+    case QM_ERR_OVERFLOW: return "OVERFLOW";
+    default: return "QUERY_FAILED";
+  }
+}
+
+/* Throw Error(message) with .code; returns NULL for the caller to return. */
+static napi_value throw_code(napi_env env, const char *code, const char *msg) {
+  napi_value ecode, emsg, err;
+  napi_create_string_utf8(env, code, NAPI_AUTO_LENGTH, &ecode);
+  napi_create_string_utf8(env, msg, NAPI_AUTO_LENGTH, &emsg);
+  napi_create_error(env, ecode, emsg, &err);
+  napi_set_named_property(env, err, "code", ecode);
+  napi_throw(env, err);
+  return NULL;
+}
+
+static napi_value throw_rc(napi_env env, int rc) {
+  char buf[1024];
+  qm_last_error(NULL, buf, sizeof buf);
+  return throw_code(env, code_name(rc), buf[0] ? buf : "device call failed");
+}
+
+#define CHECK_NAPI(call)                                                     \
+  do {                                                                       \
+    if ((call) != napi_ok) return throw_code(env, "INVALID_ARGUMENT", #call); \
+  } while (0)
+
+static void finalize_handle(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  handle *h = (handle *)data;
+  if (h->p) {
+    switch (h->kind) {
+      case H_RESULT: qm_result_close((qm_result *)h->p); break;
+      case H_PLAN: qm_plan_close((qm_plan *)h->p); break;
+      case H_TABLE: qm_table_drop((qm_table *)h->p); break;
+      case H_CTX: break; /* a context is shut down explicitly only */
+    }
+  }
+  free(h);
+}
+
+static napi_value wrap(napi_env env, int kind, void *p) {
+  handle *h = (handle *)malloc(sizeof *h);
+  napi_value v;
+  if (!h) return throw_code(env, "MEMORY_EXCEEDED", "out of host memory");
+  h->kind = kind;
+  h->p = p;
+  if (napi_create_external(env, h, finalize_handle, NULL, &v) != napi_ok) {
+    free(h);
+    return throw_code(env, "QUERY_FAILED", "napi_create_external failed");
+  }
+  return v;
+}
+
+static handle *unwrap(napi_env env, napi_value v, int kind) {
+  napi_valuetype t;
+  void *data = NULL;
+  if (napi_typeof(env, v, &t) != napi_ok || t != napi_external) return NULL;
+  if (napi_get_value_external(env, v, &data) != napi_ok || !data) return NULL;
+  handle *h = (handle *)data;
+  return (h->kind == kind && h->p) ? h : NULL;
+}
+
+static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  if (napi_get_cb_info(env, info, &argc, argv, NULL, NULL) != napi_ok) return 0;
+  for (size_t i = argc; i < want; ++i) napi_get_undefined(env, &argv[i]);
+  return 1;
+}
+
+static int to_u64(napi_env env, napi_value v, uint64_t *out) {
+  napi_valuetype t;
+  if (napi_typeof(env, v, &t) != napi_ok) return 0;
+  if (t == napi_undefined) { *out = 0; return 1; }
+  if (t == napi_bigint) {
+    bool lossless;
+    return napi_get_value_bigint_uint64(env, v, out, &lossless) == napi_ok;
+  }
+  double d;
+  if (napi_get_value_double(env, v, &d) != napi_ok || d < 0) return 0;
+  *out = (uint64_t)d;
+  return 1;
+}
+
+/* bytes of a TypedArray / ArrayBuffer / Buffer argument */
+static int get_bytes(napi_env env, napi_value v, void **data, size_t *len) {
+  bool is;
+  if (napi_is_typedarray(env, v, &is) == napi_ok && is) {
+    napi_typedarray_type tt;
+    size_t n, off;
+    napi_value ab;
+    void *base;
+    if (napi_get_typedarray_info(env, v, &tt, &n, &base, &ab, &off) != napi_ok) return 0;
+    size_t esz = 1;
+    switch (tt) {
+      case napi_int16_array: case napi_uint16_array: esz = 2; break;
+      case napi_int32_array: case napi_uint32_array: case napi_float32_array: esz = 4; break;
+      case napi_float64_array: case napi_bigint64_array: case napi_biguint64_array: esz = 8; break;
+      default: esz = 1;
+    }
+    *data = base;
+    *len = n * esz;
+    return 1;
+  }
+  if (napi_is_arraybuffer(env, v, &is) == napi_ok && is)
+    return napi_get_arraybuffer_info(env, v, data, len) == napi_ok;
+  return 0;
+}
+
+static napi_value js_init(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return throw_code(env, "INVALID_ARGUMENT", "bad arguments");
+  int32_t dev = 0;
+  napi_valuetype t;
+  napi_typeof(env, argv[0], &t);
+  if (t != napi_undefined) CHECK_NAPI(napi_get_value_int32(env, argv[0], &dev));
+  uint64_t budget = 0;
+  if (!to_u64(env, argv[1], &budget)) return throw_code(env, "INVALID_ARGUMENT", "budget must be a number");
+  qm_ctx *ctx = NULL;
+  int rc = qm_init(dev, budget, &ctx);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  return wrap(env, H_CTX, ctx);
+}
+
+static napi_value js_shutdown(napi_env env, napi_callback_info info) {
+  napi_value argv[1], undef;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], H_CTX);
+  if (h) {
+    qm_shutdown((qm_ctx *)h->p);
+    h->p = NULL;
+  }
+  napi_get_undefined(env, &undef);
+  return undef;
+}
+
+static napi_value js_last_error(napi_env env, napi_callback_info info) {
+  (void)info;
+  char buf[4096];
+  napi_value s;
+  qm_last_error(NULL, buf, sizeof buf);
+  napi_create_string_utf8(env, buf, NAPI_AUTO_LENGTH, &s);
+  return s;
+}
+
+static napi_value js_device_info(napi_env env, napi_callback_info info) {
+  napi_value argv[1], o, v;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], H_CTX);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a context");
+  char name[128];
+  uint32_t cus = 0;
+  uint64_t budget = 0, used = 0;
+  int rc = qm_device_info((qm_ctx *)h->p, name, sizeof name, &cus, &budget, &used);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  napi_create_object(env, &o);
+  napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &v);
+  napi_set_named_property(env, o, "arch", v);
+  napi_create_uint32(env, cus, &v);
+  napi_set_named_property(env, o, "cus", v);
+  napi_create_double(env, (double)budget, &v);
+  napi_set_named_property(env, o, "budget", v);
+  napi_create_double(env, (double)used, &v);
+  napi_set_named_property(env, o, "used", v);
+  return o;
+}
+
+/* tableCreate(ctx, name, nrows, [{name, dtype, data, nulls?}, ...]) */
+static napi_value js_table_create(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return throw_code(env, "INVALID_ARGUMENT", "bad arguments");
+  handle *h = unwrap(env, argv[0], H_CTX);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a context");
+  char tname[256];
+  size_t tl;
+  CHECK_NAPI(napi_get_value_string_utf8(env, argv[1], tname, sizeof tname, &tl));
+  uint64_t nrows = 0;
+  if (!to_u64(env, argv[2], &nrows)) return throw_code(env, "INVALID_ARGUMENT", "nrows must be a number");
+  uint32_t ncols = 0;
+  CHECK_NAPI(napi_get_array_length(env, argv[3], &ncols));
+  qm_column_desc *cols = (qm_column_desc *)calloc(ncols ? ncols : 1, sizeof *cols);
+  char (*names)[256] = (char (*)[256])calloc(ncols ? ncols : 1, 256);
+  if (!cols || !names) {
+    free(cols);
+    free(names);
+    return throw_code(env, "MEMORY_EXCEEDED", "out of host memory");
+  }
+  napi_value result = NULL;
+  for (uint32_t i = 0; i < ncols; ++i) {
+    napi_value c, v;
+    size_t nl, len;
+    void *data;
+    int32_t dtype;
+    bool has;
+    if (napi_get_element(env, argv[3], i, &c) != napi_ok ||
+        napi_get_named_property(env, c, "name", &v) != napi_ok ||
+        napi_get_value_string_utf8(env, v, names[i], 256, &nl) != napi_ok ||
+        napi_get_named_property(env, c, "dtype", &v) != napi_ok ||
+        napi_get_value_int32(env, v, &dtype) != napi_ok ||
+        napi_get_named_property(env, c, "data", &v) != napi_ok || !get_bytes(env, v, &data, &len)) {
+      throw_code(env, "INVALID_ARGUMENT", "column needs {name, dtype, data: TypedArray}");
+      goto done;
+    }
+    cols[i].name = names[i];
+    cols[i].dtype = dtype;
+    cols[i].data = data;
+    if (napi_has_named_property(env, c, "nulls", &has) == napi_ok && has) {
+      napi_valuetype t;
+      napi_get_named_property(env, c, "nulls", &v);
+      napi_typeof(env, v, &t);
+      if (t != napi_undefined && t != napi_null) {
+        void *nb;
+        size_t nbl;
+        if (!get_bytes(env, v, &nb, &nbl) || nbl < (nrows + 7) / 8) {
+          throw_code(env, "INVALID_ARGUMENT", "nulls must be a Uint8Array of ceil(nrows/8) bytes");
+          goto done;
+        }
+        cols[i].null_bitmap = (const uint8_t *)nb;
+      }
+    }
+  }
+  {
+    qm_table *t = NULL;
+    int rc = qm_table_create((qm_ctx *)h->p, tname, nrows, ncols, cols, &t);
+    if (rc != QM_OK) throw_rc(env, rc);
+    else result = wrap(env, H_TABLE, t);
+  }
+done:
+  free(cols);
+  free(names);
+  return result;
+}
+
+static napi_value close_kind(napi_env env, napi_callback_info info, int kind) {
+  napi_value argv[1], undef;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], kind);
+  if (h) {
+    if (kind == H_TABLE) qm_table_drop((qm_table *)h->p);
+    else if (kind == H_PLAN) qm_plan_close((qm_plan *)h->p);
+    else if (kind == H_RESULT) qm_result_close((qm_result *)h->p);
+    h->p = NULL;
+  }
+  napi_get_undefined(env, &undef);
+  return undef;
+}
+static napi_value js_table_drop(napi_env env, napi_callback_info info) { return close_kind(env, info, H_TABLE); }
+static napi_value js_plan_close(napi_env env, napi_callback_info info) { return close_kind(env, info, H_PLAN); }
+static napi_value js_result_close(napi_env env, napi_callback_info info) { return close_kind(env, info, H_RESULT); }
+
+static napi_value js_table_rows(napi_env env, napi_callback_info info) {
+  napi_value argv[1], v;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], H_TABLE);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a table");
+  napi_create_double(env, (double)qm_table_rows((qm_table *)h->p), &v);
+  return v;
+}
+
+static napi_value js_plan_compile(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  get_args(env, info, 2, argv);
+  handle *h = unwrap(env, argv[0], H_CTX);
+  void *data;
+  size_t len;
+  if (!h || !get_bytes(env, argv[1], &data, &len))
+    return throw_code(env, "INVALID_ARGUMENT", "planCompile(ctx, Uint8Array)");
+  qm_plan *p = NULL;
+  int rc = qm_plan_compile((qm_ctx *)h->p, (const uint8_t *)data, len, &p);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  return wrap(env, H_PLAN, p);
+}
+
+static napi_value js_exec(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  handle *p = unwrap(env, argv[0], H_PLAN);
+  handle *l = unwrap(env, argv[1], H_TABLE);
+  handle *r = unwrap(env, argv[2], H_TABLE);
+  if (!p || !l) return throw_code(env, "INVALID_ARGUMENT", "exec(plan, table[, rightTable])");
+  qm_result *res = NULL;
+  int rc = qm_exec((qm_plan *)p->p, (qm_table *)l->p, r ? (qm_table *)r->p : NULL, &res);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  return wrap(env, H_RESULT, res);
+}
+
+/* batchExec(ctx, [plan...], table) -> [result...] */
+static napi_value js_batch_exec(napi_env env, napi_callback_info info) {
+  napi_value argv[3], arr;
+  get_args(env, info, 3, argv);
+  handle *c = unwrap(env, argv[0], H_CTX);
+  handle *t = unwrap(env, argv[2], H_TABLE);
+  uint32_t n = 0;
+  if (!c || !t || napi_get_array_length(env, argv[1], &n) != napi_ok)
+    return throw_code(env, "INVALID_ARGUMENT", "batchExec(ctx, plans[], table)");
+  qm_plan **plans = (qm_plan **)calloc(n ? n : 1, sizeof *plans);
+  qm_result **res = (qm_result **)calloc(n ? n : 1, sizeof *res);
+  napi_value out = NULL;
+  if (!plans || !res) {
+    throw_code(env, "MEMORY_EXCEEDED", "out of host memory");
+    goto done;
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value v;
+    handle *ph;
+    if (napi_get_element(env, argv[1], i, &v) != napi_ok || !(ph = unwrap(env, v, H_PLAN))) {
+      throw_code(env, "INVALID_ARGUMENT", "plans[] must hold plan handles");
+      goto done;
+    }
+    plans[i] = (qm_plan *)ph->p;
+  }
+  {
+    int rc = qm_batch_exec((qm_ctx *)c->p, plans, n, (qm_table *)t->p, res);
+    if (rc != QM_OK) {
+      throw_rc(env, rc);
+      goto done;
+    }
+  }
+  napi_create_array_with_length(env, n, &arr);
+  for (uint32_t i = 0; i < n; ++i) napi_set_element(env, arr, i, wrap(env, H_RESULT, res[i]));
+  out = arr;
+done:
+  free(plans);
+  free(res);
+  return out;
+}
+
+static napi_value js_result_rows(napi_env env, napi_callback_info info) {
+  napi_value argv[1], v;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], H_RESULT);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a result");
+  uint64_t n = 0;
+  int rc = qm_result_rows((qm_result *)h->p, &n);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  napi_create_double(env, (double)n, &v);
+  return v;
+}
+
+/* resultQmcb(result) -> ArrayBuffer holding the QMCB bytes */
+static napi_value js_result_qmcb(napi_env env, napi_callback_info info) {
+  napi_value argv[1], ab;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], H_RESULT);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a result");
+  uint64_t size = 0, written = 0;
+  int rc = qm_result_qmcb_size((qm_result *)h->p, &size);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  void *buf;
+  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)size, &buf, &ab));
+  rc = qm_result_qmcb((qm_result *)h->p, (uint8_t *)buf, size, &written);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  return ab;
+}
+
+static void set_num(napi_env env, napi_value o, const char *k, double x) {
+  napi_value v;
+  napi_create_double(env, x, &v);
+  napi_set_named_property(env, o, k, v);
+}
+
+static napi_value js_result_stats(napi_env env, napi_callback_info info) {
+  napi_value argv[1], o;
+  get_args(env, info, 1, argv);
+  handle *h = unwrap(env, argv[0], H_RESULT);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a result");
+  qm_stats st;
+  int rc = qm_result_stats((qm_result *)h->p, &st);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  napi_create_object(env, &o);
+  set_num(env, o, "rowsScanned", (double)st.rows_scanned);
+  set_num(env, o, "rowsSelected", (double)st.rows_selected);
+  set_num(env, o, "groups", (double)st.groups);
+  set_num(env, o, "bytesAlgorithmic", (double)st.bytes_algorithmic);
+  set_num(env, o, "deviceMs", st.device_ms);
+  set_num(env, o, "scanMs", st.scan_ms);
+  set_num(env, o, "wallMs", st.wall_ms);
+  set_num(env, o, "kernelLaunches", st.kernel_launches);
+  set_num(env, o, "retries", st.retries);
+  set_num(env, o, "strategy", st.strategy);
+  return o;
+}
+
+static napi_value init_module(napi_env env, napi_value exports) {
+  napi_property_descriptor props[] = {
+    {"init", NULL, js_init, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"shutdown", NULL, js_shutdown, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"lastError", NULL, js_last_error, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"deviceInfo", NULL, js_device_info, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"tableCreate", NULL, js_table_create, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"tableDrop", NULL, js_table_drop, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"tableRows", NULL, js_table_rows, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"planCompile", NULL, js_plan_compile, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"planClose", NULL, js_plan_close, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"exec", NULL, js_exec, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"batchExec", NULL, js_batch_exec, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"resultRows", NULL, js_result_rows, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"resultQmcb", NULL, js_result_qmcb, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"resultStats", NULL, js_result_stats, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"resultClose", NULL, js_result_close, NULL, NULL, NULL, napi_enumerable, NULL},
+  };
+  if (napi_define_properties(env, exports, sizeof props / sizeof props[0], props) != napi_ok) return NULL;
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, init_module)

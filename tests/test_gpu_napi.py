@@ -1,0 +1,66 @@
+"""End to end from JavaScript: the Node-API addon and querymode_gpu.js drive
+the device path (register -> DataFrame builder -> exec -> QMCB -> rows) on the
+reference's generateRows tables (src/operators-conformance.test.ts:101-113),
+checked against the same closed forms as the oracle tests."""
+from __future__ import annotations
+
+import json
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+JS = ROOT / "querymode_amd" / "napi" / "querymode_gpu.js"
+NODE = shutil.which("node")
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(NODE is None or not (ROOT / "querymode_amd" / "lib" / "qm_napi.node").exists(),
+                                 reason="node or the addon is not available")]
+
+SCRIPT = r"""
+const q = require(%s);
+const N = 1000000;
+const id = new Int32Array(N), region = new Int32Array(N), amount = new Int32Array(N);
+for (let i = 0; i < N; i++) { id[i] = i; region[i] = i %% 4; amount[i] = (i * 7 + 13) %% 1000; }
+const R = 500000;
+const rid = new Int32Array(R), value = new Int32Array(R);
+for (let i = 0; i < R; i++) { rid[i] = i; value[i] = i * 10; }
+const qm = q.QueryMode.gpu(0);
+qm.registerTable('rows', { id, region, amount });
+qm.registerTable('right', { id: rid, value });
+const out = {};
+out.group = qm.table('rows').groupBy('region').aggregate('sum', 'amount', 'sum_amount')
+  .aggregate('avg', 'amount', 'avg_amount').aggregate('min', 'amount', 'min_amount')
+  .aggregate('max', 'amount', 'max_amount').aggregate('count', 'amount', 'cnt').exec().rows;
+out.topk = qm.table('rows').sort('amount', 'desc').limit(100).select('id', 'amount').exec().rows;
+out.join = qm.table('rows').join(qm.table('right'), { left: 'id', right: 'id' })
+  .aggregate('count', '*').aggregate('sum', 'value').exec().rows;
+(async () => {
+  const op = qm.table('rows').filter('amount', 'eq', 999).select('id').toOperator(1000);
+  out.pulled = (await q.drainPipeline(op)).length;
+  qm.close();
+  console.log(JSON.stringify(out, (k, v) => typeof v === 'bigint' ? v.toString() : v));
+})();
+"""
+
+
+def test_js_host_end_to_end():
+    print("node: generateRows(1M) group-by / top-K / join through qm_napi.node", flush=True)
+    res = subprocess.run([NODE, "-e", SCRIPT % json.dumps(str(JS))], capture_output=True, text=True, timeout=120,
+                         cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    assert len(out["group"]) == 4
+    for x in out["group"]:
+        c = (3 * x["region"] + 1) % 4
+        per = 250 * c + 4 * (249 * 250 // 2)          # one 1000-row period, 1000 periods
+        assert int(x["sum_amount"]) == 1000 * per
+        assert x["avg_amount"] == c + 498
+        assert int(x["min_amount"]) == c and int(x["max_amount"]) == 996 + c
+        assert int(x["cnt"]) == 250_000
+    assert [int(r["amount"]) for r in out["topk"]] == [999] * 100
+    assert [int(r["id"]) for r in out["topk"]] == [998 + 1000 * k for k in range(100)]
+    assert [(int(r["count_*"]), int(r["sum_value"])) for r in out["join"]] == [(500_000, 10 * 499_999 * 500_000 // 2)]
+    assert out["pulled"] == 1000
