@@ -96,6 +96,8 @@ typedef struct qm_stats {
 #define QM_STRAT_HASH_GLB    4  /* global open addressing, no LDS cache */
 #define QM_STRAT_PART_HASH   5  /* partitioned: scatter by key hash, LDS table per partition */
 #define QM_STRAT_PART_DIRECT 6  /* partitioned: scatter by dense key slice, LDS slice per partition */
+#define QM_STRAT_SHARED_INTERVAL 7 /* batched: one range filter + small key, aggregated once per range interval */
+#define QM_STRAT_SORTED_RUNS 8  /* key column stored in order: segmented reduction over its runs */
 
 /* ---- lifecycle ----
  * Replaces the executor construction QueryMode.local() does

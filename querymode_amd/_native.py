@@ -21,7 +21,7 @@ QM_F64, QM_I64, QM_UTF8, QM_BOOL, QM_F32VEC, QM_NULL, QM_I32, QM_F32 = 0, 1, 2, 
 QM_I8, QM_I16 = 16, 17
 
 STRATEGIES = {0: "none", 1: "direct_lds", 2: "hash_lds", 3: "direct_global", 4: "hash_global", 5: "part_hash",
-              6: "part_direct"}
+              6: "part_direct", 7: "shared_interval", 8: "sorted_runs"}
 
 
 class ColumnDesc(C.Structure):

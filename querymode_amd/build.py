@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_scan.hip", "k_part.hip", "k_result.hip", "k_topk.hip", "k_join.hip", "k_stats.hip", "k_sort.hip",
-               "qm_plan.cpp", "qm_api.cpp"]
+               "k_batch.hip", "qm_plan.cpp", "qm_api.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

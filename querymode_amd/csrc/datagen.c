@@ -126,3 +126,35 @@ void qg_fill_fk_i64(uint64_t seed, uint32_t stream, uint64_t row0, uint64_t n, u
     out[i] = (int64_t)(u01(a) < match_rate ? id : nids + id);
   }
 }
+
+// Sorted sample of n uniform integers over [lo, lo + span): the order
+// statistics of n uniform draws, from normalised partial sums of exponential
+// spacings (E_i = -log u_i; U_(k) = (E_1 + ... + E_k) / (E_1 + ... + E_{n+1})).
+// Deterministic in (seed, stream, n); parallel over chunks of rows.
+void qg_sorted_uniform_i64(uint64_t seed, uint32_t stream, uint64_t n, int64_t lo, uint64_t span, int64_t *out) {
+  const uint64_t chunk = 1ull << 20;
+  const uint64_t nch = (n + 1 + chunk - 1) / chunk;
+  double *sums = (double *)malloc((nch + 1) * sizeof(double));
+  if (!sums) return;
+#pragma omp parallel for schedule(static)
+  for (uint64_t c = 0; c < nch; ++c) {
+    uint64_t a = c * chunk, b = a + chunk < n + 1 ? a + chunk : n + 1;
+    double s = 0;
+    for (uint64_t i = a; i < b; ++i) s += -log(1.0 - u01(rng(seed, stream, i)));
+    sums[c + 1] = s;
+  }
+  sums[0] = 0;
+  for (uint64_t c = 0; c < nch; ++c) sums[c + 1] += sums[c];
+  const double total = sums[nch];
+#pragma omp parallel for schedule(static)
+  for (uint64_t c = 0; c < nch; ++c) {
+    uint64_t a = c * chunk, b = a + chunk < n ? a + chunk : n;
+    double s = sums[c];
+    for (uint64_t i = a; i < b; ++i) {
+      s += -log(1.0 - u01(rng(seed, stream, i)));
+      uint64_t v = (uint64_t)(s / total * (double)span);
+      out[i] = lo + (int64_t)(v < span ? v : span - 1);
+    }
+  }
+  free(sums);
+}

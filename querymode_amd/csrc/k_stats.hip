@@ -9,13 +9,15 @@
 namespace qm {
 
 // stats words: [0] max(~code) (MIN), [1] max(code) (MAX), [2] max finite |x| bits,
-// [3] NaN count, [4] NULL count, [5] non-null count
+// [3] NaN count, [4] NULL count, [5] non-null count, [6] descents of an integer
+// column without NULLs (rows i with x[i] > x[i+1]; 0 = sorted ascending)
 __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *hll) {
   __shared__ uint32_t sh_hll[4096];
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) sh_hll[i] = 0;
   __syncthreads();
-  uint64_t mn = 0, mx = 0, mabs = 0, nan = 0, nul = 0, nn = 0;
+  uint64_t mn = 0, mx = 0, mabs = 0, nan = 0, nul = 0, nn = 0, desc = 0;
   const bool isf = dt_is_float(c.dtype);
+  const bool order_check = !isf && c.nulls == nullptr;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
@@ -39,6 +41,7 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     } else {
       int64_t v = load1_i(c, i);
       keybits = (uint64_t)v;
+      if (order_check && i + 1 < n && load1_i(c, i + 1) < v) ++desc;
       uint64_t code = code_i64(v);
       if (~code > mn) mn = ~code;
       if (code > mx) mx = code;
@@ -57,6 +60,7 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     nan += __shfl_xor(nan, off);
     nul += __shfl_xor(nul, off);
     nn += __shfl_xor(nn, off);
+    desc += __shfl_xor(desc, off);
   }
   if ((threadIdx.x & 63) == 0) {
     if (mn) g_atomic_max(&stats[0], mn);
@@ -65,6 +69,7 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     if (nan) g_atomic_add(&stats[3], nan);
     if (nul) g_atomic_add(&stats[4], nul);
     if (nn) g_atomic_add(&stats[5], nn);
+    if (desc) g_atomic_add(&stats[6], desc);
   }
   __syncthreads();
   for (int r = threadIdx.x; r < 4096; r += blockDim.x)

@@ -1124,6 +1124,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
       s.imin = decode_i64(~sw[0]);
       s.imax = decode_i64(sw[1]);
       s.max_abs = std::max(std::fabs((double)s.imin), std::fabs((double)s.imax));
+      s.sorted = s.nulls == 0 && sw[6] == 0;
     }
     s.distinct = s.nonnull ? std::max(1.0, hll_estimate(regs.data())) : 0.0;
     // skew probe on a strided host sample
@@ -1649,6 +1650,468 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   res->stats.bytes_algorithmic = referenced_bytes(cs, T->rows, pl.has_join ? R->rows : 0);
 }
 
+// ---------------------------------------------------------------------------
+// batched aggregate queries over one table: one fused pass (k_batch.hip)
+// ---------------------------------------------------------------------------
+enum : int { BK_NONE = 0, BK_SHARED = 1, BK_SORTED = 2 };
+
+// one query of a fused launch
+struct BItem {
+  uint32_t idx = 0;                 // position in the caller's plan list
+  const Plan *plan = nullptr;
+  ColSpace cs;                      // the query's own column space (slots)
+  AggBinding ab;
+  ListStore ls;
+  int kind = BK_NONE;
+  int nslots = 0;
+  bool never = false;               // a predicate can never hold
+  const TCol *slot_t[MAX_PSLOTS] = {};
+  uint64_t lo[MAX_PSLOTS] = {}, hi[MAX_PSLOTS] = {};
+  const TCol *key_t = nullptr;
+  const TCol *st_t[MAX_STATES] = {};
+  DevBuf table;                     // the query's DIRECT table (final layout)
+  uint64_t e_lo = 0, e_hi = 0;      // shared: covered intervals
+  int group = -1;                   // shared group
+};
+
+// unsigned order-code range [lo, hi] of one bound predicate; false when the
+// predicate is not a plain range (negated, NaN-matching, IN, NULL tests)
+bool pred_code_range(const DPred &p, uint64_t &lo, uint64_t &hi, bool &never) {
+  never = false;
+  switch (p.kind) {
+    case PK_FALSE: never = true; lo = 0; hi = 0; return true;
+    case PK_IS_NOT_NULL: lo = 0; hi = ~0ull; return true;
+    case PK_INT_RANGE:
+      if (p.negate) return false;
+      if (p.ilo > p.ihi) { never = true; return true; }
+      lo = code_i64(p.ilo);
+      hi = code_i64(p.ihi);
+      return true;
+    case PK_F64_RANGE: {
+      if (p.negate || p.nan_result) return false;
+      if (std::isnan(p.flo) || std::isnan(p.fhi)) { never = true; return true; }
+      uint64_t a = code_f64_bits(f64_bits(p.flo)), b = code_f64_bits(f64_bits(p.fhi));
+      if (p.lo_strict) { if (a == ~0ull) { never = true; return true; } ++a; }
+      if (p.hi_strict) { if (b == 0) { never = true; return true; } --b; }
+      if (a > b) { never = true; return true; }
+      lo = a;
+      hi = b;
+      return true;
+    }
+  }
+  return false;
+}
+
+// Bind one plan and decide whether the fused pass can run it.
+void batch_classify(qm_ctx *ctx, const qm_table *T, const Plan &pl, BItem &it) {
+  it.kind = BK_NONE;
+  if (pl.has_join || pl.aggs.empty() || pl.groupby.size() > 1 || !pl.computed.empty()) return;
+  ColSpace &cs = it.cs;
+  cs.L = T;
+  cs.plan = &pl;
+  try {
+    bind_filters(ctx, cs, pl.filters, pl.groups, it.ab.q, it.ls, false);
+    bind_aggregate(ctx, cs, pl, T->rows, false, it.ab, 0);
+  } catch (const Error &) {
+    return;  // the single-query path reports the error
+  }
+  const DQuery &q = it.ab.q;
+  if (!q.and_only || q.ngroups) return;
+  // one predicate per slot (the planner merges the ranges of a column)
+  it.nslots = q.nslots;
+  for (int s = 0; s < q.nslots; ++s) {
+    int found = 0;
+    for (int i = 0; i < q.npreds; ++i) {
+      if (q.preds[i].slot != s) continue;
+      ++found;
+      bool nv = false;
+      if (!pred_code_range(q.preds[i], it.lo[s], it.hi[s], nv)) return;
+      it.never = it.never || nv;
+    }
+    if (found != 1) return;
+    it.slot_t[s] = cs.tcols[q.slot_col[s]];
+  }
+  for (int s = 0; s < q.nstates; ++s) {
+    if (q.states[s].product) return;
+    it.st_t[s] = q.states[s].col_a >= 0 ? cs.tcols[q.states[s].col_a] : nullptr;
+  }
+  if (q.gmode != GM_DIRECT) return;
+  const int nk = q.key.ncols;
+  if (nk == 1) {
+    const TCol *kt = cs.tcols[q.key.col[0]];
+    if (kt->st.nulls || dt_float(kt->dtype)) return;
+    it.key_t = kt;
+    if (q.key.dom[0] <= 64 && it.nslots == 1 && q.nstates <= B_MAX_SHARED_STATES) it.kind = BK_SHARED;
+    else if (kt->st.sorted && q.nstates <= B_MAX_STATES && it.nslots <= B_MAX_SLOTS) it.kind = BK_SORTED;
+  } else if (nk == 0 && it.nslots == 1 && q.nstates <= B_MAX_SHARED_STATES) {
+    it.kind = BK_SHARED;
+  }
+  if (it.kind == BK_SHARED && it.slot_t[0]->st.nulls == 0 && it.lo[0] == 0 && it.hi[0] == ~0ull) {
+    // an IS NOT NULL-only filter on a column without NULLs is a full range: fine
+  }
+}
+
+// a shared-interval group: queries with the same range column and key
+struct BGroup {
+  const TCol *col = nullptr, *key = nullptr;
+  std::vector<BItem *> items;
+  std::vector<uint64_t> bounds;
+  std::vector<DState> st;              // union of states, COUNT(*) first
+  std::vector<const TCol *> st_t;
+  int entry_words = 0;
+  uint32_t dom = 1;
+  int64_t kmin = 0;
+  DevBuf table, dbounds;
+  int launch = -1;
+};
+
+// index of `s` (state of item `it`) in the group's state union
+int group_state(BGroup &g, const BItem &it, int s) {
+  const DState &a = it.ab.q.states[s];
+  const TCol *t = it.st_t[s];
+  for (size_t i = 0; i < g.st.size(); ++i) {
+    const DState &b = g.st[i];
+    if (b.kind == a.kind && b.is_min == a.is_min && b.fixed == a.fixed && b.scale == a.scale && g.st_t[i] == t)
+      return (int)i;
+  }
+  DState d = a;
+  d.word = (int16_t)g.entry_words;
+  g.entry_words += (d.kind == SK_SUM_128) ? (d.fixed ? 3 : 2) : 1;
+  g.st.push_back(d);
+  g.st_t.push_back(t);
+  return (int)g.st.size() - 1;
+}
+
+struct BLaunch {
+  std::vector<const TCol *> cols;
+  std::vector<BGroup *> groups;
+  std::vector<BItem *> sorted;
+  const TCol *sorted_key = nullptr;
+  int col_of(const TCol *t) const {
+    for (size_t i = 0; i < cols.size(); ++i) if (cols[i] == t) return (int)i;
+    return -1;
+  }
+  bool fits(const std::vector<const TCol *> &more) const {
+    size_t n = cols.size();
+    for (const TCol *t : more) if (t && col_of(t) < 0) ++n;
+    return n <= (size_t)B_MAX_COLS;
+  }
+  void add(const std::vector<const TCol *> &more) {
+    for (const TCol *t : more) if (t && col_of(t) < 0) cols.push_back(t);
+  }
+};
+
+const int B_WAVES = B_BLOCK / 64;
+const uint64_t B_LDS_WORDS = 156 * 1024 / 8;
+
+uint64_t group_lds_words(const BGroup &g) {
+  return g.bounds.size() + (g.bounds.size() + 1) * g.dom * (uint64_t)g.entry_words;
+}
+
+std::vector<const TCol *> item_cols(const BItem &it) {
+  std::vector<const TCol *> v;
+  for (int s = 0; s < it.nslots; ++s) v.push_back(it.slot_t[s]);
+  if (it.key_t) v.push_back(it.key_t);
+  for (int s = 0; s < it.ab.q.nstates; ++s) if (it.st_t[s]) v.push_back(it.st_t[s]);
+  return v;
+}
+
+// Finalise many aggregate results with one synchronisation per phase.
+void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_result *> &out, DevBuf &status,
+                    DevBuf &selected) {
+  const size_t n = items.size();
+  if (!n) return;
+  std::vector<DFinal> f(n);
+  std::vector<DevBuf> cnt(n), offs(n);
+  uint64_t max_nb = 1;
+  for (size_t i = 0; i < n; ++i) max_nb = std::max<uint64_t>(max_nb, qmk_compact_blocks(items[i]->ab.nslots));
+  DevBuf stmp(ctx, qmk_exscan_tmp_words(max_nb) * 8 + 64), totals(ctx, n * 8 + 64);
+  for (size_t i = 0; i < n; ++i) {
+    BItem &it = *items[i];
+    AggBinding &b = it.ab;
+    DFinal &x = f[i];
+    memset(&x, 0, sizeof x);
+    x.table = it.table.as<uint64_t>();
+    x.nslots = b.nslots;
+    x.entry_words = b.q.entry_words;
+    x.count_word = b.q.count_word;
+    x.gmode = b.q.gmode;
+    x.always_one_row = b.always_one_row ? 1 : 0;
+    x.key = b.q.key;
+    x.status = status.as<uint32_t>() + 2 * i;
+    uint64_t nb = qmk_compact_blocks(x.nslots);
+    cnt[i] = DevBuf(ctx, nb * 8);
+    offs[i] = DevBuf(ctx, nb * 8);
+    HIPX(qmk_compact_count(&x, cnt[i].as<uint64_t>(), ctx->stream));
+    HIPX(qmk_exscan_u64(cnt[i].as<uint64_t>(), nb, offs[i].as<uint64_t>(), totals.as<uint64_t>() + i,
+                        stmp.as<uint64_t>(), ctx->stream));
+  }
+  std::vector<uint64_t> tot(n);
+  d2h(ctx, tot.data(), totals.p, n);
+  for (size_t i = 0; i < n; ++i) {
+    BItem &it = *items[i];
+    AggBinding &b = it.ab;
+    qm_result *res = new qm_result();
+    out[it.idx] = res;
+    res->ctx = ctx;
+    res->rows = tot[i];
+    f[i].nouts = (int)b.outs.size();
+    for (size_t k = 0; k < b.outs.size(); ++k) {
+      const DOut &o = b.outs[k].o;
+      OutCol oc = make_out(ctx, b.outs[k].name, o.out_dtype, tot[i], o.kind != OK_COUNT);
+      f[i].outs[k] = o;
+      f[i].outs[k].data = oc.data.p;
+      f[i].outs[k].nulls = oc.nulls.as<uint32_t>();
+      res->cols.push_back(std::move(oc));
+    }
+    if (tot[i]) HIPX(qmk_compact_write(&f[i], offs[i].as<uint64_t>(), ctx->stream));
+  }
+  std::vector<uint32_t> st(2 * n);
+  d2h(ctx, st.data(), status.p, 2 * n);
+  std::vector<uint64_t> sel(n);
+  d2h(ctx, sel.data(), selected.p, n);
+  for (size_t i = 0; i < n; ++i) {
+    if (st[2 * i + 1]) fail(QM_ERR_OVERFLOW, "SUM over an int64 column does not fit int64");
+    qm_result *res = out[items[i]->idx];
+    res->stats.rows_selected = sel[i];
+    res->stats.strategy = items[i]->kind == BK_SHARED ? QM_STRAT_SHARED_INTERVAL : QM_STRAT_SORTED_RUNS;
+  }
+  DQuery noq;
+  memset(&noq, 0, sizeof noq);
+  DevBuf dnoq(ctx, sizeof(DQuery));
+  h2d(ctx, dnoq.p, reinterpret_cast<const uint8_t *>(&noq), sizeof noq);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t l = 0;
+    sort_limit_result(out[items[i]->idx], *items[i]->plan, dnoq.as<DQuery>(), &l);
+  }
+}
+
+// Run the fusable plans of `items` in fused passes; results by item idx.
+void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std::vector<qm_result *> &out) {
+  Timer tm(ctx);
+  // shared groups
+  std::vector<std::unique_ptr<BGroup>> groups;
+  std::vector<BItem *> sorted;
+  for (BItem *it : items) {
+    if (it->kind == BK_SORTED) { sorted.push_back(it); continue; }
+    BGroup *g = nullptr;
+    for (auto &gg : groups)
+      if (gg->col == it->slot_t[0] && gg->key == it->key_t && gg->items.size() < 4 * B_MAX_BOUNDS / 8) g = gg.get();
+    if (!g) {
+      groups.emplace_back(new BGroup());
+      g = groups.back().get();
+      g->col = it->slot_t[0];
+      g->key = it->key_t;
+      DState c0;
+      memset(&c0, 0, sizeof c0);
+      c0.kind = SK_COUNT;
+      c0.col_a = -1;
+      c0.word = 0;
+      g->st.push_back(c0);
+      g->st_t.push_back(nullptr);
+      g->entry_words = 1;
+      if (it->key_t) { g->dom = (uint32_t)it->ab.q.key.dom[0]; g->kmin = it->ab.q.key.kmin[0]; }
+    }
+    for (int s = 0; s < it->ab.q.nstates; ++s) group_state(*g, *it, s);
+    g->items.push_back(it);
+    it->group = (int)(std::find_if(groups.begin(), groups.end(), [&](auto &p) { return p.get() == g; }) - groups.begin());
+  }
+  for (auto &g : groups) {
+    std::vector<uint64_t> b;
+    for (BItem *it : g->items) {
+      if (it->never) continue;
+      b.push_back(it->lo[0]);
+      if (it->hi[0] != ~0ull) b.push_back(it->hi[0] + 1);
+    }
+    std::sort(b.begin(), b.end());
+    b.erase(std::unique(b.begin(), b.end()), b.end());
+    if (b.size() > (size_t)B_MAX_BOUNDS) fail(QM_ERR_UNSUPPORTED, "too many range bounds in one shared group");
+    g->bounds = b;
+    for (BItem *it : g->items) {
+      if (it->never) { it->e_lo = it->e_hi = 0; continue; }
+      it->e_lo = (uint64_t)(std::upper_bound(b.begin(), b.end(), it->lo[0]) - b.begin());
+      it->e_hi = it->hi[0] == ~0ull ? b.size() + 1
+                                    : (uint64_t)(std::upper_bound(b.begin(), b.end(), it->hi[0] + 1) - b.begin());
+    }
+  }
+  // launches: shared groups, then sorted-key queries (one key column per launch)
+  std::vector<BLaunch> launches;
+  auto lds_of = [&](const BLaunch &L, const BGroup *extra_g, size_t extra_sorted) {
+    uint64_t w = 0;
+    for (const BGroup *g : L.groups) w += group_lds_words(*g);
+    if (extra_g) w += group_lds_words(*extra_g);
+    w += (uint64_t)B_WAVES * (L.sorted.size() + extra_sorted) * qmk_batch_carry_words();
+    return w;
+  };
+  for (auto &g : groups) {
+    std::vector<const TCol *> need{g->col, g->key};
+    for (const TCol *t : g->st_t) need.push_back(t);
+    BLaunch *L = nullptr;
+    for (auto &l : launches)
+      if (l.groups.size() < (size_t)B_MAX_SHARED && l.fits(need) && lds_of(l, g.get(), 0) <= B_LDS_WORDS) L = &l;
+    if (!L) { launches.emplace_back(); L = &launches.back(); }
+    if (lds_of(*L, g.get(), 0) > B_LDS_WORDS || !L->fits(need)) fail(QM_ERR_UNSUPPORTED, "shared group too large");
+    L->add(need);
+    L->groups.push_back(g.get());
+  }
+  for (BItem *it : sorted) {
+    std::vector<const TCol *> need = item_cols(*it);
+    BLaunch *L = nullptr;
+    for (auto &l : launches)
+      if ((!l.sorted_key || l.sorted_key == it->key_t) && l.fits(need) && lds_of(l, nullptr, 1) <= B_LDS_WORDS) L = &l;
+    if (!L) { launches.emplace_back(); L = &launches.back(); }
+    L->add(need);
+    L->sorted_key = it->key_t;
+    L->sorted.push_back(it);
+  }
+  // tables, status, selected counters
+  const size_t n = items.size();
+  DevBuf status(ctx, n * 8 + 64), selected(ctx, n * 8 + 64);
+  zero(ctx, status.p, n * 8 + 64);
+  zero(ctx, selected.p, n * 8 + 64);
+  std::vector<BItem *> order;  // finalisation order = status / selected index
+  for (auto &L : launches) {
+    for (BGroup *g : L.groups) for (BItem *it : g->items) order.push_back(it);
+    for (BItem *it : L.sorted) order.push_back(it);
+  }
+  for (BItem *it : order) {
+    it->table = DevBuf(ctx, it->ab.table_words * 8);
+    if (it->kind == BK_SORTED) zero(ctx, it->table.p, it->ab.table_words * 8);
+  }
+  for (auto &g : groups) {
+    uint64_t words = (g->bounds.size() + 1) * g->dom * (uint64_t)g->entry_words;
+    g->table = DevBuf(ctx, words * 8);
+    zero(ctx, g->table.p, words * 8);
+    g->dbounds = DevBuf(ctx, std::max<size_t>(g->bounds.size(), 1) * 8);
+    h2d(ctx, g->dbounds.p, g->bounds.data(), g->bounds.size());
+  }
+  const uint64_t ntiles = std::max<uint64_t>(1, (T->rows + 511) / 512);
+  float scan_ms_total = 0;
+  std::vector<DevBuf> qbufs;
+  for (auto &L : launches) {
+    BScan bs;
+    memset(&bs, 0, sizeof bs);
+    bs.nrows = T->rows;
+    bs.ncols = (int)L.cols.size();
+    for (size_t j = 0; j < L.cols.size(); ++j) {
+      bs.data[j] = L.cols[j]->d;
+      bs.nulls[j] = L.cols[j]->dnull;
+      bs.dtype[j] = L.cols[j]->dtype;
+    }
+    uint32_t off = 0;
+    bs.nshared = (int)L.groups.size();
+    for (size_t gi = 0; gi < L.groups.size(); ++gi) {
+      BGroup &g = *L.groups[gi];
+      BShared &S = bs.sh[gi];
+      S.col = (uint8_t)L.col_of(g.col);
+      S.key_col = g.key ? (uint8_t)L.col_of(g.key) : 0xFF;
+      S.nstates = (uint8_t)g.st.size();
+      S.nbounds = (int)g.bounds.size();
+      S.kmin = g.kmin;
+      S.dom = g.dom;
+      S.entry_words = g.entry_words;
+      for (size_t s = 0; s < g.st.size(); ++s) {
+        S.st[s] = g.st[s];
+        S.st_col[s] = g.st_t[s] ? (uint8_t)L.col_of(g.st_t[s]) : 0xFF;
+      }
+      S.bounds = g.dbounds.as<uint64_t>();
+      S.table = g.table.as<uint64_t>();
+      S.bnd_off = off;
+      off += (uint32_t)g.bounds.size();
+      S.lds_off = off;
+      off += (uint32_t)((g.bounds.size() + 1) * g.dom * (uint64_t)g.entry_words);
+    }
+    bs.carry_off = off;
+    bs.nq = (int)L.sorted.size();
+    off += (uint32_t)(B_WAVES * L.sorted.size() * qmk_batch_carry_words());
+    std::vector<BQuery> hq(L.sorted.size());
+    for (size_t i = 0; i < L.sorted.size(); ++i) {
+      BItem &it = *L.sorted[i];
+      BQuery &Q = hq[i];
+      memset(&Q, 0, sizeof Q);
+      const DQuery &q = it.ab.q;
+      Q.nslots = it.never ? (uint8_t)0xFF : (uint8_t)it.nslots;
+      for (int s = 0; s < it.nslots; ++s) {
+        Q.slot_col[s] = (uint8_t)L.col_of(it.slot_t[s]);
+        Q.lo[s] = it.lo[s];
+        Q.span[s] = it.hi[s] - it.lo[s];
+      }
+      Q.key_col = (uint8_t)L.col_of(it.key_t);
+      Q.nstates = (uint8_t)q.nstates;
+      for (int s = 0; s < q.nstates; ++s) {
+        Q.st[s] = q.states[s];
+        Q.st_col[s] = it.st_t[s] ? (uint8_t)L.col_of(it.st_t[s]) : 0xFF;
+      }
+      Q.kmin = q.key.kmin[0];
+      Q.entry_words = q.entry_words;
+      Q.table = it.table.as<uint64_t>();
+      size_t oi = std::find(order.begin(), order.end(), &it) - order.begin();
+      Q.selected = selected.as<uint64_t>() + oi;
+    }
+    DevBuf dq(ctx, std::max<size_t>(hq.size(), 1) * sizeof(BQuery));
+    h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(hq.data()), hq.size() * sizeof(BQuery));
+    const size_t lds_bytes = (size_t)off * 8;
+    const int wpc = qmk_batch_waves_per_cu(bs.ncols, lds_bytes);
+    const uint64_t blocks_per_cu = std::max(1, wpc / B_WAVES);
+    uint64_t grid = (uint64_t)ctx->num_cus * blocks_per_cu;
+    const uint64_t waves = grid * B_WAVES;
+    bs.tiles_per_wave = (ntiles + waves - 1) / waves;
+    grid = ((ntiles + bs.tiles_per_wave - 1) / bs.tiles_per_wave + B_WAVES - 1) / B_WAVES;
+    HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+    HIPX(qmk_batch_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, lds_bytes, ctx->stream));
+    HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+    HIPX(hipEventSynchronize(ctx->evk1));
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    scan_ms_total += ms;
+    qbufs.push_back(std::move(dq));
+  }
+  // shared groups: each query merges its intervals into its DIRECT table
+  std::vector<BCombine> cb;
+  uint32_t max_dom = 1;
+  for (auto &g : groups) {
+    for (BItem *it : g->items) {
+      BCombine c;
+      memset(&c, 0, sizeof c);
+      c.src = g->table.as<uint64_t>();
+      c.dst = it->table.as<uint64_t>();
+      size_t oi = std::find(order.begin(), order.end(), it) - order.begin();
+      c.selected = selected.as<uint64_t>() + oi;
+      c.dom = g->dom;
+      c.e_lo = (uint32_t)it->e_lo;
+      c.e_hi = (uint32_t)it->e_hi;
+      c.src_words = g->entry_words;
+      c.dst_words = it->ab.q.entry_words;
+      c.nstates = it->ab.q.nstates;
+      for (int s = 0; s < it->ab.q.nstates; ++s) {
+        int gs = group_state(*g, *it, s);
+        c.src_word[s] = g->st[gs].word;
+        c.dst_word[s] = it->ab.q.states[s].word;
+        c.kind[s] = it->ab.q.states[s].kind;
+        c.fixed[s] = it->ab.q.states[s].fixed;
+      }
+      cb.push_back(c);
+      max_dom = std::max(max_dom, g->dom);
+    }
+  }
+  DevBuf dcb(ctx, std::max<size_t>(cb.size(), 1) * sizeof(BCombine));
+  if (!cb.empty()) {
+    h2d(ctx, dcb.p, reinterpret_cast<const uint8_t *>(cb.data()), cb.size() * sizeof(BCombine));
+    HIPX(qmk_batch_combine(dcb.as<BCombine>(), (uint32_t)cb.size(), max_dom, ctx->stream));
+  }
+  finalize_batch(ctx, order, out, status, selected);
+  double ms = tm.stop();
+  for (BItem *it : order) {
+    qm_result *res = out[it->idx];
+    res->stats.rows_scanned = T->rows;
+    res->stats.groups = res->rows;
+    res->stats.bytes_algorithmic = referenced_bytes(it->cs, T->rows, 0);
+    res->stats.device_ms = ms;
+    res->stats.scan_ms = scan_ms_total;
+    res->stats.kernel_launches = (uint32_t)launches.size();
+  }
+}
+
 }  // namespace
 
 extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out) {
@@ -1689,8 +2152,18 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       results[0]->stats.device_ms += bms;
       results[0]->stats.kernel_launches += l;
     } else {
+      // a key column stored in order: segmented reduction over its runs
+      BItem it;
+      it.idx = 0;
+      it.plan = &pl;
+      batch_classify(ctx, left, pl, it);
       try {
-        run_aggregates(ctx, left, nullptr, plans, results, false, nullptr);
+        if (it.kind == BK_SORTED) {
+          std::vector<BItem *> items{&it};
+          run_batch(ctx, left, items, results);
+        } else {
+          run_aggregates(ctx, left, nullptr, plans, results, false, nullptr);
+        }
       } catch (...) {
         delete results[0];
         throw;
@@ -1717,14 +2190,29 @@ extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans
   HIPX(hipSetDevice(ctx->device));
   auto t0 = std::chrono::steady_clock::now();
   std::vector<uint32_t> agg_idx, other_idx;
+  std::vector<std::unique_ptr<BItem>> fused;
   for (uint32_t i = 0; i < nplans; ++i) {
     if (!plans[i]) fail(QM_ERR_INVALID_ARG, "null plan %u", i);
     const Plan &pl = plans[i]->plan;
     bool agg = !pl.aggs.empty();
-    if (agg && !pl.has_join) agg_idx.push_back(i); else other_idx.push_back(i);
+    if (agg && !pl.has_join) {
+      std::unique_ptr<BItem> it(new BItem());
+      it->idx = i;
+      it->plan = &pl;
+      batch_classify(ctx, table, pl, *it);
+      if (it->kind != BK_NONE) fused.push_back(std::move(it));
+      else agg_idx.push_back(i);
+    } else {
+      other_idx.push_back(i);
+    }
   }
   std::vector<qm_result *> all(nplans, nullptr);
   try {
+    if (!fused.empty()) {
+      std::vector<BItem *> items;
+      for (auto &it : fused) items.push_back(it.get());
+      run_batch(ctx, table, items, all);
+    }
     // fused launches of up to MAX_QUERIES aggregate queries sharing one scan
     size_t pos = 0;
     while (pos < agg_idx.size()) {

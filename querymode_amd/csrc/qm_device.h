@@ -174,4 +174,62 @@ struct DScan {
   int32_t _p;
 };
 
+// ---- batched execution: many aggregate queries over one table in one pass.
+// Every loaded column value becomes its order-preserving unsigned code
+// (code_i64 / code_f64_bits), so each AND-ed range predicate is one unsigned
+// compare `code - lo <= span`, for integer and float columns alike.
+static constexpr int B_MAX_COLS = 4;     // distinct columns a batch launch loads
+static constexpr int B_MAX_SLOTS = 4;    // AND-ed ranges per query
+static constexpr int B_MAX_STATES = 4;   // states of a sorted-key query
+static constexpr int B_MAX_SHARED_STATES = 8;
+static constexpr int B_MAX_BOUNDS = 1024;
+static constexpr int B_MAX_SHARED = 2;   // shared-interval groups per launch
+static constexpr int B_BLOCK = 512;
+
+// A query grouped by a sorted integer key: its own predicate, then a segmented
+// reduction over the key's runs into a DIRECT table (entry layout of the
+// single-query engine; slot = key - kmin).
+struct BQuery {
+  uint8_t nslots, key_col, nstates, _p0;
+  uint8_t slot_col[B_MAX_SLOTS];
+  uint8_t st_col[B_MAX_STATES];   // batch column of each state (0xFF: COUNT(*))
+  uint32_t carry_off;             // word offset of this query's per-wave carry in LDS
+  uint64_t lo[B_MAX_SLOTS], span[B_MAX_SLOTS];
+  int64_t kmin;
+  int32_t entry_words, _p1;
+  DState st[B_MAX_STATES];
+  uint64_t *table;                // zeroed, dom x entry_words
+  uint64_t *selected;             // selected-row counter
+};
+
+// Queries whose filter is one range on column `col` and whose key is a small
+// dense domain (or absent): one shared aggregation keyed by (elementary
+// interval of `col`, key). Interval e of a code x = #{bounds <= x}; a query
+// with code range [lo, hi] covers intervals [e(lo), e(hi + 1)).
+struct BShared {
+  uint8_t col, key_col, nstates, _p0;  // key_col 0xFF: no key (dom 1)
+  int32_t nbounds;
+  int64_t kmin;
+  uint32_t dom;
+  int32_t entry_words;                 // states only; state 0 is COUNT(*)
+  uint8_t st_col[B_MAX_SHARED_STATES];
+  DState st[B_MAX_SHARED_STATES];
+  const uint64_t *bounds;              // nbounds ascending codes
+  uint64_t *table;                     // (nbounds + 1) x dom x entry_words, zeroed
+  uint32_t lds_off;                    // word offset of the workgroup table in LDS
+  uint32_t bnd_off;                    // word offset of the bounds copy in LDS
+};
+
+struct BScan {
+  uint64_t nrows;
+  int32_t ncols, nq, nshared, carry_words;   // carry_words: per-wave LDS carry area
+  const void *data[B_MAX_COLS];
+  const uint8_t *nulls[B_MAX_COLS];
+  int32_t dtype[B_MAX_COLS];
+  uint64_t tiles_per_wave;             // contiguous 512-row tiles per wave range
+  uint32_t carry_off;                  // word offset of the per-wave carries in LDS
+  uint32_t _p;
+  BShared sh[B_MAX_SHARED];
+};
+
 }  // namespace qm

@@ -151,6 +151,7 @@ struct ColStats {
   double distinct = 0;       // HyperLogLog estimate
   double dup_frac = 0;       // share of a strided host sample whose value repeats in it
   std::vector<int64_t> heavy; // most frequent sample values (integer columns), >= 1/2048 of the sample
+  bool sorted = false;        // integer column without NULLs stored in ascending order
 };
 
 struct TCol {

@@ -40,6 +40,17 @@ struct DFinal {
   uint32_t *status;       // [1] <- int128 SUM did not fit int64
 };
 
+// merge of a shared-interval table into one query's DIRECT table
+struct BCombine {
+  const uint64_t *src;     // (nbounds + 1) x dom x src_words
+  uint64_t *dst;           // dom x dst_words (the query's entry layout)
+  uint64_t *selected;      // rows selected by the query
+  uint32_t dom, e_lo, e_hi;
+  int32_t src_words, dst_words, nstates;
+  int16_t src_word[B_MAX_SHARED_STATES], dst_word[B_MAX_SHARED_STATES];
+  uint8_t kind[B_MAX_SHARED_STATES], fixed[B_MAX_SHARED_STATES];
+};
+
 // top-K candidate (composite order key, lexicographic on (a, b))
 struct TopKey { uint64_t a, b; };
 
@@ -103,6 +114,13 @@ hipError_t qmk_part_starts(const uint64_t *part_off, uint32_t nparts, uint32_t g
                            uint64_t *starts, hipStream_t st);
 hipError_t qmk_part_agg(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t nseg, uint32_t grid, int entry_words,
                         hipStream_t st);
+
+// batched aggregate queries (one fused pass)
+uint32_t qmk_batch_carry_words();
+int qmk_batch_waves_per_cu(int ncols, size_t lds_bytes);
+hipError_t qmk_batch_scan(const qm::BScan *bs, const qm::BQuery *d_q, uint32_t grid, size_t lds_bytes,
+                          hipStream_t st);
+hipError_t qmk_batch_combine(const qm::BCombine *d_cb, uint32_t n, uint32_t max_dom, hipStream_t st);
 
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,

@@ -13,7 +13,11 @@ same bytes). Stream ids are ``16 * workload + column``.
   U[0,64), weight f32 U[0,1). join_probe (200M rows): fk i64, 90% uniform over
   the id set, 10% uniform over [10M, 20M); amount f64 lognormal(3, 1).
 * batch_table (500M rows): k1 i32 U[0,4), k2 i64 U[0,1e7), v1 i32 uniform
-  over the full int32 range, v2 f64 U[0,1).
+  over the full int32 range, v2 f64 U[0,1). With ``k2_order=True`` the table
+  is stored in k2 order (its sort key): k2 holds the sorted sample of 500M
+  uniform draws over [0, 1e7) (a function of (seed, stream, n, row)), the
+  other columns are unchanged.
+* batch_queries: the 256-query mix over batch_table.
 """
 from __future__ import annotations
 
@@ -46,9 +50,10 @@ def _load():
         lib.qg_fill_zipf_i64.argtypes = [u64, u32, u64, u64, dbl, u64, C.c_int, vp]
         lib.qg_permutation_i64.argtypes = [u64, u32, u64, i64, vp]
         lib.qg_fill_fk_i64.argtypes = [u64, u32, u64, u64, u64, dbl, vp]
+        lib.qg_sorted_uniform_i64.argtypes = [u64, u32, u64, i64, u64, vp]
         for f in ("qg_fill_i64", "qg_fill_i32", "qg_fill_i8", "qg_fill_uniform_f64", "qg_fill_uniform_f32",
                   "qg_fill_normal_f64", "qg_fill_lognormal_f64", "qg_fill_bernoulli_i8", "qg_fill_zipf_i64",
-                  "qg_permutation_i64", "qg_fill_fk_i64"):
+                  "qg_permutation_i64", "qg_fill_fk_i64", "qg_sorted_uniform_i64"):
             getattr(lib, f).restype = None
         _lib = lib
     return _lib
@@ -147,10 +152,38 @@ def join_probe(n=200_000_000, nids=10_000_000, seed=0, row0=0) -> dict:
     }
 
 
-def batch_table(n=500_000_000, seed=0, row0=0) -> dict:
+def sorted_uniform_i64(seed: int, stream: int, n: int, lo: int, hi: int) -> np.ndarray:
+    """Ascending sample of n uniform integers over [lo, hi)."""
+    out = np.empty(n, dtype=np.int64)
+    _load().qg_sorted_uniform_i64(seed, stream, n, lo, hi - lo, _p(out))
+    return out
+
+
+def batch_table(n=500_000_000, seed=0, row0=0, k2_order=False) -> dict:
+    if k2_order and row0:
+        raise ValueError("a k2-ordered table is generated whole")
     return {
         "k1": uniform_int(seed, 65, row0, n, 0, 4, np.int32),
-        "k2": uniform_int(seed, 66, row0, n, 0, 10_000_000, np.int64),
+        "k2": (sorted_uniform_i64(seed, 70, n, 0, 10_000_000) if k2_order
+               else uniform_int(seed, 66, row0, n, 0, 10_000_000, np.int64)),
         "v1": uniform_int(seed, 67, row0, n, -(2 ** 31), 2 ** 32, np.int32),
         "v2": uniform_f64(seed, 68, row0, n, 0.0, 1.0),
     }
+
+
+def batch_queries(nq: int = 256, seed: int = 0, min_sel: float = 1e-4, max_sel: float = 0.5):
+    """The batched mix: query q filters v2 to [lo, lo + s) with s log-uniform in
+    [min_sel, max_sel] and lo uniform in [0, 1 - s), groups by k1 (even q) or
+    k2 (odd q), and computes SUM(v1), COUNT(*), MIN(v1), MAX(v2)."""
+    from .plan import AggregateOp, FilterOp, QueryDescriptor
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 4]))
+    out = []
+    for q in range(nq):
+        s = float(np.exp(rng.uniform(np.log(min_sel), np.log(max_sel))))
+        lo = float(rng.uniform(0.0, 1.0 - s))
+        key = "k1" if q % 2 == 0 else "k2"
+        out.append(QueryDescriptor(
+            "batch", filters=[FilterOp("v2", "gte", lo), FilterOp("v2", "lt", lo + s)], groupBy=[key],
+            aggregates=[AggregateOp("sum", "v1"), AggregateOp("count", "*"), AggregateOp("min", "v1"),
+                        AggregateOp("max", "v2")]))
+    return out

@@ -1,0 +1,98 @@
+"""Batched execution (qm_batch_exec): many aggregate queries fused into one
+pass over a resident table. Every batched result must equal the oracle's and
+the same query run alone through qm_exec, bit for bit (integer SUM / COUNT /
+MIN / MAX; float MIN / MAX are exact values)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from querymode_amd import datagen
+from querymode_amd.plan import AggregateOp, FilterOp, QueryDescriptor
+
+from ._compare import assert_groups_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _table(n, kdom, seed=0):
+    cols = datagen.batch_table(n=n, seed=seed, k2_order=True)
+    if kdom != 10_000_000:
+        cols["k2"] = datagen.sorted_uniform_i64(seed, 71, n, 0, kdom)
+    return cols
+
+
+def _check(qm, name, cols, descs, nulls=None, strategies=None):
+    qm.register_table(name, cols, nulls)
+    batch = qm.executor.execute_batch(descs)
+    ot = O.Table(cols, nulls)
+    for i, (d, got) in enumerate(zip(descs, batch)):
+        want = O.run(d, {name: ot})
+        assert_groups_equal(got.rows, want, d.groupBy, ctx=f"{name} batch q{i}")
+        if strategies is not None:
+            assert got.stats["strategy"] in strategies, (i, got.stats["strategy"])
+    return batch
+
+
+@pytest.mark.parametrize("kdom", [10_000_000, 3000, 40])
+def test_batch_matches_oracle_and_solo(qm, kdom):
+    n = 3_000_000
+    _say(f"batch: {n} rows, k2 domain {kdom}, 96 queries")
+    cols = _table(n, kdom)
+    descs = [QueryDescriptor("b", filters=d.filters, groupBy=d.groupBy, aggregates=d.aggregates)
+             for d in datagen.batch_queries(96, seed=kdom, min_sel=1e-3)]
+    batch = _check(qm, "b", cols, descs, strategies={"shared_interval", "sorted_runs"})
+    # each result is the same alone
+    for i, d in enumerate(descs[:24]):
+        solo = qm.executor.execute(d)
+        assert_groups_equal(batch[i].rows, solo.rows, d.groupBy, ctx=f"solo q{i}")
+        assert solo.stats["strategy"] in ("direct_lds", "sorted_runs", "direct_global"), solo.stats["strategy"]
+
+
+def test_batch_edge_queries(qm):
+    n = 1_000_003  # ragged tail
+    _say("batch edge cases: empty ranges, global aggregates, full ranges, NULLs")
+    cols = _table(n, 5000, seed=3)
+    rng = np.random.default_rng(5)
+    nulls = {"v1": rng.random(n) < 0.1, "v2": rng.random(n) < 0.05}
+    aggs = [AggregateOp("sum", "v1"), AggregateOp("count", "*"), AggregateOp("count", "v1"), AggregateOp("min", "v1"),
+            AggregateOp("max", "v2"), AggregateOp("avg", "v1"), AggregateOp("min", "v2")]
+    descs = [
+        QueryDescriptor("e", filters=[FilterOp("v2", "gte", 0.7), FilterOp("v2", "lt", 0.2)], groupBy=["k2"],
+                        aggregates=aggs[:4]),                                    # never true, sorted key
+        QueryDescriptor("e", filters=[FilterOp("v2", "gt", 2.0)], groupBy=["k1"], aggregates=aggs[:5]),  # empty
+        QueryDescriptor("e", filters=[FilterOp("v2", "between", [0.25, 0.5])], aggregates=aggs[:5]),      # global
+        QueryDescriptor("e", filters=[FilterOp("v2", "gt", 1.5)], aggregates=aggs[:5]),                    # global, empty
+        QueryDescriptor("e", filters=[FilterOp("v2", "gte", 0.0)], groupBy=["k1"], aggregates=aggs),       # full range
+        QueryDescriptor("e", filters=[FilterOp("v2", "lt", 0.9), FilterOp("v1", "gt", 0)], groupBy=["k2"],
+                        aggregates=aggs[:4] + [AggregateOp("min", "v2")]),      # two ranges, sorted key
+        QueryDescriptor("e", filters=[FilterOp("k1", "eq", 2)], groupBy=["k2"], aggregates=aggs[:4]),
+        QueryDescriptor("e", filters=[FilterOp("v2", "is_not_null")], groupBy=["k1"], aggregates=aggs[:5]),
+    ]
+    _check(qm, "e", cols, descs, nulls)
+
+
+def test_batch_falls_back_for_other_shapes(qm):
+    n = 500_000
+    cols = _table(n, 100_000, seed=9)
+    descs = [
+        QueryDescriptor("f", filters=[FilterOp("v2", "lt", 0.5)], groupBy=["k1"],
+                        aggregates=[AggregateOp("sum", "v2")]),                                 # fused (shared)
+        QueryDescriptor("f", filterGroups=[[FilterOp("v2", "lt", 0.1)], [FilterOp("v2", "gt", 0.9)]],
+                        groupBy=["k1"], aggregates=[AggregateOp("count", "*")]),                # OR groups
+        QueryDescriptor("f", filters=[FilterOp("k1", "in", [1, 3])], groupBy=["k2"],
+                        aggregates=[AggregateOp("sum", "v1")]),                                 # IN list
+        QueryDescriptor("f", filters=[FilterOp("v2", "lt", 0.3)], groupBy=["v1"],
+                        aggregates=[AggregateOp("count", "*")]),                                # unsorted hash key
+        QueryDescriptor("f", filters=[FilterOp("v2", "lt", 0.3)], groupBy=["k2"],
+                        aggregates=[AggregateOp("sum", "v2"), AggregateOp("avg", "v2")]),        # f64 sums, sorted
+    ]
+    _check(qm, "f", cols, descs)
