@@ -986,6 +986,7 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   if (const char *e = getenv("QM_TUNE_SCATTER_RPB")) c->tune.scatter_rpb = std::max(0, atoi(e));
   if (const char *e = getenv("QM_TUNE_PART_LOAD")) c->tune.part_load = std::min(0.9, std::max(0.1, atof(e)));
   if (const char *e = getenv("QM_TUNE_HOT")) c->tune.hot = atoi(e) != 0;
+  if (const char *e = getenv("QM_TUNE_QP_MIN")) c->tune.qp_min = std::max(1, atoi(e));
   HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&c->ev0));
   HIPX(hipEventCreate(&c->ev1));
@@ -1792,14 +1793,28 @@ struct BLaunch {
     return -1;
   }
   bool fits(const std::vector<const TCol *> &more) const {
-    size_t n = cols.size();
-    for (const TCol *t : more) if (t && col_of(t) < 0) ++n;
-    return n <= (size_t)B_MAX_COLS;
+    std::vector<const TCol *> u = cols;
+    for (const TCol *t : more)
+      if (t && std::find(u.begin(), u.end(), t) == u.end()) u.push_back(t);
+    return u.size() <= (size_t)B_MAX_COLS;
   }
   void add(const std::vector<const TCol *> &more) {
     for (const TCol *t : more) if (t && col_of(t) < 0) cols.push_back(t);
   }
 };
+
+// the aggregate list of a sorted-key query (query-parallel launches share one)
+bool same_signature(const BItem &a, const BItem &b) {
+  const DQuery &x = a.ab.q, &y = b.ab.q;
+  if (x.nstates != y.nstates || x.entry_words != y.entry_words) return false;
+  for (int s = 0; s < x.nstates; ++s) {
+    const DState &p = x.states[s], &q = y.states[s];
+    if (p.kind != q.kind || p.is_min != q.is_min || p.fixed != q.fixed || p.scale != q.scale || p.word != q.word ||
+        a.st_t[s] != b.st_t[s])
+      return false;
+  }
+  return true;
+}
 
 const int B_WAVES = B_BLOCK / 64;
 const uint64_t B_LDS_WORDS = 156 * 1024 / 8;
@@ -1941,6 +1956,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     for (const BGroup *g : L.groups) w += group_lds_words(*g);
     if (extra_g) w += group_lds_words(*extra_g);
     w += (uint64_t)B_WAVES * (L.sorted.size() + extra_sorted) * qmk_batch_carry_words();
+    w += (L.sorted.size() + extra_sorted) * sizeof(BQuery) / 8;
     return w;
   };
   for (auto &g : groups) {
@@ -1958,8 +1974,15 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     std::vector<const TCol *> need = item_cols(*it);
     BLaunch *L = nullptr;
     for (auto &l : launches)
-      if ((!l.sorted_key || l.sorted_key == it->key_t) && l.fits(need) && lds_of(l, nullptr, 1) <= B_LDS_WORDS) L = &l;
-    if (!L) { launches.emplace_back(); L = &launches.back(); }
+      if (l.groups.empty() && (!l.sorted_key || l.sorted_key == it->key_t) && l.fits(need) &&
+          l.sorted.size() < 256 && same_signature(*l.sorted[0], *it) && it->nslots <= B_QP_SLOTS &&
+          l.sorted[0]->nslots <= B_QP_SLOTS)
+        L = &l;
+    if (!L) {
+      launches.emplace_back();
+      L = &launches.back();
+      L->cols.push_back(it->key_t);  // the sorted key is batch column 0
+    }
     L->add(need);
     L->sorted_key = it->key_t;
     L->sorted.push_back(it);
@@ -2024,6 +2047,8 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     bs.carry_off = off;
     bs.nq = (int)L.sorted.size();
     off += (uint32_t)(B_WAVES * L.sorted.size() * qmk_batch_carry_words());
+    bs.qdesc_off = off;
+    off += (uint32_t)(L.sorted.size() * sizeof(BQuery) / 8);
     std::vector<BQuery> hq(L.sorted.size());
     for (size_t i = 0; i < L.sorted.size(); ++i) {
       BItem &it = *L.sorted[i];
@@ -2051,14 +2076,68 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     DevBuf dq(ctx, std::max<size_t>(hq.size(), 1) * sizeof(BQuery));
     h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(hq.data()), hq.size() * sizeof(BQuery));
     const size_t lds_bytes = (size_t)off * 8;
-    const int wpc = qmk_batch_waves_per_cu(bs.ncols, lds_bytes);
+    int wide = 0;
+    for (const BQuery &Q : hq)
+      for (int s2 = 0; s2 < Q.nstates; ++s2) if (Q.st[s2].kind == SK_SUM_128) wide = 1;
+    // lean query-parallel launch: many queries sharing one aggregate list
+    // (COUNT / integer SUM / MIN / MAX), each filtered by one range on one
+    // common column
+    bool qp = !L.sorted.empty() && (int)L.sorted.size() >= ctx->tune.qp_min && !wide;
+    int pcol = -1;
+    for (BItem *it : L.sorted) {
+      if (!same_signature(*L.sorted[0], *it) || it->nslots > 1) qp = false;
+      if (!it->never && it->nslots == 1) {
+        int c = L.col_of(it->slot_t[0]);
+        if (pcol >= 0 && c != pcol) qp = false;
+        pcol = c;
+      }
+    }
+    std::vector<int> order_st;  // additive states first, then order codes
+    if (qp) {
+      const DQuery &q0 = L.sorted[0]->ab.q;
+      if (q0.nstates < 1 || q0.states[0].kind != SK_COUNT || q0.states[0].col_a >= 0) qp = false;
+      for (int s2 = 0; s2 < q0.nstates && qp; ++s2) {
+        const uint8_t k = q0.states[s2].kind;
+        if (k == SK_COUNT || k == SK_SUM_I64) order_st.push_back(s2);
+        else if (k != SK_MAX_CODE) qp = false;
+      }
+      const int nadd = (int)order_st.size();
+      for (int s2 = 0; s2 < q0.nstates; ++s2) if (q0.states[s2].kind == SK_MAX_CODE) order_st.push_back(s2);
+      const int nmax = (int)order_st.size() - nadd;
+      if (qp && qmk_batch_qpf_supported(nadd, nmax)) {
+        bs.sig_nstates = q0.nstates;
+        bs.sig_entry_words = q0.entry_words;
+        for (size_t i = 0; i < order_st.size(); ++i) {
+          const int s2 = order_st[i];
+          bs.sig_st[i] = q0.states[s2];
+          bs.sig_col[i] = L.sorted[0]->st_t[s2] ? (uint8_t)L.col_of(L.sorted[0]->st_t[s2]) : 0xFF;
+        }
+        bs.qp_pred_col = pcol < 0 ? 0 : pcol;
+        const int qpl = L.sorted.size() <= 64 ? 1 : (L.sorted.size() <= 128 ? 2 : 4);
+        const int wpc = qmk_batch_qpf_waves_per_cu(nadd, nmax, qpl);
+        uint64_t grid = (uint64_t)ctx->num_cus * std::max(1, wpc / B_WAVES);
+        const uint64_t waves = grid * B_WAVES;
+        bs.tiles_per_wave = (ntiles + waves - 1) / waves;
+        grid = ((ntiles + bs.tiles_per_wave - 1) / bs.tiles_per_wave + B_WAVES - 1) / B_WAVES;
+        HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+        HIPX(qmk_batch_qpf_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, nadd, nmax, qpl, ctx->stream));
+        HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+        HIPX(hipEventSynchronize(ctx->evk1));
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+        scan_ms_total += ms;
+        qbufs.push_back(std::move(dq));
+        continue;
+      }
+    }
+    const int wpc = qmk_batch_waves_per_cu(bs.ncols, wide, bs.nshared > 0 ? 1 : 2, lds_bytes);
     const uint64_t blocks_per_cu = std::max(1, wpc / B_WAVES);
     uint64_t grid = (uint64_t)ctx->num_cus * blocks_per_cu;
     const uint64_t waves = grid * B_WAVES;
     bs.tiles_per_wave = (ntiles + waves - 1) / waves;
     grid = ((ntiles + bs.tiles_per_wave - 1) / bs.tiles_per_wave + B_WAVES - 1) / B_WAVES;
     HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-    HIPX(qmk_batch_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, lds_bytes, ctx->stream));
+    HIPX(qmk_batch_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, lds_bytes, wide, ctx->stream));
     HIPX(hipEventRecord(ctx->evk1, ctx->stream));
     HIPX(hipEventSynchronize(ctx->evk1));
     float ms = 0;

@@ -220,6 +220,8 @@ struct BShared {
   uint32_t bnd_off;                    // word offset of the bounds copy in LDS
 };
 
+static_assert(sizeof(BQuery) % 8 == 0, "BQuery is copied to LDS in 8-byte words");
+
 struct BScan {
   uint64_t nrows;
   int32_t ncols, nq, nshared, carry_words;   // carry_words: per-wave LDS carry area
@@ -228,8 +230,15 @@ struct BScan {
   int32_t dtype[B_MAX_COLS];
   uint64_t tiles_per_wave;             // contiguous 512-row tiles per wave range
   uint32_t carry_off;                  // word offset of the per-wave carries in LDS
-  uint32_t _p;
+  uint32_t qdesc_off;                  // word offset of the LDS copy of the query descriptors
   BShared sh[B_MAX_SHARED];
+  // query-parallel sorted-key launches: the aggregate list every query of
+  // the launch shares (states, their batch columns, entry layout)
+  int32_t sig_nstates, sig_entry_words;
+  DState sig_st[B_MAX_STATES];
+  uint8_t sig_col[B_MAX_STATES];
+  int32_t qp_pred_col;                 // uniform predicate column (single-range queries)
 };
+static constexpr int B_QP_SLOTS = 2;   // AND-ed ranges per query on the query-parallel path
 
 }  // namespace qm

@@ -111,6 +111,7 @@ struct qm_ctx {
     int scatter_rpb = 0;       // pass-1 regions per scatter workgroup (0 = auto)
     double part_load = 0.5;    // target fill of a pass-3 LDS table
     int hot = 1;               // aggregate heavy-hitter keys during the scatter
+    int qp_min = 2;            // sorted-key launches with at least this many queries run query-parallel
   } tune;
 
   void *alloc(size_t bytes);

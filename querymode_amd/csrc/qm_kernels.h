@@ -117,11 +117,14 @@ hipError_t qmk_part_agg(const qm::DQuery *d_q, const qm::DPart *pp, uint32_t nse
 
 // batched aggregate queries (one fused pass)
 uint32_t qmk_batch_carry_words();
-int qmk_batch_waves_per_cu(int ncols, size_t lds_bytes);
-hipError_t qmk_batch_scan(const qm::BScan *bs, const qm::BQuery *d_q, uint32_t grid, size_t lds_bytes,
+int qmk_batch_waves_per_cu(int ncols, int wide, int mode, size_t lds_bytes);
+hipError_t qmk_batch_scan(const qm::BScan *bs, const qm::BQuery *d_q, uint32_t grid, size_t lds_bytes, int wide,
                           hipStream_t st);
 hipError_t qmk_batch_combine(const qm::BCombine *d_cb, uint32_t n, uint32_t max_dom, hipStream_t st);
-
+int qmk_batch_qpf_supported(int nadd, int nmax);
+int qmk_batch_qpf_waves_per_cu(int nadd, int nmax, int qpl);
+hipError_t qmk_batch_qpf_scan(const qm::BScan *bs, const qm::BQuery *d_q, uint32_t grid, int nadd, int nmax,
+                              int qpl, hipStream_t st);
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);

@@ -31,7 +31,8 @@ def test_header_declares_the_entry_points():
 
 
 def test_library_exports_every_declared_symbol():
-    lib = C.CDLL(str(N.LIB_PATH))
+    import os
+    lib = C.CDLL(str(N.LIB_PATH), mode=os.RTLD_NOW)  # every internal symbol must resolve
     for name in _declared():
         assert hasattr(lib, name), f"{name} missing from {N.LIB_PATH.name}"
 
