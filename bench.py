@@ -191,6 +191,7 @@ def bench_batch(args) -> dict:
     for s in pq:
         k = N.STRATEGIES.get(s.strategy)
         strategies[k] = strategies.get(k, 0) + 1
+    served = serve_mix(args, dev, cols, rows, descs) if args.serve_seconds > 0 else None
     if args.copy_calibration:
         copy_calibration(1 << 30)
     cpu = None
@@ -220,6 +221,7 @@ def bench_batch(args) -> dict:
         "batched_vs_per_query": batched_rows_s / per_query_rows_s,
         "per_query_ms_total": pq_wall * 1e3,
         "per_query_strategies": strategies,
+        "served": served,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": _traffic("batch_kernel<4>", rows, "traffic_batch_mix.json")},
@@ -234,15 +236,214 @@ def bench_batch(args) -> dict:
     return out
 
 
+def serve_mix(args, dev, cols, rows, descs) -> dict:
+    """Poisson arrivals at a fixed QPS through the continuous-batching server:
+    40% group-by queries of the batched mix, 30% the 200M x 10M join +
+    aggregate, 30% sort(v2 desc).limit(k), k in {10, 100, 1000}."""
+    import random
+    import threading
+
+    from querymode_amd import datagen
+    from querymode_amd.client import QueryMode
+    from querymode_amd.plan import AggregateOp, JoinDescriptor, QueryDescriptor
+    from querymode_amd.serve import Server, percentile
+
+    qm = QueryMode(dev)
+    nbuild, nprobe = 10_000_000, 200_000_000
+    qm.register_table("build", datagen.join_build(n=nbuild, seed=0))
+    qm.register_table("probe", datagen.join_probe(n=nprobe, nids=nbuild, seed=0))
+    join = QueryDescriptor("probe", join=JoinDescriptor(QueryDescriptor("build"), "fk", "id"), groupBy=["region"],
+                           computedColumns=[("aw", "amount", "weight")],
+                           aggregates=[AggregateOp("sum", "aw"), AggregateOp("count", "*")])
+    rng = random.Random(11)
+    mix, nrows = [], []
+    t = 0.0
+    while True:
+        t += rng.expovariate(args.qps)
+        if t > args.serve_seconds:
+            break
+        u = rng.random()
+        if u < 0.4:
+            mix.append((t, descs[rng.randrange(len(descs))]))
+            nrows.append(rows)
+        elif u < 0.7:
+            mix.append((t, join))
+            nrows.append(nprobe + nbuild)
+        else:
+            mix.append((t, QueryDescriptor("batch", sortColumn="v2", sortDirection="desc",
+                                           limit=rng.choice([10, 100, 1000]))))
+            nrows.append(rows)
+    # warm the join index, top-K and group tables once
+    for d in (join, mix[0][1], QueryDescriptor("batch", sortColumn="v2", sortDirection="desc", limit=10)):
+        qm.executor.execute(d)
+    srv = Server(qm)
+    reqs = [None] * len(mix)
+    t0 = time.perf_counter()
+
+    def producer():
+        for i, (at, d) in enumerate(mix):
+            delay = t0 + at - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            reqs[i] = srv.submit(d)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    for r in reqs:
+        r.wait(600)
+    elapsed = time.perf_counter() - t0
+    srv.close()
+    lat = [r.latency_ms for r in reqs]
+    kinds = {"group-by": [], "join": [], "top-k": []}
+    for r in reqs:
+        k = "join" if r.desc.join is not None else ("group-by" if r.desc.aggregates else "top-k")
+        kinds[k].append(r.latency_ms)
+    return {"qps": args.qps, "seconds": args.serve_seconds, "requests": len(reqs),
+            "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+            "rows_per_s": sum(nrows) / elapsed, "achieved_qps": len(reqs) / elapsed,
+            "cycles": srv.cycles, "mean_requests_per_cycle": len(reqs) / max(1, srv.cycles),
+            "p50_ms_by_kind": {k: (percentile(v, 50) if v else None) for k, v in kinds.items()},
+            "p99_ms_by_kind": {k: (percentile(v, 99) if v else None) for k, v in kinds.items()},
+            "mix": "40% group-by (batched mix), 30% join 200M x 10M + aggregate, 30% top-K v2 desc k in {10,100,1000}"}
+
+
+def bench_join(args) -> dict:
+    import numpy as np
+    from querymode_amd import _native as N
+    from querymode_amd import datagen
+    from querymode_amd.client import Device, _Plan
+    from querymode_amd.plan import AggregateOp, JoinDescriptor, QueryDescriptor
+
+    nprobe = args.rows or 200_000_000
+    nbuild = 10_000_000
+    t0 = time.time()
+    build = datagen.join_build(n=nbuild, seed=0)
+    probe = datagen.join_probe(n=nprobe, nids=nbuild, seed=0)
+    gen_s = time.time() - t0
+    desc = QueryDescriptor("probe", join=JoinDescriptor(QueryDescriptor("build"), "fk", "id"), groupBy=["region"],
+                           computedColumns=[("aw", "amount", "weight")],
+                           aggregates=[AggregateOp("sum", "aw"), AggregateOp("count", "*")])
+    dev = Device(0)
+    t0 = time.time()
+    tb = dev.register("build", build)
+    tp = dev.register("probe", probe)
+    upload_s = time.time() - t0
+    plan = _Plan(dev, desc)
+
+    def step():
+        rh = C.c_void_p()
+        N.check(dev.lib.qm_exec(plan.handle, tp.handle, tb.handle, C.byref(rh)))
+        st = N.Stats()
+        N.check(dev.lib.qm_result_stats(rh, C.byref(st)))
+        dev.lib.qm_result_close(rh)
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+    wall = time.perf_counter() - t0
+    rows = nprobe + nbuild
+    scan_ms = float(np.mean([s.scan_ms for s in stats]))
+    alg = nprobe * 16 + nbuild * 16  # fk + amount; id + region + weight (+ pad) read once
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        n = min(args.cpu_sample_rows or 10_000_000, nprobe)
+        t1 = time.perf_counter()
+        O.run(desc, {"build": O.Table(build), "probe": O.Table({k: v[:n] for k, v in probe.items()})})
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": (n + nbuild) / cpu_s, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"the whole 10M-row build side and the first {n} probe rows, same query: C restatement "
+                         f"of the reference hash join (build right, probe left) + aggregate, one thread, {cpu_s:.2f} s"}
+    out = {
+        "metric": METRIC, "value": rows / (wall / args.steps), "unit": "rows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64/f64",
+        "data": "synthetic (deterministic splitmix64 generator, seed 0)",
+        "config": {"workload": "hash join 200M probe x 10M build -> GROUP BY region SUM(amount*weight), COUNT(*)",
+                   "probe_rows": nprobe, "build_rows": nbuild, "selected": int(stats[-1].rows_selected),
+                   "strategy": N.STRATEGIES.get(stats[-1].strategy), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": alg / (scan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (scan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "kernel": {"name": "scan_agg_kernel (fused join probe)", "avg_ms": scan_ms, "algorithmic_bytes": alg,
+                   "query_device_ms": float(np.mean([s.device_ms for s in stats]))},
+        "cpu_baseline": cpu,
+        "setup": {"generate_s": gen_s, "upload_s": upload_s},
+    }
+    plan.close()
+    dev.close()
+    return out
+
+
+def bench_topk(args) -> dict:
+    import numpy as np
+    from querymode_amd import _native as N
+    from querymode_amd import datagen
+    from querymode_amd.client import Device, _Plan
+    from querymode_amd.plan import QueryDescriptor
+
+    rows = args.rows or 500_000_000
+    t0 = time.time()
+    cols = datagen.batch_table(n=rows, seed=0, k2_order=True)
+    gen_s = time.time() - t0
+    dev = Device(0)
+    table = dev.register("batch", cols)
+    per_k = {}
+    for k in (10, 100, 1000):
+        plan = _Plan(dev, QueryDescriptor("batch", sortColumn="v2", sortDirection="desc", limit=k))
+        for _ in range(args.warmup):
+            _exec_stats(dev.lib, N, plan, table)
+        t0 = time.perf_counter()
+        st = [_exec_stats(dev.lib, N, plan, table) for _ in range(args.steps)]
+        wall = time.perf_counter() - t0
+        per_k[k] = {"ms": wall / args.steps * 1e3, "scan_ms": float(np.mean([s.scan_ms for s in st])),
+                    "device_ms": float(np.mean([s.device_ms for s in st]))}
+        plan.close()
+    alg = rows * 8
+    scan_ms = per_k[100]["scan_ms"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        n = min(args.cpu_sample_rows or 100_000_000, rows)
+        t = O.Table({"v2": cols["v2"][:n]})
+        t1 = time.perf_counter()
+        O.topk_heap_rows(t, "v2", True, 100)
+        cpu_s = time.perf_counter() - t1
+        cpu = {"value": n / cpu_s, "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {n} rows of v2, k = 100: C restatement of the reference binary-heap TopK, one "
+                         f"thread, {cpu_s:.2f} s"}
+    out = {
+        "metric": METRIC, "value": rows / (per_k[100]["ms"] * 1e-3), "unit": "rows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": per_k[100]["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (deterministic splitmix64 generator, seed 0)",
+        "config": {"workload": "top-K: sort(v2 desc).limit(k) over 500M rows (value at k = 100)", "rows": rows,
+                   "per_k": per_k, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": alg / (scan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (scan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "kernel": {"name": "topk_scan_kernel", "avg_ms": scan_ms, "algorithmic_bytes": alg},
+        "cpu_baseline": cpu,
+        "setup": {"generate_s": gen_s},
+    }
+    dev.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["batch", "zipf"], default="batch")
+    ap.add_argument("--workload", choices=["batch", "zipf", "join", "topk"], default="batch")
     ap.add_argument("--rows", type=int, default=0, help="table rows (default: the workload's)")
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
+    ap.add_argument("--qps", type=float, default=40.0, help="served phase: Poisson arrival rate")
+    ap.add_argument("--serve-seconds", type=float, default=6.0, help="served phase length (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run one 1 GiB device-to-device copy (the known-size dispatch the "
@@ -252,7 +453,7 @@ def main() -> int:
         print("bench.py measures one GPU: run it with --gpus 1 (this build has no multi-GPU path)",
               file=sys.stderr)
         return 2
-    out = bench_batch(args) if args.workload == "batch" else bench_zipf(args)
+    out = {"batch": bench_batch, "zipf": bench_zipf, "join": bench_join, "topk": bench_topk}[args.workload](args)
     print(json.dumps(out))
     return 0
 

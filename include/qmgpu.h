@@ -144,6 +144,11 @@ int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out);
 int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
                   qm_table *table, qm_result **results_out);
 
+/* Device bytes one execution of `p` needs beyond the resident tables (no
+ * reference counterpart: the reference checks its memoryBudget per operator,
+ * src/operators.ts:2383-2462; the serving loop admits requests with this). */
+int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *bytes);
+
 /* ---- results (QueryResult.columnarData, src/types.ts:262-295) ---- */
 int qm_result_rows(qm_result *r, uint64_t *rows);
 int qm_result_qmcb_size(qm_result *r, uint64_t *size);

@@ -57,6 +57,7 @@ SIGNATURES = [
     ("qm_plan_close", C.c_int, [C.c_void_p]),
     ("qm_exec", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("qm_batch_exec", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("qm_plan_device_bytes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_rows", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_qmcb_size", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_qmcb", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),

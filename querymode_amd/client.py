@@ -27,20 +27,49 @@ _NP_TO_QM = {
 }
 
 
-@dataclass
 class QueryResult:
-    """src/types.ts:262-295 (the fields this path fills)."""
-    rows: list[dict]
-    rowCount: int
-    columns: list[str]
-    columnarData: bytes | None = None
-    bytesRead: int = 0
-    pagesSkipped: int = 0
-    durationMs: float = 0.0
-    stats: dict = field(default_factory=dict)
+    """src/types.ts:262-295 (the fields this path fills). The result arrives
+    as QMCB (``columnarData``); ``rows`` materialises one dict per row on first
+    access only, so a large result is never turned into Python objects unless
+    a caller asks for them."""
+
+    def __init__(self, rowCount: int, columns: list[str], columnarData: bytes | memoryview | None = None,
+                 durationMs: float = 0.0, stats: dict | None = None, rows: list[dict] | None = None,
+                 bytesRead: int = 0, pagesSkipped: int = 0):
+        self.rowCount = rowCount
+        self.columns = columns
+        self.columnarData = columnarData
+        self.durationMs = durationMs
+        self.stats = stats or {}
+        self.bytesRead = bytesRead
+        self.pagesSkipped = pagesSkipped
+        self._rows = rows
+
+    @property
+    def rows(self) -> list[dict]:
+        if self._rows is None:
+            self._rows = qmcb.to_rows(self.batch()) if self.columnarData is not None else []
+        return self._rows
 
     def batch(self) -> qmcb.ColumnarBatch:
         return qmcb.decode(self.columnarData)
+
+    def __repr__(self) -> str:  # pragma: no cover - cosmetic
+        return f"QueryResult(rowCount={self.rowCount}, columns={self.columns})"
+
+
+def _qmcb_header(data: bytes) -> tuple[int, list[str]]:
+    """Row count and column names from a QMCB header (src/columnar.ts:438-470)."""
+    import struct
+    magic, rows, ncols = struct.unpack_from("<IIH", data, 0)
+    if magic != qmcb.QMCB_MAGIC:
+        raise QueryModeError("INVALID_FORMAT", "device result is not QMCB")
+    pos, names = 10, []
+    for _ in range(ncols):
+        (nl,) = struct.unpack_from("<H", data, pos)
+        names.append(bytes(data[pos + 2:pos + 2 + nl]).decode("utf-8"))
+        pos += 2 + nl + 2
+    return rows, names
 
 
 class Device:
@@ -158,16 +187,17 @@ class _Plan:
             self.handle = None
 
 
-def _read_result(lib, rh) -> tuple[bytes, dict]:
+def _read_result(lib, rh) -> tuple[memoryview, dict]:
     try:
         size = C.c_uint64()
         N.check(lib.qm_result_qmcb_size(rh, C.byref(size)))
-        buf = (C.c_uint8 * max(1, size.value))()
+        # an uninitialised host buffer the library writes the QMCB bytes into
+        buf = np.empty(max(1, size.value), dtype=np.uint8)
         wr = C.c_uint64()
-        N.check(lib.qm_result_qmcb(rh, buf, size.value, C.byref(wr)))
+        N.check(lib.qm_result_qmcb(rh, buf.ctypes.data_as(C.c_void_p), size.value, C.byref(wr)))
         st = N.Stats()
         N.check(lib.qm_result_stats(rh, C.byref(st)))
-        return bytes(buf)[: wr.value], st.as_dict()
+        return memoryview(buf)[: wr.value], st.as_dict()
     finally:
         lib.qm_result_close(rh)
 
@@ -198,9 +228,8 @@ class GpuExecutor:
     def execute(self, desc: QueryDescriptor) -> QueryResult:
         t0 = time.perf_counter()
         data, st = self.run_qmcb(desc)
-        b = qmcb.decode(data)
-        rows = qmcb.to_rows(b)
-        return QueryResult(rows=rows, rowCount=b.row_count, columns=[c.name for c in b.columns], columnarData=data,
+        rows, names = _qmcb_header(data)
+        return QueryResult(rowCount=rows, columns=names, columnarData=data,
                            durationMs=(time.perf_counter() - t0) * 1e3, stats=st)
 
     def execute_batch(self, descs: list[QueryDescriptor]) -> list[QueryResult]:
@@ -223,8 +252,8 @@ class GpuExecutor:
         res = []
         for i in range(len(descs)):
             data, st = _read_result(self.dev.lib, C.c_void_p(outs[i]))
-            b = qmcb.decode(data)
-            res.append(QueryResult(rows=qmcb.to_rows(b), rowCount=b.row_count, columns=[c.name for c in b.columns],
+            nrow, names = _qmcb_header(data)
+            res.append(QueryResult(rowCount=nrow, columns=names,
                                    columnarData=data, durationMs=(time.perf_counter() - t0) * 1e3, stats=st))
         return res
 

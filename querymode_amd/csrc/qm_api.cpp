@@ -1503,6 +1503,7 @@ std::vector<std::pair<std::string, int>> projection(ColSpace &cs, const Plan &pl
 void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Plan &pl, qm_result *res) {
   Timer tm(ctx);
   uint32_t launches = 0;
+  bool timed_scan = false;
   ColSpace cs;
   cs.L = T;
   cs.R = pl.has_join ? R : nullptr;
@@ -1570,8 +1571,11 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
       if (k > 0) {
         DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
         DevBuf ccount(ctx, (size_t)grid * 4 + 64);
+        HIPX(hipEventRecord(ctx->evk0, ctx->stream));
         HIPX(qmk_topk_scan(&sc, dq.as<DQuery>(), sslot, pl.desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(),
                            ccount.as<uint32_t>(), ctx->stream));
+        HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+        timed_scan = true;
         ++launches;
         const TopKey *lists = cand.as<TopKey>();
         const uint32_t *lc = ccount.as<uint32_t>();
@@ -1645,6 +1649,11 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
     }
   }
   res->stats.device_ms = tm.stop();
+  if (timed_scan) {
+    float sm = 0;
+    HIPX(hipEventElapsedTime(&sm, ctx->evk0, ctx->evk1));
+    res->stats.scan_ms = sm;
+  }
   res->stats.groups = res->rows;
   res->stats.rows_selected = n;
   res->stats.kernel_launches = launches;
@@ -2257,6 +2266,55 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
   res->stats.wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   *out = res.release();
+  API_END
+}
+
+// Device memory one execution of `p` needs beyond the resident tables
+// (group tables, partition records, join index, result columns), from the
+// same binding the execution uses; the serving loop admits requests against
+// the HBM budget with it.
+extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *bytes) {
+  API_BEGIN
+  if (!p || !left || !bytes) fail(QM_ERR_INVALID_ARG, "null argument");
+  *bytes = 0;
+  qm_ctx *ctx = p->ctx;
+  const Plan &pl = p->plan;
+  if (pl.has_join && !right) fail(QM_ERR_TABLE_NOT_FOUND, "join needs the right (build) table");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  uint64_t b = 0;
+  const uint64_t n = left->rows;
+  if (pl.has_join) {
+    // join index: open addressing at twice the build rows (16-byte slots) + row lists
+    b += right->rows * 2 * 16 + right->rows * 4 + 4096;
+  }
+  if (!pl.aggs.empty()) {
+    BItem it;
+    it.plan = &pl;
+    ColSpace &cs = it.cs;
+    cs.L = left;
+    cs.R = pl.has_join ? right : nullptr;
+    cs.rkey = pl.rkey;
+    cs.plan = &pl;
+    bind_filters(ctx, cs, pl.filters, pl.groups, it.ab.q, it.ls, false);
+    bind_aggregate(ctx, cs, pl, n, pl.has_join, it.ab, 0);
+    b += it.ab.table_words * 8;
+    if (it.ab.part) {
+      const uint64_t R = (1 + it.ab.srcs.size() + (it.ab.has_mask ? 1 : 0)) <= 4 ? 4 : 8;
+      b += 2 * n * R * 8;  // pass-1 regions (upper bound) + partition-ordered records
+    }
+    b += it.ab.nslots * 8 * (it.ab.outs.size() + 1);  // compaction and result columns
+  } else {
+    // rows: selection ids + gathered columns (bounded by limit when present)
+    uint64_t out = (pl.limit >= 0) ? (uint64_t)(pl.limit + pl.offset) : n;
+    if (pl.has_join) out = std::max(out, n);
+    uint64_t width = 8;
+    for (const TCol &c : left->cols) width += dt_size(c.dtype);
+    if (pl.has_join) for (const TCol &c : right->cols) width += dt_size(c.dtype);
+    b += std::min(out, n * (pl.has_join ? 4 : 1)) * width;
+    if (pl.has_sort && pl.limit < 0) b += n * 24;  // radix sort keys + values
+  }
+  *bytes = b;
   API_END
 }
 
