@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_scan.hip", "k_part.hip", "k_result.hip", "k_topk.hip", "k_join.hip", "k_stats.hip", "k_sort.hip",
-               "k_batch.hip", "qm_plan.cpp", "qm_api.cpp"]
+               "k_batch.hip", "k_runs.hip", "qm_plan.cpp", "qm_api.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
@@ -43,9 +43,18 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"build step failed: {' '.join(cmd)}")
 
 
+def _up_to_date(src: Path, out: Path) -> bool:
+    """QM_BUILD_INCREMENTAL=1: keep an object newer than its source and every header."""
+    if os.environ.get("QM_BUILD_INCREMENTAL") != "1" or not out.exists():
+        return False
+    deps = [src, *CSRC.glob("*.h"), *INCLUDE.glob("*.h")]
+    return all(d.stat().st_mtime < out.stat().st_mtime for d in deps)
+
+
 def _compile_one(src: str) -> Path:
     out = OBJ / (src.rsplit(".", 1)[0] + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(out)])
+    if not _up_to_date(CSRC / src, out):
+        _run([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(CSRC / src), "-o", str(out)])
     return out
 
 

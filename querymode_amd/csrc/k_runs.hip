@@ -1,0 +1,579 @@
+// k_runs.hip — many aggregate queries grouped by a sorted integer key.
+//
+// The table is stored in ascending key order, so a group is a run of
+// consecutive rows. Every query of a launch filters by (at most) one range on
+// one common column and shares one aggregate list. The launch's sorted
+// distinct range bounds cut that column into buckets: bucket(x) = #{bounds <= x}
+// (NULLs: bucket nbounds + 1), and a query's range is the bucket range
+// [e_lo, e_hi).
+//
+// A wave owns the runs that start inside its chunk of rows and walks them in
+// segments of at most 64 rows (one row per lane):
+//   1. each lane finds its row's bucket (binary search over the bounds in LDS);
+//   2. a histogram over buckets and its prefix sum give every bucket's first
+//      position; scattering the rows there counting-sorts the segment by
+//      bucket;
+//   3. additive states (COUNT(col), integer SUM) get a prefix sum over the
+//      sorted order, order-code states (MIN / MAX) a 6-level sparse max table;
+//   4. then lanes switch to queries: a query's rows in the segment are the
+//      sorted span [C(e_lo), C(e_hi)), so COUNT is a difference of two prefix
+//      entries, a SUM another difference, a MIN / MAX the max of two table
+//      entries. The per-(segment, query) cost is a handful of LDS reads,
+//      independent of the run length.
+// A run's groups are final when the run ends, so each non-empty (run, query)
+// is written once, with plain stores, straight into the query's result
+// columns. A count pass computes every (chunk, query) group count first; the
+// per-query exclusive scan over chunks gives each wave its output positions,
+// so results come out in ascending key order (the reference's first-seen
+// group order on a table stored in key order).
+//
+// Aggregate semantics are those of src/partial-agg.ts:38-134 (per group:
+// COUNT, exact integer SUM, MIN / MAX by value, NaN never updates MIN / MAX);
+// the filter semantics those of src/decode.ts:477-578 (NULL never matches).
+#include "k_agg.hip.h"
+#include "qm_kernels.h"
+
+namespace qm {
+
+static constexpr int R_WAVES = R_BLOCK / 64;
+
+// order the LDS accesses of the lanes of one wave (the LDS serves a wave's
+// requests in issue order; this keeps the compiler from moving them across)
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ DCol rcol(const void *data, int32_t dt) {
+  DCol c;
+  c.data = data; c.nulls = nullptr; c.dtype = dt; c.side = 0;
+  return c;
+}
+
+__device__ __forceinline__ int64_t rkey(const RScan &rs, uint64_t row) { return load1_i(rcol(rs.key, rs.key_dt), row); }
+
+__device__ __forceinline__ bool rnull(const uint8_t *nb, uint64_t row) {
+  return nb != nullptr && ((nb[row >> 3] >> (row & 7)) & 1);
+}
+
+__device__ __forceinline__ uint64_t rcode(const void *data, int32_t dt, uint64_t row) {
+  const DCol c = rcol(data, dt);
+  if (dt_is_float(dt)) return code_f64_bits(f64_bits(load1_f(c, row)));
+  return code_i64(load1_i(c, row));
+}
+
+// contribution of one row to state s (0 = the state's identity)
+__device__ __forceinline__ uint64_t rcontrib(const RScan &rs, int s, uint64_t row) {
+  const DState &st = rs.st[s];
+  if (rnull(rs.snull[s], row)) return 0;
+  if (st.kind == SK_COUNT) return 1;
+  const DCol c = rcol(rs.scol[s], rs.s_dt[s]);
+  Contrib k;
+  if (dt_is_float(rs.s_dt[s])) k = contrib_of(st, true, 0, load1_f(c, row));
+  else k = contrib_of(st, false, load1_i(c, row), 0.0);
+  return k.skip ? 0 : k.a;
+}
+
+__device__ __forceinline__ void rstore_i(void *out, int32_t dt, uint64_t i, int64_t v) {
+  switch (dt) {
+    case DT_I8: static_cast<int8_t *>(out)[i] = (int8_t)v; break;
+    case DT_BOOL: static_cast<uint8_t *>(out)[i] = (uint8_t)v; break;
+    case DT_I16: static_cast<int16_t *>(out)[i] = (int16_t)v; break;
+    case DT_I32: static_cast<int32_t *>(out)[i] = (int32_t)v; break;
+    default: static_cast<int64_t *>(out)[i] = v; break;
+  }
+}
+__device__ __forceinline__ void rstore_f(void *out, int32_t dt, uint64_t i, double v) {
+  if (dt == DT_F32) static_cast<float *>(out)[i] = (float)v;
+  else static_cast<double *>(out)[i] = v;
+}
+
+// value of state si of a finished run (0: the row count, then st[si - 1])
+template <int NS>
+__device__ __forceinline__ uint64_t rstate(int si, uint64_t cnt, const uint64_t (&st)[NS]) {
+  uint64_t v = cnt;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) v = (si == j + 1) ? st[j] : v;
+  return v;
+}
+
+// write one group (a run's result for query q) at row pos of q's columns
+// (global row g = row_off[q] + pos); the values are those finalize_slot
+// derives from a table entry
+template <int NS>
+__device__ void remit(const RScan &rs, int q, uint64_t pos, uint64_t g, int64_t key, uint64_t cnt,
+                      const uint64_t (&st)[NS]) {
+  for (int o = 0; o < rs.nouts; ++o) {
+    const ROut ro = rs.outs[o];
+    void *dst = rs.out_base[o];
+    const uint64_t c = ro.cnt_state < 0 ? 0 : rstate<NS>(ro.cnt_state, cnt, st);
+    const uint64_t v = rstate<NS>(ro.state, cnt, st);
+    bool isnull = false;
+    switch (ro.kind) {
+      case OK_KEY: rstore_i(dst, ro.out_dtype, g, key); break;
+      case OK_COUNT: static_cast<int64_t *>(dst)[g] = (int64_t)c; break;
+      case OK_SUM_I64:
+        isnull = c == 0;
+        static_cast<int64_t *>(dst)[g] = isnull ? 0 : (int64_t)v;
+        break;
+      case OK_AVG_I64:
+        isnull = c == 0;
+        static_cast<double *>(dst)[g] = isnull ? 0.0 : (double)(int64_t)v / (double)c;
+        break;
+      default: {  // OK_MIN / OK_MAX: v is the max of (MIN: complemented) order codes
+        const bool mn = ro.kind == OK_MIN;
+        if (c == 0) {
+          isnull = true;
+          if (ro.src_is_float) rstore_f(dst, ro.out_dtype, g, 0.0);
+          else rstore_i(dst, ro.out_dtype, g, 0);
+        } else if (ro.src_is_float) {
+          double d;
+          if (v == 0) d = mn ? bits_f64(0x7FF0000000000000ull) : bits_f64(0xFFF0000000000000ull);
+          else d = bits_f64(decode_f64_code(mn ? ~v : v));
+          rstore_f(dst, ro.out_dtype, g, d);
+        } else {
+          rstore_i(dst, ro.out_dtype, g, decode_i64(mn ? ~v : v));
+        }
+        break;
+      }
+    }
+    if (isnull) {
+      uint32_t *nb = rs.out_nulls[(uint64_t)q * rs.nouts + o];
+      if (nb) atomicOr(&nb[pos >> 5], 1u << (pos & 31));
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+// PASS 0: groups per (chunk, query) into rs.counts.
+// PASS 1: the groups themselves, at rs.offsets.
+template <int PASS, int NSUM, int NMAX, int QPL>
+__global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
+  constexpr int NS = NSUM + NMAX;
+  constexpr int NSA = NS > 0 ? NS : 1;
+  __shared__ uint64_t s_bounds[R_MAX_BUCKETS];
+  __shared__ __align__(16) uint32_t s_cnt[R_WAVES][R_MAX_BUCKETS];
+  __shared__ uint64_t s_sum[R_WAVES][NSUM > 0 ? NSUM : 1][64];
+  __shared__ uint64_t s_tab[R_WAVES][NMAX > 0 ? NMAX : 1][6][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nbp = (1 << rs.bsteps) - 1;  // padded bound count
+  for (int i = threadIdx.x; i < nbp; i += R_BLOCK) s_bounds[i] = rs.bounds[i];
+  __syncthreads();
+
+  const uint64_t chunk = (uint64_t)blockIdx.x * R_WAVES + wave;
+  if (chunk >= rs.nchunks) return;  // no block barrier below
+  uint32_t *cnt = s_cnt[wave];
+  const uint32_t nb = (uint32_t)rs.nbounds;
+
+  bool qon[QPL];
+  uint32_t elo[QPL], ehi[QPL], groups[QPL];
+  uint64_t opos[QPL], ocap[QPL], roff[QPL], nsel[QPL], acnt[QPL], ast[QPL][NSA];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int q = k * 64 + lane;
+    qon[k] = q < rs.nq;
+    const uint32_t w = qon[k] ? rs.qe[q] : 0u;
+    elo[k] = w & 0xFFFFu;
+    ehi[k] = w >> 16;
+    groups[k] = 0;
+    nsel[k] = 0;
+    opos[k] = (PASS == 1 && qon[k]) ? rs.offsets[chunk * (uint64_t)rs.nq + q] : 0;
+    ocap[k] = (PASS == 1 && qon[k]) ? rs.totals[q] : 0;
+    roff[k] = (PASS == 1 && qon[k]) ? rs.row_off[q] : 0;
+  }
+
+  const uint64_t sbase = PASS == 1 ? rs.stg_base[chunk] : 0;
+  const uint64_t send = PASS == 1 ? rs.stg_base[chunk + 1] : 0;
+  uint64_t sdone = 0;
+  const uint64_t c0 = chunk * rs.chunk_rows;
+  const uint64_t c1 = (c0 + rs.chunk_rows < rs.nrows) ? c0 + rs.chunk_rows : rs.nrows;
+  uint64_t s = c0;
+  if (c0 > 0 && c0 < c1) {
+    // skip the rows of the run that started in the previous chunk
+    const int64_t prev = rkey(rs, c0 - 1);
+    for (;;) {
+      const uint64_t row = s + lane;
+      const bool diff = row >= rs.nrows || rkey(rs, row) != prev;
+      const uint64_t m = __ballot(diff);
+      if (m) { s += (uint64_t)__builtin_ctzll(m); break; }
+      s += 64;
+    }
+  }
+
+  while (s < c1) {
+    const int64_t rk = rkey(rs, s);
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      acnt[k] = 0;
+#pragma unroll
+      for (int j = 0; j < NSA; ++j) ast[k][j] = 0;
+    }
+    uint64_t p = s;
+    for (;;) {
+      const uint64_t row = p + lane;
+      const bool inr = row < rs.nrows && rkey(rs, row) == rk;
+      const uint64_t m = __ballot(inr);
+      const int n = (~m == 0ull) ? 64 : __builtin_ctzll(~m);  // the run's rows: a prefix of the window
+      if (n == 0) break;
+      const bool valid = lane < n;
+      // 1. bucket of the row's filter value
+      uint32_t e = 0;
+      if (valid) {
+        if (rnull(rs.pnull, row)) {
+          e = nb + 1;
+        } else {
+          const uint64_t x = rcode(rs.pcol, rs.p_dt, row);
+          uint32_t f = 0;
+          for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)
+            if (s_bounds[f + step - 1] <= x) f += step;
+          e = f < nb ? f : nb;
+        }
+      }
+      // 2. histogram and prefix over buckets (lane l owns buckets 8l .. 8l+7)
+      uint4 *c4 = reinterpret_cast<uint4 *>(cnt) + 2 * lane;
+      c4[0] = make_uint4(0, 0, 0, 0);
+      c4[1] = make_uint4(0, 0, 0, 0);
+      wave_lds_sync();
+      if (valid) atomicAdd(&cnt[e], 1u);
+      wave_lds_sync();
+      {
+        const uint4 a = c4[0], b = c4[1];
+        uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t t = 0;
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+          const uint32_t c = v[g];
+          v[g] = (PASS == 0) ? t + c : t;  // inclusive (count pass) / exclusive (scatter turns it inclusive)
+          t += c;
+        }
+        uint32_t inc = t;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const uint32_t y = __shfl_up(inc, off);
+          if (lane >= off) inc += y;
+        }
+        const uint32_t ex = inc - t;
+        c4[0] = make_uint4(v[0] + ex, v[1] + ex, v[2] + ex, v[3] + ex);
+        c4[1] = make_uint4(v[4] + ex, v[5] + ex, v[6] + ex, v[7] + ex);
+      }
+      wave_lds_sync();
+      if constexpr (PASS == 1) {
+        // counting sort: each row to its position in bucket order
+        if (valid) {
+          const uint32_t pos = atomicAdd(&cnt[e], 1u);
+#pragma unroll
+          for (int j = 0; j < NSUM; ++j) s_sum[wave][j][pos] = rcontrib(rs, 1 + j, row);
+#pragma unroll
+          for (int j = 0; j < NMAX; ++j) s_tab[wave][j][0][pos] = rcontrib(rs, 1 + NSUM + j, row);
+        }
+        wave_lds_sync();
+        // 3. prefix sums and sparse max tables over the sorted order
+#pragma unroll
+        for (int j = 0; j < NSUM; ++j) {
+          uint64_t x = valid ? s_sum[wave][j][lane] : 0ull;
+#pragma unroll
+          for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+          }
+          if (valid) s_sum[wave][j][lane] = x;
+        }
+#pragma unroll
+        for (int j = 0; j < NMAX; ++j) {
+          uint64_t t = valid ? s_tab[wave][j][0][lane] : 0ull;
+#pragma unroll
+          for (int l = 1; l < 6; ++l) {
+            const uint64_t u = __shfl_down(t, 1 << (l - 1));  // lanes past 63 read their own value
+            t = umax64(t, u);
+            s_tab[wave][j][l][lane] = t;
+          }
+        }
+        wave_lds_sync();
+      }
+      // 4. lanes = queries: the query's span [a, b) of the sorted segment
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        if (!qon[k]) continue;
+        const uint32_t a = elo[k] ? cnt[elo[k] - 1] : 0u;
+        const uint32_t b = ehi[k] ? cnt[ehi[k] - 1] : 0u;
+        if (b <= a) continue;
+        acnt[k] += b - a;
+        if constexpr (PASS == 1) {
+#pragma unroll
+          for (int j = 0; j < NSUM; ++j) ast[k][j] += s_sum[wave][j][b - 1] - (a ? s_sum[wave][j][a - 1] : 0ull);
+          if constexpr (NMAX > 0) {
+            // two windows of 2^l rows cover [a, b); level 5 (32 rows) covers a full segment
+            const int l0 = 31 - __builtin_clz(b - a);
+            const int l = l0 < 5 ? l0 : 5;
+            const uint32_t b2 = b - (1u << l);
+#pragma unroll
+            for (int j = 0; j < NMAX; ++j)
+              ast[k][NSUM + j] = umax64(ast[k][NSUM + j], umax64(s_tab[wave][j][l][a], s_tab[wave][j][l][b2]));
+          }
+        }
+      }
+      wave_lds_sync();
+      p += (uint64_t)n;
+      if (n < 64) break;
+    }
+    // the run is complete: one group per query that selected a row of it
+#pragma unroll
+    for (int k = 0; k < QPL; ++k) {
+      const bool has = qon[k] && acnt[k] != 0;
+      if constexpr (PASS == 0) {
+        groups[k] += has ? 1u : 0u;
+      } else {
+        // records of this run: consecutive in the chunk's staging range
+        const uint64_t m = __ballot(has);
+        if (has) {
+          const uint64_t i = sbase + sdone + (uint64_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+          if (i < send && opos[k] < ocap[k]) {
+            rs.stg_q[i] = (uint16_t)(k * 64 + lane);
+            rs.stg_key[i] = rk;
+            rs.stg_row[i] = roff[k] + opos[k];
+            rs.stg_val[0][i] = acnt[k];
+#pragma unroll
+            for (int j = 0; j < NS; ++j) rs.stg_val[1 + j][i] = ast[k][j];
+          }
+          ++opos[k];
+          nsel[k] += acnt[k];
+        }
+        sdone += (uint64_t)__builtin_popcountll(m);
+      }
+    }
+    s = p;
+  }
+  uint32_t tot = 0;
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    if (!qon[k]) continue;
+    const int q = k * 64 + lane;
+    tot += groups[k];
+    if constexpr (PASS == 0) rs.counts[chunk * (uint64_t)rs.nq + q] = groups[k];
+    else if (nsel[k]) atomicAdd((unsigned long long *)&rs.selected[q], (unsigned long long)nsel[k]);
+  }
+  if constexpr (PASS == 0) {
+    uint64_t t = tot;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    if (lane == 0) rs.chunk_groups[chunk] = t;
+  }
+}
+
+// Place pass: a workgroup moves one chunk's staged records into the result
+// columns, one window of R_WIN records at a time: coalesced loads of the
+// window into LDS slots ordered by query (a counting sort), then each thread
+// writes one slot, so consecutive threads write consecutive rows of one
+// query's columns.
+static constexpr int R_WIN = 2048;
+static constexpr int R_PBLOCK = 512;
+
+template <int NS>
+__global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
+  constexpr int NV = NS + 1;  // count + states
+  extern __shared__ __align__(16) uint8_t s_dyn[];
+  uint64_t *s_key = reinterpret_cast<uint64_t *>(s_dyn);  // [R_WIN]
+  uint64_t *s_row = s_key + R_WIN;                           // [R_WIN]
+  uint64_t *s_val = s_row + R_WIN;                           // [NV][R_WIN]
+  uint16_t *s_qs = reinterpret_cast<uint16_t *>(s_val + NV * R_WIN);  // query of each slot
+  __shared__ uint32_t s_cnt[R_MAX_QUERIES];
+  __shared__ uint32_t s_wsum[R_PBLOCK / 64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const uint64_t c = blockIdx.x;
+  const uint64_t b0 = rs.stg_base[c], b1 = rs.stg_base[c + 1];
+  const uint32_t nq = (uint32_t)rs.nq;
+  constexpr int PER = R_WIN / R_PBLOCK;
+  for (uint64_t w0 = b0; w0 < b1; w0 += R_WIN) {
+    const int n = (int)((b1 - w0 < (uint64_t)R_WIN) ? (b1 - w0) : (uint64_t)R_WIN);
+    if (tid < R_MAX_QUERIES) s_cnt[tid] = 0;
+    __syncthreads();
+    uint32_t q[PER];
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = tid + t * R_PBLOCK;
+      q[t] = 0;
+      if (i < n) {
+        const uint32_t x = rs.stg_q[w0 + i];
+        q[t] = x < nq ? x : 0u;
+        atomicAdd(&s_cnt[q[t]], 1u);
+      }
+    }
+    __syncthreads();
+    {  // exclusive scan over the queries (thread t < R_MAX_QUERIES owns query t)
+      const uint32_t v = tid < R_MAX_QUERIES ? s_cnt[tid] : 0u;
+      uint32_t inc = v;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(inc, off);
+        if (lane >= off) inc += y;
+      }
+      if (lane == 63) s_wsum[wave] = inc;
+      __syncthreads();
+      uint32_t base = 0;
+      for (int w = 0; w < wave; ++w) base += s_wsum[w];
+      if (tid < R_MAX_QUERIES) s_cnt[tid] = base + inc - v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int i = tid + t * R_PBLOCK;
+      if (i < n) {
+        const uint64_t r = w0 + (uint64_t)i;
+        const uint32_t slot = atomicAdd(&s_cnt[q[t]], 1u);
+        s_qs[slot] = (uint16_t)q[t];
+        s_key[slot] = (uint64_t)rs.stg_key[r];
+        s_row[slot] = rs.stg_row[r];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s_val[k * R_WIN + slot] = rs.stg_val[k][r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      const int j = tid + t * R_PBLOCK;
+      if (j >= n) continue;
+      const int qq = s_qs[j];
+      const uint64_t g = s_row[j];
+      const uint64_t ro = rs.row_off[qq];
+      if (g < ro || g - ro >= rs.totals[qq]) continue;  // the write pass staged every record
+      constexpr int NSA = NS > 0 ? NS : 1;
+      uint64_t st[NSA];
+#pragma unroll
+      for (int k = 0; k < NSA; ++k) st[k] = k < NS ? s_val[(1 + k) * R_WIN + j] : 0ull;
+      remit<NSA>(rs, qq, g - ro, g, (int64_t)s_key[j], s_val[j], st);
+    }
+    __syncthreads();
+  }
+}
+
+static size_t runs_place_lds(int ns) { return (size_t)R_WIN * (8 * (2 + ns + 1) + 2); }
+
+// exclusive scan over chunks, per query: a block serves 64 queries (lanes),
+// its 16 waves one slice of the chunks each
+__global__ void __launch_bounds__(1024) runs_offsets_kernel(const uint32_t *__restrict__ counts, uint32_t nchunks,
+                                                            uint32_t nq, uint64_t *offsets, uint64_t *totals) {
+  __shared__ uint64_t part[16][64];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const uint32_t q = blockIdx.x * 64 + lane;
+  const uint32_t per = (nchunks + 15) / 16;
+  const uint32_t a = w * per < nchunks ? w * per : nchunks;
+  const uint32_t b = a + per < nchunks ? a + per : nchunks;
+  uint64_t s = 0;
+  if (q < nq)
+    for (uint32_t c = a; c < b; ++c) s += counts[(uint64_t)c * nq + q];
+  part[w][lane] = s;
+  __syncthreads();
+  uint64_t base = 0;
+  for (int i = 0; i < w; ++i) base += part[i][lane];
+  if (q < nq) {
+    for (uint32_t c = a; c < b; ++c) {
+      offsets[(uint64_t)c * nq + q] = base;
+      base += counts[(uint64_t)c * nq + q];
+    }
+    if (w == 15) totals[q] = base;
+  }
+}
+
+#define R_FN(P, S, M)                                                                   \
+  case (S) * 4 + (M):                                                                   \
+    return qpl == 1 ? (const void *)runs_kernel<P, S, M, 1>                             \
+                    : (qpl == 2 ? (const void *)runs_kernel<P, S, M, 2> : (const void *)runs_kernel<P, S, M, 4>);
+
+static const void *runs_write_fn(int nsum, int nmax, int qpl) {
+  switch (nsum * 4 + nmax) {
+    R_FN(1, 0, 0)
+    R_FN(1, 0, 1)
+    R_FN(1, 0, 2)
+    R_FN(1, 0, 3)
+    R_FN(1, 1, 0)
+    R_FN(1, 1, 1)
+    R_FN(1, 1, 2)
+    R_FN(1, 2, 0)
+    R_FN(1, 2, 1)
+    R_FN(1, 3, 0)
+  }
+  return nullptr;
+}
+#undef R_FN
+
+static int runs_qpl(int nq) { return nq <= 64 ? 1 : (nq <= 128 ? 2 : 4); }
+
+static const void *runs_place_fn(int ns) {
+  switch (ns) {
+    case 0: return (const void *)runs_place_kernel<0>;
+    case 1: return (const void *)runs_place_kernel<1>;
+    case 2: return (const void *)runs_place_kernel<2>;
+    case 3: return (const void *)runs_place_kernel<3>;
+  }
+  return nullptr;
+}
+
+static bool runs_args_ok(const RScan *rs) {
+  return rs->nq >= 1 && rs->nq <= R_MAX_QUERIES && rs->nbounds >= 0 && rs->nbounds + 2 <= R_MAX_BUCKETS &&
+         rs->bsteps >= 0 && rs->bsteps <= 9 && ((1 << rs->bsteps) - 1) >= rs->nbounds && rs->nchunks >= 1 &&
+         rs->chunk_rows >= 1 && rs->nsum >= 0 && rs->nsum <= rs->nstates - 1 && rs->nstates >= 1 &&
+         rs->nstates <= R_MAX_STATES && rs->nouts >= 1 && rs->nouts <= R_MAX_OUTS &&
+         (uint64_t)rs->nchunks * rs->chunk_rows >= rs->nrows;
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" int qmk_runs_supported(int nsum, int nmax) {
+  return nsum >= 0 && nmax >= 0 && 1 + nsum + nmax <= R_MAX_STATES && runs_write_fn(nsum, nmax, 1) != nullptr;
+}
+
+extern "C" hipError_t qmk_runs_count(const RScan *rs, hipStream_t st) {
+  if (!runs_args_ok(rs)) return hipErrorInvalidValue;
+  const int qpl = runs_qpl(rs->nq);
+  const void *fn = qpl == 1 ? (const void *)runs_kernel<0, 0, 0, 1>
+                            : (qpl == 2 ? (const void *)runs_kernel<0, 0, 0, 2> : (const void *)runs_kernel<0, 0, 0, 4>);
+  RScan a = *rs;
+  void *args[] = {&a};
+  const uint32_t grid = (rs->nchunks + R_WAVES - 1) / R_WAVES;
+  return hipLaunchKernel(fn, dim3(grid), dim3(R_BLOCK), args, 0, st);
+}
+
+extern "C" hipError_t qmk_runs_offsets(const uint32_t *counts, uint32_t nchunks, uint32_t nq, uint64_t *offsets,
+                                       uint64_t *totals, hipStream_t st) {
+  if (nq == 0 || nchunks == 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(runs_offsets_kernel, dim3((nq + 63) / 64), dim3(1024), 0, st, counts, nchunks, nq, offsets,
+                     totals);
+  return hipGetLastError();
+}
+
+// lds_pad: extra dynamic LDS per workgroup, which caps the waves in flight
+// (fewer concurrent output streams per cache)
+extern "C" hipError_t qmk_runs_write(const RScan *rs, uint32_t lds_pad, hipStream_t st) {
+  if (!runs_args_ok(rs)) return hipErrorInvalidValue;
+  const void *fn = runs_write_fn(rs->nsum, rs->nstates - 1 - rs->nsum, runs_qpl(rs->nq));
+  if (!fn) return hipErrorInvalidValue;
+  if (lds_pad) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pad);
+    if (e != hipSuccess) return e;
+  }
+  RScan a = *rs;
+  void *args[] = {&a};
+  const uint32_t grid = (rs->nchunks + R_WAVES - 1) / R_WAVES;
+  return hipLaunchKernel(fn, dim3(grid), dim3(R_BLOCK), args, lds_pad, st);
+}
+
+extern "C" hipError_t qmk_runs_place(const RScan *rs, hipStream_t st) {
+  if (!runs_args_ok(rs)) return hipErrorInvalidValue;
+  const int ns = rs->nstates - 1;
+  const void *fn = runs_place_fn(ns);
+  if (!fn) return hipErrorInvalidValue;
+  const size_t lds = runs_place_lds(ns);
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  RScan a = *rs;
+  void *args[] = {&a};
+  return hipLaunchKernel(fn, dim3(rs->nchunks), dim3(R_PBLOCK), args, lds, st);
+}

@@ -987,6 +987,10 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   if (const char *e = getenv("QM_TUNE_PART_LOAD")) c->tune.part_load = std::min(0.9, std::max(0.1, atof(e)));
   if (const char *e = getenv("QM_TUNE_HOT")) c->tune.hot = atoi(e) != 0;
   if (const char *e = getenv("QM_TUNE_QP_MIN")) c->tune.qp_min = std::max(1, atoi(e));
+  if (const char *e = getenv("QM_TUNE_RUNS_MIN_Q")) c->tune.runs_min_q = std::max(1, atoi(e));
+  if (const char *e = getenv("QM_TUNE_RUNS_MIN_LEN")) c->tune.runs_min_len = std::max(0.0, atof(e));
+  if (const char *e = getenv("QM_TUNE_RUNS_CHUNKS")) c->tune.runs_chunks_per_cu = std::max(1, atoi(e));
+  if (const char *e = getenv("QM_TUNE_RUNS_LDS_PAD")) c->tune.runs_lds_pad = std::min(96 * 1024, std::max(0, atoi(e)));
   HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&c->ev0));
   HIPX(hipEventCreate(&c->ev1));
@@ -1910,6 +1914,270 @@ void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_res
   }
 }
 
+// ---- run-sorted launches: queries grouped by a sorted key, each filtered by
+// at most one range on one common column (k_runs.hip) ----
+struct RunsPlan {
+  const TCol *pcol = nullptr;
+  int perm[R_MAX_STATES] = {};       // run state i = query state perm[i]
+  int nsum = 0, nmax = 0;
+  ROut outs[R_MAX_OUTS] = {};
+  int nouts = 0;
+  std::vector<uint64_t> bounds;      // sorted distinct bound codes
+};
+
+bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan &rp) {
+  const size_t n = L.sorted.size();
+  if (n == 0 || n > (size_t)R_MAX_QUERIES || (int)n < ctx->tune.runs_min_q || !L.groups.empty()) return false;
+  const TCol *key = L.sorted_key;
+  if (!key || !key->st.sorted || key->st.nulls || dt_float(key->dtype)) return false;
+  if ((double)T->rows / std::max(1.0, key->st.distinct) < ctx->tune.runs_min_len) return false;
+  for (const BItem *it : L.sorted) {
+    if (it->nslots > 1) return false;
+    if (it->nslots == 1 && !it->never) {
+      if (rp.pcol && rp.pcol != it->slot_t[0]) return false;
+      rp.pcol = it->slot_t[0];
+    }
+  }
+  if (!rp.pcol) rp.pcol = key;
+  const BItem &i0 = *L.sorted[0];
+  const DQuery &q0 = i0.ab.q;
+  if (q0.nstates < 1 || q0.nstates > R_MAX_STATES) return false;
+  if (q0.states[0].kind != SK_COUNT || q0.states[0].col_a >= 0) return false;
+  std::vector<int> adds, maxs;
+  for (int s = 1; s < q0.nstates; ++s) {
+    const DState &d = q0.states[s];
+    if (d.product || !i0.st_t[s]) return false;
+    if (d.kind == SK_SUM_I64) {
+      if (T->rows >= (1ull << 32)) return false;  // |sum| of < 2^32 values of <= 32 bits fits int64
+      adds.push_back(s);
+    } else if (d.kind == SK_COUNT) {
+      adds.push_back(s);
+    } else if (d.kind == SK_MAX_CODE) {
+      maxs.push_back(s);
+    } else {
+      return false;
+    }
+  }
+  rp.nsum = (int)adds.size();
+  rp.nmax = (int)maxs.size();
+  if (!qmk_runs_supported(rp.nsum, rp.nmax)) return false;
+  int k = 0;
+  rp.perm[k++] = 0;
+  for (int s : adds) rp.perm[k++] = s;
+  for (int s : maxs) rp.perm[k++] = s;
+  auto state_of_word = [&](int w) -> int {
+    for (int i = 0; i < q0.nstates; ++i) if (q0.states[rp.perm[i]].word == w) return i;
+    return -1;
+  };
+  const auto &o0 = i0.ab.outs;
+  if (o0.empty() || o0.size() > (size_t)R_MAX_OUTS) return false;
+  for (const BItem *it : L.sorted) {
+    const auto &o = it->ab.outs;
+    if (o.size() != o0.size()) return false;
+    for (size_t j = 0; j < o.size(); ++j) {
+      const DOut &a = o[j].o, &b = o0[j].o;
+      if (a.kind != b.kind || a.out_dtype != b.out_dtype || a.src_is_float != b.src_is_float || a.word != b.word ||
+          a.cnt_word != b.cnt_word || a.key_idx != b.key_idx)
+        return false;
+    }
+  }
+  for (size_t j = 0; j < o0.size(); ++j) {
+    const DOut &d = o0[j].o;
+    ROut &r = rp.outs[j];
+    switch (d.kind) {
+      case OK_KEY: case OK_COUNT: case OK_SUM_I64: case OK_AVG_I64: case OK_MIN: case OK_MAX: break;
+      default: return false;
+    }
+    r.kind = d.kind;
+    r.src_is_float = d.src_is_float;
+    r.out_dtype = d.out_dtype;
+    r.state = (int8_t)(d.kind == OK_KEY ? 0 : state_of_word(d.word));
+    r.cnt_state = (int8_t)(d.cnt_word >= 0 ? state_of_word(d.cnt_word) : -1);
+    if (r.state < 0 || (d.cnt_word >= 0 && r.cnt_state < 0)) return false;
+  }
+  rp.nouts = (int)o0.size();
+  std::vector<uint64_t> &b = rp.bounds;
+  for (const BItem *it : L.sorted) {
+    if (it->never || it->nslots == 0) continue;
+    b.push_back(it->lo[0]);
+    if (it->hi[0] != ~0ull) b.push_back(it->hi[0] + 1);
+  }
+  std::sort(b.begin(), b.end());
+  b.erase(std::unique(b.begin(), b.end()), b.end());
+  return b.size() + 2 <= (size_t)R_MAX_BUCKETS;
+}
+
+// Count pass, offsets, then the write pass straight into each query's result
+// columns (ascending key order). Adds the kernel time to scan_ms.
+void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &rp, std::vector<qm_result *> &out,
+              float &scan_ms) {
+  const size_t n = L.sorted.size();
+  const std::vector<uint64_t> &b = rp.bounds;
+  const uint32_t nb = (uint32_t)b.size();
+  int steps = 0;
+  while (((1u << steps) - 1) < nb) ++steps;
+  std::vector<uint64_t> padded((1u << steps) - 1, ~0ull);
+  std::copy(b.begin(), b.end(), padded.begin());
+  std::vector<uint32_t> qe(n);
+  for (size_t i = 0; i < n; ++i) {
+    const BItem &it = *L.sorted[i];
+    uint32_t lo, hi;
+    if (it.never) {
+      lo = hi = 0;
+    } else if (it.nslots == 0) {
+      lo = 0;
+      hi = nb + 2;  // every row, NULL filter values included
+    } else {
+      lo = (uint32_t)(std::upper_bound(b.begin(), b.end(), it.lo[0]) - b.begin());
+      hi = it.hi[0] == ~0ull ? nb + 1 : (uint32_t)(std::upper_bound(b.begin(), b.end(), it.hi[0] + 1) - b.begin());
+    }
+    qe[i] = lo | (hi << 16);
+  }
+  const uint64_t rows = T->rows;
+  const uint64_t target = (uint64_t)ctx->num_cus * (uint64_t)ctx->tune.runs_chunks_per_cu;
+  uint64_t chunk_rows = std::max<uint64_t>(1024, (rows + target - 1) / target);
+  chunk_rows = (chunk_rows + 63) / 64 * 64;
+  const uint32_t nchunks = (uint32_t)std::max<uint64_t>(1, (rows + chunk_rows - 1) / chunk_rows);
+
+  DevBuf dbounds(ctx, std::max<size_t>(padded.size(), 1) * 8), dqe(ctx, n * 4 + 64);
+  DevBuf counts(ctx, (size_t)nchunks * n * 4 + 64), offs(ctx, (size_t)nchunks * n * 8 + 64);
+  DevBuf totals(ctx, n * 8 + 64), selected(ctx, n * 8 + 64);
+  DevBuf cgroups(ctx, (size_t)nchunks * 8 + 64), sbase(ctx, ((size_t)nchunks + 1) * 8 + 64);
+  DevBuf stmp(ctx, qmk_exscan_tmp_words(nchunks) * 8 + 64);
+  if (!padded.empty()) h2d(ctx, dbounds.p, padded.data(), padded.size());
+  h2d(ctx, dqe.p, qe.data(), n);
+  zero(ctx, selected.p, n * 8 + 64);
+
+  const BItem &i0 = *L.sorted[0];
+  const DQuery &q0 = i0.ab.q;
+  RScan rs;
+  memset(&rs, 0, sizeof rs);
+  rs.nrows = rows;
+  rs.chunk_rows = chunk_rows;
+  rs.nchunks = nchunks;
+  rs.nq = (int)n;
+  rs.key = L.sorted_key->d;
+  rs.key_dt = L.sorted_key->dtype;
+  rs.pcol = rp.pcol->d;
+  rs.p_dt = rp.pcol->dtype;
+  rs.pnull = rp.pcol->st.nulls ? rp.pcol->dnull : nullptr;
+  rs.nstates = 1 + rp.nsum + rp.nmax;
+  rs.nsum = rp.nsum;
+  for (int i = 0; i < rs.nstates; ++i) {
+    const int s = rp.perm[i];
+    rs.st[i] = q0.states[s];
+    const TCol *t = i0.st_t[s];
+    rs.scol[i] = t ? t->d : nullptr;
+    rs.snull[i] = (t && t->st.nulls) ? t->dnull : nullptr;
+    rs.s_dt[i] = t ? t->dtype : 0;
+  }
+  rs.nbounds = (int)nb;
+  rs.bsteps = steps;
+  rs.bounds = dbounds.as<uint64_t>();
+  rs.qe = dqe.as<uint32_t>();
+  rs.counts = counts.as<uint32_t>();
+  rs.offsets = offs.as<uint64_t>();
+  rs.totals = totals.as<uint64_t>();
+  rs.nouts = rp.nouts;
+  for (int j = 0; j < rp.nouts; ++j) rs.outs[j] = rp.outs[j];
+  rs.selected = selected.as<uint64_t>();
+  rs.chunk_groups = cgroups.as<uint64_t>();
+  rs.stg_base = sbase.as<uint64_t>();
+
+  std::vector<uint64_t> tot(n + 1, 0);  // per query, then all staged records
+  if (rows) {
+    HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+    HIPX(qmk_runs_count(&rs, ctx->stream));
+    HIPX(qmk_runs_offsets(rs.counts, nchunks, (uint32_t)n, offs.as<uint64_t>(), totals.as<uint64_t>(), ctx->stream));
+    HIPX(qmk_exscan_u64(cgroups.as<uint64_t>(), nchunks, sbase.as<uint64_t>(), totals.as<uint64_t>() + n,
+                        stmp.as<uint64_t>(), ctx->stream));
+    HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+    d2h(ctx, tot.data(), totals.p, n + 1);
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    scan_ms += ms;
+  }
+  // stg_base[nchunks] = all records (the exclusive scan leaves it to the total)
+  if (rows) h2d(ctx, sbase.as<uint64_t>() + nchunks, &tot[n], 1);
+  // result columns: one arena per output column shared by the launch's
+  // results (query i's rows at row_off[i], padded like any result column);
+  // the write pass fills them
+  std::vector<uint64_t> roff(n);
+  uint64_t arows = 0;
+  for (size_t i = 0; i < n; ++i) {
+    roff[i] = arows;
+    arows += pad_rows(tot[i]);
+  }
+  std::vector<std::shared_ptr<DevBuf>> arenas(rp.nouts);
+  for (int j = 0; j < rp.nouts; ++j)
+    arenas[j] = std::make_shared<DevBuf>(ctx, std::max<uint64_t>(arows, 1) * dt_size(rp.outs[j].out_dtype) + 64);
+  std::vector<uint32_t *> nptrs(n * rp.nouts, nullptr);
+  for (size_t i = 0; i < n; ++i) {
+    const BItem &it = *L.sorted[i];
+    qm_result *res = new qm_result();
+    out[it.idx] = res;
+    res->ctx = ctx;
+    res->rows = tot[i];
+    res->arenas = arenas;
+    for (int j = 0; j < rp.nouts; ++j) {
+      const ROut &r = rp.outs[j];
+      OutCol oc;
+      oc.name = it.ab.outs[j].name;
+      oc.dtype = r.out_dtype;
+      oc.data.p = static_cast<uint8_t *>(arenas[j]->p) + roff[i] * dt_size(r.out_dtype);
+      oc.data.n = pad_rows(tot[i]) * dt_size(r.out_dtype);  // a view: ctx stays null, nothing to release
+      // an emitted group has COUNT(*) >= 1: only a COUNT(col)-guarded output can be NULL
+      if (r.kind != OK_KEY && r.kind != OK_COUNT && r.cnt_state > 0) {
+        const uint64_t nb = pad_rows(tot[i]) / 8 + 64;
+        oc.nulls = DevBuf(ctx, nb);
+        zero(ctx, oc.nulls.p, nb);
+        nptrs[i * rp.nouts + j] = oc.nulls.as<uint32_t>();
+      }
+      res->cols.push_back(std::move(oc));
+    }
+  }
+  const uint64_t all = tot[n];
+  uint64_t check = 0;
+  for (size_t i = 0; i < n; ++i) check += tot[i];
+  if (check != all) fail(QM_ERR_QUERY_FAILED, "run-sorted batch: %llu staged groups, %llu result rows",
+                         (unsigned long long)all, (unsigned long long)check);
+  std::vector<uint64_t> sel(n, 0);
+  if (all) {
+    DevBuf droff(ctx, n * 8 + 64), dnptr(ctx, nptrs.size() * 8 + 64);
+    h2d(ctx, droff.p, roff.data(), n);
+    h2d(ctx, dnptr.p, nptrs.data(), nptrs.size());
+    for (int j = 0; j < rp.nouts; ++j) rs.out_base[j] = arenas[j]->p;
+    rs.row_off = droff.as<uint64_t>();
+    rs.out_nulls = dnptr.as<uint32_t *const>();
+    DevBuf sq(ctx, all * 2 + 64), skey(ctx, all * 8 + 64), srow(ctx, all * 8 + 64);
+    std::vector<DevBuf> sval;
+    for (int j = 0; j < rs.nstates; ++j) sval.emplace_back(ctx, all * 8 + 64);
+    rs.stg_q = sq.as<uint16_t>();
+    rs.stg_key = skey.as<int64_t>();
+    rs.stg_row = srow.as<uint64_t>();
+    for (int j = 0; j < rs.nstates; ++j) rs.stg_val[j] = sval[j].as<uint64_t>();
+    HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+    HIPX(qmk_runs_write(&rs, (uint32_t)ctx->tune.runs_lds_pad, ctx->stream));
+    HIPX(qmk_runs_place(&rs, ctx->stream));
+    HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+    d2h(ctx, sel.data(), selected.p, n);
+    float ms = 0;
+    HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    scan_ms += ms;
+  }
+  DQuery noq;
+  memset(&noq, 0, sizeof noq);
+  DevBuf dnoq(ctx, sizeof(DQuery));
+  h2d(ctx, dnoq.p, reinterpret_cast<const uint8_t *>(&noq), sizeof noq);
+  for (size_t i = 0; i < n; ++i) {
+    qm_result *res = out[L.sorted[i]->idx];
+    res->stats.rows_selected = sel[i];
+    res->stats.strategy = QM_STRAT_SORTED_RUNS;
+    uint32_t l = 0;
+    sort_limit_result(res, *L.sorted[i]->plan, dnoq.as<DQuery>(), &l);
+  }
+}
+
 // Run the fusable plans of `items` in fused passes; results by item idx.
 void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std::vector<qm_result *> &out) {
   Timer tm(ctx);
@@ -1996,6 +2264,23 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     L->sorted_key = it->key_t;
     L->sorted.push_back(it);
   }
+  // run-sorted launches write their results directly
+  float scan_ms_total = 0;
+  std::vector<BItem *> runs_done;
+  const size_t nlaunches = launches.size();
+  {
+    std::vector<BLaunch> rest;
+    for (auto &L : launches) {
+      RunsPlan rp;
+      if (!L.sorted.empty() && runs_plan(ctx, T, L, rp)) {
+        run_runs(ctx, T, L, rp, out, scan_ms_total);
+        for (BItem *it : L.sorted) runs_done.push_back(it);
+      } else {
+        rest.push_back(std::move(L));
+      }
+    }
+    launches = std::move(rest);
+  }
   // tables, status, selected counters
   const size_t n = items.size();
   DevBuf status(ctx, n * 8 + 64), selected(ctx, n * 8 + 64);
@@ -2018,7 +2303,6 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     h2d(ctx, g->dbounds.p, g->bounds.data(), g->bounds.size());
   }
   const uint64_t ntiles = std::max<uint64_t>(1, (T->rows + 511) / 512);
-  float scan_ms_total = 0;
   std::vector<DevBuf> qbufs;
   for (auto &L : launches) {
     BScan bs;
@@ -2189,6 +2473,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
   }
   finalize_batch(ctx, order, out, status, selected);
   double ms = tm.stop();
+  for (BItem *it : runs_done) order.push_back(it);
   for (BItem *it : order) {
     qm_result *res = out[it->idx];
     res->stats.rows_scanned = T->rows;
@@ -2196,7 +2481,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     res->stats.bytes_algorithmic = referenced_bytes(it->cs, T->rows, 0);
     res->stats.device_ms = ms;
     res->stats.scan_ms = scan_ms_total;
-    res->stats.kernel_launches = (uint32_t)launches.size();
+    res->stats.kernel_launches = (uint32_t)nlaunches;
   }
 }
 

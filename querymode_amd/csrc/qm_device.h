@@ -241,4 +241,67 @@ struct BScan {
 };
 static constexpr int B_QP_SLOTS = 2;   // AND-ed ranges per query on the query-parallel path
 
+// ---- run-sorted batch: queries grouped by a sorted integer key, each
+// filtered by (at most) one range on a common column. A run is the rows of
+// one key value. The sorted distinct query bounds cut the filter column into
+// buckets (bucket of code x = #{bounds <= x}; bucket nbounds + 1 holds NULLs),
+// so each query's range is a contiguous bucket range [e_lo, e_hi). A wave
+// counting-sorts each run segment (<= 64 rows) by bucket; a query's rows in
+// the segment are then one contiguous span of the sorted order, answered from
+// prefix sums (additive states) and a sparse max table (order codes).
+static constexpr int R_MAX_STATES = 4;    // state 0 (COUNT(*)) + 3 more
+static constexpr int R_MAX_BUCKETS = 512; // nbounds + 2
+static constexpr int R_MAX_QUERIES = 256;
+static constexpr int R_MAX_OUTS = 8;
+static constexpr int R_BLOCK = 256;       // 4 waves, one chunk each
+
+// one output column of every query of the launch (same for all queries)
+struct ROut {
+  uint8_t kind;          // OK_KEY, OK_COUNT, OK_SUM_I64, OK_AVG_I64, OK_MIN, OK_MAX
+  uint8_t src_is_float;
+  int8_t state;          // state holding the value (OK_KEY: unused)
+  int8_t cnt_state;      // state deciding null-ness / the divisor (-1: none)
+  int32_t out_dtype;
+};
+
+struct RScan {
+  uint64_t nrows;
+  uint64_t chunk_rows;           // a wave owns the runs that start in its chunk
+  uint32_t nchunks;
+  int32_t nq;
+  const void *key;
+  int32_t key_dt;
+  int32_t p_dt;                  // filter column (codes), its NULL bitmap
+  const void *pcol;
+  const uint8_t *pnull;
+  int32_t nstates;               // 1 + nsum + nmax
+  int32_t nsum;                  // states 1..nsum: additive; the rest: order codes
+  DState st[R_MAX_STATES];       // st[0] = COUNT(*)
+  const void *scol[R_MAX_STATES];
+  const uint8_t *snull[R_MAX_STATES];
+  int32_t s_dt[R_MAX_STATES];
+  int32_t nbounds;               // sorted distinct bound codes
+  int32_t bsteps;                // search steps: the bounds are padded to 2^bsteps - 1 entries
+  const uint64_t *bounds;        // 2^bsteps - 1 entries (padding: all-ones)
+  const uint32_t *qe;            // per query: e_lo | e_hi << 16
+  uint32_t *counts;              // [nchunks][nq] groups per chunk (count pass)
+  const uint64_t *offsets;       // [nchunks][nq] first output row (write pass)
+  const uint64_t *totals;        // [nq] result rows (write pass bound)
+  int32_t nouts, _p0;
+  ROut outs[R_MAX_OUTS];
+  void *out_base[R_MAX_OUTS];    // column o of every query: query q's row i at global row row_off[q] + i
+  const uint64_t *row_off;       // [nq]
+  uint32_t *const *out_nulls;    // [nq][nouts] per-query NULL bitmaps (nullptr: never null)
+  uint64_t *selected;            // [nq] selected-row counters
+  // staging: the write pass appends each finished run's groups (one record
+  // per query that selected a row of it) to its chunk's record range; the
+  // place pass moves them into the result columns, query by query
+  uint64_t *chunk_groups;        // [nchunks] records per chunk (count pass)
+  const uint64_t *stg_base;      // [nchunks + 1] first record of each chunk
+  uint16_t *stg_q;               // query
+  int64_t *stg_key;              // run key
+  uint64_t *stg_row;             // global result row (row_off[q] + position)
+  uint64_t *stg_val[R_MAX_STATES]; // state values (0: the row count)
+};
+
 }  // namespace qm

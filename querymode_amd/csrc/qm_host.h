@@ -112,6 +112,10 @@ struct qm_ctx {
     double part_load = 0.5;    // target fill of a pass-3 LDS table
     int hot = 1;               // aggregate heavy-hitter keys during the scatter
     int qp_min = 2;            // sorted-key launches with at least this many queries run query-parallel
+    int runs_min_q = 2;        // ... and with at least this many (and runs this long) run-sorted
+    double runs_min_len = 8.0; // mean rows per key run for the run-sorted kernel
+    int runs_chunks_per_cu = 16;
+    int runs_lds_pad = 0;      // extra LDS bytes per write-pass workgroup (caps waves in flight)
   } tune;
 
   void *alloc(size_t bytes);
@@ -195,6 +199,7 @@ struct OutCol {
 struct qm_result {
   qm_ctx *ctx = nullptr;
   uint64_t rows = 0;
-  std::vector<qm::OutCol> cols;
+  std::vector<qm::OutCol> cols;  // a column may be a view into one of `arenas`
+  std::vector<std::shared_ptr<qm::DevBuf>> arenas;  // buffers shared by the results of one launch
   qm_stats stats{};
 };

@@ -125,6 +125,14 @@ int qmk_batch_qpf_supported(int nadd, int nmax);
 int qmk_batch_qpf_waves_per_cu(int nadd, int nmax, int qpl);
 hipError_t qmk_batch_qpf_scan(const qm::BScan *bs, const qm::BQuery *d_q, uint32_t grid, int nadd, int nmax,
                               int qpl, hipStream_t st);
+// run-sorted batch (sorted key, one range column): count pass, per-query
+// offsets over chunks, write pass into the result columns
+hipError_t qmk_runs_count(const qm::RScan *rs, hipStream_t st);
+hipError_t qmk_runs_offsets(const uint32_t *counts, uint32_t nchunks, uint32_t nq, uint64_t *offsets,
+                            uint64_t *totals, hipStream_t st);
+int qmk_runs_supported(int nsum, int nmax);
+hipError_t qmk_runs_write(const qm::RScan *rs, uint32_t lds_pad, hipStream_t st);
+hipError_t qmk_runs_place(const qm::RScan *rs, hipStream_t st);
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);
