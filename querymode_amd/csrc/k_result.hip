@@ -269,6 +269,33 @@ __global__ void compact_write_kernel(DFinal f, const uint64_t *block_offsets) {
   }
 }
 
+// Many small tables compacted in one launch: workgroup i serves fs[i] (at
+// most CMP_SMALL_SLOTS slots), clears its output NULL bitmaps, then writes its
+// occupied slots in slot order; counts[i] = rows written.
+static constexpr uint64_t CMP_SMALL_SLOTS = 4096;
+__global__ void __launch_bounds__(CMP_BLOCK) compact_small_kernel(const DFinal *fs, uint64_t *counts) {
+  __shared__ uint32_t sh[17];
+  const DFinal &f = fs[blockIdx.x];
+  const uint64_t nslots = f.nslots < CMP_SMALL_SLOTS ? f.nslots : CMP_SMALL_SLOTS;
+  const uint64_t words = (nslots + 31) / 32;
+  for (int oi = 0; oi < f.nouts; ++oi) {
+    uint32_t *nb = f.outs[oi].nulls;
+    if (nb)
+      for (uint64_t w = threadIdx.x; w < words; w += blockDim.x) nb[w] = 0;
+  }
+  __syncthreads();
+  uint64_t out = 0;
+  for (uint64_t i0 = 0; i0 < nslots; i0 += blockDim.x) {
+    const uint64_t slot = i0 + threadIdx.x;
+    const uint32_t occ = (slot < nslots && slot_occupied(f, slot)) ? 1u : 0u;
+    uint32_t total;
+    const uint32_t off = block_exscan_u32(occ, &total, sh);
+    if (occ) finalize_slot(f, slot, out + off);
+    out += total;
+  }
+  if (threadIdx.x == 0) counts[blockIdx.x] = out;
+}
+
 // ---------------------------------------------------------------------------
 // filter-only selection: ascending row ids of rows passing the predicates.
 // Each block owns a contiguous range of tiles so the output order is the
@@ -367,6 +394,12 @@ extern "C" uint64_t qmk_compact_blocks(uint64_t nslots) {
 extern "C" hipError_t qmk_compact_count(const DFinal *f, uint64_t *block_counts, hipStream_t st) {
   uint64_t nb = qmk_compact_blocks(f->nslots);
   hipLaunchKernelGGL(compact_count_kernel, dim3((unsigned)nb), dim3(CMP_BLOCK), 0, st, *f, block_counts);
+  return hipGetLastError();
+}
+extern "C" uint64_t qmk_compact_small_slots() { return CMP_SMALL_SLOTS; }
+extern "C" hipError_t qmk_compact_small(const DFinal *d_f, uint32_t n, uint64_t *counts, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(compact_small_kernel, dim3(n), dim3(CMP_BLOCK), 0, st, d_f, counts);
   return hipGetLastError();
 }
 extern "C" hipError_t qmk_compact_write(const DFinal *f, const uint64_t *block_offsets, hipStream_t st) {

@@ -369,10 +369,9 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
 // window into LDS slots ordered by query (a counting sort), then each thread
 // writes one slot, so consecutive threads write consecutive rows of one
 // query's columns.
-static constexpr int R_WIN = 2048;
 static constexpr int R_PBLOCK = 512;
 
-template <int NS>
+template <int NS, int R_WIN>
 __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   constexpr int NV = NS + 1;  // count + states
   extern __shared__ __align__(16) uint8_t s_dyn[];
@@ -451,7 +450,7 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   }
 }
 
-static size_t runs_place_lds(int ns) { return (size_t)R_WIN * (8 * (2 + ns + 1) + 2); }
+static size_t runs_place_lds(int ns, int win) { return (size_t)win * (8 * (2 + ns + 1) + 2); }
 
 // exclusive scan over chunks, per query: a block serves 64 queries (lanes),
 // its 16 waves one slice of the chunks each
@@ -504,14 +503,18 @@ static const void *runs_write_fn(int nsum, int nmax, int qpl) {
 
 static int runs_qpl(int nq) { return nq <= 64 ? 1 : (nq <= 128 ? 2 : 4); }
 
-static const void *runs_place_fn(int ns) {
+template <int WIN>
+static const void *runs_place_fn_w(int ns) {
   switch (ns) {
-    case 0: return (const void *)runs_place_kernel<0>;
-    case 1: return (const void *)runs_place_kernel<1>;
-    case 2: return (const void *)runs_place_kernel<2>;
-    case 3: return (const void *)runs_place_kernel<3>;
+    case 0: return (const void *)runs_place_kernel<0, WIN>;
+    case 1: return (const void *)runs_place_kernel<1, WIN>;
+    case 2: return (const void *)runs_place_kernel<2, WIN>;
+    case 3: return (const void *)runs_place_kernel<3, WIN>;
   }
   return nullptr;
+}
+static const void *runs_place_fn(int ns, int win) {
+  return win >= 2048 ? runs_place_fn_w<2048>(ns) : runs_place_fn_w<1024>(ns);
 }
 
 static bool runs_args_ok(const RScan *rs) {
@@ -565,12 +568,15 @@ extern "C" hipError_t qmk_runs_write(const RScan *rs, uint32_t lds_pad, hipStrea
   return hipLaunchKernel(fn, dim3(grid), dim3(R_BLOCK), args, lds_pad, st);
 }
 
-extern "C" hipError_t qmk_runs_place(const RScan *rs, hipStream_t st) {
+// win: records per window (1024 or 2048; larger windows give longer
+// contiguous runs per query, smaller ones more workgroups per CU)
+extern "C" hipError_t qmk_runs_place(const RScan *rs, int win, hipStream_t st) {
   if (!runs_args_ok(rs)) return hipErrorInvalidValue;
   const int ns = rs->nstates - 1;
-  const void *fn = runs_place_fn(ns);
+  win = win >= 2048 ? 2048 : 1024;
+  const void *fn = runs_place_fn(ns, win);
   if (!fn) return hipErrorInvalidValue;
-  const size_t lds = runs_place_lds(ns);
+  const size_t lds = runs_place_lds(ns, win);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   RScan a = *rs;

@@ -713,7 +713,7 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
 // ===========================================================================
 // result assembly helpers
 // ===========================================================================
-OutCol make_out(qm_ctx *ctx, const std::string &name, int dt, uint64_t rows, bool with_nulls) {
+OutCol make_out(qm_ctx *ctx, const std::string &name, int dt, uint64_t rows, bool with_nulls, bool zero_nulls = true) {
   OutCol oc;
   oc.name = name;
   oc.dtype = dt;
@@ -721,7 +721,7 @@ OutCol make_out(qm_ctx *ctx, const std::string &name, int dt, uint64_t rows, boo
   oc.data = DevBuf(ctx, pr * dt_size(dt));
   if (with_nulls) {
     oc.nulls = DevBuf(ctx, pr / 8 + 64);
-    zero(ctx, oc.nulls.p, pr / 8 + 64);
+    if (zero_nulls) zero(ctx, oc.nulls.p, pr / 8 + 64);
   }
   return oc;
 }
@@ -990,6 +990,7 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   if (const char *e = getenv("QM_TUNE_RUNS_MIN_Q")) c->tune.runs_min_q = std::max(1, atoi(e));
   if (const char *e = getenv("QM_TUNE_RUNS_MIN_LEN")) c->tune.runs_min_len = std::max(0.0, atof(e));
   if (const char *e = getenv("QM_TUNE_RUNS_CHUNKS")) c->tune.runs_chunks_per_cu = std::max(1, atoi(e));
+  if (const char *e = getenv("QM_TUNE_RUNS_WIN")) c->tune.runs_win = atoi(e) >= 2048 ? 2048 : 1024;
   if (const char *e = getenv("QM_TUNE_RUNS_LDS_PAD")) c->tune.runs_lds_pad = std::min(96 * 1024, std::max(0, atoi(e)));
   HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&c->ev0));
@@ -1844,16 +1845,16 @@ std::vector<const TCol *> item_cols(const BItem &it) {
   return v;
 }
 
-// Finalise many aggregate results with one synchronisation per phase.
+// Finalise many aggregate results with one synchronisation per phase. Small
+// tables (the shared-interval queries' DIRECT tables) are compacted together
+// in one launch into columns sized for every slot; larger ones take the
+// count / scan / write sequence each.
 void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_result *> &out, DevBuf &status,
                     DevBuf &selected) {
   const size_t n = items.size();
   if (!n) return;
   std::vector<DFinal> f(n);
-  std::vector<DevBuf> cnt(n), offs(n);
-  uint64_t max_nb = 1;
-  for (size_t i = 0; i < n; ++i) max_nb = std::max<uint64_t>(max_nb, qmk_compact_blocks(items[i]->ab.nslots));
-  DevBuf stmp(ctx, qmk_exscan_tmp_words(max_nb) * 8 + 64), totals(ctx, n * 8 + 64);
+  std::vector<size_t> small, large;
   for (size_t i = 0; i < n; ++i) {
     BItem &it = *items[i];
     AggBinding &b = it.ab;
@@ -1867,23 +1868,58 @@ void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_res
     x.always_one_row = b.always_one_row ? 1 : 0;
     x.key = b.q.key;
     x.status = status.as<uint32_t>() + 2 * i;
-    uint64_t nb = qmk_compact_blocks(x.nslots);
+    x.nouts = (int)b.outs.size();
+    (b.nslots <= qmk_compact_small_slots() ? small : large).push_back(i);
+  }
+  // large tables: per-table block counts and their scan
+  std::vector<DevBuf> cnt(n), offs(n);
+  uint64_t max_nb = 1;
+  for (size_t i : large) max_nb = std::max<uint64_t>(max_nb, qmk_compact_blocks(f[i].nslots));
+  DevBuf stmp(ctx, qmk_exscan_tmp_words(max_nb) * 8 + 64), totals(ctx, n * 8 + 64);
+  for (size_t i : large) {
+    uint64_t nb = qmk_compact_blocks(f[i].nslots);
     cnt[i] = DevBuf(ctx, nb * 8);
     offs[i] = DevBuf(ctx, nb * 8);
-    HIPX(qmk_compact_count(&x, cnt[i].as<uint64_t>(), ctx->stream));
+    HIPX(qmk_compact_count(&f[i], cnt[i].as<uint64_t>(), ctx->stream));
     HIPX(qmk_exscan_u64(cnt[i].as<uint64_t>(), nb, offs[i].as<uint64_t>(), totals.as<uint64_t>() + i,
                         stmp.as<uint64_t>(), ctx->stream));
   }
-  std::vector<uint64_t> tot(n);
-  d2h(ctx, tot.data(), totals.p, n);
-  for (size_t i = 0; i < n; ++i) {
+  // small tables: outputs sized for every slot, one compaction launch
+  for (size_t i : small) {
+    BItem &it = *items[i];
+    AggBinding &b = it.ab;
+    qm_result *res = new qm_result();
+    out[it.idx] = res;
+    res->ctx = ctx;
+    for (size_t k = 0; k < b.outs.size(); ++k) {
+      const DOut &o = b.outs[k].o;
+      OutCol oc = make_out(ctx, b.outs[k].name, o.out_dtype, f[i].nslots, o.kind != OK_COUNT, false);
+      f[i].outs[k] = o;
+      f[i].outs[k].data = oc.data.p;
+      f[i].outs[k].nulls = oc.nulls.as<uint32_t>();
+      res->cols.push_back(std::move(oc));
+    }
+  }
+  DevBuf dsmall(ctx, std::max<size_t>(small.size(), 1) * sizeof(DFinal));
+  if (!small.empty()) {
+    std::vector<DFinal> fs;
+    for (size_t i : small) fs.push_back(f[i]);
+    h2d(ctx, dsmall.p, reinterpret_cast<const uint8_t *>(fs.data()), fs.size() * sizeof(DFinal));
+    DevBuf scount(ctx, small.size() * 8 + 64);
+    HIPX(qmk_compact_small(dsmall.as<DFinal>(), (uint32_t)small.size(), scount.as<uint64_t>(), ctx->stream));
+    std::vector<uint64_t> sc(small.size());
+    d2h(ctx, sc.data(), scount.p, small.size());
+    for (size_t j = 0; j < small.size(); ++j) out[items[small[j]]->idx]->rows = sc[j];
+  }
+  std::vector<uint64_t> tot(n, 0);
+  if (!large.empty()) d2h(ctx, tot.data(), totals.p, n);
+  for (size_t i : large) {
     BItem &it = *items[i];
     AggBinding &b = it.ab;
     qm_result *res = new qm_result();
     out[it.idx] = res;
     res->ctx = ctx;
     res->rows = tot[i];
-    f[i].nouts = (int)b.outs.size();
     for (size_t k = 0; k < b.outs.size(); ++k) {
       const DOut &o = b.outs[k].o;
       OutCol oc = make_out(ctx, b.outs[k].name, o.out_dtype, tot[i], o.kind != OK_COUNT);
@@ -2158,7 +2194,7 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     for (int j = 0; j < rs.nstates; ++j) rs.stg_val[j] = sval[j].as<uint64_t>();
     HIPX(hipEventRecord(ctx->evk0, ctx->stream));
     HIPX(qmk_runs_write(&rs, (uint32_t)ctx->tune.runs_lds_pad, ctx->stream));
-    HIPX(qmk_runs_place(&rs, ctx->stream));
+    HIPX(qmk_runs_place(&rs, ctx->tune.runs_win, ctx->stream));
     HIPX(hipEventRecord(ctx->evk1, ctx->stream));
     d2h(ctx, sel.data(), selected.p, n);
     float ms = 0;

@@ -116,6 +116,7 @@ struct qm_ctx {
     double runs_min_len = 8.0; // mean rows per key run for the run-sorted kernel
     int runs_chunks_per_cu = 16;
     int runs_lds_pad = 0;      // extra LDS bytes per write-pass workgroup (caps waves in flight)
+    int runs_win = 1024;       // place-pass window (records)
   } tune;
 
   void *alloc(size_t bytes);

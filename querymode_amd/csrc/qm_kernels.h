@@ -68,6 +68,10 @@ uint64_t qmk_exscan_tmp_words(uint64_t n);
 uint64_t qmk_compact_blocks(uint64_t nslots);
 hipError_t qmk_compact_count(const qm::DFinal *f, uint64_t *block_counts, hipStream_t st);
 hipError_t qmk_compact_write(const qm::DFinal *f, const uint64_t *block_offsets, hipStream_t st);
+// many tables of at most qmk_compact_small_slots() slots in one launch; the
+// output columns hold nslots rows and their NULL bitmaps are cleared here
+uint64_t qmk_compact_small_slots();
+hipError_t qmk_compact_small(const qm::DFinal *d_f, uint32_t n, uint64_t *counts, hipStream_t st);
 
 hipError_t qmk_select_count(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t nblocks, uint64_t tpb,
                             uint64_t *block_counts, hipStream_t st);
@@ -132,7 +136,7 @@ hipError_t qmk_runs_offsets(const uint32_t *counts, uint32_t nchunks, uint32_t n
                             uint64_t *totals, hipStream_t st);
 int qmk_runs_supported(int nsum, int nmax);
 hipError_t qmk_runs_write(const qm::RScan *rs, uint32_t lds_pad, hipStream_t st);
-hipError_t qmk_runs_place(const qm::RScan *rs, hipStream_t st);
+hipError_t qmk_runs_place(const qm::RScan *rs, int win, hipStream_t st);
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);
