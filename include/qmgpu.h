@@ -81,7 +81,7 @@ typedef struct qm_stats {
   uint64_t groups;            /* output rows */
   uint64_t bytes_algorithmic; /* bytes of every referenced column, read once */
   double   device_ms;         /* device time of the query's kernels (HIP events) */
-  double   scan_ms;           /* device time of the dominant scan kernel alone */
+  double   scan_ms;           /* device time of the dominant scan kernel alone (batches: the longest launch) */
   double   wall_ms;           /* host wall time of qm_exec */
   uint32_t kernel_launches;
   uint32_t retries;           /* hash-table regrowths */

@@ -98,7 +98,7 @@ struct qm_ctx {
   std::atomic<uint64_t> used{0};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t evk0 = nullptr, evk1 = nullptr;  // around the dominant scan kernel
+  hipEvent_t evk0 = nullptr, evk1 = nullptr, evk2 = nullptr;  // around the dominant scan kernel(s)
   std::mutex mu;           // one query on the device at a time per context
   std::mutex pool_mu;
   std::multimap<size_t, void *> pool;  // free device buffers by size
