@@ -10,7 +10,7 @@
  * (src/errors.ts:7-32). Results cross as QMCB ArrayBuffers (src/columnar.ts).
  *
  * Exports: init, shutdown, lastError, deviceInfo, tableCreate, tableDrop,
- * tableRows, planCompile, planClose, exec, batchExec, resultRows,
+ * tableRows, planCompile, planClose, exec, batchExec, planDeviceBytes, resultRows,
  * resultQmcb, resultStats, resultClose.
  */
 #define NAPI_VERSION 6
@@ -327,6 +327,21 @@ static napi_value js_exec(napi_env env, napi_callback_info info) {
   return wrap(env, H_RESULT, res);
 }
 
+/* planDeviceBytes(plan, table[, rightTable]) -> device bytes beyond the resident tables */
+static napi_value js_plan_device_bytes(napi_env env, napi_callback_info info) {
+  napi_value argv[3], v;
+  get_args(env, info, 3, argv);
+  handle *p = unwrap(env, argv[0], H_PLAN);
+  handle *l = unwrap(env, argv[1], H_TABLE);
+  handle *r = unwrap(env, argv[2], H_TABLE);
+  if (!p || !l) return throw_code(env, "INVALID_ARGUMENT", "planDeviceBytes(plan, table[, rightTable])");
+  uint64_t b = 0;
+  int rc = qm_plan_device_bytes((qm_plan *)p->p, (qm_table *)l->p, r ? (qm_table *)r->p : NULL, &b);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  napi_create_double(env, (double)b, &v);
+  return v;
+}
+
 /* batchExec(ctx, [plan...], table) -> [result...] */
 static napi_value js_batch_exec(napi_env env, napi_callback_info info) {
   napi_value argv[3], arr;
@@ -437,6 +452,7 @@ static napi_value init_module(napi_env env, napi_value exports) {
     {"planClose", NULL, js_plan_close, NULL, NULL, NULL, napi_enumerable, NULL},
     {"exec", NULL, js_exec, NULL, NULL, NULL, napi_enumerable, NULL},
     {"batchExec", NULL, js_batch_exec, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"planDeviceBytes", NULL, js_plan_device_bytes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultRows", NULL, js_result_rows, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultQmcb", NULL, js_result_qmcb, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultStats", NULL, js_result_stats, NULL, NULL, NULL, napi_enumerable, NULL},
