@@ -151,6 +151,10 @@ def bench_batch(args) -> dict:
     t0 = time.time()
     table = dev.register("batch", cols)
     upload_s = time.time() - t0
+    tuned = None
+    if not args.no_autotune:
+        from querymode_amd.autotune import Autotuner
+        tuned = Autotuner(dev).tune_batch(table, descs).as_dict()
     plans = [_Plan(dev, d) for d in descs]
     arr = (C.c_void_p * nq)(*[p.handle for p in plans])
 
@@ -176,7 +180,7 @@ def bench_batch(args) -> dict:
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     batched_rows_s = nq * rows / (wall / args.steps)
-    scan_ms = float(np.mean([s.scan_ms for s, _ in stats]))  # the batch's longest launch
+    scan_ms = float(np.mean([s.scan_ms for s, _ in stats]))  # the batch's longest launch reading the table
     dev_ms = float(np.mean([s.device_ms for s, _ in stats]))
     batch_bytes = rows * sum(np.dtype(c.dtype).itemsize for c in cols.values())  # every column once per pass
     # the dominant launch is the run-sorted write pass over the k2 queries: it
@@ -226,6 +230,7 @@ def bench_batch(args) -> dict:
         "per_query_ms_total": pq_wall * 1e3,
         "per_query_strategies": strategies,
         "served": served,
+        "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": _traffic(kname, rows, "traffic_batch_mix.json")},
@@ -452,6 +457,7 @@ def main() -> int:
     ap.add_argument("--qps", type=float, default=40.0, help="served phase: Poisson arrival rate")
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="served phase length (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the planner's default knobs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run one 1 GiB device-to-device copy (the known-size dispatch the "
                          "FETCH_SIZE / WRITE_SIZE counters are scaled by)")

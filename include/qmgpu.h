@@ -81,7 +81,7 @@ typedef struct qm_stats {
   uint64_t groups;            /* output rows */
   uint64_t bytes_algorithmic; /* bytes of every referenced column, read once */
   double   device_ms;         /* device time of the query's kernels (HIP events) */
-  double   scan_ms;           /* device time of the dominant scan kernel alone (batches: the longest launch) */
+  double   scan_ms;           /* device time of the dominant scan kernel alone (batches: the longest launch reading the table) */
   double   wall_ms;           /* host wall time of qm_exec */
   uint32_t kernel_launches;
   uint32_t retries;           /* hash-table regrowths */
@@ -148,6 +148,15 @@ int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
  * reference counterpart: the reference checks its memoryBudget per operator,
  * src/operators.ts:2383-2462; the serving loop admits requests with this). */
 int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *bytes);
+
+/* Tuning knobs of the device planner, set at run time by an empirical tuner
+ * (querymode_amd/autotune.py) or read at qm_init from QM_TUNE_<NAME>. No
+ * reference counterpart (the reference has no device planner). Knobs:
+ * scatter_rpb, part_load, hot, qp_min, runs_min_q, runs_min_len,
+ * runs_chunks_per_cu, runs_win, runs_lds_pad; values are clamped to each
+ * knob's range; an unknown name is QM_ERR_INVALID_ARG. */
+int qm_tune_set(qm_ctx *ctx, const char *name, double value);
+int qm_tune_get(qm_ctx *ctx, const char *name, double *value);
 
 /* ---- results (QueryResult.columnarData, src/types.ts:262-295) ---- */
 int qm_result_rows(qm_result *r, uint64_t *rows);

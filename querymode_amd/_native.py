@@ -58,6 +58,8 @@ SIGNATURES = [
     ("qm_exec", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("qm_batch_exec", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("qm_plan_device_bytes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("qm_tune_set", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("qm_tune_get", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),
     ("qm_result_rows", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_qmcb_size", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_qmcb", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),

@@ -88,6 +88,15 @@ class Device:
         N.check(self.lib.qm_device_info(self.handle, name, 128, C.byref(cus), C.byref(budget), C.byref(used)))
         return {"arch": name.value.decode(), "cus": cus.value, "budget": budget.value, "used": used.value}
 
+    # ---- planner tuning knobs (qm_tune_set / qm_tune_get) ----
+    def tune_get(self, name: str) -> float:
+        v = C.c_double()
+        N.check(self.lib.qm_tune_get(self.handle, name.encode(), C.byref(v)))
+        return v.value
+
+    def tune_set(self, name: str, value: float) -> None:
+        N.check(self.lib.qm_tune_set(self.handle, name.encode(), float(value)))
+
     def register(self, name: str, columns: dict[str, np.ndarray],
                  nulls: dict[str, np.ndarray] | None = None) -> "GpuTable":
         """Upload decoded columns once; they stay resident in HBM."""

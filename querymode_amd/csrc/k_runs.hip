@@ -45,6 +45,40 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Inclusive wave scans with DPP moves (row shifts within 16-lane rows, then
+// row broadcasts of lanes 15 and 31): VALU only, no LDS crossbar traffic.
+// Lanes whose source is outside the row (or whose row is masked) add 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ uint64_t dpp64(uint64_t x) {
+  const uint32_t lo = dpp32<CTRL, ROW_MASK>((uint32_t)x);
+  const uint32_t hi = dpp32<CTRL, ROW_MASK>((uint32_t)(x >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+template <class T, class F>
+__device__ __forceinline__ T wave_incl_scan(T x, F sh) {
+  x += sh.template operator()<0x111, 0xF>(x);  // row_shr:1
+  x += sh.template operator()<0x112, 0xF>(x);  // row_shr:2
+  x += sh.template operator()<0x114, 0xF>(x);  // row_shr:4
+  x += sh.template operator()<0x118, 0xF>(x);  // row_shr:8
+  x += sh.template operator()<0x142, 0xA>(x);  // row_bcast:15 -> rows 1, 3
+  x += sh.template operator()<0x143, 0xC>(x);  // row_bcast:31 -> rows 2, 3
+  return x;
+}
+struct Dpp32 { template <int C, int M> __device__ uint32_t operator()(uint32_t x) const { return dpp32<C, M>(x); } };
+struct Dpp64 { template <int C, int M> __device__ uint64_t operator()(uint64_t x) const { return dpp64<C, M>(x); } };
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x) { return wave_incl_scan<uint32_t>(x, Dpp32{}); }
+__device__ __forceinline__ uint64_t wave_scan_u64(uint64_t x) { return wave_incl_scan<uint64_t>(x, Dpp64{}); }
+
+__device__ __forceinline__ uint64_t rfirst64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ DCol rcol(const void *data, int32_t dt) {
   DCol c;
   c.data = data; c.nulls = nullptr; c.dtype = dt; c.side = 0;
@@ -147,6 +181,92 @@ __device__ void remit(const RScan &rs, int q, uint64_t pos, uint64_t g, int64_t 
 
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
+// Raw column words: 8 bytes from the 4-byte-aligned address holding the row's
+// value (columns are padded well past their last row), decoded only where the
+// value is used, so the loads of a window stay in flight while the previous
+// window is processed (a decode at the load would wait for it at once).
+__device__ __forceinline__ int rbytes(int32_t dt) {
+  return (dt == DT_F64 || dt == DT_I64) ? 8 : ((dt == DT_I32 || dt == DT_F32) ? 4 : (dt == DT_I16 ? 2 : 1));
+}
+__device__ __forceinline__ uint64_t rraw(const void *data, int32_t dt, uint64_t row) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(data) + row * (uint64_t)rbytes(dt);
+  const uint32_t *w = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+  return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+}
+__device__ __forceinline__ uint32_t rraw_null(const uint8_t *nb, uint64_t row) {
+  if (!nb) return 0u;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(nb) + (row >> 3);
+  return *reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+}
+__device__ __forceinline__ bool rdec_null(uint32_t raw, uint64_t row) {
+  return (raw >> ((((row >> 3) & 3) << 3) + (row & 7))) & 1u;
+}
+__device__ __forceinline__ int64_t rdec_i(uint64_t raw, int32_t dt, uint64_t row) {
+  const int sh = (int)(((row * (uint64_t)rbytes(dt)) & 3) << 3);
+  switch (dt) {
+    case DT_I8: return (int8_t)(raw >> sh);
+    case DT_BOOL: return (uint8_t)(raw >> sh);
+    case DT_I16: return (int16_t)(raw >> sh);
+    case DT_I32: return (int32_t)(uint32_t)raw;
+    default: return (int64_t)raw;
+  }
+}
+__device__ __forceinline__ double rdec_f(uint64_t raw, int32_t dt) {
+  return dt == DT_F32 ? (double)__uint_as_float((uint32_t)raw) : bits_f64(raw);
+}
+__device__ __forceinline__ uint64_t rdec_code(uint64_t raw, int32_t dt, uint64_t row) {
+  if (dt_is_float(dt)) return code_f64_bits(f64_bits(rdec_f(raw, dt)));
+  return code_i64(rdec_i(raw, dt, row));
+}
+
+// One 64-row window (one row per lane, rows p + lane): raw words of the key,
+// the filter column (+ NULL word) and (write pass) the state columns.
+template <int NS>
+struct RWin {
+  uint64_t p;
+  bool ok;        // row < nrows
+  uint64_t kr, xr;
+  uint32_t xn;
+  uint64_t cr[NS > 0 ? NS : 1];
+  uint32_t cn[NS > 0 ? NS : 1];
+};
+
+template <int PASS, int NS>
+__device__ __forceinline__ void rload(const RScan &rs, uint64_t p, int lane, RWin<NS> &w) {
+  const uint64_t row = p + (uint64_t)lane;
+  w.p = p;
+  w.ok = row < rs.nrows;
+  w.kr = w.xr = 0;
+  w.xn = 0;
+#pragma unroll
+  for (int j = 0; j < (NS > 0 ? NS : 1); ++j) { w.cr[j] = 0; w.cn[j] = 0; }
+  if (w.ok) {
+    w.kr = rraw(rs.key, rs.key_dt, row);
+    w.xr = rraw(rs.pcol, rs.p_dt, row);
+    w.xn = rraw_null(rs.pnull, row);
+    if constexpr (PASS == 1) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        if (rs.scol[1 + j]) w.cr[j] = rraw(rs.scol[1 + j], rs.s_dt[1 + j], row);
+        w.cn[j] = rraw_null(rs.snull[1 + j], row);
+      }
+    }
+  }
+}
+
+// contribution of the window row to state s = 1 + j (0: the state's identity)
+template <int NS>
+__device__ __forceinline__ uint64_t rcontrib_w(const RScan &rs, const RWin<NS> &w, int j, uint64_t row) {
+  const int s = 1 + j;
+  const DState &st = rs.st[s];
+  if (rs.snull[s] && rdec_null(w.cn[j], row)) return 0;
+  if (st.kind == SK_COUNT) return 1;
+  Contrib k;
+  if (dt_is_float(rs.s_dt[s])) k = contrib_of(st, true, 0, rdec_f(w.cr[j], rs.s_dt[s]));
+  else k = contrib_of(st, false, rdec_i(w.cr[j], rs.s_dt[s], row), 0.0);
+  return k.skip ? 0 : k.a;
+}
+
 // PASS 0: groups per (chunk, query) into rs.counts.
 // PASS 1: the groups themselves, at rs.offsets.
 template <int PASS, int NSUM, int NMAX, int QPL>
@@ -203,29 +323,43 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
     }
   }
 
-  while (s < c1) {
-    const int64_t rk = rkey(rs, s);
+  // Windows of 64 rows, each loaded while the previous one is processed. A
+  // window's leading rows with the current run key are the run's segment.
+  uint64_t p = s;
+  RWin<NS> cur, nxt;
+  if (p < c1) rload<PASS, NS>(rs, p, lane, cur);
+  bool fresh = true;  // `cur` starts a new run
+  int64_t rk = 0;
+  while (true) {
+    if (fresh) {
+      if (p >= c1) break;  // runs starting at or after the chunk end belong to the next chunk
+      rk = (int64_t)rfirst64((uint64_t)rdec_i(cur.kr, rs.key_dt, cur.p + lane));
 #pragma unroll
-    for (int k = 0; k < QPL; ++k) {
-      acnt[k] = 0;
+      for (int k = 0; k < QPL; ++k) {
+        acnt[k] = 0;
 #pragma unroll
-      for (int j = 0; j < NSA; ++j) ast[k][j] = 0;
+        for (int j = 0; j < NSA; ++j) ast[k][j] = 0;
+      }
+      fresh = false;
     }
-    uint64_t p = s;
-    for (;;) {
-      const uint64_t row = p + lane;
-      const bool inr = row < rs.nrows && rkey(rs, row) == rk;
+    bool run_end;
+    {
+      const uint64_t row = cur.p + (uint64_t)lane;
+      const bool inr = cur.ok && rdec_i(cur.kr, rs.key_dt, row) == rk;
       const uint64_t m = __ballot(inr);
       const int n = (~m == 0ull) ? 64 : __builtin_ctzll(~m);  // the run's rows: a prefix of the window
-      if (n == 0) break;
+      run_end = n < 64;
+      const uint64_t pn = p + (uint64_t)n;
+      if (n > 0) rload<PASS, NS>(rs, pn, lane, nxt);
       const bool valid = lane < n;
+      if (n > 0) {
       // 1. bucket of the row's filter value
       uint32_t e = 0;
       if (valid) {
-        if (rnull(rs.pnull, row)) {
+        if (rs.pnull && rdec_null(cur.xn, row)) {
           e = nb + 1;
         } else {
-          const uint64_t x = rcode(rs.pcol, rs.p_dt, row);
+          const uint64_t x = rdec_code(cur.xr, rs.p_dt, row);
           uint32_t f = 0;
           for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)
             if (s_bounds[f + step - 1] <= x) f += step;
@@ -249,12 +383,7 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
           v[g] = (PASS == 0) ? t + c : t;  // inclusive (count pass) / exclusive (scatter turns it inclusive)
           t += c;
         }
-        uint32_t inc = t;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const uint32_t y = __shfl_up(inc, off);
-          if (lane >= off) inc += y;
-        }
+        const uint32_t inc = wave_scan_u32(t);
         const uint32_t ex = inc - t;
         c4[0] = make_uint4(v[0] + ex, v[1] + ex, v[2] + ex, v[3] + ex);
         c4[1] = make_uint4(v[4] + ex, v[5] + ex, v[6] + ex, v[7] + ex);
@@ -265,30 +394,32 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
         if (valid) {
           const uint32_t pos = atomicAdd(&cnt[e], 1u);
 #pragma unroll
-          for (int j = 0; j < NSUM; ++j) s_sum[wave][j][pos] = rcontrib(rs, 1 + j, row);
+          for (int j = 0; j < NSUM; ++j) s_sum[wave][j][pos] = rcontrib_w<NS>(rs, cur, j, row);
 #pragma unroll
-          for (int j = 0; j < NMAX; ++j) s_tab[wave][j][0][pos] = rcontrib(rs, 1 + NSUM + j, row);
+          for (int j = 0; j < NMAX; ++j) s_tab[wave][j][0][pos] = rcontrib_w<NS>(rs, cur, NSUM + j, row);
+        } else {
+          // positions n..63 of level 0 hold the identity (the sorted rows fill 0..n-1)
+#pragma unroll
+          for (int j = 0; j < NMAX; ++j) s_tab[wave][j][0][lane] = 0ull;
         }
         wave_lds_sync();
         // 3. prefix sums and sparse max tables over the sorted order
 #pragma unroll
         for (int j = 0; j < NSUM; ++j) {
-          uint64_t x = valid ? s_sum[wave][j][lane] : 0ull;
-#pragma unroll
-          for (int off = 1; off < 64; off <<= 1) {
-            const uint64_t y = __shfl_up(x, off);
-            if (lane >= off) x += y;
-          }
+          const uint64_t x = wave_scan_u64(valid ? s_sum[wave][j][lane] : 0ull);
           if (valid) s_sum[wave][j][lane] = x;
         }
 #pragma unroll
         for (int j = 0; j < NMAX; ++j) {
-          uint64_t t = valid ? s_tab[wave][j][0][lane] : 0ull;
+          // level l at i = max of level l-1 at i and i + 2^(l-1) (lanes past 63 keep their value)
+          uint64_t t = s_tab[wave][j][0][lane];
 #pragma unroll
           for (int l = 1; l < 6; ++l) {
-            const uint64_t u = __shfl_down(t, 1 << (l - 1));  // lanes past 63 read their own value
+            const int src = lane + (1 << (l - 1));
+            const uint64_t u = s_tab[wave][j][l - 1][src < 64 ? src : lane];
             t = umax64(t, u);
             s_tab[wave][j][l][lane] = t;
+            wave_lds_sync();
           }
         }
         wave_lds_sync();
@@ -316,36 +447,41 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
         }
       }
       wave_lds_sync();
-      p += (uint64_t)n;
-      if (n < 64) break;
-    }
-    // the run is complete: one group per query that selected a row of it
-#pragma unroll
-    for (int k = 0; k < QPL; ++k) {
-      const bool has = qon[k] && acnt[k] != 0;
-      if constexpr (PASS == 0) {
-        groups[k] += has ? 1u : 0u;
-      } else {
-        // records of this run: consecutive in the chunk's staging range
-        const uint64_t m = __ballot(has);
-        if (has) {
-          const uint64_t i = sbase + sdone + (uint64_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-          if (i < send && opos[k] < ocap[k]) {
-            rs.stg_q[i] = (uint16_t)(k * 64 + lane);
-            rs.stg_key[i] = rk;
-            rs.stg_row[i] = roff[k] + opos[k];
-            rs.stg_val[0][i] = acnt[k];
-#pragma unroll
-            for (int j = 0; j < NS; ++j) rs.stg_val[1 + j][i] = ast[k][j];
-          }
-          ++opos[k];
-          nsel[k] += acnt[k];
-        }
-        sdone += (uint64_t)__builtin_popcountll(m);
+      }  // n > 0
+      if (n > 0) {
+        p = pn;
+        cur = nxt;
       }
     }
-    s = p;
+    if (run_end) {
+      // the run is complete: one group per query that selected a row of it
+#pragma unroll
+      for (int k = 0; k < QPL; ++k) {
+        const bool has = qon[k] && acnt[k] != 0;
+        if constexpr (PASS == 0) {
+          groups[k] += has ? 1u : 0u;
+        } else {
+          // records of this run: consecutive in the chunk's staging range
+          const uint64_t m = __ballot(has);
+          if (has) {
+            const uint64_t i = sbase + sdone + (uint64_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (i < send && opos[k] < ocap[k]) {
+              rs.stg_q[i] = (uint16_t)(k * 64 + lane);
+              rs.stg_key[i] = rk;
+              rs.stg_row[i] = roff[k] + opos[k];
+              rs.stg_val[0][i] = acnt[k];
+#pragma unroll
+              for (int j = 0; j < NS; ++j) rs.stg_val[1 + j][i] = ast[k][j];
+            }
+            ++opos[k];
+            nsel[k] += acnt[k];
+          }
+          sdone += (uint64_t)__builtin_popcountll(m);
+        }
+      }
+      fresh = true;
+    }
   }
   uint32_t tot = 0;
 #pragma unroll
@@ -374,6 +510,7 @@ static constexpr int R_PBLOCK = 512;
 template <int NS, int R_WIN>
 __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   constexpr int NV = NS + 1;  // count + states
+  constexpr int PER = R_WIN / R_PBLOCK;
   extern __shared__ __align__(16) uint8_t s_dyn[];
   uint64_t *s_key = reinterpret_cast<uint64_t *>(s_dyn);  // [R_WIN]
   uint64_t *s_row = s_key + R_WIN;                           // [R_WIN]
@@ -386,31 +523,37 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   const uint64_t c = blockIdx.x;
   const uint64_t b0 = rs.stg_base[c], b1 = rs.stg_base[c + 1];
   const uint32_t nq = (uint32_t)rs.nq;
-  constexpr int PER = R_WIN / R_PBLOCK;
-  for (uint64_t w0 = b0; w0 < b1; w0 += R_WIN) {
-    const int n = (int)((b1 - w0 < (uint64_t)R_WIN) ? (b1 - w0) : (uint64_t)R_WIN);
-    if (tid < R_MAX_QUERIES) s_cnt[tid] = 0;
-    __syncthreads();
-    uint32_t q[PER];
+  // the records of one window, one window ahead (loads in flight while the
+  // previous window is sorted and written)
+  struct Rec { uint32_t q; uint64_t key, row, v[NV]; };
+  Rec cur[PER], nxt[PER];
+  auto load = [&](uint64_t w0, Rec (&r)[PER]) {
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      const int i = tid + t * R_PBLOCK;
-      q[t] = 0;
-      if (i < n) {
-        const uint32_t x = rs.stg_q[w0 + i];
-        q[t] = x < nq ? x : 0u;
-        atomicAdd(&s_cnt[q[t]], 1u);
+      const uint64_t i = w0 + (uint64_t)(tid + t * R_PBLOCK);
+      r[t].q = 0xFFFFu;
+      if (i < b1) {
+        r[t].q = rs.stg_q[i];
+        r[t].key = (uint64_t)rs.stg_key[i];
+        r[t].row = rs.stg_row[i];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) r[t].v[k] = rs.stg_val[k][i];
       }
     }
+  };
+  if (b0 < b1) load(b0, cur);
+  for (uint64_t w0 = b0; w0 < b1; w0 += R_WIN) {
+    const int n = (int)((b1 - w0 < (uint64_t)R_WIN) ? (b1 - w0) : (uint64_t)R_WIN);
+    if (w0 + R_WIN < b1) load(w0 + R_WIN, nxt);
+    if (tid < R_MAX_QUERIES) s_cnt[tid] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PER; ++t)
+      if (cur[t].q != 0xFFFFu) atomicAdd(&s_cnt[cur[t].q < nq ? cur[t].q : 0u], 1u);
     __syncthreads();
     {  // exclusive scan over the queries (thread t < R_MAX_QUERIES owns query t)
       const uint32_t v = tid < R_MAX_QUERIES ? s_cnt[tid] : 0u;
-      uint32_t inc = v;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(inc, off);
-        if (lane >= off) inc += y;
-      }
+      const uint32_t inc = wave_scan_u32(v);
       if (lane == 63) s_wsum[wave] = inc;
       __syncthreads();
       uint32_t base = 0;
@@ -420,16 +563,14 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      const int i = tid + t * R_PBLOCK;
-      if (i < n) {
-        const uint64_t r = w0 + (uint64_t)i;
-        const uint32_t slot = atomicAdd(&s_cnt[q[t]], 1u);
-        s_qs[slot] = (uint16_t)q[t];
-        s_key[slot] = (uint64_t)rs.stg_key[r];
-        s_row[slot] = rs.stg_row[r];
+      if (cur[t].q == 0xFFFFu) continue;
+      const uint32_t q = cur[t].q < nq ? cur[t].q : 0u;
+      const uint32_t slot = atomicAdd(&s_cnt[q], 1u);
+      s_qs[slot] = (uint16_t)q;
+      s_key[slot] = cur[t].key;
+      s_row[slot] = cur[t].row;
 #pragma unroll
-        for (int k = 0; k < NV; ++k) s_val[k * R_WIN + slot] = rs.stg_val[k][r];
-      }
+      for (int k = 0; k < NV; ++k) s_val[k * R_WIN + slot] = cur[t].v[k];
     }
     __syncthreads();
 #pragma unroll
@@ -447,6 +588,8 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
       remit<NSA>(rs, qq, g - ro, g, (int64_t)s_key[j], s_val[j], st);
     }
     __syncthreads();
+#pragma unroll
+    for (int t = 0; t < PER; ++t) cur[t] = nxt[t];
   }
 }
 

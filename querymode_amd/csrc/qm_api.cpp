@@ -7,6 +7,7 @@
 // with every stage on the device; the host only binds names, picks the
 // group-by strategy from column statistics and sizes the device tables.
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -963,6 +964,44 @@ DevBuf select_rows(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, uint64_t lim
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+namespace {
+// The planner's tuning knobs: name, environment override read at qm_init,
+// range (values are clamped), setter and getter.
+struct KnobDef {
+  const char *name, *env;
+  double lo, hi;
+  void (*set)(qm_ctx::Tune &, double);
+  double (*get)(const qm_ctx::Tune &);
+};
+const KnobDef kKnobs[] = {
+    {"scatter_rpb", "QM_TUNE_SCATTER_RPB", 0, 64, [](qm_ctx::Tune &t, double v) { t.scatter_rpb = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.scatter_rpb; }},
+    {"part_load", "QM_TUNE_PART_LOAD", 0.1, 0.9, [](qm_ctx::Tune &t, double v) { t.part_load = v; },
+     [](const qm_ctx::Tune &t) { return t.part_load; }},
+    {"hot", "QM_TUNE_HOT", 0, 1, [](qm_ctx::Tune &t, double v) { t.hot = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.hot; }},
+    {"qp_min", "QM_TUNE_QP_MIN", 1, 1 << 20, [](qm_ctx::Tune &t, double v) { t.qp_min = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.qp_min; }},
+    {"runs_min_q", "QM_TUNE_RUNS_MIN_Q", 1, 1 << 20, [](qm_ctx::Tune &t, double v) { t.runs_min_q = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.runs_min_q; }},
+    {"runs_min_len", "QM_TUNE_RUNS_MIN_LEN", 0, 1e12, [](qm_ctx::Tune &t, double v) { t.runs_min_len = v; },
+     [](const qm_ctx::Tune &t) { return t.runs_min_len; }},
+    {"runs_chunks_per_cu", "QM_TUNE_RUNS_CHUNKS", 1, 1024,
+     [](qm_ctx::Tune &t, double v) { t.runs_chunks_per_cu = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.runs_chunks_per_cu; }},
+    {"runs_win", "QM_TUNE_RUNS_WIN", 1024, 2048, [](qm_ctx::Tune &t, double v) { t.runs_win = v >= 2048 ? 2048 : 1024; },
+     [](const qm_ctx::Tune &t) { return (double)t.runs_win; }},
+    {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
+};
+const KnobDef *find_knob(const char *name) {
+  if (!name) return nullptr;
+  for (const KnobDef &k : kKnobs)
+    if (!strcmp(k.name, name)) return &k;
+  return nullptr;
+}
+}  // namespace
+
 extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   API_BEGIN
   if (!out) fail(QM_ERR_INVALID_ARG, "null out");
@@ -983,15 +1022,8 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(tot * 0.9);
   c->num_cus = prop.multiProcessorCount;
   c->name = prop.gcnArchName;
-  if (const char *e = getenv("QM_TUNE_SCATTER_RPB")) c->tune.scatter_rpb = std::max(0, atoi(e));
-  if (const char *e = getenv("QM_TUNE_PART_LOAD")) c->tune.part_load = std::min(0.9, std::max(0.1, atof(e)));
-  if (const char *e = getenv("QM_TUNE_HOT")) c->tune.hot = atoi(e) != 0;
-  if (const char *e = getenv("QM_TUNE_QP_MIN")) c->tune.qp_min = std::max(1, atoi(e));
-  if (const char *e = getenv("QM_TUNE_RUNS_MIN_Q")) c->tune.runs_min_q = std::max(1, atoi(e));
-  if (const char *e = getenv("QM_TUNE_RUNS_MIN_LEN")) c->tune.runs_min_len = std::max(0.0, atof(e));
-  if (const char *e = getenv("QM_TUNE_RUNS_CHUNKS")) c->tune.runs_chunks_per_cu = std::max(1, atoi(e));
-  if (const char *e = getenv("QM_TUNE_RUNS_WIN")) c->tune.runs_win = atoi(e) >= 2048 ? 2048 : 1024;
-  if (const char *e = getenv("QM_TUNE_RUNS_LDS_PAD")) c->tune.runs_lds_pad = std::min(96 * 1024, std::max(0, atoi(e)));
+  for (const KnobDef &k : kKnobs)
+    if (const char *e = getenv(k.env)) k.set(c->tune, std::min(k.hi, std::max(k.lo, atof(e))));
   HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPX(hipEventCreate(&c->ev0));
   HIPX(hipEventCreate(&c->ev1));
@@ -2046,7 +2078,8 @@ bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan 
 }
 
 // Count pass, offsets, then the write pass straight into each query's result
-// columns (ascending key order). scan_ms: the longest of its launches so far.
+// columns (ascending key order). scan_ms: the longest of its launches that
+// read the table so far.
 void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &rp, std::vector<qm_result *> &out,
               float &scan_ms) {
   const size_t n = L.sorted.size();
@@ -2200,10 +2233,9 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     HIPX(qmk_runs_place(&rs, ctx->tune.runs_win, ctx->stream));
     HIPX(hipEventRecord(ctx->evk2, ctx->stream));
     d2h(ctx, sel.data(), selected.p, n);
-    float mw = 0, mp = 0;
+    float mw = 0;
     HIPX(hipEventElapsedTime(&mw, ctx->evk0, ctx->evk1));
-    HIPX(hipEventElapsedTime(&mp, ctx->evk1, ctx->evk2));
-    scan_ms = std::max(scan_ms, std::max(mw, mp));
+    scan_ms = std::max(scan_ms, mw);  // the place pass reads no table column
   }
   DQuery noq;
   memset(&noq, 0, sizeof noq);
@@ -2640,6 +2672,27 @@ extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right,
     if (pl.has_sort && pl.limit < 0) b += n * 24;  // radix sort keys + values
   }
   *bytes = b;
+  API_END
+}
+
+extern "C" int qm_tune_set(qm_ctx *ctx, const char *name, double value) {
+  API_BEGIN
+  if (!ctx) fail(QM_ERR_INVALID_ARG, "null context");
+  const KnobDef *k = find_knob(name);
+  if (!k) fail(QM_ERR_INVALID_ARG, "unknown tuning knob: %s", name ? name : "(null)");
+  if (std::isnan(value)) fail(QM_ERR_INVALID_ARG, "knob %s: NaN", name);
+  std::lock_guard<std::mutex> g(ctx->mu);
+  k->set(ctx->tune, std::min(k->hi, std::max(k->lo, value)));
+  API_END
+}
+
+extern "C" int qm_tune_get(qm_ctx *ctx, const char *name, double *value) {
+  API_BEGIN
+  if (!ctx || !value) fail(QM_ERR_INVALID_ARG, "null argument");
+  const KnobDef *k = find_knob(name);
+  if (!k) fail(QM_ERR_INVALID_ARG, "unknown tuning knob: %s", name ? name : "(null)");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  *value = k->get(ctx->tune);
   API_END
 }
 
