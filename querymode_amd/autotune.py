@@ -29,6 +29,7 @@ from .plan import QueryDescriptor, encode_plan
 # candidate values per knob (the first pass starts from the library defaults)
 BATCH_SPACE: dict[str, list[float]] = {
     "runs_chunks_per_cu": [8, 16, 32, 64, 128],
+    "runs_qblock": [0, 16, 32, 64],
     "runs_win": [1024, 2048],
     "runs_min_len": [4, 8, 16, 32],
 }

@@ -290,7 +290,8 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
 
   bool qon[QPL];
   uint32_t elo[QPL], ehi[QPL], groups[QPL];
-  uint64_t opos[QPL], ocap[QPL], roff[QPL], nsel[QPL], acnt[QPL], ast[QPL][NSA];
+  uint32_t opos[QPL], nsel[QPL];
+  uint64_t acnt[QPL], ast[QPL][NSA];
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
     const int q = k * 64 + lane;
@@ -300,14 +301,21 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
     ehi[k] = w >> 16;
     groups[k] = 0;
     nsel[k] = 0;
-    opos[k] = (PASS == 1 && qon[k]) ? rs.offsets[chunk * (uint64_t)rs.nq + q] : 0;
-    ocap[k] = (PASS == 1 && qon[k]) ? rs.totals[q] : 0;
-    roff[k] = (PASS == 1 && qon[k]) ? rs.row_off[q] : 0;
+    opos[k] = (PASS == 1 && qon[k]) ? (uint32_t)rs.offsets[chunk * (uint64_t)rs.nq + q] : 0u;
   }
 
-  const uint64_t sbase = PASS == 1 ? rs.stg_base[chunk] : 0;
-  const uint64_t send = PASS == 1 ? rs.stg_base[chunk + 1] : 0;
-  uint64_t sdone = 0;
+  // staging range of this lane's query block in each slot k (the 16 lanes
+  // of a DPP row share one block)
+  const int qbw = rs.qbw;  // 0: one block (every slot of the wave appends to one range)
+  uint64_t sbase[QPL];
+  uint32_t sdone[QPL];
+#pragma unroll
+  for (int k = 0; k < QPL; ++k) {
+    const int qb = qbw ? (k * 64 + lane) / qbw : 0;
+    const bool inb = PASS == 1 && qb < rs.nqb;
+    sbase[k] = inb ? rs.stg_base[chunk * (uint64_t)rs.nqb + qb] : 0;
+    sdone[k] = 0;
+  }
   const uint64_t c0 = chunk * rs.chunk_rows;
   const uint64_t c1 = (c0 + rs.chunk_rows < rs.nrows) ? c0 + rs.chunk_rows : rs.nrows;
   uint64_t s = c0;
@@ -461,42 +469,59 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
         if constexpr (PASS == 0) {
           groups[k] += has ? 1u : 0u;
         } else {
-          // records of this run: consecutive in the chunk's staging range
+          // records of this run: consecutive in the (chunk, query block) range
           const uint64_t m = __ballot(has);
+          // this lane's block within the wave: all 64 lanes (qbw 0 or 64) or a row of qbw lanes
+          const int bw = (qbw == 0 || qbw >= 64) ? 64 : qbw;
+          const uint64_t bmask = bw == 64 ? ~0ull : ((1ull << bw) - 1ull);
+          const int b0 = lane & ~(bw - 1);
+          const uint64_t gm = (m >> b0) & bmask;
+          const uint32_t nrec = (uint32_t)__builtin_popcountll(gm);
           if (has) {
-            const uint64_t i = sbase + sdone + (uint64_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (i < send && opos[k] < ocap[k]) {
+            const uint64_t below = (lane - b0) ? (gm & ((1ull << (lane - b0)) - 1ull)) : 0ull;
+            const uint64_t i = sbase[k] + sdone[k] + (uint64_t)__builtin_popcountll(below);
+            if (i < rs.stg_total) {  // the count pass sized every range
               rs.stg_q[i] = (uint16_t)(k * 64 + lane);
               rs.stg_key[i] = rk;
-              rs.stg_row[i] = roff[k] + opos[k];
+              rs.stg_pos[i] = opos[k];
               rs.stg_val[0][i] = acnt[k];
 #pragma unroll
               for (int j = 0; j < NS; ++j) rs.stg_val[1 + j][i] = ast[k][j];
             }
             ++opos[k];
-            nsel[k] += acnt[k];
+            nsel[k] += (uint32_t)acnt[k];
           }
-          sdone += (uint64_t)__builtin_popcountll(m);
+          if (qbw == 0) {
+#pragma unroll
+            for (int k2 = 0; k2 < QPL; ++k2) sdone[k2] += nrec;  // one range for every slot
+          } else {
+            sdone[k] += nrec;
+          }
         }
       }
       fresh = true;
     }
   }
-  uint32_t tot = 0;
+  uint64_t gsum = 0;
 #pragma unroll
   for (int k = 0; k < QPL; ++k) {
-    if (!qon[k]) continue;
     const int q = k * 64 + lane;
-    tot += groups[k];
-    if constexpr (PASS == 0) rs.counts[chunk * (uint64_t)rs.nq + q] = groups[k];
-    else if (nsel[k]) atomicAdd((unsigned long long *)&rs.selected[q], (unsigned long long)nsel[k]);
-  }
-  if constexpr (PASS == 0) {
-    uint64_t t = tot;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
-    if (lane == 0) rs.chunk_groups[chunk] = t;
+    if constexpr (PASS == 0) {
+      if (qon[k]) rs.counts[chunk * (uint64_t)rs.nq + q] = groups[k];
+      // groups of the query block: qbw lanes (qbw 0: accumulated over the wave and its slots)
+      const int bw = (qbw == 0 || qbw >= 64) ? 64 : qbw;
+      uint32_t t = qon[k] ? groups[k] : 0u;
+      for (int off = bw >> 1; off > 0; off >>= 1) t += __shfl_xor(t, off, bw);
+      if (qbw == 0) {
+        gsum += t;
+        if (k == QPL - 1 && lane == 0) rs.chunk_groups[chunk] = gsum;
+      } else {
+        const int qb = (k * 64 + lane) / qbw;
+        if ((lane & (bw - 1)) == 0 && qb < rs.nqb) rs.chunk_groups[chunk * (uint64_t)rs.nqb + qb] = t;
+      }
+    } else {
+      if (qon[k] && nsel[k]) atomicAdd((unsigned long long *)&rs.selected[q], (unsigned long long)nsel[k]);
+    }
   }
 }
 
@@ -518,14 +543,21 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   uint16_t *s_qs = reinterpret_cast<uint16_t *>(s_val + NV * R_WIN);  // query of each slot
   __shared__ uint32_t s_cnt[R_MAX_QUERIES];
   __shared__ uint32_t s_wsum[R_PBLOCK / 64];
+  __shared__ uint64_t s_roff[R_MAX_QUERIES], s_rend[R_MAX_QUERIES];  // each query's result rows [roff, rend)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const uint64_t c = blockIdx.x;
-  const uint64_t b0 = rs.stg_base[c], b1 = rs.stg_base[c + 1];
+  // workgroups in query-block-major order: the ones running together write
+  // the columns of the same few queries
+  const uint64_t qb = blockIdx.x / rs.nchunks, c = blockIdx.x % rs.nchunks;
+  const uint64_t b0 = rs.stg_base[c * rs.nqb + qb], b1 = rs.stg_base[c * rs.nqb + qb + 1];
   const uint32_t nq = (uint32_t)rs.nq;
+  if (tid < R_MAX_QUERIES) {
+    s_roff[tid] = tid < (int)nq ? rs.row_off[tid] : 0;
+    s_rend[tid] = tid < (int)nq ? rs.row_off[tid] + rs.totals[tid] : 0;
+  }
   // the records of one window, one window ahead (loads in flight while the
   // previous window is sorted and written)
-  struct Rec { uint32_t q; uint64_t key, row, v[NV]; };
+  struct Rec { uint32_t q, pos; uint64_t key, v[NV]; };
   Rec cur[PER], nxt[PER];
   auto load = [&](uint64_t w0, Rec (&r)[PER]) {
 #pragma unroll
@@ -535,7 +567,7 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
       if (i < b1) {
         r[t].q = rs.stg_q[i];
         r[t].key = (uint64_t)rs.stg_key[i];
-        r[t].row = rs.stg_row[i];
+        r[t].pos = rs.stg_pos[i];
 #pragma unroll
         for (int k = 0; k < NV; ++k) r[t].v[k] = rs.stg_val[k][i];
       }
@@ -568,24 +600,71 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
       const uint32_t slot = atomicAdd(&s_cnt[q], 1u);
       s_qs[slot] = (uint16_t)q;
       s_key[slot] = cur[t].key;
-      s_row[slot] = cur[t].row;
+      s_row[slot] = cur[t].pos;
 #pragma unroll
       for (int k = 0; k < NV; ++k) s_val[k * R_WIN + slot] = cur[t].v[k];
     }
     __syncthreads();
+    // one output column at a time over this thread's slots (the output's
+    // kind and type are decided once per column, not per row)
+    int qq[PER];
+    uint64_t g[PER];
+    bool ok[PER];
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
       const int j = tid + t * R_PBLOCK;
-      if (j >= n) continue;
-      const int qq = s_qs[j];
-      const uint64_t g = s_row[j];
-      const uint64_t ro = rs.row_off[qq];
-      if (g < ro || g - ro >= rs.totals[qq]) continue;  // the write pass staged every record
-      constexpr int NSA = NS > 0 ? NS : 1;
-      uint64_t st[NSA];
+      ok[t] = j < n;
+      qq[t] = ok[t] ? s_qs[j] : 0;
+      const uint64_t pos = ok[t] ? s_row[j] : 0;  // row within the query's result
+      g[t] = s_roff[qq[t]] + pos;
+      ok[t] = ok[t] && g[t] < s_rend[qq[t]];  // the write pass staged every record
+    }
+    for (int o = 0; o < rs.nouts; ++o) {
+      const ROut ro = rs.outs[o];
+      void *dst = rs.out_base[o];
+      const int vs = ro.state, cs = ro.cnt_state;
 #pragma unroll
-      for (int k = 0; k < NSA; ++k) st[k] = k < NS ? s_val[(1 + k) * R_WIN + j] : 0ull;
-      remit<NSA>(rs, qq, g - ro, g, (int64_t)s_key[j], s_val[j], st);
+      for (int t = 0; t < PER; ++t) {
+        if (!ok[t]) continue;
+        const int j = tid + t * R_PBLOCK;
+        const uint64_t cnt = cs < 0 ? 0 : s_val[cs * R_WIN + j];
+        const uint64_t v = s_val[vs * R_WIN + j];
+        const uint64_t gg = g[t];
+        bool isnull = false;
+        switch (ro.kind) {
+          case OK_KEY: rstore_i(dst, ro.out_dtype, gg, (int64_t)s_key[j]); break;
+          case OK_COUNT: static_cast<int64_t *>(dst)[gg] = (int64_t)cnt; break;
+          case OK_SUM_I64:
+            isnull = cnt == 0;
+            static_cast<int64_t *>(dst)[gg] = isnull ? 0 : (int64_t)v;
+            break;
+          case OK_AVG_I64:
+            isnull = cnt == 0;
+            static_cast<double *>(dst)[gg] = isnull ? 0.0 : (double)(int64_t)v / (double)cnt;
+            break;
+          default: {  // OK_MIN / OK_MAX (the values finalize_slot derives)
+            const bool mn = ro.kind == OK_MIN;
+            if (cnt == 0) {
+              isnull = true;
+              if (ro.src_is_float) rstore_f(dst, ro.out_dtype, gg, 0.0);
+              else rstore_i(dst, ro.out_dtype, gg, 0);
+            } else if (ro.src_is_float) {
+              double d;
+              if (v == 0) d = mn ? bits_f64(0x7FF0000000000000ull) : bits_f64(0xFFF0000000000000ull);
+              else d = bits_f64(decode_f64_code(mn ? ~v : v));
+              rstore_f(dst, ro.out_dtype, gg, d);
+            } else {
+              rstore_i(dst, ro.out_dtype, gg, decode_i64(mn ? ~v : v));
+            }
+            break;
+          }
+        }
+        if (isnull) {
+          uint32_t *nbm = rs.out_nulls[(uint64_t)qq[t] * rs.nouts + o];
+          const uint64_t pos = gg - s_roff[qq[t]];
+          if (nbm) atomicOr(&nbm[pos >> 5], 1u << (pos & 31));
+        }
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -665,7 +744,9 @@ static bool runs_args_ok(const RScan *rs) {
          rs->bsteps >= 0 && rs->bsteps <= 9 && ((1 << rs->bsteps) - 1) >= rs->nbounds && rs->nchunks >= 1 &&
          rs->chunk_rows >= 1 && rs->nsum >= 0 && rs->nsum <= rs->nstates - 1 && rs->nstates >= 1 &&
          rs->nstates <= R_MAX_STATES && rs->nouts >= 1 && rs->nouts <= R_MAX_OUTS &&
-         (uint64_t)rs->nchunks * rs->chunk_rows >= rs->nrows;
+         (uint64_t)rs->nchunks * rs->chunk_rows >= rs->nrows &&
+         (rs->qbw == 0 || rs->qbw == 16 || rs->qbw == 32 || rs->qbw == 64) &&
+         rs->nqb == (rs->qbw ? (rs->nq + rs->qbw - 1) / rs->qbw : 1);
 }
 
 }  // namespace qm
@@ -724,5 +805,5 @@ extern "C" hipError_t qmk_runs_place(const RScan *rs, int win, hipStream_t st) {
   if (e != hipSuccess) return e;
   RScan a = *rs;
   void *args[] = {&a};
-  return hipLaunchKernel(fn, dim3(rs->nchunks), dim3(R_PBLOCK), args, lds, st);
+  return hipLaunchKernel(fn, dim3(rs->nchunks * (uint32_t)rs->nqb), dim3(R_PBLOCK), args, lds, st);
 }

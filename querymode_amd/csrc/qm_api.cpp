@@ -991,6 +991,9 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.runs_chunks_per_cu; }},
     {"runs_win", "QM_TUNE_RUNS_WIN", 1024, 2048, [](qm_ctx::Tune &t, double v) { t.runs_win = v >= 2048 ? 2048 : 1024; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_win; }},
+    {"runs_qblock", "QM_TUNE_RUNS_QBLOCK", 0, 64,
+     [](qm_ctx::Tune &t, double v) { t.runs_qblock = v >= 48 ? 64 : (v >= 24 ? 32 : (v >= 8 ? 16 : 0)); },
+     [](const qm_ctx::Tune &t) { return (double)t.runs_qblock; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
 };
@@ -1998,6 +2001,7 @@ struct RunsPlan {
 bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan &rp) {
   const size_t n = L.sorted.size();
   if (n == 0 || n > (size_t)R_MAX_QUERIES || (int)n < ctx->tune.runs_min_q || !L.groups.empty()) return false;
+  if (T->rows >= (1ull << 32)) return false;  // 32-bit positions and per-chunk counters
   const TCol *key = L.sorted_key;
   if (!key || !key->st.sorted || key->st.nulls || dt_float(key->dtype)) return false;
   if ((double)T->rows / std::max(1.0, key->st.distinct) < ctx->tune.runs_min_len) return false;
@@ -2113,8 +2117,11 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
   DevBuf dbounds(ctx, std::max<size_t>(padded.size(), 1) * 8), dqe(ctx, n * 4 + 64);
   DevBuf counts(ctx, (size_t)nchunks * n * 4 + 64), offs(ctx, (size_t)nchunks * n * 8 + 64);
   DevBuf totals(ctx, n * 8 + 64), selected(ctx, n * 8 + 64);
-  DevBuf cgroups(ctx, (size_t)nchunks * 8 + 64), sbase(ctx, ((size_t)nchunks + 1) * 8 + 64);
-  DevBuf stmp(ctx, qmk_exscan_tmp_words(nchunks) * 8 + 64);
+  const int qbw = ctx->tune.runs_qblock;  // 0, 16, 32 or 64
+  const uint32_t nqb = qbw ? (uint32_t)((n + qbw - 1) / qbw) : 1u;
+  const uint64_t nseg = (uint64_t)nchunks * nqb;  // staging ranges: (chunk, query block)
+  DevBuf cgroups(ctx, nseg * 8 + 64), sbase(ctx, (nseg + 1) * 8 + 64);
+  DevBuf stmp(ctx, qmk_exscan_tmp_words(nseg) * 8 + 64);
   if (!padded.empty()) h2d(ctx, dbounds.p, padded.data(), padded.size());
   h2d(ctx, dqe.p, qe.data(), n);
   zero(ctx, selected.p, n * 8 + 64);
@@ -2152,6 +2159,8 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
   rs.nouts = rp.nouts;
   for (int j = 0; j < rp.nouts; ++j) rs.outs[j] = rp.outs[j];
   rs.selected = selected.as<uint64_t>();
+  rs.nqb = (int)nqb;
+  rs.qbw = qbw;
   rs.chunk_groups = cgroups.as<uint64_t>();
   rs.stg_base = sbase.as<uint64_t>();
 
@@ -2160,7 +2169,7 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     HIPX(hipEventRecord(ctx->evk0, ctx->stream));
     HIPX(qmk_runs_count(&rs, ctx->stream));
     HIPX(qmk_runs_offsets(rs.counts, nchunks, (uint32_t)n, offs.as<uint64_t>(), totals.as<uint64_t>(), ctx->stream));
-    HIPX(qmk_exscan_u64(cgroups.as<uint64_t>(), nchunks, sbase.as<uint64_t>(), totals.as<uint64_t>() + n,
+    HIPX(qmk_exscan_u64(cgroups.as<uint64_t>(), nseg, sbase.as<uint64_t>(), totals.as<uint64_t>() + n,
                         stmp.as<uint64_t>(), ctx->stream));
     HIPX(hipEventRecord(ctx->evk1, ctx->stream));
     d2h(ctx, tot.data(), totals.p, n + 1);
@@ -2168,8 +2177,8 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
     scan_ms = std::max(scan_ms, ms);
   }
-  // stg_base[nchunks] = all records (the exclusive scan leaves it to the total)
-  if (rows) h2d(ctx, sbase.as<uint64_t>() + nchunks, &tot[n], 1);
+  // stg_base[nseg] = all records (the exclusive scan leaves it to the total)
+  if (rows) h2d(ctx, sbase.as<uint64_t>() + nseg, &tot[n], 1);
   // result columns: one arena per output column shared by the launch's
   // results (query i's rows at row_off[i], padded like any result column);
   // the write pass fills them
@@ -2220,12 +2229,13 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     for (int j = 0; j < rp.nouts; ++j) rs.out_base[j] = arenas[j]->p;
     rs.row_off = droff.as<uint64_t>();
     rs.out_nulls = dnptr.as<uint32_t *const>();
-    DevBuf sq(ctx, all * 2 + 64), skey(ctx, all * 8 + 64), srow(ctx, all * 8 + 64);
+    DevBuf sq(ctx, all * 2 + 64), skey(ctx, all * 8 + 64), spos(ctx, all * 4 + 64);
     std::vector<DevBuf> sval;
     for (int j = 0; j < rs.nstates; ++j) sval.emplace_back(ctx, all * 8 + 64);
     rs.stg_q = sq.as<uint16_t>();
     rs.stg_key = skey.as<int64_t>();
-    rs.stg_row = srow.as<uint64_t>();
+    rs.stg_pos = spos.as<uint32_t>();
+    rs.stg_total = all;
     for (int j = 0; j < rs.nstates; ++j) rs.stg_val[j] = sval[j].as<uint64_t>();
     HIPX(hipEventRecord(ctx->evk0, ctx->stream));
     HIPX(qmk_runs_write(&rs, (uint32_t)ctx->tune.runs_lds_pad, ctx->stream));

@@ -296,11 +296,15 @@ struct RScan {
   // staging: the write pass appends each finished run's groups (one record
   // per query that selected a row of it) to its chunk's record range; the
   // place pass moves them into the result columns, query by query
-  uint64_t *chunk_groups;        // [nchunks] records per chunk (count pass)
-  const uint64_t *stg_base;      // [nchunks + 1] first record of each chunk
+  // records are grouped by (chunk, query block of qbw queries; qbw 0: all
+  // queries in one block), chunk-major
+  int32_t nqb, qbw;              // query blocks, queries per block (0, 16, 32 or 64)
+  uint64_t *chunk_groups;        // [nchunks][nqb] records per (chunk, query block) (count pass)
+  const uint64_t *stg_base;      // [nchunks * nqb + 1] first record of each (chunk, query block)
   uint16_t *stg_q;               // query
   int64_t *stg_key;              // run key
-  uint64_t *stg_row;             // global result row (row_off[q] + position)
+  uint32_t *stg_pos;             // row within the query's result
+  uint64_t stg_total;            // records staged (the staging arrays' length)
   uint64_t *stg_val[R_MAX_STATES]; // state values (0: the row count)
 };
 

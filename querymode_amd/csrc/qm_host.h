@@ -114,9 +114,10 @@ struct qm_ctx {
     int qp_min = 2;            // sorted-key launches with at least this many queries run query-parallel
     int runs_min_q = 2;        // ... and with at least this many (and runs this long) run-sorted
     double runs_min_len = 8.0; // mean rows per key run for the run-sorted kernel
-    int runs_chunks_per_cu = 16;
+    int runs_chunks_per_cu = 32;
     int runs_lds_pad = 0;      // extra LDS bytes per write-pass workgroup (caps waves in flight)
     int runs_win = 1024;       // place-pass window (records)
+    int runs_qblock = 64;      // queries per staging block (0 = one block per chunk; 16, 32, 64)
   } tune;
 
   void *alloc(size_t bytes);
