@@ -701,6 +701,126 @@ __global__ void __launch_bounds__(1024) runs_offsets_kernel(const uint32_t *__re
   }
 }
 
+// ---------------------------------------------------------------------------
+// Lean shared-interval pass (SScan). Each workgroup owns a contiguous row
+// range; a thread takes every R-th row of it (one row per lane, coalesced),
+// with the next row's raw words in flight while the current row is applied:
+// interval of the range value (binary search over the bounds in LDS), then
+// each state's contribution into the (interval, key) entry with LDS atomics.
+// Rows whose range value is NULL match no query of the group.
+template <int NSV>
+struct SRow {
+  bool ok;
+  uint64_t kr;
+  uint64_t vr[NSV];
+  uint32_t vn[NSV];
+};
+
+template <int NSV>
+__device__ __forceinline__ void sload(const SScan &ss, uint64_t row, uint64_t r1, SRow<NSV> &w) {
+  w.ok = row < r1;
+  w.kr = 0;
+#pragma unroll
+  for (int j = 0; j < NSV; ++j) { w.vr[j] = 0; w.vn[j] = 0; }
+  if (w.ok) {
+    if (ss.key) w.kr = rraw(ss.key, ss.key_dt, row);
+#pragma unroll
+    for (int j = 0; j < NSV; ++j) {
+      if (j >= ss.nv) break;
+      w.vr[j] = rraw(ss.vcol[j], ss.v_dt[j], row);
+      w.vn[j] = rraw_null(ss.vnull[j], row);
+    }
+  }
+}
+
+template <int NSV>
+__global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
+  extern __shared__ __align__(16) uint8_t s_dyn[];
+  const int nbp = (1 << ss.bsteps) - 1;
+  uint64_t *sb = reinterpret_cast<uint64_t *>(s_dyn);     // padded bounds
+  lds_u64 *tab = (lds_u64 *)(sb + nbp + 1);               // (interval, key) entries
+  const uint32_t W = (uint32_t)ss.entry_words;
+  const uint32_t nent = (uint32_t)(ss.nbounds + 1) * ss.dom;
+  for (int i = threadIdx.x; i < nbp; i += S_BLOCK) sb[i] = ss.bounds[i];
+  for (uint32_t i = threadIdx.x; i < nent * W; i += S_BLOCK) tab[i] = 0;
+  __syncthreads();
+  const uint64_t r0 = (uint64_t)blockIdx.x * ss.rows_per_block;
+  const uint64_t r1 = (r0 + ss.rows_per_block < ss.nrows) ? r0 + ss.rows_per_block : ss.nrows;
+  const uint32_t nb = (uint32_t)ss.nbounds;
+  SRow<NSV> cur, nxt;
+  uint64_t row = r0 + threadIdx.x;
+  sload<NSV>(ss, row, r1, cur);
+  for (; row < r1; row += S_BLOCK) {
+    sload<NSV>(ss, row + S_BLOCK, r1, nxt);
+    if (!(ss.vnull[0] && rdec_null(cur.vn[0], row))) {
+      const uint64_t x = rdec_code(cur.vr[0], ss.v_dt[0], row);
+      uint32_t f = 0;
+      for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)
+        if (sb[f + step - 1] <= x) f += step;
+      const uint32_t e = f < nb ? f : nb;
+      uint32_t d = 0;
+      if (ss.key) {
+        const int64_t kv = rdec_i(cur.kr, ss.key_dt, row) - ss.kmin;
+        d = (kv >= 0 && kv < (int64_t)ss.dom) ? (uint32_t)kv : 0u;  // the key's statistics bound it
+      }
+      lds_u64 *ent = tab + ((uint64_t)e * ss.dom + d) * W;
+#pragma unroll
+      for (int si = 0; si < B_MAX_SHARED_STATES; ++si) {
+        if (si >= ss.nstates) break;
+        const DState &st = ss.st[si];
+        const int vi = ss.st_v[si];
+        Contrib k;
+        k.a = 1; k.b = k.c = 0; k.skip = false;
+        if (vi >= 0) {
+          uint64_t raw = 0;
+          uint32_t rn = 0;
+          int32_t dt = 0;
+#pragma unroll
+          for (int j = 0; j < NSV; ++j)
+            if (vi == j) { raw = cur.vr[j]; rn = cur.vn[j]; dt = ss.v_dt[j]; }
+          if (ss.vnull[vi] && rdec_null(rn, row)) continue;
+          if (st.kind != SK_COUNT) {
+            if (dt_is_float(dt)) k = contrib_of(st, true, 0, rdec_f(raw, dt));
+            else k = contrib_of(st, false, rdec_i(raw, dt, row), 0.0);
+            if (k.skip) continue;
+          }
+        }
+        apply_lds(st, ent, k);
+      }
+    }
+    cur = nxt;
+  }
+  __syncthreads();
+  // merge into the global table: one entry per thread, its states in order
+  for (uint32_t en = threadIdx.x; en < nent; en += S_BLOCK) {
+    const lds_u64 *src = tab + (uint64_t)en * W;
+    if (src[ss.st[0].word] == 0) continue;  // state 0 is COUNT(*): no row reached the entry
+    uint64_t *dst = ss.table + (uint64_t)en * W;
+    for (int si = 0; si < ss.nstates; ++si) {
+      const DState &st = ss.st[si];
+      const uint32_t w = (uint32_t)st.word;
+      switch (st.kind) {
+        case SK_COUNT: case SK_SUM_I64: if (src[w]) g_atomic_add(dst + w, src[w]); break;
+        case SK_SUM_128:
+          g_add128(dst + w, src[w], src[w + 1]);
+          if (st.fixed && src[w + 2]) g_atomic_or(dst + w + 2, src[w + 2]);
+          break;
+        default: if (src[w]) g_atomic_max(dst + w, src[w]); break;
+      }
+    }
+  }
+}
+
+static const void *shared_fn(int nv) {
+  switch (nv) {
+    case 1: return (const void *)shared_kernel<1>;
+    case 2: return (const void *)shared_kernel<2>;
+    case 3: return (const void *)shared_kernel<3>;
+    case 4: return (const void *)shared_kernel<4>;
+  }
+  return nullptr;
+}
+
 #define R_FN(P, S, M)                                                                   \
   case (S) * 4 + (M):                                                                   \
     return qpl == 1 ? (const void *)runs_kernel<P, S, M, 1>                             \
@@ -806,4 +926,38 @@ extern "C" hipError_t qmk_runs_place(const RScan *rs, int win, hipStream_t st) {
   RScan a = *rs;
   void *args[] = {&a};
   return hipLaunchKernel(fn, dim3(rs->nchunks * (uint32_t)rs->nqb), dim3(R_PBLOCK), args, lds, st);
+}
+
+extern "C" size_t qmk_shared_lds_bytes(const SScan *ss) {
+  const uint64_t nbp = (1ull << ss->bsteps) - 1;
+  return (size_t)(nbp + 1 + (uint64_t)(ss->nbounds + 1) * ss->dom * (uint64_t)ss->entry_words) * 8;
+}
+
+extern "C" int qmk_shared_blocks_per_cu(const SScan *ss) {
+  const void *fn = shared_fn(ss->nv);
+  if (!fn) return 0;
+  const size_t lds = qmk_shared_lds_bytes(ss);
+  if (lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, S_BLOCK, lds) != hipSuccess) nb = 1;
+  return nb < 1 ? 1 : nb;
+}
+
+extern "C" hipError_t qmk_shared_scan(const SScan *ss, uint32_t grid, hipStream_t st) {
+  const void *fn = shared_fn(ss->nv);
+  if (!fn || grid == 0 || ss->nstates < 1 || ss->nstates > B_MAX_SHARED_STATES || ss->dom < 1 ||
+      ss->bsteps < 0 || ss->bsteps > 10 || ((1 << ss->bsteps) - 1) < ss->nbounds ||
+      (uint64_t)grid * ss->rows_per_block < ss->nrows)
+    return hipErrorInvalidValue;
+  for (int i = 0; i < ss->nstates; ++i)
+    if (ss->st_v[i] >= ss->nv) return hipErrorInvalidValue;
+  const size_t lds = qmk_shared_lds_bytes(ss);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  if (lds > 64 * 1024) {
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
+  SScan a = *ss;
+  void *args[] = {&a};
+  return hipLaunchKernel(fn, dim3(grid), dim3(S_BLOCK), args, lds, st);
 }

@@ -994,6 +994,10 @@ const KnobDef kKnobs[] = {
     {"runs_qblock", "QM_TUNE_RUNS_QBLOCK", 0, 64,
      [](qm_ctx::Tune &t, double v) { t.runs_qblock = v >= 48 ? 64 : (v >= 24 ? 32 : (v >= 8 ? 16 : 0)); },
      [](const qm_ctx::Tune &t) { return (double)t.runs_qblock; }},
+    {"shared_lean", "QM_TUNE_SHARED_LEAN", 0, 1, [](qm_ctx::Tune &t, double v) { t.shared_lean = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.shared_lean; }},
+    {"shared_waves", "QM_TUNE_SHARED_WAVES", 1, 64, [](qm_ctx::Tune &t, double v) { t.shared_waves = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.shared_waves; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
 };
@@ -1815,6 +1819,7 @@ struct BGroup {
   int64_t kmin = 0;
   DevBuf table, dbounds;
   int launch = -1;
+  bool lean = false;  // runs in its own lean pass (k_runs.hip shared_kernel)
 };
 
 // index of `s` (state of item `it`) in the group's state union
@@ -2260,6 +2265,60 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
   }
 }
 
+// One shared-interval group through the lean pass: bounds padded for the
+// branch-free search, the group's distinct value columns (the range column
+// first), one workgroup range of rows per block.
+void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms) {
+  SScan ss;
+  memset(&ss, 0, sizeof ss);
+  ss.nrows = T->rows;
+  std::vector<const TCol *> vc{g.col};
+  for (const TCol *t : g.st_t)
+    if (t && std::find(vc.begin(), vc.end(), t) == vc.end()) vc.push_back(t);
+  ss.nv = (int)vc.size();
+  for (size_t j = 0; j < vc.size(); ++j) {
+    ss.vcol[j] = vc[j]->d;
+    ss.vnull[j] = vc[j]->st.nulls ? vc[j]->dnull : nullptr;
+    ss.v_dt[j] = vc[j]->dtype;
+  }
+  if (g.key) {
+    ss.key = g.key->d;
+    ss.key_dt = g.key->dtype;
+  }
+  ss.dom = g.dom;
+  ss.kmin = g.kmin;
+  ss.nstates = (int)g.st.size();
+  for (size_t i = 0; i < g.st.size(); ++i) {
+    ss.st[i] = g.st[i];
+    ss.st_v[i] = g.st_t[i] ? (int8_t)(std::find(vc.begin(), vc.end(), g.st_t[i]) - vc.begin()) : (int8_t)-1;
+  }
+  ss.entry_words = g.entry_words;
+  const uint32_t nb = (uint32_t)g.bounds.size();
+  int steps = 0;
+  while (((1u << steps) - 1) < nb) ++steps;
+  std::vector<uint64_t> padded((1u << steps) - 1, ~0ull);
+  std::copy(g.bounds.begin(), g.bounds.end(), padded.begin());
+  DevBuf db(ctx, std::max<size_t>(padded.size(), 1) * 8);
+  if (!padded.empty()) h2d(ctx, db.p, padded.data(), padded.size());
+  ss.nbounds = (int)nb;
+  ss.bsteps = steps;
+  ss.bounds = db.as<uint64_t>();
+  ss.table = g.table.as<uint64_t>();
+  const int bpc = qmk_shared_blocks_per_cu(&ss);
+  uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)std::max(1, bpc) * (uint64_t)ctx->tune.shared_waves;
+  uint64_t rpb = (T->rows + grid - 1) / grid;
+  rpb = std::max<uint64_t>(S_BLOCK, (rpb + S_BLOCK - 1) / S_BLOCK * S_BLOCK);
+  grid = (T->rows + rpb - 1) / rpb;
+  ss.rows_per_block = rpb;
+  HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+  HIPX(qmk_shared_scan(&ss, (uint32_t)grid, ctx->stream));
+  HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+  HIPX(hipEventSynchronize(ctx->evk1));
+  float ms = 0;
+  HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+  scan_ms = std::max(scan_ms, ms);
+}
+
 // Run the fusable plans of `items` in fused passes; results by item idx.
 void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std::vector<qm_result *> &out) {
   Timer tm(ctx);
@@ -2308,6 +2367,17 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
                                     : (uint64_t)(std::upper_bound(b.begin(), b.end(), it->hi[0] + 1) - b.begin());
     }
   }
+  // groups whose value columns fit the lean pass run it (one group per pass)
+  for (auto &g : groups) {
+    std::vector<const TCol *> vc{g->col};
+    for (const TCol *t : g->st_t)
+      if (t && std::find(vc.begin(), vc.end(), t) == vc.end()) vc.push_back(t);
+    int steps = 0;
+    while (((1u << steps) - 1) < g->bounds.size()) ++steps;
+    const uint64_t lds = (((1ull << steps) - 1) + 1 + (g->bounds.size() + 1) * g->dom * (uint64_t)g->entry_words) * 8;
+    g->lean = ctx->tune.shared_lean && vc.size() <= (size_t)S_MAX_V && g->st.size() <= (size_t)B_MAX_SHARED_STATES &&
+              steps <= 10 && lds <= 96 * 1024;
+  }
   // launches: shared groups, then sorted-key queries (one key column per launch)
   std::vector<BLaunch> launches;
   auto lds_of = [&](const BLaunch &L, const BGroup *extra_g, size_t extra_sorted) {
@@ -2319,6 +2389,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     return w;
   };
   for (auto &g : groups) {
+    if (g->lean) continue;
     std::vector<const TCol *> need{g->col, g->key};
     for (const TCol *t : g->st_t) need.push_back(t);
     BLaunch *L = nullptr;
@@ -2369,10 +2440,16 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
   zero(ctx, status.p, n * 8 + 64);
   zero(ctx, selected.p, n * 8 + 64);
   std::vector<BItem *> order;  // finalisation order = status / selected index
+  for (auto &g : groups)  // lean groups run outside the launches
+    if (g->lean) for (BItem *it : g->items) order.push_back(it);
   for (auto &L : launches) {
     for (BGroup *g : L.groups) for (BItem *it : g->items) order.push_back(it);
     for (BItem *it : L.sorted) order.push_back(it);
   }
+  for (auto &g : groups)
+    for (BItem *it : g->items)
+      if (std::find(order.begin(), order.end(), it) == order.end())
+        fail(QM_ERR_QUERY_FAILED, "batched query %u has no pass", it->idx);
   for (BItem *it : order) {
     it->table = DevBuf(ctx, it->ab.table_words * 8);
     if (it->kind == BK_SORTED) zero(ctx, it->table.p, it->ab.table_words * 8);
@@ -2384,6 +2461,8 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     g->dbounds = DevBuf(ctx, std::max<size_t>(g->bounds.size(), 1) * 8);
     h2d(ctx, g->dbounds.p, g->bounds.data(), g->bounds.size());
   }
+  for (auto &g : groups)
+    if (g->lean && T->rows) run_shared_lean(ctx, T, *g, scan_ms_total);
   const uint64_t ntiles = std::max<uint64_t>(1, (T->rows + 511) / 512);
   std::vector<DevBuf> qbufs;
   for (auto &L : launches) {
@@ -2529,6 +2608,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
       memset(&c, 0, sizeof c);
       c.src = g->table.as<uint64_t>();
       c.dst = it->table.as<uint64_t>();
+      if (!c.src || !c.dst) fail(QM_ERR_QUERY_FAILED, "shared-interval merge without a table");
       size_t oi = std::find(order.begin(), order.end(), it) - order.begin();
       c.selected = selected.as<uint64_t>() + oi;
       c.dom = g->dom;
@@ -2613,7 +2693,7 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       it.plan = &pl;
       batch_classify(ctx, left, pl, it);
       try {
-        if (it.kind == BK_SORTED) {
+        if (it.kind == BK_SORTED || (it.kind == BK_SHARED && ctx->tune.shared_lean)) {
           std::vector<BItem *> items{&it};
           run_batch(ctx, left, items, results);
         } else {

@@ -308,4 +308,31 @@ struct RScan {
   uint64_t *stg_val[R_MAX_STATES]; // state values (0: the row count)
 };
 
+// ---- lean shared-interval pass: one group of queries with a range on one
+// common column and a small key domain; rows across lanes, raw column words
+// loaded one row step ahead, (interval, key) table in LDS merged into
+// `table` with atomics. Interval e of code x = #{bounds <= x}.
+static constexpr int S_MAX_V = 4;     // distinct value columns (0 = the range column)
+static constexpr int S_BLOCK = 256;
+struct SScan {
+  uint64_t nrows;
+  uint64_t rows_per_block;
+  const void *key;               // nullptr: no key (dom 1)
+  int32_t key_dt;
+  uint32_t dom;
+  int64_t kmin;
+  int32_t nv;
+  int32_t nstates;
+  const void *vcol[S_MAX_V];
+  const uint8_t *vnull[S_MAX_V];
+  int32_t v_dt[S_MAX_V];
+  DState st[B_MAX_SHARED_STATES];
+  int8_t st_v[B_MAX_SHARED_STATES];  // value column of each state (-1: COUNT(*))
+  int32_t entry_words;
+  int32_t nbounds, bsteps;           // bounds padded to 2^bsteps - 1 entries (all-ones)
+  int32_t _p0;
+  const uint64_t *bounds;
+  uint64_t *table;                   // (nbounds + 1) x dom x entry_words, zeroed
+};
+
 }  // namespace qm

@@ -137,6 +137,10 @@ hipError_t qmk_runs_offsets(const uint32_t *counts, uint32_t nchunks, uint32_t n
 int qmk_runs_supported(int nsum, int nmax);
 hipError_t qmk_runs_write(const qm::RScan *rs, uint32_t lds_pad, hipStream_t st);
 hipError_t qmk_runs_place(const qm::RScan *rs, int win, hipStream_t st);
+// lean shared-interval pass (one group per launch)
+size_t qmk_shared_lds_bytes(const qm::SScan *ss);
+int qmk_shared_blocks_per_cu(const qm::SScan *ss);
+hipError_t qmk_shared_scan(const qm::SScan *ss, uint32_t grid, hipStream_t st);
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);

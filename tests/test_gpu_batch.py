@@ -54,7 +54,8 @@ def test_batch_matches_oracle_and_solo(qm, kdom):
     for i, d in enumerate(descs[:24]):
         solo = qm.executor.execute(d)
         assert_groups_equal(batch[i].rows, solo.rows, d.groupBy, ctx=f"solo q{i}")
-        assert solo.stats["strategy"] in ("direct_lds", "sorted_runs", "direct_global"), solo.stats["strategy"]
+        assert solo.stats["strategy"] in ("shared_interval", "direct_lds", "sorted_runs", "direct_global"), \
+            solo.stats["strategy"]
 
 
 def test_batch_edge_queries(qm):

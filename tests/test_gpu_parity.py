@@ -30,14 +30,21 @@ def run_both(qm, desc, tables, nulls=None):
     return got, want
 
 
-def test_filter_groupby_small(qm):
-    _say("case filter_groupby_small n=1M")
+@pytest.mark.parametrize("lean", [1, 0])
+def test_filter_groupby_small(qm, lean):
+    # lean 1: the shared-interval pass (the default for a small key);
+    # lean 0: the fused scan with a replicated LDS table
+    _say(f"case filter_groupby_small n=1M lean={lean}")
     cols = datagen.filter_groupby_small(n=1_000_000, seed=0)
     d = QueryDescriptor("t", filters=[FilterOp("status", "eq", 1)], groupBy=["category"],
                         aggregates=[AggregateOp("sum", "amount", "total"), AggregateOp("count", "*")],
                         sortColumn="total", sortDirection="desc")
-    got, want = run_both(qm, d, {"t": cols})
-    assert got.stats["strategy"] == "direct_lds"
+    qm.device.tune_set("shared_lean", lean)
+    try:
+        got, want = run_both(qm, d, {"t": cols})
+    finally:
+        qm.device.tune_set("shared_lean", 1)
+    assert got.stats["strategy"] == ("shared_interval" if lean else "direct_lds")
     assert_rows_equal(got.rows, want, float_cols={"total"}, ctx="c1")
 
 
