@@ -821,6 +821,148 @@ static const void *shared_fn(int nv) {
   return nullptr;
 }
 
+// ---------------------------------------------------------------------------
+// One query grouped by a sorted key (S1Scan).
+struct S1Win {
+  uint64_t p;
+  bool ok;
+  uint64_t kr;
+  uint64_t pr[B_MAX_SLOTS];
+  uint32_t pn[B_MAX_SLOTS];
+  uint64_t sr[R_MAX_STATES];
+  uint32_t sn[R_MAX_STATES];
+};
+
+__device__ __forceinline__ void s1load(const S1Scan &q, uint64_t p, int lane, S1Win &w) {
+  const uint64_t row = p + (uint64_t)lane;
+  w.p = p;
+  w.ok = row < q.nrows;
+  w.kr = 0;
+#pragma unroll
+  for (int j = 0; j < B_MAX_SLOTS; ++j) { w.pr[j] = 0; w.pn[j] = 0; }
+#pragma unroll
+  for (int j = 0; j < R_MAX_STATES; ++j) { w.sr[j] = 0; w.sn[j] = 0; }
+  if (!w.ok) return;
+  w.kr = rraw(q.key, q.key_dt, row);
+#pragma unroll
+  for (int j = 0; j < B_MAX_SLOTS; ++j) {
+    if (j >= q.nslots) break;
+    w.pr[j] = rraw(q.pcol[j], q.p_dt[j], row);
+    w.pn[j] = rraw_null(q.pnull[j], row);
+  }
+#pragma unroll
+  for (int j = 1; j < R_MAX_STATES; ++j) {
+    if (j >= q.nstates) break;
+    if (q.scol[j]) w.sr[j] = rraw(q.scol[j], q.s_dt[j], row);
+    w.sn[j] = rraw_null(q.snull[j], row);
+  }
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t x) {
+  // inclusive max scan (out-of-row sources read 0, the identity), lane 63 holds the max
+  x = umax64(x, dpp64<0x111, 0xF>(x));
+  x = umax64(x, dpp64<0x112, 0xF>(x));
+  x = umax64(x, dpp64<0x114, 0xF>(x));
+  x = umax64(x, dpp64<0x118, 0xF>(x));
+  x = umax64(x, dpp64<0x142, 0xA>(x));
+  x = umax64(x, dpp64<0x143, 0xC>(x));
+  return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63) |
+         ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63) << 32);
+}
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t x) {
+  x = wave_scan_u64(x);
+  return (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 63) |
+         ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 63) << 32);
+}
+
+__global__ void __launch_bounds__(R_BLOCK) sorted1_kernel(S1Scan q) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint64_t chunk = (uint64_t)blockIdx.x * R_WAVES + wave;
+  if (chunk >= q.nchunks || q.nslots < 0) return;
+  const uint64_t c0 = chunk * q.chunk_rows;
+  const uint64_t c1 = (c0 + q.chunk_rows < q.nrows) ? c0 + q.chunk_rows : q.nrows;
+  uint64_t s = c0;
+  if (c0 > 0 && c0 < c1) {  // skip the run that started in the previous chunk
+    const int64_t prev = rdec_i(rraw(q.key, q.key_dt, c0 - 1), q.key_dt, c0 - 1);
+    for (;;) {
+      const uint64_t row = s + lane;
+      const bool diff = row >= q.nrows || rdec_i(rraw(q.key, q.key_dt, row), q.key_dt, row) != prev;
+      const uint64_t m = __ballot(diff);
+      if (m) { s += (uint64_t)__builtin_ctzll(m); break; }
+      s += 64;
+    }
+  }
+  const int W = q.entry_words;
+  uint64_t p = s, nsel = 0;
+  S1Win cur, nxt;
+  if (p < c1) s1load(q, p, lane, cur);
+  bool fresh = true;
+  int64_t rk = 0;
+  uint64_t acc[R_MAX_STATES];  // wave-uniform run accumulators
+  while (true) {
+    if (fresh) {
+      if (p >= c1) break;
+      rk = (int64_t)rfirst64((uint64_t)rdec_i(cur.kr, q.key_dt, cur.p + lane));
+#pragma unroll
+      for (int j = 0; j < R_MAX_STATES; ++j) acc[j] = 0;
+      fresh = false;
+    }
+    const uint64_t row = cur.p + (uint64_t)lane;
+    const bool inr = cur.ok && rdec_i(cur.kr, q.key_dt, row) == rk;
+    const uint64_t m = __ballot(inr);
+    const int n = (~m == 0ull) ? 64 : __builtin_ctzll(~m);
+    const bool run_end = n < 64;
+    if (n > 0) {
+      const uint64_t pn = p + (uint64_t)n;
+      s1load(q, pn, lane, nxt);
+      bool sel = lane < n;
+#pragma unroll
+      for (int j = 0; j < B_MAX_SLOTS; ++j) {
+        if (j >= q.nslots) break;
+        if (q.pnull[j] && rdec_null(cur.pn[j], row)) sel = false;
+        sel = sel && (rdec_code(cur.pr[j], q.p_dt[j], row) - q.lo[j]) <= q.span[j];
+      }
+      const uint64_t sm = __ballot(sel);
+      if (sm) {
+        acc[0] += (uint64_t)__builtin_popcountll(sm);
+#pragma unroll
+        for (int j = 1; j < R_MAX_STATES; ++j) {
+          if (j >= q.nstates) break;
+          const DState &st = q.st[j];
+          uint64_t v = 0;
+          if (sel && !(q.snull[j] && rdec_null(cur.sn[j], row))) {
+            if (st.kind == SK_COUNT) {
+              v = 1;
+            } else {
+              Contrib k;
+              if (dt_is_float(q.s_dt[j])) k = contrib_of(st, true, 0, rdec_f(cur.sr[j], q.s_dt[j]));
+              else k = contrib_of(st, false, rdec_i(cur.sr[j], q.s_dt[j], row), 0.0);
+              v = k.skip ? 0 : k.a;
+            }
+          }
+          if (st.kind == SK_MAX_CODE) acc[j] = umax64(acc[j], wave_max_u64(v));
+          else acc[j] += wave_sum_u64(v);
+        }
+      }
+      p = pn;
+      cur = nxt;
+    }
+    if (run_end) {
+      if (acc[0]) {
+        nsel += acc[0];
+        if (lane < q.nstates) {  // one state word per lane, plain stores (this wave owns the run)
+          uint64_t v = acc[0];
+#pragma unroll
+          for (int j = 1; j < R_MAX_STATES; ++j) v = lane == j ? acc[j] : v;
+          q.table[(uint64_t)(rk - q.kmin) * (uint64_t)W + (uint64_t)q.st[lane].word] = v;
+        }
+      }
+      fresh = true;
+    }
+  }
+  if (lane == 0 && nsel) atomicAdd((unsigned long long *)q.selected, (unsigned long long)nsel);
+}
+
 #define R_FN(P, S, M)                                                                   \
   case (S) * 4 + (M):                                                                   \
     return qpl == 1 ? (const void *)runs_kernel<P, S, M, 1>                             \
@@ -960,4 +1102,16 @@ extern "C" hipError_t qmk_shared_scan(const SScan *ss, uint32_t grid, hipStream_
   SScan a = *ss;
   void *args[] = {&a};
   return hipLaunchKernel(fn, dim3(grid), dim3(S_BLOCK), args, lds, st);
+}
+
+extern "C" hipError_t qmk_sorted1_scan(const S1Scan *q, hipStream_t st) {
+  if (q->nchunks < 1 || q->nstates < 1 || q->nstates > R_MAX_STATES || q->nslots > B_MAX_SLOTS ||
+      (uint64_t)q->nchunks * q->chunk_rows < q->nrows || q->entry_words < q->nstates)
+    return hipErrorInvalidValue;
+  for (int j = 0; j < q->nstates; ++j)
+    if (q->st[j].kind == SK_SUM_128 || q->st[j].word < 0 || q->st[j].word >= q->entry_words) return hipErrorInvalidValue;
+  S1Scan a = *q;
+  void *args[] = {&a};
+  return hipLaunchKernel((const void *)sorted1_kernel, dim3((q->nchunks + R_WAVES - 1) / R_WAVES), dim3(R_BLOCK),
+                         args, 0, st);
 }

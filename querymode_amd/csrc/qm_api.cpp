@@ -998,6 +998,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.shared_lean; }},
     {"shared_waves", "QM_TUNE_SHARED_WAVES", 1, 64, [](qm_ctx::Tune &t, double v) { t.shared_waves = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.shared_waves; }},
+    {"sorted1", "QM_TUNE_SORTED1", 0, 1, [](qm_ctx::Tune &t, double v) { t.sorted1 = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.sorted1; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
 };
@@ -2533,6 +2535,62 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     int wide = 0;
     for (const BQuery &Q : hq)
       for (int s2 = 0; s2 < Q.nstates; ++s2) if (Q.st[s2].kind == SK_SUM_128) wide = 1;
+    // one sorted-key query with long runs: the lean single-query pass
+    if (L.sorted.size() == 1 && !wide && ctx->tune.sorted1 && L.groups.empty()) {
+      BItem &it = *L.sorted[0];
+      const DQuery &q = it.ab.q;
+      bool ok = q.nstates <= R_MAX_STATES && it.nslots <= B_MAX_SLOTS &&
+                (double)T->rows / std::max(1.0, it.key_t->st.distinct) >= ctx->tune.runs_min_len;
+      for (int s2 = 0; s2 < q.nstates && ok; ++s2) {
+        const DState &d = q.states[s2];
+        ok = !d.product && (d.kind == SK_COUNT || d.kind == SK_SUM_I64 || d.kind == SK_MAX_CODE) &&
+             (s2 == 0 ? (d.kind == SK_COUNT && d.col_a < 0) : it.st_t[s2] != nullptr);
+      }
+      if (ok) {
+        S1Scan q1;
+        memset(&q1, 0, sizeof q1);
+        q1.nrows = T->rows;
+        const uint64_t target = (uint64_t)ctx->num_cus * (uint64_t)ctx->tune.runs_chunks_per_cu;
+        uint64_t cr = std::max<uint64_t>(1024, (T->rows + target - 1) / target);
+        cr = (cr + 63) / 64 * 64;
+        q1.chunk_rows = cr;
+        q1.nchunks = (uint32_t)std::max<uint64_t>(1, (T->rows + cr - 1) / cr);
+        q1.nslots = it.never ? -1 : it.nslots;
+        q1.key = it.key_t->d;
+        q1.key_dt = it.key_t->dtype;
+        q1.kmin = q.key.kmin[0];
+        for (int j = 0; j < it.nslots; ++j) {
+          q1.pcol[j] = it.slot_t[j]->d;
+          q1.pnull[j] = it.slot_t[j]->st.nulls ? it.slot_t[j]->dnull : nullptr;
+          q1.p_dt[j] = it.slot_t[j]->dtype;
+          q1.lo[j] = it.lo[j];
+          q1.span[j] = it.hi[j] - it.lo[j];
+        }
+        q1.nstates = q.nstates;
+        for (int s2 = 0; s2 < q.nstates; ++s2) {
+          q1.st[s2] = q.states[s2];
+          const TCol *t = it.st_t[s2];
+          q1.scol[s2] = t ? t->d : nullptr;
+          q1.snull[s2] = (t && t->st.nulls) ? t->dnull : nullptr;
+          q1.s_dt[s2] = t ? t->dtype : 0;
+        }
+        q1.entry_words = q.entry_words;
+        q1.table = it.table.as<uint64_t>();
+        size_t oi = std::find(order.begin(), order.end(), &it) - order.begin();
+        q1.selected = selected.as<uint64_t>() + oi;
+        if (T->rows) {
+          HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+          HIPX(qmk_sorted1_scan(&q1, ctx->stream));
+          HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+          HIPX(hipEventSynchronize(ctx->evk1));
+          float ms = 0;
+          HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+          scan_ms_total = std::max(scan_ms_total, ms);
+        }
+        qbufs.push_back(std::move(dq));
+        continue;
+      }
+    }
     // lean query-parallel launch: many queries sharing one aggregate list
     // (COUNT / integer SUM / MIN / MAX), each filtered by one range on one
     // common column

@@ -335,4 +335,31 @@ struct SScan {
   uint64_t *table;                   // (nbounds + 1) x dom x entry_words, zeroed
 };
 
+// ---- one query grouped by a sorted key (lean): a wave owns the runs that
+// start in its chunk; per 64-row window the rows of the current run that pass
+// the AND-ed ranges are reduced across lanes (DPP), and a finished run with
+// selected rows is written once into the query's DIRECT table (slot = key -
+// kmin) with plain stores.
+struct S1Scan {
+  uint64_t nrows;
+  uint64_t chunk_rows;
+  uint32_t nchunks;
+  int32_t nslots;                   // AND-ed ranges (code lo + span); -1: never true
+  const void *key;
+  int32_t key_dt;
+  int32_t nstates;                  // states of the query's entry layout (state 0 COUNT(*))
+  int64_t kmin;
+  const void *pcol[B_MAX_SLOTS];
+  const uint8_t *pnull[B_MAX_SLOTS];
+  int32_t p_dt[B_MAX_SLOTS];
+  uint64_t lo[B_MAX_SLOTS], span[B_MAX_SLOTS];
+  DState st[R_MAX_STATES];
+  const void *scol[R_MAX_STATES];
+  const uint8_t *snull[R_MAX_STATES];
+  int32_t s_dt[R_MAX_STATES];
+  int32_t entry_words, _p0;
+  uint64_t *table;                  // zeroed, dom x entry_words
+  uint64_t *selected;
+};
+
 }  // namespace qm

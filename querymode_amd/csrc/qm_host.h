@@ -120,6 +120,7 @@ struct qm_ctx {
     int runs_qblock = 64;      // queries per staging block (0 = one block per chunk; 16, 32, 64)
     int shared_lean = 1;       // shared-interval groups (and single small-key queries) take the lean pass
     int shared_waves = 4;      // lean pass: workgroup rounds per resident slot
+    int sorted1 = 1;           // a single sorted-key query takes the lean single-query pass
   } tune;
 
   void *alloc(size_t bytes);

@@ -141,6 +141,8 @@ hipError_t qmk_runs_place(const qm::RScan *rs, int win, hipStream_t st);
 size_t qmk_shared_lds_bytes(const qm::SScan *ss);
 int qmk_shared_blocks_per_cu(const qm::SScan *ss);
 hipError_t qmk_shared_scan(const qm::SScan *ss, uint32_t grid, hipStream_t st);
+// one query grouped by a sorted key into its DIRECT table
+hipError_t qmk_sorted1_scan(const qm::S1Scan *q, hipStream_t st);
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);
