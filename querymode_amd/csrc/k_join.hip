@@ -212,6 +212,26 @@ static unsigned gridn(uint64_t n) {
 }
 
 // tmp must hold 2*nent + qmk_exscan_tmp_words(nent) u64 words, cursor nent u32
+// DIRECT index with unique keys: per slot the build row + 1 (0: none);
+// status[1] <- 1 when a key has several build rows (the index is then unused)
+__global__ void join_unique_kernel(const uint64_t *table, uint64_t nent, const uint32_t *rows, uint32_t *uniq,
+                                   uint32_t *status) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < nent; i += stride) {
+    const uint64_t v = table[i];
+    const uint32_t cnt = (uint32_t)v, start = (uint32_t)(v >> 32);
+    if (cnt > 1) atomicOr(&status[1], 1u);
+    uniq[i] = cnt == 1 ? rows[start] + 1u : 0u;
+  }
+}
+
+extern "C" hipError_t qmk_join_unique(const uint64_t *table, uint64_t nent, const uint32_t *rows, uint32_t *uniq,
+                                      uint32_t *status, hipStream_t st) {
+  hipLaunchKernelGGL(join_unique_kernel, dim3(gridn(nent)), dim3(256), 0, st, table, nent, rows, uniq, status);
+  return hipGetLastError();
+}
+
 extern "C" hipError_t qmk_join_build(const DCol *key, uint64_t nrows, const uint64_t *sel, int mode, int key_float,
                                      int64_t kmin, uint64_t cap, uint64_t *table, uint32_t *rows, uint32_t *cursor,
                                      uint64_t *tmp, uint32_t *status, hipStream_t st) {

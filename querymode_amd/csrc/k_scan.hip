@@ -166,6 +166,70 @@ __device__ void flush_lds(const DQuery &q, lds_u64 *lds) {
   }
 }
 
+// One joined row (probe row prow, build row brow) into the query's group
+// table: key and state values through single-row gathers (build columns are
+// the side-1 columns).
+__device__ void join_apply(const DScan &sc, const DQuery &q, uint64_t prow, uint32_t brow, lds_u64 *lds) {
+  // One joined row: key and state values through single-row gathers.
+  const int W = q.entry_words;
+  uint64_t ent = ~0ull;
+  {
+    int64_t v[MAX_KEYS]; uint32_t n[MAX_KEYS];
+    for (int i = 0; i < q.key.ncols; ++i) {
+      const DCol &c = sc.cols[q.key.col[i]];
+      uint64_t row = c.side ? brow : (prow);
+      n[i] = load1_null(c, row);
+      v[i] = q.key.is_float[i] ? (int64_t)canon_f64_key_bits(f64_bits(load1_f(c, row)))
+                               : load1_i(c, row);
+    }
+    if (q.key.ncols == 0) {
+      ent = (q.lmode == LM_NONE) ? (1ull << 63) : 0;
+    } else if (q.gmode == GM_DIRECT) {
+      uint64_t idx = key_index_direct(q.key, v, n);
+      ent = (q.lmode == LM_DIRECT) ? idx : (idx | (1ull << 63));
+    } else {
+      int side;
+      uint64_t stored = key_code_hash(q.key, v, n, &side);
+      if (side >= 0) ent = (q.gcap + (uint64_t)side) | (1ull << 63);
+      else {
+        uint64_t g = glb_find(q, stored);
+        ent = (g == ~0ull) ? ~0ull : (g | (1ull << 63));
+      }
+    }
+  }
+  if (ent == ~0ull) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t rep_base = (q.lmode == LM_DIRECT)
+      ? (uint64_t)(lane & (uint32_t)(q.lds_replicas - 1)) * q.lds_stride : 0;
+  for (int si = 0; si < q.nstates; ++si) {
+    const DState &s = q.states[si];
+    int64_t iv = 0; double fv = 0.0; bool isf = false; bool isnull = false;
+    if (s.col_a >= 0) {
+      const DCol &a = sc.cols[s.col_a];
+      uint64_t ra = a.side ? brow : (prow);
+      isnull = load1_null(a, ra);
+      if (s.product) {
+        const DCol &b = sc.cols[s.col_b];
+        uint64_t rb = b.side ? brow : (prow);
+        isnull = isnull || load1_null(b, rb);
+        double x = dt_is_float(a.dtype) ? load1_f(a, ra) : (double)load1_i(a, ra);
+        double y = dt_is_float(b.dtype) ? load1_f(b, rb) : (double)load1_i(b, rb);
+        fv = x * y; isf = true;
+      } else if (dt_is_float(a.dtype)) {
+        fv = load1_f(a, ra); isf = true;
+      } else {
+        iv = load1_i(a, ra);
+      }
+    }
+    if (isnull) continue;  // this state skips a NULL value
+    Contrib k = contrib_of(s, isf, iv, fv);
+    if (k.skip) continue;
+    if (ent >> 63) apply_glb(s, q.table + (ent & ~(1ull << 63)) * W, k);
+    else apply_lds(s, lds + q.lds_offset + rep_base + ent * W, k);
+  }
+
+}
+
 __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *__restrict__ qs) {
   extern __shared__ uint64_t lds_raw[];
   // LDS: [0, MAX_QUERIES) selected-row counters, then the query tables.
@@ -178,6 +242,7 @@ __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *_
 
   const uint64_t tile_rows = (uint64_t)blockDim.x * RPT;
   const uint64_t ntiles = (sc.nrows + tile_rows - 1) / tile_rows;
+  uint64_t nmatch = 0;  // joined rows (fused probe), added to the counter once
 
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const uint64_t row0 = tile * tile_rows + (uint64_t)threadIdx.x * RPT;
@@ -208,74 +273,36 @@ __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *_
         load8_i(kc, row0, kv);
       }
       m &= ~load8_nulls(kc, row0);  // NULL never matches in joins
-      for (int r = 0; r < RPT; ++r) {
-        if (!((m >> r) & 1)) continue;
-        uint32_t start, count;
-        if (!join_lookup(sc, kv[r], &start, &count)) continue;
-        for (uint32_t j = 0; j < count; ++j) {
-          uint32_t brow = sc.join_rows[start + j];
-          l_atomic_add(lds_cnt, 1);
-          // One joined row: key and state values through single-row gathers.
-          const int W = q.entry_words;
-          uint64_t ent = ~0ull;
-          {
-            int64_t v[MAX_KEYS]; uint32_t n[MAX_KEYS];
-            for (int i = 0; i < q.key.ncols; ++i) {
-              const DCol &c = sc.cols[q.key.col[i]];
-              uint64_t row = c.side ? brow : (row0 + r);
-              n[i] = load1_null(c, row);
-              v[i] = q.key.is_float[i] ? (int64_t)canon_f64_key_bits(f64_bits(load1_f(c, row)))
-                                       : load1_i(c, row);
-            }
-            if (q.key.ncols == 0) {
-              ent = (q.lmode == LM_NONE) ? (1ull << 63) : 0;
-            } else if (q.gmode == GM_DIRECT) {
-              uint64_t idx = key_index_direct(q.key, v, n);
-              ent = (q.lmode == LM_DIRECT) ? idx : (idx | (1ull << 63));
-            } else {
-              int side;
-              uint64_t stored = key_code_hash(q.key, v, n, &side);
-              if (side >= 0) ent = (q.gcap + (uint64_t)side) | (1ull << 63);
-              else {
-                uint64_t g = glb_find(q, stored);
-                ent = (g == ~0ull) ? ~0ull : (g | (1ull << 63));
-              }
-            }
+      if (sc.join_unique) {
+        // unique build keys: the index holds the build row; all 8 lookups in flight at once
+        uint32_t b1[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          b1[r] = 0;
+          if ((m >> r) & 1) {
+            const uint64_t idx = (uint64_t)(kv[r] - sc.join_min);
+            if (idx < sc.join_cap) b1[r] = sc.join_unique[idx];
           }
-          if (ent == ~0ull) continue;
-          const uint32_t lane = threadIdx.x & 63;
-          const uint64_t rep_base = (q.lmode == LM_DIRECT)
-              ? (uint64_t)(lane & (uint32_t)(q.lds_replicas - 1)) * q.lds_stride : 0;
-          for (int si = 0; si < q.nstates; ++si) {
-            const DState &s = q.states[si];
-            int64_t iv = 0; double fv = 0.0; bool isf = false; bool isnull = false;
-            if (s.col_a >= 0) {
-              const DCol &a = sc.cols[s.col_a];
-              uint64_t ra = a.side ? brow : (row0 + r);
-              isnull = load1_null(a, ra);
-              if (s.product) {
-                const DCol &b = sc.cols[s.col_b];
-                uint64_t rb = b.side ? brow : (row0 + r);
-                isnull = isnull || load1_null(b, rb);
-                double x = dt_is_float(a.dtype) ? load1_f(a, ra) : (double)load1_i(a, ra);
-                double y = dt_is_float(b.dtype) ? load1_f(b, rb) : (double)load1_i(b, rb);
-                fv = x * y; isf = true;
-              } else if (dt_is_float(a.dtype)) {
-                fv = load1_f(a, ra); isf = true;
-              } else {
-                iv = load1_i(a, ra);
-              }
-            }
-            if (isnull) continue;
-            Contrib k = contrib_of(s, isf, iv, fv);
-            if (k.skip) continue;
-            if (ent >> 63) apply_glb(s, q.table + (ent & ~(1ull << 63)) * W, k);
-            else apply_lds(s, lds + q.lds_offset + rep_base + ent * W, k);
+        }
+        for (int r = 0; r < RPT; ++r) {
+          if (!b1[r]) continue;
+          ++nmatch;
+          join_apply(sc, q, row0 + r, b1[r] - 1u, lds);
+        }
+      } else {
+        for (int r = 0; r < RPT; ++r) {
+          if (!((m >> r) & 1)) continue;
+          uint32_t start, count;
+          if (!join_lookup(sc, kv[r], &start, &count)) continue;
+          for (uint32_t j = 0; j < count; ++j) {
+            ++nmatch;
+            join_apply(sc, q, row0 + r, sc.join_rows[start + j], lds);
           }
         }
       }
     }
   }
+  if (nmatch) l_atomic_add(lds_cnt, nmatch);
   __syncthreads();
   // selected-row counters: one global atomic per workgroup per query
   if (threadIdx.x < (unsigned)nq && lds_cnt[threadIdx.x])

@@ -847,6 +847,7 @@ void sort_limit_result(qm_result *res, const Plan &pl, const DQuery *d_noq, uint
 namespace {
 struct JoinIndex {
   DevBuf table, rows, cursor, tmp;
+  DevBuf unique;     // DIRECT with unique keys: build row + 1 per slot (empty otherwise)
   int mode = 0;      // 1 DIRECT, 2 HASH
   uint64_t cap = 0;
   int64_t kmin = 0;
@@ -922,9 +923,16 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
                       ji.rows.as<uint32_t>(), ji.cursor.as<uint32_t>(), ji.tmp.as<uint64_t>(), status.as<uint32_t>(),
                       ctx->stream));
   *launches += 8;
+  if (ji.mode == 1 && ctx->tune.join_unique) {
+    ji.unique = DevBuf(ctx, ji.cap * 4 + 64);
+    HIPX(qmk_join_unique(ji.table.as<uint64_t>(), ji.cap, ji.rows.as<uint32_t>(), ji.unique.as<uint32_t>(),
+                         status.as<uint32_t>(), ctx->stream));
+    *launches += 1;
+  }
   uint32_t st[2];
   d2h(ctx, st, status.p, 2);
   if (st[0]) fail(QM_ERR_QUERY_FAILED, "join hash table overflow");
+  if (st[1]) ji.unique.reset();  // duplicate build keys: probe through the row lists
 }
 
 void attach_join(DScan &sc, const JoinIndex &ji, int probe_slot) {
@@ -1000,6 +1008,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.shared_waves; }},
     {"sorted1", "QM_TUNE_SORTED1", 0, 1, [](qm_ctx::Tune &t, double v) { t.sorted1 = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.sorted1; }},
+    {"join_unique", "QM_TUNE_JOIN_UNIQUE", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_unique = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
 };
@@ -1432,6 +1442,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       sc.ncols = (int)cs.cols.size();
       for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
       attach_join(sc, *ji, ps);
+      sc.join_unique = ji->unique.as<uint32_t>();  // fused aggregate: one gather per probe row
     }
     std::vector<DQuery> hq(nq);
     for (size_t i = 0; i < nq; ++i) hq[i] = ab[i].q;

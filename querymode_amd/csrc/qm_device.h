@@ -172,6 +172,9 @@ struct DScan {
   int32_t join_mode;           // 0 none, 1 DIRECT, 2 HASH
   int32_t join_key_float;      // probe key column is f64 (canonical bits)
   int32_t _p;
+  // DIRECT with unique build keys (fused aggregates only): build row + 1 per
+  // key slot (0 = no build row), one gather per probe row
+  const uint32_t *join_unique; // nullptr: use join_table / join_rows
 };
 
 // ---- batched execution: many aggregate queries over one table in one pass.

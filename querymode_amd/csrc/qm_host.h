@@ -121,6 +121,7 @@ struct qm_ctx {
     int shared_lean = 1;       // shared-interval groups (and single small-key queries) take the lean pass
     int shared_waves = 4;      // lean pass: workgroup rounds per resident slot
     int sorted1 = 1;           // a single sorted-key query takes the lean single-query pass
+    int join_unique = 1;       // DIRECT join indexes of unique build keys hold the build row itself
   } tune;
 
   void *alloc(size_t bytes);
