@@ -1722,7 +1722,10 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
 // ---------------------------------------------------------------------------
 // batched aggregate queries over one table: one fused pass (k_batch.hip)
 // ---------------------------------------------------------------------------
-enum : int { BK_NONE = 0, BK_SHARED = 1, BK_SORTED = 2 };
+// BK_RUNS: grouped by a sorted key too sparse for a DIRECT table; only the
+// run-sorted kernels fuse such queries (they emit one group per run and need
+// no table), the rest run through the per-query path
+enum : int { BK_NONE = 0, BK_SHARED = 1, BK_SORTED = 2, BK_RUNS = 3 };
 
 // one query of a fused launch
 struct BItem {
@@ -1804,8 +1807,15 @@ void batch_classify(qm_ctx *ctx, const qm_table *T, const Plan &pl, BItem &it) {
     if (q.states[s].product) return;
     it.st_t[s] = q.states[s].col_a >= 0 ? cs.tcols[q.states[s].col_a] : nullptr;
   }
-  if (q.gmode != GM_DIRECT) return;
   const int nk = q.key.ncols;
+  if (q.gmode == GM_HASH && nk == 1 && it.nslots <= 1 && q.nstates <= R_MAX_STATES) {
+    const TCol *kt = cs.tcols[q.key.col[0]];
+    if (kt->st.sorted && !kt->st.nulls && !dt_float(kt->dtype)) {
+      it.key_t = kt;
+      it.kind = BK_RUNS;
+    }
+  }
+  if (q.gmode != GM_DIRECT) return;
   if (nk == 1) {
     const TCol *kt = cs.tcols[q.key.col[0]];
     if (kt->st.nulls || dt_float(kt->dtype)) return;
@@ -2083,7 +2093,8 @@ bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan 
     r.kind = d.kind;
     r.src_is_float = d.src_is_float;
     r.out_dtype = d.out_dtype;
-    r.state = (int8_t)(d.kind == OK_KEY ? 0 : state_of_word(d.word));
+    // a COUNT output reads its count state only (its value word is unused)
+    r.state = (int8_t)((d.kind == OK_KEY || d.kind == OK_COUNT) ? 0 : state_of_word(d.word));
     r.cnt_state = (int8_t)(d.cnt_word >= 0 ? state_of_word(d.cnt_word) : -1);
     if (r.state < 0 || (d.cnt_word >= 0 && r.cnt_state < 0)) return false;
   }
@@ -2333,13 +2344,17 @@ void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms) 
 }
 
 // Run the fusable plans of `items` in fused passes; results by item idx.
-void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std::vector<qm_result *> &out) {
+// BK_RUNS items the run-sorted kernels cannot take are left to the caller:
+// their idx go to `rejected` (their out[] entries stay null).
+void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std::vector<qm_result *> &out,
+               std::vector<uint32_t> *rejected) {
   Timer tm(ctx);
   // shared groups
   std::vector<std::unique_ptr<BGroup>> groups;
-  std::vector<BItem *> sorted;
+  std::vector<BItem *> sorted, sparse;
   for (BItem *it : items) {
     if (it->kind == BK_SORTED) { sorted.push_back(it); continue; }
+    if (it->kind == BK_RUNS) { sparse.push_back(it); continue; }
     BGroup *g = nullptr;
     for (auto &gg : groups)
       if (gg->col == it->slot_t[0] && gg->key == it->key_t && gg->items.size() < 4 * B_MAX_BOUNDS / 8) g = gg.get();
@@ -2430,10 +2445,44 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     L->sorted_key = it->key_t;
     L->sorted.push_back(it);
   }
+  // sparse sorted keys: one run-sorted launch per (key, range column, state
+  // signature), at most R_MAX_QUERIES queries each
+  std::vector<BLaunch> sparse_launches;
+  std::vector<const TCol *> sparse_pcol;
+  for (BItem *it : sparse) {
+    const TCol *pc = (it->nslots == 1 && !it->never) ? it->slot_t[0] : nullptr;
+    BLaunch *L = nullptr;
+    for (size_t l = 0; l < sparse_launches.size() && !L; ++l) {
+      BLaunch &c = sparse_launches[l];
+      if (c.sorted_key == it->key_t && c.sorted.size() < (size_t)R_MAX_QUERIES && same_signature(*c.sorted[0], *it) &&
+          (!pc || !sparse_pcol[l] || pc == sparse_pcol[l])) {
+        L = &c;
+        if (pc) sparse_pcol[l] = pc;
+      }
+    }
+    if (!L) {
+      sparse_launches.emplace_back();
+      sparse_pcol.push_back(pc);
+      L = &sparse_launches.back();
+      L->sorted_key = it->key_t;
+    }
+    L->sorted.push_back(it);
+  }
   // run-sorted launches write their results directly
   float scan_ms_total = 0;
   std::vector<BItem *> runs_done;
-  const size_t nlaunches = launches.size();
+  size_t nlaunches = launches.size();
+  for (auto &L : sparse_launches) {
+    RunsPlan rp;
+    if (runs_plan(ctx, T, L, rp)) {
+      run_runs(ctx, T, L, rp, out, scan_ms_total);
+      for (BItem *it : L.sorted) runs_done.push_back(it);
+      ++nlaunches;
+    } else {
+      if (!rejected) fail(QM_ERR_QUERY_FAILED, "sparse sorted-key query without a run-sorted launch");
+      for (BItem *it : L.sorted) rejected->push_back(it->idx);
+    }
+  }
   {
     std::vector<BLaunch> rest;
     for (auto &L : launches) {
@@ -2764,7 +2813,7 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       try {
         if (it.kind == BK_SORTED || (it.kind == BK_SHARED && ctx->tune.shared_lean)) {
           std::vector<BItem *> items{&it};
-          run_batch(ctx, left, items, results);
+          run_batch(ctx, left, items, results, nullptr);
         } else {
           run_aggregates(ctx, left, nullptr, plans, results, false, nullptr);
         }
@@ -2885,7 +2934,10 @@ extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans
     if (!fused.empty()) {
       std::vector<BItem *> items;
       for (auto &it : fused) items.push_back(it.get());
-      run_batch(ctx, table, items, all);
+      std::vector<uint32_t> rejected;
+      run_batch(ctx, table, items, all, &rejected);
+      agg_idx.insert(agg_idx.end(), rejected.begin(), rejected.end());
+      std::sort(agg_idx.begin(), agg_idx.end());
     }
     // fused launches of up to MAX_QUERIES aggregate queries sharing one scan
     size_t pos = 0;
