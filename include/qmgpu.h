@@ -144,6 +144,23 @@ int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out);
 int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
                   qm_table *table, qm_result **results_out);
 
+/* Asynchronous form of qm_batch_exec, for callers that keep working (or
+ * queue the next batch) while one runs: the batch executes on a worker
+ * thread of the library; batches of one context run one after another. The
+ * plans and the table must stay open until the batch is done. Replaces the
+ * reference's Promise-returning execute (src/client.ts:1280-1304) for a set
+ * of queries. */
+int qm_batch_submit(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
+                    qm_table *table, qm_batch **out);
+/* *done = 1 once the batch has finished (successfully or not), else 0. */
+int qm_batch_poll(qm_batch *b, int *done);
+/* Blocks until the batch has finished and returns its status (the message is
+ * then qm_last_error's). On QM_OK the nplans result handles go to
+ * results_out (the caller closes them); they can be taken once. */
+int qm_batch_wait(qm_batch *b, qm_result **results_out);
+/* Waits for the batch if it still runs; frees results not taken. */
+int qm_batch_close(qm_batch *b);
+
 /* Device bytes one execution of `p` needs beyond the resident tables (no
  * reference counterpart: the reference checks its memoryBudget per operator,
  * src/operators.ts:2383-2462; the serving loop admits requests with this). */

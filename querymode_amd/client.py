@@ -211,6 +211,54 @@ def _read_result(lib, rh) -> tuple[memoryview, dict]:
         lib.qm_result_close(rh)
 
 
+def _batch_results(lib, outs, n: int, t0: float) -> list[QueryResult]:
+    res = []
+    for i in range(n):
+        data, st = _read_result(lib, C.c_void_p(outs[i]))
+        nrow, names = _qmcb_header(data)
+        res.append(QueryResult(rowCount=nrow, columns=names,
+                               columnarData=data, durationMs=(time.perf_counter() - t0) * 1e3, stats=st))
+    return res
+
+
+class PendingBatch:
+    """A batch running on the library's worker thread (qm_batch_submit). The
+    plans and the table stay referenced until the batch is closed."""
+
+    def __init__(self, lib, handle, plans, table, t0: float):
+        self._lib, self._h, self._plans, self._table, self._t0 = lib, handle, plans, table, t0
+        self.size = len(plans)
+
+    def done(self) -> bool:
+        d = C.c_int()
+        N.check(self._lib.qm_batch_poll(self._h, C.byref(d)))
+        return bool(d.value)
+
+    def wait(self) -> list[QueryResult]:
+        if not self._h:
+            raise QueryModeError("INVALID_ARGUMENT", "the batch is closed")
+        outs = (C.c_void_p * self.size)()
+        try:
+            N.check(self._lib.qm_batch_wait(self._h, outs))
+            return _batch_results(self._lib, outs, self.size, self._t0)
+        finally:
+            self.close()
+
+    def close(self) -> None:
+        """Wait for the batch if it still runs, then release it and its plans."""
+        if self._h:
+            self._lib.qm_batch_close(self._h)
+            self._h = None
+            for p in self._plans:
+                p.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
 class GpuExecutor:
     """QueryExecutor (src/client.ts:1280-1304) backed by the device library."""
 
@@ -241,14 +289,17 @@ class GpuExecutor:
         return QueryResult(rowCount=rows, columns=names, columnarData=data,
                            durationMs=(time.perf_counter() - t0) * 1e3, stats=st)
 
+    def _batch_table(self, descs: list[QueryDescriptor]) -> GpuTable:
+        tname = descs[0].table
+        if any(d.table != tname for d in descs):
+            raise QueryModeError("INVALID_ARGUMENT", "a batch runs over one table")
+        return self._table(tname)
+
     def execute_batch(self, descs: list[QueryDescriptor]) -> list[QueryResult]:
         """Many queries over one resident table in fused launches (qm_batch_exec)."""
         if not descs:
             return []
-        tname = descs[0].table
-        if any(d.table != tname for d in descs):
-            raise QueryModeError("INVALID_ARGUMENT", "a batch runs over one table")
-        table = self._table(tname)
+        table = self._batch_table(descs)
         plans = [_Plan(self.dev, d) for d in descs]
         try:
             arr = (C.c_void_p * len(plans))(*[p.handle for p in plans])
@@ -258,13 +309,25 @@ class GpuExecutor:
         finally:
             for p in plans:
                 p.close()
-        res = []
-        for i in range(len(descs)):
-            data, st = _read_result(self.dev.lib, C.c_void_p(outs[i]))
-            nrow, names = _qmcb_header(data)
-            res.append(QueryResult(rowCount=nrow, columns=names,
-                                   columnarData=data, durationMs=(time.perf_counter() - t0) * 1e3, stats=st))
-        return res
+        return _batch_results(self.dev.lib, outs, len(descs), t0)
+
+    def submit_batch(self, descs: list[QueryDescriptor]) -> "PendingBatch":
+        """Start a batch on the library's worker thread (qm_batch_submit) and
+        return at once; ``PendingBatch.wait()`` gives the results."""
+        if not descs:
+            raise QueryModeError("INVALID_ARGUMENT", "an empty batch")
+        table = self._batch_table(descs)
+        plans = [_Plan(self.dev, d) for d in descs]
+        try:
+            arr = (C.c_void_p * len(plans))(*[p.handle for p in plans])
+            h = C.c_void_p()
+            t0 = time.perf_counter()
+            N.check(self.dev.lib.qm_batch_submit(self.dev.handle, arr, len(plans), table.handle, C.byref(h)))
+        except BaseException:
+            for p in plans:
+                p.close()
+            raise
+        return PendingBatch(self.dev.lib, h, plans, table, t0)
 
     def explain(self, desc: QueryDescriptor) -> dict:
         ops = ["GpuScan(filter)" if desc.filters or desc.filterGroups else "GpuScan"]

@@ -185,6 +185,18 @@ __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > 
 // value (columns are padded well past their last row), decoded only where the
 // value is used, so the loads of a window stay in flight while the previous
 // window is processed (a decode at the load would wait for it at once).
+// staged values of 4 or 8 bytes (the width is uniform over a launch)
+__device__ __forceinline__ void stg_put(void *p, int32_t w, uint64_t i, uint64_t v) {
+  if (w == 4) static_cast<uint32_t *>(p)[i] = (uint32_t)v;
+  else static_cast<uint64_t *>(p)[i] = v;
+}
+// one 8-byte load at p + w * i either way (the arrays end 64 bytes past their
+// last entry), so the loads of a window issue back to back without branches
+__device__ __forceinline__ uint64_t stg_get(const void *p, int32_t w, uint64_t i) {
+  const uint32_t *a = reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(p) + (uint64_t)w * i);
+  const uint64_t v = (uint64_t)a[0] | ((uint64_t)a[1] << 32);
+  return w == 4 ? (v & 0xFFFFFFFFull) : v;
+}
 __device__ __forceinline__ int rbytes(int32_t dt) {
   return (dt == DT_F64 || dt == DT_I64) ? 8 : ((dt == DT_I32 || dt == DT_F32) ? 4 : (dt == DT_I16 ? 2 : 1));
 }
@@ -482,11 +494,11 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
             const uint64_t i = sbase[k] + sdone[k] + (uint64_t)__builtin_popcountll(below);
             if (i < rs.stg_total) {  // the count pass sized every range
               rs.stg_q[i] = (uint16_t)(k * 64 + lane);
-              rs.stg_key[i] = rk;
+              stg_put(rs.stg_key, rs.key_w, i, (uint64_t)rk - (uint64_t)rs.kbase);
               rs.stg_pos[i] = opos[k];
-              rs.stg_val[0][i] = acnt[k];
+              stg_put(rs.stg_val[0], rs.val_w[0], i, acnt[k]);
 #pragma unroll
-              for (int j = 0; j < NS; ++j) rs.stg_val[1 + j][i] = ast[k][j];
+              for (int j = 0; j < NS; ++j) stg_put(rs.stg_val[1 + j], rs.val_w[1 + j], i, ast[k][j] - rs.vbase[1 + j]);
             }
             ++opos[k];
             nsel[k] += (uint32_t)acnt[k];
@@ -566,10 +578,10 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
       r[t].q = 0xFFFFu;
       if (i < b1) {
         r[t].q = rs.stg_q[i];
-        r[t].key = (uint64_t)rs.stg_key[i];
+        r[t].key = stg_get(rs.stg_key, rs.key_w, i) + (uint64_t)rs.kbase;
         r[t].pos = rs.stg_pos[i];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) r[t].v[k] = rs.stg_val[k][i];
+        for (int k = 0; k < NV; ++k) r[t].v[k] = stg_get(rs.stg_val[k], rs.val_w[k], i) + rs.vbase[k];
       }
     }
   };

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <thread>
 #include <unordered_map>
 #include "qm_host.h"
 
@@ -2258,14 +2259,37 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     for (int j = 0; j < rp.nouts; ++j) rs.out_base[j] = arenas[j]->p;
     rs.row_off = droff.as<uint64_t>();
     rs.out_nulls = dnptr.as<uint32_t *const>();
-    DevBuf sq(ctx, all * 2 + 64), skey(ctx, all * 8 + 64), spos(ctx, all * 4 + 64);
+    // staged widths: counts are below 2^32 (rows < 2^32); the order code of a
+    // value of <= 32 bits (or its complement, for MIN) lies in
+    // [2^63 - 2^31, 2^63 + 2^31); a key whose range spans < 2^32 values is
+    // stored as an offset from its minimum
+    const TCol *kt = L.sorted_key;
+    rs.key_w = 8;
+    rs.kbase = 0;
+    if (kt->st.has_int && (uint64_t)kt->st.imax - (uint64_t)kt->st.imin < (1ull << 32)) {
+      rs.key_w = 4;
+      rs.kbase = kt->st.imin;
+    }
+    for (int j = 0; j < rs.nstates; ++j) {
+      const DState &d = rs.st[j];
+      rs.val_w[j] = 8;
+      rs.vbase[j] = 0;
+      if (d.kind == SK_COUNT) {
+        rs.val_w[j] = 4;
+      } else if (d.kind == SK_MAX_CODE && (rs.s_dt[j] == DT_I8 || rs.s_dt[j] == DT_I16 || rs.s_dt[j] == DT_I32 ||
+                                           rs.s_dt[j] == DT_BOOL)) {
+        rs.val_w[j] = 4;
+        rs.vbase[j] = (1ull << 63) - (1ull << 31);
+      }
+    }
+    DevBuf sq(ctx, all * 2 + 64), skey(ctx, all * rs.key_w + 64), spos(ctx, all * 4 + 64);
     std::vector<DevBuf> sval;
-    for (int j = 0; j < rs.nstates; ++j) sval.emplace_back(ctx, all * 8 + 64);
+    for (int j = 0; j < rs.nstates; ++j) sval.emplace_back(ctx, all * rs.val_w[j] + 64);
     rs.stg_q = sq.as<uint16_t>();
-    rs.stg_key = skey.as<int64_t>();
+    rs.stg_key = skey.p;
     rs.stg_pos = spos.as<uint32_t>();
     rs.stg_total = all;
-    for (int j = 0; j < rs.nstates; ++j) rs.stg_val[j] = sval[j].as<uint64_t>();
+    for (int j = 0; j < rs.nstates; ++j) rs.stg_val[j] = sval[j].p;
     HIPX(hipEventRecord(ctx->evk0, ctx->stream));
     HIPX(qmk_runs_write(&rs, (uint32_t)ctx->tune.runs_lds_pad, ctx->stream));
     HIPX(hipEventRecord(ctx->evk1, ctx->stream));
@@ -3110,4 +3134,71 @@ extern "C" int qm_result_close(qm_result *r) {
     delete r;
   }
   return QM_OK;
+}
+
+// ---- asynchronous batches: qm_batch_exec on a worker thread ----
+struct qm_batch {
+  qm_ctx *ctx = nullptr;
+  qm_table *table = nullptr;
+  std::vector<qm_plan *> plans;
+  std::vector<qm_result *> results;
+  std::thread worker;
+  std::atomic<int> done{0};
+  int status = QM_OK;
+  std::string error;
+  bool taken = false;
+};
+
+extern "C" int qm_batch_submit(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_table *table,
+                               qm_batch **out) {
+  API_BEGIN
+  if (!out) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (!ctx || !table || (nplans && !plans)) fail(QM_ERR_INVALID_ARG, "null argument");
+  for (uint32_t i = 0; i < nplans; ++i)
+    if (!plans[i]) fail(QM_ERR_INVALID_ARG, "null plan %u", i);
+  std::unique_ptr<qm_batch> b(new qm_batch());
+  b->ctx = ctx;
+  b->table = table;
+  b->plans.assign(plans, plans + nplans);
+  b->results.assign(nplans, nullptr);
+  qm_batch *bp = b.get();
+  bp->worker = std::thread([bp] {
+    const int rc = qm_batch_exec(bp->ctx, bp->plans.data(), (uint32_t)bp->plans.size(), bp->table,
+                                 bp->results.data());
+    bp->status = rc;
+    if (rc != QM_OK) bp->error = g_last_error;  // this worker thread's message
+    bp->done.store(1, std::memory_order_release);
+  });
+  *out = b.release();
+  API_END
+}
+
+extern "C" int qm_batch_poll(qm_batch *b, int *done) {
+  API_BEGIN
+  if (!b || !done) fail(QM_ERR_INVALID_ARG, "null argument");
+  *done = b->done.load(std::memory_order_acquire);
+  API_END
+}
+
+extern "C" int qm_batch_wait(qm_batch *b, qm_result **results_out) {
+  API_BEGIN
+  if (!b) fail(QM_ERR_INVALID_ARG, "null argument");
+  if (b->worker.joinable()) b->worker.join();
+  if (b->status != QM_OK) return set_error(b->status, b->error);
+  if (b->taken) fail(QM_ERR_INVALID_ARG, "the batch's results were already taken");
+  if (!results_out && !b->results.empty()) fail(QM_ERR_INVALID_ARG, "null argument");
+  for (size_t i = 0; i < b->results.size(); ++i) results_out[i] = b->results[i];
+  b->taken = true;
+  API_END
+}
+
+extern "C" int qm_batch_close(qm_batch *b) {
+  API_BEGIN
+  if (!b) return QM_OK;
+  if (b->worker.joinable()) b->worker.join();
+  if (!b->taken)
+    for (qm_result *r : b->results) qm_result_close(r);
+  delete b;
+  API_END
 }

@@ -304,11 +304,17 @@ struct RScan {
   int32_t nqb, qbw;              // query blocks, queries per block (0, 16, 32 or 64)
   uint64_t *chunk_groups;        // [nchunks][nqb] records per (chunk, query block) (count pass)
   const uint64_t *stg_base;      // [nchunks * nqb + 1] first record of each (chunk, query block)
+  // key and state values are stored as (value - base) in key_w / val_w
+  // bytes (4 where every value of the launch fits 32 bits after the base)
   uint16_t *stg_q;               // query
-  int64_t *stg_key;              // run key
+  void *stg_key;                 // run key - kbase
   uint32_t *stg_pos;             // row within the query's result
   uint64_t stg_total;            // records staged (the staging arrays' length)
-  uint64_t *stg_val[R_MAX_STATES]; // state values (0: the row count)
+  void *stg_val[R_MAX_STATES];   // state values - vbase (0: the row count)
+  int64_t kbase;
+  uint64_t vbase[R_MAX_STATES];
+  int32_t key_w;                 // 4 or 8
+  int32_t val_w[R_MAX_STATES];   // 4 or 8
 };
 
 // ---- lean shared-interval pass: one group of queries with a range on one

@@ -10,8 +10,9 @@
  * (src/errors.ts:7-32). Results cross as QMCB ArrayBuffers (src/columnar.ts).
  *
  * Exports: init, shutdown, lastError, deviceInfo, tableCreate, tableDrop,
- * tableRows, planCompile, planClose, exec, batchExec, planDeviceBytes, resultRows,
- * resultQmcb, resultStats, resultClose.
+ * tableRows, planCompile, planClose, exec, batchExec, batchSubmit (a Promise of
+ * the results; the batch runs while the event loop goes on), planDeviceBytes,
+ * resultRows, resultQmcb, resultStats, resultClose.
  */
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -383,6 +384,99 @@ done:
   return out;
 }
 
+/* batchSubmit(ctx, [plan...], table) -> Promise<[result...]>: qm_batch_submit
+ * on the JS thread, qm_batch_wait on a libuv pool thread; the plans array and
+ * the table are referenced until the promise settles (a finalised plan would
+ * otherwise be closed under the running batch). */
+typedef struct batch_work {
+  napi_async_work work;
+  napi_deferred deferred;
+  napi_ref keep_plans, keep_table;
+  qm_batch *batch;
+  uint32_t n;
+  qm_result **res;
+  int rc;
+  char err[1024];
+} batch_work;
+
+static void batch_wait_exec(napi_env env, void *data) {
+  (void)env;
+  batch_work *w = (batch_work *)data;
+  w->rc = qm_batch_wait(w->batch, w->res);
+  if (w->rc != QM_OK) qm_last_error(NULL, w->err, sizeof w->err);  /* this pool thread's message */
+  qm_batch_close(w->batch);
+  w->batch = NULL;
+}
+
+static void batch_wait_done(napi_env env, napi_status status, void *data) {
+  batch_work *w = (batch_work *)data;
+  if (status == napi_ok && w->rc == QM_OK) {
+    napi_value arr;
+    napi_create_array_with_length(env, w->n, &arr);
+    for (uint32_t i = 0; i < w->n; ++i) napi_set_element(env, arr, i, wrap(env, H_RESULT, w->res[i]));
+    napi_resolve_deferred(env, w->deferred, arr);
+  } else {
+    napi_value ecode, emsg, err;
+    const char *code = w->rc == QM_OK ? "QUERY_FAILED" : code_name(w->rc);
+    napi_create_string_utf8(env, code, NAPI_AUTO_LENGTH, &ecode);
+    napi_create_string_utf8(env, w->err[0] ? w->err : "batch failed", NAPI_AUTO_LENGTH, &emsg);
+    napi_create_error(env, ecode, emsg, &err);
+    napi_set_named_property(env, err, "code", ecode);
+    napi_reject_deferred(env, w->deferred, err);
+  }
+  napi_delete_reference(env, w->keep_plans);
+  napi_delete_reference(env, w->keep_table);
+  napi_delete_async_work(env, w->work);
+  free(w->res);
+  free(w);
+}
+
+static napi_value js_batch_submit(napi_env env, napi_callback_info info) {
+  napi_value argv[3], promise, name;
+  get_args(env, info, 3, argv);
+  handle *c = unwrap(env, argv[0], H_CTX);
+  handle *t = unwrap(env, argv[2], H_TABLE);
+  uint32_t n = 0;
+  if (!c || !t || napi_get_array_length(env, argv[1], &n) != napi_ok || n == 0)
+    return throw_code(env, "INVALID_ARGUMENT", "batchSubmit(ctx, plans[], table)");
+  qm_plan **plans = (qm_plan **)calloc(n, sizeof *plans);
+  batch_work *w = (batch_work *)calloc(1, sizeof *w);
+  qm_result **res = (qm_result **)calloc(n, sizeof *res);
+  if (!plans || !w || !res) {
+    free(plans);
+    free(w);
+    free(res);
+    return throw_code(env, "MEMORY_EXCEEDED", "out of host memory");
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    napi_value v;
+    handle *ph;
+    if (napi_get_element(env, argv[1], i, &v) != napi_ok || !(ph = unwrap(env, v, H_PLAN))) {
+      free(plans);
+      free(w);
+      free(res);
+      return throw_code(env, "INVALID_ARGUMENT", "plans[] must hold plan handles");
+    }
+    plans[i] = (qm_plan *)ph->p;
+  }
+  int rc = qm_batch_submit((qm_ctx *)c->p, plans, n, (qm_table *)t->p, &w->batch);
+  free(plans);
+  if (rc != QM_OK) {
+    free(w);
+    free(res);
+    return throw_rc(env, rc);
+  }
+  w->n = n;
+  w->res = res;
+  napi_create_reference(env, argv[1], 1, &w->keep_plans);
+  napi_create_reference(env, argv[2], 1, &w->keep_table);
+  napi_create_promise(env, &w->deferred, &promise);
+  napi_create_string_utf8(env, "qm_batch", NAPI_AUTO_LENGTH, &name);
+  napi_create_async_work(env, NULL, name, batch_wait_exec, batch_wait_done, w, &w->work);
+  napi_queue_async_work(env, w->work);
+  return promise;
+}
+
 static napi_value js_result_rows(napi_env env, napi_callback_info info) {
   napi_value argv[1], v;
   get_args(env, info, 1, argv);
@@ -452,6 +546,7 @@ static napi_value init_module(napi_env env, napi_value exports) {
     {"planClose", NULL, js_plan_close, NULL, NULL, NULL, napi_enumerable, NULL},
     {"exec", NULL, js_exec, NULL, NULL, NULL, napi_enumerable, NULL},
     {"batchExec", NULL, js_batch_exec, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"batchSubmit", NULL, js_batch_submit, NULL, NULL, NULL, napi_enumerable, NULL},
     {"planDeviceBytes", NULL, js_plan_device_bytes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultRows", NULL, js_result_rows, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultQmcb", NULL, js_result_qmcb, NULL, NULL, NULL, napi_enumerable, NULL},

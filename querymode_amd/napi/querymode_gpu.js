@@ -205,6 +205,33 @@ class GpuExecutor {
       bytesRead: r.stats.bytesAlgorithmic, pagesSkipped: 0, durationMs: Date.now() - t0, stats: r.stats };
   }
 
+  /**
+   * Many aggregate queries over one registered table in fused launches; the
+   * Promise settles when the batch is done (the event loop runs meanwhile).
+   */
+  async executeBatch(descs) {
+    if (!descs.length) return [];
+    const name = descs[0].table;
+    const table = this.tables.get(name);
+    if (!table) throw qmError('TABLE_NOT_FOUND', 'table ' + name + ' is not registered');
+    if (descs.some((d) => d.table !== name)) throw qmError('INVALID_ARGUMENT', 'a batch runs over one table');
+    const t0 = Date.now();
+    const plans = descs.map((d) => addon.planCompile(this.ctx, encodePlan(d)));
+    try {
+      const results = await addon.batchSubmit(this.ctx, plans, table);
+      return results.map((res) => {
+        try {
+          const qmcb = addon.resultQmcb(res);
+          const stats = addon.resultStats(res);
+          const batch = decodeQMCB(qmcb);
+          const rows = qmcbToRows(batch);
+          return { rows, rowCount: rows.length, columns: batch.columns.map((c) => c.name), columnarData: qmcb,
+            bytesRead: stats.bytesAlgorithmic, pagesSkipped: 0, durationMs: Date.now() - t0, stats };
+        } finally { addon.resultClose(res); }
+      });
+    } finally { plans.forEach((p) => addon.planClose(p)); }
+  }
+
   count(desc) { return this.execute(Object.assign({}, desc, { aggregates: [{ fn: 'count', column: '*' }], groupBy: [] })).rows[0]['count_*']; }
 
   /** A pull-based Operator over the device result: next() / nextColumnar() / close(). */

@@ -77,6 +77,13 @@ def test_null_handles_are_rejected():
     n = C.c_uint64()
     assert lib.qm_result_rows(None, C.byref(n)) != 0
     assert lib.qm_plan_compile(None, b"", 0, C.byref(out)) != 0
+    # asynchronous batches: argument errors come back at submission
+    plans = (C.c_void_p * 1)()
+    assert lib.qm_batch_submit(None, plans, 1, None, C.byref(out)) == -10 and out.value is None
+    done = C.c_int()
+    assert lib.qm_batch_poll(None, C.byref(done)) == -10
+    assert lib.qm_batch_wait(None, plans) == -10
+    assert lib.qm_batch_close(None) == 0
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):

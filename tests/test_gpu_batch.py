@@ -11,6 +11,7 @@ import pytest
 
 from oracle import oracle as O
 from querymode_amd import datagen
+from querymode_amd.errors import QueryModeError
 from querymode_amd.plan import AggregateOp, FilterOp, QueryDescriptor
 
 from ._compare import assert_groups_equal
@@ -97,3 +98,29 @@ def test_batch_falls_back_for_other_shapes(qm):
                         aggregates=[AggregateOp("sum", "v2"), AggregateOp("avg", "v2")]),        # f64 sums, sorted
     ]
     _check(qm, "f", cols, descs)
+
+
+def test_submitted_batches_match_synchronous(qm):
+    """qm_batch_submit: two batches in flight from one thread; each result
+    equals the synchronous batch's, byte for byte, whichever is waited first."""
+    n = 2_000_000
+    _say("async batches: two submissions, waited in reverse order")
+    qm.register_table("a", _table(n, 3000, seed=13))
+    qa = [QueryDescriptor("a", filters=d.filters, groupBy=d.groupBy, aggregates=d.aggregates)
+          for d in datagen.batch_queries(64, seed=13, min_sel=1e-3)]
+    qb = [QueryDescriptor("a", filters=d.filters, groupBy=d.groupBy, aggregates=d.aggregates)
+          for d in datagen.batch_queries(32, seed=14, min_sel=1e-3)]
+    ex = qm.executor
+    want_a = [bytes(r.columnarData) for r in ex.execute_batch(qa)]
+    want_b = [bytes(r.columnarData) for r in ex.execute_batch(qb)]
+    pa = ex.submit_batch(qa)
+    pb = ex.submit_batch(qb)
+    got_b = [bytes(r.columnarData) for r in pb.wait()]
+    assert pa.done() in (True, False)
+    got_a = [bytes(r.columnarData) for r in pa.wait()]
+    assert got_a == want_a and got_b == want_b
+    # a batch closed without waiting releases its results
+    ex.submit_batch(qb).close()
+    # errors surface at wait
+    with pytest.raises(QueryModeError):
+        ex.submit_batch([QueryDescriptor("a", groupBy=["k2"], aggregates=[AggregateOp("sum", "nope")])]).wait()

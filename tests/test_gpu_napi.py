@@ -40,6 +40,12 @@ out.join = qm.table('rows').join(qm.table('right'), { left: 'id', right: 'id' })
 (async () => {
   const op = qm.table('rows').filter('amount', 'eq', 999).select('id').toOperator(1000);
   out.pulled = (await q.drainPipeline(op)).length;
+  // a batch through the Promise-returning batchSubmit equals each query run alone
+  const frames = [0, 250, 500, 750].map((lo) => qm.table('rows').filter('amount', 'gte', lo)
+    .filter('amount', 'lt', lo + 300).groupBy('region').aggregate('sum', 'amount').aggregate('count', '*'));
+  const pending = qm.executor.executeBatch(frames.map((f) => f.desc));
+  out.batch = (await pending).map((r) => r.rows);
+  out.solo = frames.map((f) => f.exec().rows);
   qm.close();
   console.log(JSON.stringify(out, (k, v) => typeof v === 'bigint' ? v.toString() : v));
 })();
@@ -47,7 +53,7 @@ out.join = qm.table('rows').join(qm.table('right'), { left: 'id', right: 'id' })
 
 
 def test_js_host_end_to_end():
-    print("node: generateRows(1M) group-by / top-K / join through qm_napi.node", flush=True)
+    print("node: generateRows(1M) group-by / top-K / join / batch through qm_napi.node", flush=True)
     res = subprocess.run([NODE, "-e", SCRIPT % json.dumps(str(JS))], capture_output=True, text=True, timeout=120,
                          cwd=ROOT)
     assert res.returncode == 0, res.stderr
@@ -64,3 +70,6 @@ def test_js_host_end_to_end():
     assert [int(r["id"]) for r in out["topk"]] == [998 + 1000 * k for k in range(100)]
     assert [(int(r["count_*"]), int(r["sum_value"])) for r in out["join"]] == [(500_000, 10 * 499_999 * 500_000 // 2)]
     assert out["pulled"] == 1000
+    key = lambda r: int(r["region"])  # noqa: E731
+    assert [sorted(b, key=key) for b in out["batch"]] == [sorted(x, key=key) for x in out["solo"]]
+    assert all(len(b) == 4 for b in out["batch"])
