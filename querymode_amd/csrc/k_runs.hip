@@ -132,53 +132,6 @@ __device__ __forceinline__ uint64_t rstate(int si, uint64_t cnt, const uint64_t 
   return v;
 }
 
-// write one group (a run's result for query q) at row pos of q's columns
-// (global row g = row_off[q] + pos); the values are those finalize_slot
-// derives from a table entry
-template <int NS>
-__device__ void remit(const RScan &rs, int q, uint64_t pos, uint64_t g, int64_t key, uint64_t cnt,
-                      const uint64_t (&st)[NS]) {
-  for (int o = 0; o < rs.nouts; ++o) {
-    const ROut ro = rs.outs[o];
-    void *dst = rs.out_base[o];
-    const uint64_t c = ro.cnt_state < 0 ? 0 : rstate<NS>(ro.cnt_state, cnt, st);
-    const uint64_t v = rstate<NS>(ro.state, cnt, st);
-    bool isnull = false;
-    switch (ro.kind) {
-      case OK_KEY: rstore_i(dst, ro.out_dtype, g, key); break;
-      case OK_COUNT: static_cast<int64_t *>(dst)[g] = (int64_t)c; break;
-      case OK_SUM_I64:
-        isnull = c == 0;
-        static_cast<int64_t *>(dst)[g] = isnull ? 0 : (int64_t)v;
-        break;
-      case OK_AVG_I64:
-        isnull = c == 0;
-        static_cast<double *>(dst)[g] = isnull ? 0.0 : (double)(int64_t)v / (double)c;
-        break;
-      default: {  // OK_MIN / OK_MAX: v is the max of (MIN: complemented) order codes
-        const bool mn = ro.kind == OK_MIN;
-        if (c == 0) {
-          isnull = true;
-          if (ro.src_is_float) rstore_f(dst, ro.out_dtype, g, 0.0);
-          else rstore_i(dst, ro.out_dtype, g, 0);
-        } else if (ro.src_is_float) {
-          double d;
-          if (v == 0) d = mn ? bits_f64(0x7FF0000000000000ull) : bits_f64(0xFFF0000000000000ull);
-          else d = bits_f64(decode_f64_code(mn ? ~v : v));
-          rstore_f(dst, ro.out_dtype, g, d);
-        } else {
-          rstore_i(dst, ro.out_dtype, g, decode_i64(mn ? ~v : v));
-        }
-        break;
-      }
-    }
-    if (isnull) {
-      uint32_t *nb = rs.out_nulls[(uint64_t)q * rs.nouts + o];
-      if (nb) atomicOr(&nb[pos >> 5], 1u << (pos & 31));
-    }
-  }
-}
-
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
 // Raw column words: 8 bytes from the 4-byte-aligned address holding the row's
@@ -190,12 +143,16 @@ __device__ __forceinline__ void stg_put(void *p, int32_t w, uint64_t i, uint64_t
   if (w == 4) static_cast<uint32_t *>(p)[i] = (uint32_t)v;
   else static_cast<uint64_t *>(p)[i] = v;
 }
-// one 8-byte load at p + w * i either way (the arrays end 64 bytes past their
-// last entry), so the loads of a window issue back to back without branches
-__device__ __forceinline__ uint64_t stg_get(const void *p, int32_t w, uint64_t i) {
-  const uint32_t *a = reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(p) + (uint64_t)w * i);
-  const uint64_t v = (uint64_t)a[0] | ((uint64_t)a[1] << 32);
-  return w == 4 ? (v & 0xFFFFFFFFull) : v;
+// one 8-byte load at p + (i << sh) (sh = 2 or 3) either way (the arrays end 64
+// bytes past their last entry), so the loads of a window issue back to back
+// without branches; stg_dec keeps the entry's bytes and adds the base where
+// the value is used, so the loads stay in flight until then
+__device__ __forceinline__ uint64_t stg_raw(const void *p, int32_t sh, uint64_t i) {
+  const uint32_t *a = reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(p) + (i << sh));
+  return (uint64_t)a[0] | ((uint64_t)a[1] << 32);
+}
+__device__ __forceinline__ uint64_t stg_dec(uint64_t raw, int32_t sh, uint64_t base) {
+  return (sh == 2 ? (raw & 0xFFFFFFFFull) : raw) + base;
 }
 __device__ __forceinline__ int rbytes(int32_t dt) {
   return (dt == DT_F64 || dt == DT_I64) ? 8 : ((dt == DT_I32 || dt == DT_F32) ? 4 : (dt == DT_I16 ? 2 : 1));
@@ -545,14 +502,14 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
 static constexpr int R_PBLOCK = 512;
 
 template <int NS, int R_WIN>
-__global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
+__global__ void __launch_bounds__(R_PBLOCK) __attribute__((amdgpu_waves_per_eu(R_WIN <= 1024 ? 6 : 1))) runs_place_kernel(RScan rs) {
   constexpr int NV = NS + 1;  // count + states
   constexpr int PER = R_WIN / R_PBLOCK;
   extern __shared__ __align__(16) uint8_t s_dyn[];
   uint64_t *s_key = reinterpret_cast<uint64_t *>(s_dyn);  // [R_WIN]
-  uint64_t *s_row = s_key + R_WIN;                           // [R_WIN]
-  uint64_t *s_val = s_row + R_WIN;                           // [NV][R_WIN]
-  uint16_t *s_qs = reinterpret_cast<uint16_t *>(s_val + NV * R_WIN);  // query of each slot
+  uint64_t *s_val = s_key + R_WIN;                           // [NV][R_WIN]
+  uint32_t *s_row = reinterpret_cast<uint32_t *>(s_val + NV * R_WIN);  // [R_WIN] row within the query's result
+  uint16_t *s_qs = reinterpret_cast<uint16_t *>(s_row + R_WIN);       // [R_WIN] query of each slot
   __shared__ uint32_t s_cnt[R_MAX_QUERIES];
   __shared__ uint32_t s_wsum[R_PBLOCK / 64];
   __shared__ uint64_t s_roff[R_MAX_QUERIES], s_rend[R_MAX_QUERIES];  // each query's result rows [roff, rend)
@@ -569,31 +526,49 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   }
   // the records of one window, one window ahead (loads in flight while the
   // previous window is sorted and written)
-  struct Rec { uint32_t q, pos; uint64_t key, v[NV]; };
+  // raw staged words (decoded at the scatter into LDS)
+  // (loaded unconditionally from a clamped index, `in` = the slot holds a
+  // record: a fixed number of loads per window lets each buffer's loads be
+  // waited for exactly where they are used)
+  struct Rec { uint64_t key, v[NV]; uint32_t pos; uint16_t q; bool in; };
   Rec cur[PER], nxt[PER];
+  const int32_t ksh = rs.key_w == 4 ? 2 : 3;
+  int32_t vsh[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) vsh[k] = rs.val_w[k] == 4 ? 2 : 3;
   auto load = [&](uint64_t w0, Rec (&r)[PER]) {
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      const uint64_t i = w0 + (uint64_t)(tid + t * R_PBLOCK);
-      r[t].q = 0xFFFFu;
-      if (i < b1) {
-        r[t].q = rs.stg_q[i];
-        r[t].key = stg_get(rs.stg_key, rs.key_w, i) + (uint64_t)rs.kbase;
-        r[t].pos = rs.stg_pos[i];
+      const uint64_t i0 = w0 + (uint64_t)(tid + t * R_PBLOCK);
+      r[t].in = i0 < b1;
+      const uint64_t i = r[t].in ? i0 : b1 - 1;  // b1 > w0 >= b0: a staged record
+      r[t].q = rs.stg_q[i];
+      r[t].key = stg_raw(rs.stg_key, ksh, i);
+      r[t].pos = rs.stg_pos[i];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) r[t].v[k] = stg_get(rs.stg_val[k], rs.val_w[k], i) + rs.vbase[k];
-      }
+      for (int k = 0; k < NV; ++k) r[t].v[k] = stg_raw(rs.stg_val[k], vsh[k], i);
     }
   };
-  if (b0 < b1) load(b0, cur);
-  for (uint64_t w0 = b0; w0 < b1; w0 += R_WIN) {
+  // one window: counting sort of its records by query in LDS, then the
+  // result columns; windows alternate between two record buffers, so each
+  // buffer's loads stay in flight for a whole window (no copy between them)
+  auto process = [&](uint64_t w0, Rec (&cur)[PER]) {
     const int n = (int)((b1 - w0 < (uint64_t)R_WIN) ? (b1 - w0) : (uint64_t)R_WIN);
-    if (w0 + R_WIN < b1) load(w0 + R_WIN, nxt);
+    // every lane takes this buffer's words here, loaded one window ago (the
+    // uses below are conditional; without this the compiler cannot tell
+    // they completed and waits for all loads, the next window's included,
+    // where their registers are reused)
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      __asm__ volatile("" ::"v"(cur[t].q), "v"(cur[t].pos), "v"(cur[t].key));
+#pragma unroll
+      for (int k = 0; k < NV; ++k) __asm__ volatile("" ::"v"(cur[t].v[k]));
+    }
     if (tid < R_MAX_QUERIES) s_cnt[tid] = 0;
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < PER; ++t)
-      if (cur[t].q != 0xFFFFu) atomicAdd(&s_cnt[cur[t].q < nq ? cur[t].q : 0u], 1u);
+      if (cur[t].in) atomicAdd(&s_cnt[cur[t].q < nq ? cur[t].q : 0u], 1u);
     __syncthreads();
     {  // exclusive scan over the queries (thread t < R_MAX_QUERIES owns query t)
       const uint32_t v = tid < R_MAX_QUERIES ? s_cnt[tid] : 0u;
@@ -607,14 +582,14 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < PER; ++t) {
-      if (cur[t].q == 0xFFFFu) continue;
+      if (!cur[t].in) continue;
       const uint32_t q = cur[t].q < nq ? cur[t].q : 0u;
       const uint32_t slot = atomicAdd(&s_cnt[q], 1u);
       s_qs[slot] = (uint16_t)q;
-      s_key[slot] = cur[t].key;
+      s_key[slot] = stg_dec(cur[t].key, ksh, (uint64_t)rs.kbase);
       s_row[slot] = cur[t].pos;
 #pragma unroll
-      for (int k = 0; k < NV; ++k) s_val[k * R_WIN + slot] = cur[t].v[k];
+      for (int k = 0; k < NV; ++k) s_val[k * R_WIN + slot] = stg_dec(cur[t].v[k], vsh[k], rs.vbase[k]);
     }
     __syncthreads();
     // one output column at a time over this thread's slots (the output's
@@ -634,6 +609,7 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
     for (int o = 0; o < rs.nouts; ++o) {
       const ROut ro = rs.outs[o];
       void *dst = rs.out_base[o];
+      uint32_t *const nbm = rs.out_nbase[o];
       const int vs = ro.state, cs = ro.cnt_state;
 #pragma unroll
       for (int t = 0; t < PER; ++t) {
@@ -671,20 +647,25 @@ __global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
             break;
           }
         }
-        if (isnull) {
-          uint32_t *nbm = rs.out_nulls[(uint64_t)qq[t] * rs.nouts + o];
-          const uint64_t pos = gg - s_roff[qq[t]];
-          if (nbm) atomicOr(&nbm[pos >> 5], 1u << (pos & 31));
-        }
+        if (isnull && nbm) atomicOr(&nbm[gg >> 5], 1u << (gg & 31));
       }
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < PER; ++t) cur[t] = nxt[t];
+  };
+  // the next window is always loaded (past the end: the last record again,
+  // unused), so every path issues the same loads
+  if (b0 >= b1) return;
+  load(b0, cur);
+  for (uint64_t w0 = b0; w0 < b1; w0 += 2 * (uint64_t)R_WIN) {
+    load(w0 + R_WIN, nxt);
+    process(w0, cur);
+    if (w0 + R_WIN >= b1) break;
+    load(w0 + 2 * (uint64_t)R_WIN, cur);
+    process(w0 + R_WIN, nxt);
   }
 }
 
-static size_t runs_place_lds(int ns, int win) { return (size_t)win * (8 * (2 + ns + 1) + 2); }
+static size_t runs_place_lds(int ns, int win) { return (size_t)win * (8 * (1 + ns + 1) + 4 + 2); }
 
 // exclusive scan over chunks, per query: a block serves 64 queries (lanes),
 // its 16 waves one slice of the chunks each

@@ -2221,7 +2221,19 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
   std::vector<std::shared_ptr<DevBuf>> arenas(rp.nouts);
   for (int j = 0; j < rp.nouts; ++j)
     arenas[j] = std::make_shared<DevBuf>(ctx, std::max<uint64_t>(arows, 1) * dt_size(rp.outs[j].out_dtype) + 64);
-  std::vector<uint32_t *> nptrs(n * rp.nouts, nullptr);
+  // NULL bitmaps likewise: one per nullable output column over the arena's
+  // rows (bit g for arena row g; row_off[i] is a multiple of 8192, so query
+  // i's bitmap starts at word row_off[i] / 32). An emitted group has
+  // COUNT(*) >= 1: only a COUNT(col)-guarded output can be NULL.
+  std::vector<uint32_t *> nbase(rp.nouts, nullptr);
+  for (int j = 0; j < rp.nouts; ++j) {
+    const ROut &r = rp.outs[j];
+    if (r.kind == OK_KEY || r.kind == OK_COUNT || r.cnt_state <= 0) continue;
+    const uint64_t nb = std::max<uint64_t>(arows, 1) / 8 + 64;
+    arenas.push_back(std::make_shared<DevBuf>(ctx, nb));
+    zero(ctx, arenas.back()->p, nb);
+    nbase[j] = arenas.back()->as<uint32_t>();
+  }
   for (size_t i = 0; i < n; ++i) {
     const BItem &it = *L.sorted[i];
     qm_result *res = new qm_result();
@@ -2234,14 +2246,12 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
       OutCol oc;
       oc.name = it.ab.outs[j].name;
       oc.dtype = r.out_dtype;
+      // views: ctx stays null, nothing to release
       oc.data.p = static_cast<uint8_t *>(arenas[j]->p) + roff[i] * dt_size(r.out_dtype);
-      oc.data.n = pad_rows(tot[i]) * dt_size(r.out_dtype);  // a view: ctx stays null, nothing to release
-      // an emitted group has COUNT(*) >= 1: only a COUNT(col)-guarded output can be NULL
-      if (r.kind != OK_KEY && r.kind != OK_COUNT && r.cnt_state > 0) {
-        const uint64_t nb = pad_rows(tot[i]) / 8 + 64;
-        oc.nulls = DevBuf(ctx, nb);
-        zero(ctx, oc.nulls.p, nb);
-        nptrs[i * rp.nouts + j] = oc.nulls.as<uint32_t>();
+      oc.data.n = pad_rows(tot[i]) * dt_size(r.out_dtype);
+      if (nbase[j]) {
+        oc.nulls.p = reinterpret_cast<uint8_t *>(nbase[j]) + roff[i] / 8;
+        oc.nulls.n = pad_rows(tot[i]) / 8 + 64;
       }
       res->cols.push_back(std::move(oc));
     }
@@ -2253,12 +2263,13 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
                          (unsigned long long)all, (unsigned long long)check);
   std::vector<uint64_t> sel(n, 0);
   if (all) {
-    DevBuf droff(ctx, n * 8 + 64), dnptr(ctx, nptrs.size() * 8 + 64);
+    DevBuf droff(ctx, n * 8 + 64);
     h2d(ctx, droff.p, roff.data(), n);
-    h2d(ctx, dnptr.p, nptrs.data(), nptrs.size());
-    for (int j = 0; j < rp.nouts; ++j) rs.out_base[j] = arenas[j]->p;
+    for (int j = 0; j < rp.nouts; ++j) {
+      rs.out_base[j] = arenas[j]->p;
+      rs.out_nbase[j] = nbase[j];
+    }
     rs.row_off = droff.as<uint64_t>();
-    rs.out_nulls = dnptr.as<uint32_t *const>();
     // staged widths: counts are below 2^32 (rows < 2^32); the order code of a
     // value of <= 32 bits (or its complement, for MIN) lies in
     // [2^63 - 2^31, 2^63 + 2^31); a key whose range spans < 2^32 values is

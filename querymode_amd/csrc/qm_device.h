@@ -294,7 +294,7 @@ struct RScan {
   ROut outs[R_MAX_OUTS];
   void *out_base[R_MAX_OUTS];    // column o of every query: query q's row i at global row row_off[q] + i
   const uint64_t *row_off;       // [nq]
-  uint32_t *const *out_nulls;    // [nq][nouts] per-query NULL bitmaps (nullptr: never null)
+  uint32_t *out_nbase[R_MAX_OUTS]; // column o's NULL bitmap over the arena rows (nullptr: never null)
   uint64_t *selected;            // [nq] selected-row counters
   // staging: the write pass appends each finished run's groups (one record
   // per query that selected a row of it) to its chunk's record range; the
