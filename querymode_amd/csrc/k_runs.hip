@@ -157,15 +157,18 @@ __device__ __forceinline__ uint64_t stg_dec(uint64_t raw, int32_t sh, uint64_t b
 __device__ __forceinline__ int rbytes(int32_t dt) {
   return (dt == DT_F64 || dt == DT_I64) ? 8 : ((dt == DT_I32 || dt == DT_F32) ? 4 : (dt == DT_I16 ? 2 : 1));
 }
+// (the word addresses are formed by pointer arithmetic from the column
+// pointer, so the compiler keeps them global: global loads count only
+// against vmcnt, where flat loads would also hold every LDS wait)
 __device__ __forceinline__ uint64_t rraw(const void *data, int32_t dt, uint64_t row) {
-  const uintptr_t a = reinterpret_cast<uintptr_t>(data) + row * (uint64_t)rbytes(dt);
-  const uint32_t *w = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+  const uint8_t *b = static_cast<const uint8_t *>(data) + row * (uint64_t)rbytes(dt);
+  const uint32_t *w = reinterpret_cast<const uint32_t *>(b - (reinterpret_cast<uintptr_t>(b) & 3));
   return (uint64_t)w[0] | ((uint64_t)w[1] << 32);
 }
 __device__ __forceinline__ uint32_t rraw_null(const uint8_t *nb, uint64_t row) {
   if (!nb) return 0u;
-  const uintptr_t a = reinterpret_cast<uintptr_t>(nb) + (row >> 3);
-  return *reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+  const uint8_t *b = nb + (row >> 3);
+  return *reinterpret_cast<const uint32_t *>(b - (reinterpret_cast<uintptr_t>(b) & 3));
 }
 __device__ __forceinline__ bool rdec_null(uint32_t raw, uint64_t row) {
   return (raw >> ((((row >> 3) & 3) << 3) + (row & 7))) & 1u;
@@ -502,7 +505,7 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
 static constexpr int R_PBLOCK = 512;
 
 template <int NS, int R_WIN>
-__global__ void __launch_bounds__(R_PBLOCK) __attribute__((amdgpu_waves_per_eu(R_WIN <= 1024 ? 6 : 1))) runs_place_kernel(RScan rs) {
+__global__ void __launch_bounds__(R_PBLOCK) runs_place_kernel(RScan rs) {
   constexpr int NV = NS + 1;  // count + states
   constexpr int PER = R_WIN / R_PBLOCK;
   extern __shared__ __align__(16) uint8_t s_dyn[];

@@ -116,7 +116,7 @@ struct qm_ctx {
     double runs_min_len = 8.0; // mean rows per key run for the run-sorted kernel
     int runs_chunks_per_cu = 32;
     int runs_lds_pad = 0;      // extra LDS bytes per write-pass workgroup (caps waves in flight)
-    int runs_win = 1024;       // place-pass window (records)
+    int runs_win = 2048;       // place-pass window (records)
     int runs_qblock = 64;      // queries per staging block (0 = one block per chunk; 16, 32, 64)
     int shared_lean = 1;       // shared-interval groups (and single small-key queries) take the lean pass
     int shared_waves = 4;      // lean pass: workgroup rounds per resident slot

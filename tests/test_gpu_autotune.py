@@ -46,6 +46,9 @@ def test_tune_knob_errors(qm):
     dev = qm.device
     with pytest.raises(QueryModeError):
         dev.tune_set("no_such_knob", 1)
+    old = dev.tune_get("runs_win")
     dev.tune_set("runs_win", 5000)        # clamped to the knob's range
     assert dev.tune_get("runs_win") == 2048
-    dev.tune_set("runs_win", 1024)
+    dev.tune_set("runs_win", 100)
+    assert dev.tune_get("runs_win") == 1024
+    dev.tune_set("runs_win", old)

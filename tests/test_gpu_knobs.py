@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 SETTINGS = [
     {"runs_qblock": 0}, {"runs_qblock": 16}, {"runs_qblock": 32},
-    {"runs_win": 2048}, {"runs_chunks_per_cu": 2}, {"runs_chunks_per_cu": 128},
+    {"runs_win": 1024}, {"runs_chunks_per_cu": 2}, {"runs_chunks_per_cu": 128},
     {"runs_min_q": 100000},                       # sorted keys: query-parallel kernel
     {"runs_min_q": 100000, "qp_min": 100000},     # sorted keys: segmented kernel
     {"shared_lean": 0},                           # shared groups: fused multi-group pass
