@@ -61,10 +61,27 @@ __device__ __forceinline__ void l_add128(lds_u64 *p, uint64_t lo, uint64_t hi) {
 
 // ---- 8-row loads. row0 is a multiple of 8 and columns are padded, so these
 // are aligned 64/128-bit loads that never leave the allocation. ----
+// Column memory (tables, results, join payloads, IN lists) is device global
+// memory. The pointers reach the kernels inside descriptor structs, where the
+// compiler cannot tell their address space; gld() loads through a global
+// pointer, so these are global loads (a flat load would also count against
+// every LDS wait of the kernel).
+typedef uint32_t gv2u __attribute__((ext_vector_type(2)));
+typedef uint32_t gv4u __attribute__((ext_vector_type(4)));
+typedef int32_t gv4i __attribute__((ext_vector_type(4)));
+typedef int64_t gv2l __attribute__((ext_vector_type(2)));
+typedef float gv4f __attribute__((ext_vector_type(4)));
+typedef double gv2d __attribute__((ext_vector_type(2)));
+template <class V> __device__ __forceinline__ V gld(const void *p, uint64_t byte_off) {
+  using G8 = const __attribute__((address_space(1))) uint8_t;
+  using GV = const __attribute__((address_space(1))) V;
+  return *(GV *)((G8 *)p + byte_off);
+}
+
 __device__ __forceinline__ void load8_i(const DCol &c, uint64_t row0, int64_t v[RPT]) {
   switch (c.dtype) {
     case DT_I8: {
-      uint2 w = *reinterpret_cast<const uint2 *>(static_cast<const int8_t *>(c.data) + row0);
+      const gv2u w = gld<gv2u>(c.data, row0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) v[r] = (int8_t)((w.x >> (8 * r)) & 0xFF);
 #pragma unroll
@@ -72,7 +89,7 @@ __device__ __forceinline__ void load8_i(const DCol &c, uint64_t row0, int64_t v[
       break;
     }
     case DT_BOOL: {
-      uint2 w = *reinterpret_cast<const uint2 *>(static_cast<const uint8_t *>(c.data) + row0);
+      const gv2u w = gld<gv2u>(c.data, row0);
 #pragma unroll
       for (int r = 0; r < 4; ++r) v[r] = (w.x >> (8 * r)) & 0xFF;
 #pragma unroll
@@ -80,7 +97,7 @@ __device__ __forceinline__ void load8_i(const DCol &c, uint64_t row0, int64_t v[
       break;
     }
     case DT_I16: {
-      uint4 w = *reinterpret_cast<const uint4 *>(static_cast<const int16_t *>(c.data) + row0);
+      const gv4u w = gld<gv4u>(c.data, row0 * 2);
       uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -90,16 +107,18 @@ __device__ __forceinline__ void load8_i(const DCol &c, uint64_t row0, int64_t v[
       break;
     }
     case DT_I32: {
-      const int4 *p = reinterpret_cast<const int4 *>(static_cast<const int32_t *>(c.data) + row0);
-      int4 a = p[0], b = p[1];
+      const gv4i a = gld<gv4i>(c.data, row0 * 4), b = gld<gv4i>(c.data, row0 * 4 + 16);
       v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
       v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
       break;
     }
     default: {  // DT_I64
-      const longlong2 *p = reinterpret_cast<const longlong2 *>(static_cast<const int64_t *>(c.data) + row0);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) { longlong2 x = p[r]; v[2 * r] = x.x; v[2 * r + 1] = x.y; }
+      for (int r = 0; r < 4; ++r) {
+        const gv2l x = gld<gv2l>(c.data, row0 * 8 + 16 * r);
+        v[2 * r] = x.x;
+        v[2 * r + 1] = x.y;
+      }
       break;
     }
   }
@@ -107,14 +126,16 @@ __device__ __forceinline__ void load8_i(const DCol &c, uint64_t row0, int64_t v[
 
 __device__ __forceinline__ void load8_f(const DCol &c, uint64_t row0, double v[RPT]) {
   if (c.dtype == DT_F32) {
-    const float4 *p = reinterpret_cast<const float4 *>(static_cast<const float *>(c.data) + row0);
-    float4 a = p[0], b = p[1];
+    const gv4f a = gld<gv4f>(c.data, row0 * 4), b = gld<gv4f>(c.data, row0 * 4 + 16);
     v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
   } else {
-    const double2 *p = reinterpret_cast<const double2 *>(static_cast<const double *>(c.data) + row0);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) { double2 x = p[r]; v[2 * r] = x.x; v[2 * r + 1] = x.y; }
+    for (int r = 0; r < 4; ++r) {
+      const gv2d x = gld<gv2d>(c.data, row0 * 8 + 16 * r);
+      v[2 * r] = x.x;
+      v[2 * r + 1] = x.y;
+    }
   }
 }
 
@@ -122,25 +143,25 @@ __device__ __forceinline__ bool dt_is_float(int32_t dt) { return dt == DT_F64 ||
 
 // 8 null bits (bit r = row0 + r is null). row0 % 8 == 0 -> one byte.
 __device__ __forceinline__ uint32_t load8_nulls(const DCol &c, uint64_t row0) {
-  return c.nulls ? (uint32_t)c.nulls[row0 >> 3] : 0u;
+  return c.nulls ? (uint32_t)gld<uint8_t>(c.nulls, row0 >> 3) : 0u;
 }
 
 // single-row loads (gathers: join build side, compaction)
 __device__ __forceinline__ int64_t load1_i(const DCol &c, uint64_t row) {
   switch (c.dtype) {
-    case DT_I8: return static_cast<const int8_t *>(c.data)[row];
-    case DT_BOOL: return static_cast<const uint8_t *>(c.data)[row];
-    case DT_I16: return static_cast<const int16_t *>(c.data)[row];
-    case DT_I32: return static_cast<const int32_t *>(c.data)[row];
-    default: return static_cast<const int64_t *>(c.data)[row];
+    case DT_I8: return gld<int8_t>(c.data, row);
+    case DT_BOOL: return gld<uint8_t>(c.data, row);
+    case DT_I16: return gld<int16_t>(c.data, row * 2);
+    case DT_I32: return gld<int32_t>(c.data, row * 4);
+    default: return gld<int64_t>(c.data, row * 8);
   }
 }
 __device__ __forceinline__ double load1_f(const DCol &c, uint64_t row) {
-  if (c.dtype == DT_F32) return static_cast<const float *>(c.data)[row];
-  return static_cast<const double *>(c.data)[row];
+  if (c.dtype == DT_F32) return gld<float>(c.data, row * 4);
+  return gld<double>(c.data, row * 8);
 }
 __device__ __forceinline__ bool load1_null(const DCol &c, uint64_t row) {
-  return c.nulls ? ((c.nulls[row >> 3] >> (row & 7)) & 1) : false;
+  return c.nulls ? ((gld<uint8_t>(c.nulls, row >> 3) >> (row & 7)) & 1) : false;
 }
 
 // ---- predicate evaluation on one value ----
@@ -149,10 +170,12 @@ __device__ __forceinline__ bool pred_int(const DPred &p, int64_t v) {
   if (p.kind == PK_INT_RANGE) {
     in = (v >= p.ilo) && (v <= p.ihi);
   } else {  // PK_INT_IN: binary search in the sorted list
-    const int64_t *L = static_cast<const int64_t *>(p.list);
     int lo = 0, hi = p.list_n;
-    while (lo < hi) { int mid = (lo + hi) >> 1; if (L[mid] < v) lo = mid + 1; else hi = mid; }
-    in = (lo < p.list_n) && (L[lo] == v);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (gld<int64_t>(p.list, (uint64_t)mid * 8) < v) lo = mid + 1; else hi = mid;
+    }
+    in = (lo < p.list_n) && (gld<int64_t>(p.list, (uint64_t)lo * 8) == v);
   }
   return in != (p.negate != 0);
 }
@@ -168,10 +191,12 @@ __device__ __forceinline__ bool pred_f64(const DPred &p, double v) {
     bool b = p.hi_strict ? (v < p.fhi) : (v <= p.fhi);
     in = a && b;
   } else {
-    const double *L = static_cast<const double *>(p.list);
     int lo = 0, hi = p.list_n;
-    while (lo < hi) { int mid = (lo + hi) >> 1; if (L[mid] < v) lo = mid + 1; else hi = mid; }
-    in = (lo < p.list_n) && (L[lo] == v);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (gld<double>(p.list, (uint64_t)mid * 8) < v) lo = mid + 1; else hi = mid;
+    }
+    in = (lo < p.list_n) && (gld<double>(p.list, (uint64_t)lo * 8) == v);
   }
   return in != (p.negate != 0);
 }

@@ -22,7 +22,15 @@ __device__ __forceinline__ int64_t u64s(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 __device__ __forceinline__ double f64s(double x) { return bits_f64((uint64_t)u64s((int64_t)f64_bits(x))); }
-template <class T> __device__ __forceinline__ T *ptrs(T *p) { return (T *)(uintptr_t)u64s((int64_t)(uintptr_t)p); }
+// A column pointer pinned to scalar registers. The integer round trip would
+// hide that it points to global memory (loads through it would become flat
+// loads, which also count against LDS waits), so it goes back to a pointer
+// through the global address space.
+template <class T> __device__ __forceinline__ T *ptrs(T *p) {
+  using G = __attribute__((address_space(1))) T;
+  G *g = (G *)(uintptr_t)u64s((int64_t)(uintptr_t)p);
+  return (T *)g;
+}
 
 struct FastFilter {
   int n;
