@@ -97,18 +97,6 @@ __device__ __forceinline__ uint64_t rcode(const void *data, int32_t dt, uint64_t
   return code_i64(load1_i(c, row));
 }
 
-// contribution of one row to state s (0 = the state's identity)
-__device__ __forceinline__ uint64_t rcontrib(const RScan &rs, int s, uint64_t row) {
-  const DState &st = rs.st[s];
-  if (rnull(rs.snull[s], row)) return 0;
-  if (st.kind == SK_COUNT) return 1;
-  const DCol c = rcol(rs.scol[s], rs.s_dt[s]);
-  Contrib k;
-  if (dt_is_float(rs.s_dt[s])) k = contrib_of(st, true, 0, load1_f(c, row));
-  else k = contrib_of(st, false, load1_i(c, row), 0.0);
-  return k.skip ? 0 : k.a;
-}
-
 __device__ __forceinline__ void rstore_i(void *out, int32_t dt, uint64_t i, int64_t v) {
   switch (dt) {
     case DT_I8: static_cast<int8_t *>(out)[i] = (int8_t)v; break;
@@ -227,16 +215,29 @@ __device__ __forceinline__ void rload(const RScan &rs, uint64_t p, int lane, RWi
 }
 
 // contribution of the window row to state s = 1 + j (0: the state's identity)
+// The run-sorted launches hold only COUNT / SUM_I64 (additive, states
+// 1..nsum) and MAX_CODE states (the rest), so the write pass takes the two
+// kinds apart; the values equal contrib_of's for those kinds.
 template <int NS>
-__device__ __forceinline__ uint64_t rcontrib_w(const RScan &rs, const RWin<NS> &w, int j, uint64_t row) {
+__device__ __forceinline__ uint64_t radd_w(const RScan &rs, const RWin<NS> &w, int j, uint64_t row) {
   const int s = 1 + j;
-  const DState &st = rs.st[s];
   if (rs.snull[s] && rdec_null(w.cn[j], row)) return 0;
-  if (st.kind == SK_COUNT) return 1;
-  Contrib k;
-  if (dt_is_float(rs.s_dt[s])) k = contrib_of(st, true, 0, rdec_f(w.cr[j], rs.s_dt[s]));
-  else k = contrib_of(st, false, rdec_i(w.cr[j], rs.s_dt[s], row), 0.0);
-  return k.skip ? 0 : k.a;
+  if (rs.st[s].kind == SK_COUNT) return 1;
+  return (uint64_t)rdec_i(w.cr[j], rs.s_dt[s], row);  // SK_SUM_I64
+}
+template <int NS>
+__device__ __forceinline__ uint64_t rmax_w(const RScan &rs, const RWin<NS> &w, int j, uint64_t row) {
+  const int s = 1 + j;
+  if (rs.snull[s] && rdec_null(w.cn[j], row)) return 0;
+  uint64_t code;
+  if (dt_is_float(rs.s_dt[s])) {
+    const double fv = rdec_f(w.cr[j], rs.s_dt[s]);
+    if (fv != fv) return 0;  // NaN never updates MIN / MAX
+    code = code_f64_bits(f64_bits(fv));
+  } else {
+    code = code_i64(rdec_i(w.cr[j], rs.s_dt[s], row));
+  }
+  return rs.st[s].is_min ? ~code : code;
 }
 
 // PASS 0: groups per (chunk, query) into rs.counts.
@@ -374,9 +375,9 @@ __global__ void __launch_bounds__(R_BLOCK) runs_kernel(RScan rs) {
         if (valid) {
           const uint32_t pos = atomicAdd(&cnt[e], 1u);
 #pragma unroll
-          for (int j = 0; j < NSUM; ++j) s_sum[wave][j][pos] = rcontrib_w<NS>(rs, cur, j, row);
+          for (int j = 0; j < NSUM; ++j) s_sum[wave][j][pos] = radd_w<NS>(rs, cur, j, row);
 #pragma unroll
-          for (int j = 0; j < NMAX; ++j) s_tab[wave][j][0][pos] = rcontrib_w<NS>(rs, cur, NSUM + j, row);
+          for (int j = 0; j < NMAX; ++j) s_tab[wave][j][0][pos] = rmax_w<NS>(rs, cur, NSUM + j, row);
         } else {
           // positions n..63 of level 0 hold the identity (the sorted rows fill 0..n-1)
 #pragma unroll
