@@ -86,7 +86,7 @@ typedef struct qm_stats {
   uint32_t kernel_launches;
   uint32_t retries;           /* hash-table regrowths */
   uint32_t strategy;          /* group-by strategy picked by the tuner (QM_STRAT_*) */
-  uint32_t _pad;
+  uint32_t pages_skipped;     /* zone-map pages (65,536 rows) not read (QueryResult.pagesSkipped) */
 } qm_stats;
 
 #define QM_STRAT_NONE        0
@@ -171,7 +171,7 @@ int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *
  * reference counterpart (the reference has no device planner). Knobs:
  * scatter_rpb, part_load, hot, qp_min, runs_min_q, runs_min_len,
  * runs_chunks_per_cu, runs_win, runs_qblock, runs_lds_pad, shared_lean, shared_waves, sorted1,
- * join_unique;
+ * join_unique, zone_maps;
  * values are clamped to each
  * knob's range; an unknown name is QM_ERR_INVALID_ARG. */
 int qm_tune_set(qm_ctx *ctx, const char *name, double value);

@@ -33,7 +33,7 @@ class Stats(C.Structure):
         ("rows_scanned", C.c_uint64), ("rows_selected", C.c_uint64), ("groups", C.c_uint64),
         ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double), ("scan_ms", C.c_double),
         ("wall_ms", C.c_double),
-        ("kernel_launches", C.c_uint32), ("retries", C.c_uint32), ("strategy", C.c_uint32), ("_pad", C.c_uint32),
+        ("kernel_launches", C.c_uint32), ("retries", C.c_uint32), ("strategy", C.c_uint32), ("pages_skipped", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

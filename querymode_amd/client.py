@@ -217,7 +217,8 @@ def _batch_results(lib, outs, n: int, t0: float) -> list[QueryResult]:
         data, st = _read_result(lib, C.c_void_p(outs[i]))
         nrow, names = _qmcb_header(data)
         res.append(QueryResult(rowCount=nrow, columns=names,
-                               columnarData=data, durationMs=(time.perf_counter() - t0) * 1e3, stats=st))
+                               columnarData=data, durationMs=(time.perf_counter() - t0) * 1e3, stats=st,
+                               pagesSkipped=int(st.get("pages_skipped", 0))))
     return res
 
 
@@ -287,7 +288,8 @@ class GpuExecutor:
         data, st = self.run_qmcb(desc)
         rows, names = _qmcb_header(data)
         return QueryResult(rowCount=rows, columns=names, columnarData=data,
-                           durationMs=(time.perf_counter() - t0) * 1e3, stats=st)
+                           durationMs=(time.perf_counter() - t0) * 1e3, stats=st,
+                           pagesSkipped=int(st.get("pages_skipped", 0)))
 
     def _batch_table(self, descs: list[QueryDescriptor]) -> GpuTable:
         tname = descs[0].table

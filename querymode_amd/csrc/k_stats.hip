@@ -87,3 +87,50 @@ extern "C" hipError_t qmk_col_stats(const DCol *c, uint64_t n, uint64_t *stats, 
   hipLaunchKernelGGL(col_stats_kernel, dim3((unsigned)g), dim3(256), 0, st, *c, n, stats, hll);
   return hipGetLastError();
 }
+
+namespace qm {
+
+// Zone maps: per page of ZONE_ROWS rows, the min and max order code of its
+// non-NULL values (a page of NULLs only: min ~0, max 0), so a range filter
+// can skip pages it cannot match (the reference's canSkipPage on page
+// min/max, src/decode.ts:9-100). One workgroup per page.
+__global__ void __launch_bounds__(256) zone_kernel(DCol c, uint64_t n, uint64_t *zmin, uint64_t *zmax) {
+  __shared__ uint64_t s_mn[4], s_mx[4];
+  const uint64_t p = blockIdx.x;
+  const uint64_t r0 = p * ZONE_ROWS, r1 = (r0 + ZONE_ROWS < n) ? r0 + ZONE_ROWS : n;
+  const bool isf = dt_is_float(c.dtype);
+  uint64_t mn = ~0ull, mx = 0;
+  for (uint64_t r = r0 + threadIdx.x; r < r1; r += 256) {
+    if (load1_null(c, r)) continue;
+    const uint64_t code = isf ? code_f64_bits(f64_bits(load1_f(c, r))) : code_i64(load1_i(c, r));
+    mn = code < mn ? code : mn;
+    mx = code > mx ? code : mx;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
+    mn = a < mn ? a : mn;
+    mx = b > mx ? b : mx;
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s_mn[w] = mn; s_mx[w] = mx; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; ++k) {
+      mn = s_mn[k] < mn ? s_mn[k] : mn;
+      mx = s_mx[k] > mx ? s_mx[k] : mx;
+    }
+    zmin[p] = mn;
+    zmax[p] = mx;
+  }
+}
+
+}  // namespace qm
+
+using namespace qm;
+
+extern "C" hipError_t qmk_zone_maps(const DCol *c, uint64_t n, uint64_t *zmin, uint64_t *zmax, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const uint64_t pages = (n + ZONE_ROWS - 1) / ZONE_ROWS;
+  hipLaunchKernelGGL(zone_kernel, dim3((unsigned)pages), dim3(256), 0, st, *c, n, zmin, zmax);
+  return hipGetLastError();
+}

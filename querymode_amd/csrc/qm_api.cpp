@@ -1011,6 +1011,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.sorted1; }},
     {"join_unique", "QM_TUNE_JOIN_UNIQUE", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_unique = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
+    {"zone_maps", "QM_TUNE_ZONE_MAPS", 0, 1, [](qm_ctx::Tune &t, double v) { t.zone_maps = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.zone_maps; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
 };
@@ -1168,6 +1170,15 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
     std::vector<uint32_t> regs(4096);
     d2h(ctx, sw, stats.p, 8);
     d2h(ctx, regs.data(), hll.p, 4096);
+    if (nrows) {
+      const uint64_t pages = (nrows + ZONE_ROWS - 1) / ZONE_ROWS;
+      DevBuf zb(ctx, pages * 16 + 64);
+      HIPX(qmk_zone_maps(&dc, nrows, zb.as<uint64_t>(), zb.as<uint64_t>() + pages, ctx->stream));
+      tc.zmin.resize(pages);
+      tc.zmax.resize(pages);
+      d2h(ctx, tc.zmin.data(), zb.p, pages);
+      d2h(ctx, tc.zmax.data(), zb.as<uint64_t>() + pages, pages);
+    }
     ColStats &s = tc.st;
     s.nan = sw[3];
     s.nulls = sw[4];
@@ -2800,6 +2811,60 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
   }
 }
 
+// Zone maps: the pages [p0, p1] that an AND of range filters can match at
+// either end of the table. Returns true (and a view of those rows in `v`)
+// when pages are skipped; the view shares the table's device columns
+// (offsets of whole pages keep every column aligned; a qm_table releases
+// nothing by itself). Only aggregate queries take it: they emit no row ids.
+bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint32_t *skipped) {
+  *skipped = 0;
+  if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || pl.has_join || !pl.computed.empty() || pl.filters.empty()) return false;
+  ColSpace cs;
+  cs.L = T;
+  cs.plan = &pl;
+  DQuery q;
+  memset(&q, 0, sizeof q);
+  ListStore ls;
+  try {
+    bind_filters(ctx, cs, pl.filters, pl.groups, q, ls, false);
+  } catch (const Error &) {
+    return false;  // the query itself reports the error
+  }
+  if (!q.and_only || q.ngroups) return false;
+  const uint64_t pages = (T->rows + ZONE_ROWS - 1) / ZONE_ROWS;
+  std::vector<uint8_t> can(pages, 1);
+  bool any = false;
+  for (int i = 0; i < q.npreds; ++i) {
+    uint64_t lo = 0, hi = 0;
+    bool never = false;
+    if (!pred_code_range(q.preds[i], lo, hi, never) || never) continue;
+    const TCol *c = cs.tcols[q.slot_col[q.preds[i].slot]];
+    if (!c || c->zmin.size() != pages) continue;
+    any = true;
+    for (uint64_t p = 0; p < pages; ++p)
+      if (c->zmax[p] < lo || c->zmin[p] > hi || c->zmin[p] > c->zmax[p]) can[p] = 0;
+  }
+  if (!any) return false;
+  uint64_t p0 = 0, p1 = pages - 1;
+  while (p0 < pages && !can[p0]) ++p0;
+  if (p0 == pages) p0 = p1 = 0;  // nothing can match: one page keeps the launch shapes
+  else while (!can[p1]) --p1;
+  if (p0 == 0 && p1 == pages - 1) return false;
+  const uint64_t r0 = p0 * ZONE_ROWS, r1 = std::min(T->rows, (p1 + 1) * ZONE_ROWS);
+  v.ctx = T->ctx;
+  v.name = T->name;
+  v.rows = r1 - r0;
+  v.cols = T->cols;
+  for (TCol &c : v.cols) {
+    c.d = static_cast<uint8_t *>(c.d) + r0 * (uint64_t)dt_size(c.dtype);
+    if (c.dnull) c.dnull += r0 / 8;
+    c.zmin.clear();
+    c.zmax.clear();
+  }
+  *skipped = (uint32_t)(pages - (p1 - p0 + 1));
+  return true;
+}
+
 }  // namespace
 
 extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out) {
@@ -2840,22 +2905,27 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       results[0]->stats.device_ms += bms;
       results[0]->stats.kernel_launches += l;
     } else {
+      // pages the filter cannot match at either end are not read (zone maps)
+      qm_table view;
+      uint32_t skipped = 0;
+      qm_table *T = zone_view(ctx, left, pl, view, &skipped) ? &view : left;
       // a key column stored in order: segmented reduction over its runs
       BItem it;
       it.idx = 0;
       it.plan = &pl;
-      batch_classify(ctx, left, pl, it);
+      batch_classify(ctx, T, pl, it);
       try {
         if (it.kind == BK_SORTED || (it.kind == BK_SHARED && ctx->tune.shared_lean)) {
           std::vector<BItem *> items{&it};
-          run_batch(ctx, left, items, results, nullptr);
+          run_batch(ctx, T, items, results, nullptr);
         } else {
-          run_aggregates(ctx, left, nullptr, plans, results, false, nullptr);
+          run_aggregates(ctx, T, nullptr, plans, results, false, nullptr);
         }
       } catch (...) {
         delete results[0];
         throw;
       }
+      results[0]->stats.pages_skipped = skipped;
     }
     res.reset(results[0]);
   } else {

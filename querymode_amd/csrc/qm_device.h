@@ -25,6 +25,8 @@ static constexpr int RPT = 8;
 // Columns are allocated padded to this many rows so full-tile vector loads
 // never leave the allocation.
 static constexpr uint64_t ROW_PAD = 8192;
+// rows per zone-map page (per-page min / max order codes kept at upload)
+static constexpr uint64_t ZONE_ROWS = 65536;
 
 struct DCol {
   const void *data;

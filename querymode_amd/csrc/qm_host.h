@@ -122,6 +122,7 @@ struct qm_ctx {
     int shared_waves = 4;      // lean pass: workgroup rounds per resident slot
     int sorted1 = 1;           // a single sorted-key query takes the lean single-query pass
     int join_unique = 1;       // DIRECT join indexes of unique build keys hold the build row itself
+    int zone_maps = 1;         // single aggregate queries skip the pages a range filter cannot match
   } tune;
 
   void *alloc(size_t bytes);
@@ -173,6 +174,9 @@ struct TCol {
   uint8_t *dnull = nullptr;
   size_t null_bytes = 0;
   ColStats st;
+  // zone maps: per ZONE_ROWS-row page, min / max order code of the non-NULL
+  // values (empty page: min ~0, max 0)
+  std::vector<uint64_t> zmin, zmax;
 };
 
 }  // namespace qm

@@ -149,4 +149,6 @@ hipError_t qmk_sorted1_scan(const qm::S1Scan *q, hipStream_t st);
 // column statistics: min/max codes, finite max |x|, NaN count, HLL registers
 hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 words*/, uint32_t *hll /*4096*/,
                          hipStream_t st);
+// zone maps: per ZONE_ROWS-row page, min / max order code of the non-NULL values
+hipError_t qmk_zone_maps(const qm::DCol *c, uint64_t n, uint64_t *zmin, uint64_t *zmax, hipStream_t st);
 }

@@ -530,6 +530,7 @@ static napi_value js_result_stats(napi_env env, napi_callback_info info) {
   set_num(env, o, "kernelLaunches", st.kernel_launches);
   set_num(env, o, "retries", st.retries);
   set_num(env, o, "strategy", st.strategy);
+  set_num(env, o, "pagesSkipped", st.pages_skipped);
   return o;
 }
 

@@ -202,7 +202,7 @@ class GpuExecutor {
     const batch = decodeQMCB(r.qmcb);
     const rows = qmcbToRows(batch);
     return { rows, rowCount: rows.length, columns: batch.columns.map((c) => c.name), columnarData: r.qmcb,
-      bytesRead: r.stats.bytesAlgorithmic, pagesSkipped: 0, durationMs: Date.now() - t0, stats: r.stats };
+      bytesRead: r.stats.bytesAlgorithmic, pagesSkipped: r.stats.pagesSkipped, durationMs: Date.now() - t0, stats: r.stats };
   }
 
   /**
@@ -226,7 +226,7 @@ class GpuExecutor {
           const batch = decodeQMCB(qmcb);
           const rows = qmcbToRows(batch);
           return { rows, rowCount: rows.length, columns: batch.columns.map((c) => c.name), columnarData: qmcb,
-            bytesRead: stats.bytesAlgorithmic, pagesSkipped: 0, durationMs: Date.now() - t0, stats };
+            bytesRead: stats.bytesAlgorithmic, pagesSkipped: stats.pagesSkipped, durationMs: Date.now() - t0, stats };
         } finally { addon.resultClose(res); }
       });
     } finally { plans.forEach((p) => addon.planClose(p)); }
