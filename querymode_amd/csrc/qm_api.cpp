@@ -2815,7 +2815,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
 // either end of the table. Returns true (and a view of those rows in `v`)
 // when pages are skipped; the view shares the table's device columns
 // (offsets of whole pages keep every column aligned; a qm_table releases
-// nothing by itself). Only aggregate queries take it: they emit no row ids.
+// nothing by itself). Joins read their whole probe table.
 bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint32_t *skipped) {
   *skipped = 0;
   if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || pl.has_join || !pl.computed.empty() || pl.filters.empty()) return false;
@@ -2930,7 +2930,13 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
     res.reset(results[0]);
   } else {
     res.reset(new qm_result());
-    run_rows_query(ctx, left, right, pl, res.get());
+    // row outputs over a view: the selected rows, their order (row id ties)
+    // and the gathered values are all taken within the same view
+    qm_table view;
+    uint32_t skipped = 0;
+    qm_table *T = zone_view(ctx, left, pl, view, &skipped) ? &view : left;
+    run_rows_query(ctx, T, right, pl, res.get());
+    res->stats.pages_skipped = skipped;
   }
   HIPX(hipStreamSynchronize(ctx->stream));
   res->stats.wall_ms =

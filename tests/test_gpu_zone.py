@@ -13,7 +13,7 @@ import pytest
 from oracle import oracle as O
 from querymode_amd.plan import AggregateOp, FilterOp, QueryDescriptor
 
-from ._compare import assert_groups_equal
+from ._compare import assert_groups_equal, assert_rows_equal
 
 pytestmark = pytest.mark.gpu
 
@@ -114,3 +114,44 @@ def test_zone_maps_skip_pages_and_keep_results(qm):
         dev.tune_set("zone_maps", 1)
     # the first cases do skip pages
     assert _expected_skipped(cols, nulls, cases[0][1]) == 42
+
+
+def test_zone_maps_row_outputs(qm):
+    """Top-K, ORDER BY and plain selections over the skipped-page view: the
+    same rows in the same order (ties by row id) as the oracle and as the
+    query with zone maps off."""
+    _say("zone maps: row outputs")
+    cols, nulls = _table()
+    qm.register_table("z", cols, nulls)
+    ot = O.Table(cols, nulls)
+    k = cols["k"]
+    lo, hi = int(k[20 * Z + 5]), int(k[23 * Z + 100])
+    rng_k = [FilterOp("k", "between", [lo, hi])]
+    descs = [
+        (QueryDescriptor("z", filters=rng_k, projections=["k", "v"], sortColumn="v", sortDirection="desc", limit=100),
+         42),
+        (QueryDescriptor("z", filters=rng_k + [FilterOp("x", "lt", 0.01)], projections=["k", "x", "v"],
+                         limit=50, offset=7), 42),
+        (QueryDescriptor("z", filters=[FilterOp("t", "between", [40.0, 40.05])], projections=["t", "g"],
+                         sortColumn="t", sortDirection="asc"), None),
+        (QueryDescriptor("z", filters=[FilterOp("k", "lt", int(k[Z + 3]))], projections=["k", "v", "t"],
+                         sortColumn="k", sortDirection="desc", limit=10, offset=2), 44),
+    ]
+    dev = qm.device
+    ex = qm.executor
+    try:
+        for i, (d, skipped) in enumerate(descs):
+            _say(f"  rows case {i}")
+            dev.tune_set("zone_maps", 1)
+            got = ex.execute(d)
+            dev.tune_set("zone_maps", 0)
+            off = ex.execute(d)
+            assert bytes(got.columnarData) == bytes(off.columnarData), i
+            want = O.run(d, {"z": ot})
+            assert_rows_equal(got.rows, want, ctx=f"zone rows case {i}")
+            if skipped is not None:
+                assert got.pagesSkipped == skipped, (i, got.pagesSkipped)
+            else:
+                assert got.pagesSkipped > 0, i
+    finally:
+        dev.tune_set("zone_maps", 1)
