@@ -230,6 +230,106 @@ __device__ void join_apply(const DScan &sc, const DQuery &q, uint64_t prow, uint
 
 }
 
+// The joined rows of one lane's 8 probe rows (unique build keys: b1[r] =
+// build row + 1, 0 = no match), with the loads of all 8 issued together:
+// first every key gather, then per state every value load, then the 8
+// updates. The values and updates equal join_apply's row by row (the updates
+// are commutative: exact sums, counts, MIN / MAX codes).
+__device__ void join_apply8(const DScan &sc, const DQuery &q, uint64_t row0, const uint32_t b1[RPT], lds_u64 *lds) {
+  const int W = q.entry_words;
+  uint64_t ent[RPT];
+  {
+    int64_t v[MAX_KEYS][RPT];
+    uint32_t n[MAX_KEYS][RPT];
+#pragma unroll
+    for (int i = 0; i < MAX_KEYS; ++i) {
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) { v[i][r] = 0; n[i][r] = 0; }
+      if (i >= q.key.ncols) continue;
+      const DCol &c = sc.cols[q.key.col[i]];
+      const bool isf = q.key.is_float[i] != 0;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        if (!b1[r]) continue;
+        const uint64_t row = c.side ? (uint64_t)(b1[r] - 1u) : row0 + r;
+        n[i][r] = load1_null(c, row);
+        v[i][r] = isf ? (int64_t)canon_f64_key_bits(f64_bits(load1_f(c, row))) : load1_i(c, row);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      ent[r] = ~0ull;
+      if (!b1[r]) continue;
+      int64_t kv[MAX_KEYS];
+      uint32_t kn[MAX_KEYS];
+#pragma unroll
+      for (int i = 0; i < MAX_KEYS; ++i) { kv[i] = v[i][r]; kn[i] = n[i][r]; }
+      if (q.key.ncols == 0) {
+        ent[r] = (q.lmode == LM_NONE) ? (1ull << 63) : 0;
+      } else if (q.gmode == GM_DIRECT) {
+        const uint64_t idx = key_index_direct(q.key, kv, kn);
+        ent[r] = (q.lmode == LM_DIRECT) ? idx : (idx | (1ull << 63));
+      } else {
+        int side;
+        const uint64_t stored = key_code_hash(q.key, kv, kn, &side);
+        if (side >= 0) {
+          ent[r] = (q.gcap + (uint64_t)side) | (1ull << 63);
+        } else {
+          const uint64_t g = glb_find(q, stored);
+          ent[r] = (g == ~0ull) ? ~0ull : (g | (1ull << 63));
+        }
+      }
+    }
+  }
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t rep_base = (q.lmode == LM_DIRECT)
+      ? (uint64_t)(lane & (uint32_t)(q.lds_replicas - 1)) * q.lds_stride : 0;
+  for (int si = 0; si < q.nstates; ++si) {
+    const DState &s = q.states[si];
+    int64_t iv[RPT];
+    double fv[RPT];
+    bool isnull[RPT];
+    bool isf = false;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) { iv[r] = 0; fv[r] = 0.0; isnull[r] = false; }
+    if (s.col_a >= 0) {
+      const DCol &a = sc.cols[s.col_a];
+      if (s.product) {
+        const DCol &b = sc.cols[s.col_b];
+        isf = true;
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          if (ent[r] == ~0ull) continue;
+          const uint64_t ra = a.side ? (uint64_t)(b1[r] - 1u) : row0 + r;
+          const uint64_t rb = b.side ? (uint64_t)(b1[r] - 1u) : row0 + r;
+          isnull[r] = load1_null(a, ra) || load1_null(b, rb);
+          const double x = dt_is_float(a.dtype) ? load1_f(a, ra) : (double)load1_i(a, ra);
+          const double y = dt_is_float(b.dtype) ? load1_f(b, rb) : (double)load1_i(b, rb);
+          fv[r] = x * y;
+        }
+      } else {
+        isf = dt_is_float(a.dtype);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          if (ent[r] == ~0ull) continue;
+          const uint64_t ra = a.side ? (uint64_t)(b1[r] - 1u) : row0 + r;
+          isnull[r] = load1_null(a, ra);
+          if (isf) fv[r] = load1_f(a, ra);
+          else iv[r] = load1_i(a, ra);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      if (ent[r] == ~0ull || isnull[r]) continue;  // no match / this state skips a NULL value
+      const Contrib k = contrib_of(s, isf, iv[r], fv[r]);
+      if (k.skip) continue;
+      if (ent[r] >> 63) apply_glb(s, q.table + (ent[r] & ~(1ull << 63)) * W, k);
+      else apply_lds(s, lds + q.lds_offset + rep_base + ent[r] * W, k);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *__restrict__ qs) {
   extern __shared__ uint64_t lds_raw[];
   // LDS: [0, MAX_QUERIES) selected-row counters, then the query tables.
@@ -284,11 +384,9 @@ __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *_
             if (idx < sc.join_cap) b1[r] = sc.join_unique[idx];
           }
         }
-        for (int r = 0; r < RPT; ++r) {
-          if (!b1[r]) continue;
-          ++nmatch;
-          join_apply(sc, q, row0 + r, b1[r] - 1u, lds);
-        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) nmatch += b1[r] ? 1u : 0u;
+        join_apply8(sc, q, row0, b1, lds);
       } else {
         for (int r = 0; r < RPT; ++r) {
           if (!((m >> r) & 1)) continue;
