@@ -98,7 +98,7 @@ def bench_zipf(args) -> dict:
     alg_bytes = stats[-1].bytes_algorithmic
     achieved = alg_bytes / (scan_ms * 1e-3) / 1e9
     strategy = N.STRATEGIES.get(stats[-1].strategy)
-    kname = "part_scan_fast_kernel<3, 1>" if strategy in ("part_hash", "part_direct") else "scan_agg_kernel"
+    kname = "part_scan_fast_kernel<3, 1>" if strategy in ("part_hash", "part_direct") else "scan_agg_kernel<false>"
     if args.copy_calibration:
         copy_calibration(1 << 30)
     cpu = None
@@ -381,7 +381,7 @@ def bench_join(args) -> dict:
                    "strategy": N.STRATEGIES.get(stats[-1].strategy), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": alg / (scan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": alg / (scan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
-        "kernel": {"name": "scan_agg_kernel (fused join probe)", "avg_ms": scan_ms, "algorithmic_bytes": alg,
+        "kernel": {"name": "scan_agg_kernel<true> (fused join probe)", "avg_ms": scan_ms, "algorithmic_bytes": alg,
                    "query_device_ms": float(np.mean([s.device_ms for s in stats]))},
         "cpu_baseline": cpu,
         "setup": {"generate_s": gen_s, "upload_s": upload_s},

@@ -25,7 +25,9 @@ __device__ __forceinline__ uint64_t key_code_hash(const DKey &k, const int64_t k
   // packed codes: code_i = v - kmin (null -> dom - 1)
   uint64_t packed = 0;
   int shift = 0;
-  for (int i = 0; i < k.ncols; ++i) {
+#pragma unroll
+  for (int i = 0; i < MAX_KEYS; ++i) {  // fixed bound: the key arrays stay in registers
+    if (i >= k.ncols) break;
     uint64_t code = knull[i] ? (k.dom[i] - 1) : (uint64_t)(kv[i] - k.kmin[i]);
     packed |= code << shift;
     shift += k.bits[i];
@@ -36,7 +38,9 @@ __device__ __forceinline__ uint64_t key_code_hash(const DKey &k, const int64_t k
 __device__ __forceinline__ uint64_t key_index_direct(const DKey &k, const int64_t kv[MAX_KEYS],
                                                      const uint32_t knull[MAX_KEYS]) {
   uint64_t idx = 0;
-  for (int i = 0; i < k.ncols; ++i) {
+#pragma unroll
+  for (int i = 0; i < MAX_KEYS; ++i) {  // fixed bound: the key arrays stay in registers
+    if (i >= k.ncols) break;
     uint64_t code = knull[i] ? (k.dom[i] - 1) : (uint64_t)(kv[i] - k.kmin[i]);
     idx += code * k.stride[i];
   }

@@ -330,6 +330,9 @@ __device__ void join_apply8(const DScan &sc, const DQuery &q, uint64_t row0, con
   }
 }
 
+// JOIN: the fused join probe (one query); else up to MAX_QUERIES queries over
+// one table. Two instantiations, so each keeps only its own registers.
+template <bool JOIN>
 __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *__restrict__ qs) {
   extern __shared__ uint64_t lds_raw[];
   // LDS: [0, MAX_QUERIES) selected-row counters, then the query tables.
@@ -351,7 +354,7 @@ __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *_
       uint64_t left = sc.nrows - row0;
       valid = left >= RPT ? 0xFFu : ((1u << left) - 1);
     }
-    if (sc.join_mode == 0) {
+    if constexpr (!JOIN) {
       for (int qi = 0; qi < nq; ++qi) {
         const DQuery &q = qs[qi];
         uint32_t m = eval_query8(q, sc.cols, row0, valid);
@@ -423,13 +426,15 @@ __global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t v) {
 // ---- host launchers ----
 extern "C" hipError_t qmk_scan_agg(const qm::DScan *sc, const qm::DQuery *d_queries, int grid,
                                    int block, size_t lds_bytes, hipStream_t st) {
+  const void *fn = sc->join_mode ? (const void *)qm::scan_agg_kernel<true> : (const void *)qm::scan_agg_kernel<false>;
   if (lds_bytes > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)qm::scan_agg_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(qm::scan_agg_kernel, dim3(grid), dim3(block), lds_bytes, st, *sc, d_queries);
-  return hipGetLastError();
+  qm::DScan a = *sc;
+  const qm::DQuery *q = d_queries;
+  void *args[] = {&a, &q};
+  return hipLaunchKernel(fn, dim3(grid), dim3(block), args, lds_bytes, st);
 }
 
 extern "C" hipError_t qmk_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st) {
