@@ -235,7 +235,8 @@ __device__ void join_apply(const DScan &sc, const DQuery &q, uint64_t prow, uint
 // first every key gather, then per state every value load, then the 8
 // updates. The values and updates equal join_apply's row by row (the updates
 // are commutative: exact sums, counts, MIN / MAX codes).
-__device__ void join_apply8(const DScan &sc, const DQuery &q, uint64_t row0, const uint32_t b1[RPT], lds_u64 *lds) {
+__device__ __forceinline__ void join_apply8(const DScan &sc, const DQuery &q, uint64_t row0, const uint32_t b1[RPT],
+                                            lds_u64 *lds) {
   const int W = q.entry_words;
   uint64_t ent[RPT];
   {
@@ -333,7 +334,13 @@ __device__ void join_apply8(const DScan &sc, const DQuery &q, uint64_t row0, con
 // JOIN: the fused join probe (one query); else up to MAX_QUERIES queries over
 // one table. Two instantiations, so each keeps only its own registers.
 template <bool JOIN>
-__global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc, const DQuery *__restrict__ qs) {
+__global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc_arg, const DQuery *__restrict__ qs) {
+  // The descriptor is read in place in the kernel-argument segment (it is the
+  // first argument, at offset 0): its column array is indexed with
+  // per-query slots, which on the by-value copy would move the whole struct
+  // to per-lane scratch memory; in place they are scalar loads.
+  (void)sc_arg;
+  const DScan &sc = *(const DScan *)__builtin_amdgcn_kernarg_segment_ptr();
   extern __shared__ uint64_t lds_raw[];
   // LDS: [0, MAX_QUERIES) selected-row counters, then the query tables.
   lds_u64 *lds_cnt = (lds_u64 *)lds_raw;
