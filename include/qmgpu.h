@@ -171,7 +171,7 @@ int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *
  * reference counterpart (the reference has no device planner). Knobs:
  * scatter_rpb, part_load, hot, qp_min, runs_min_q, runs_min_len,
  * runs_chunks_per_cu, runs_win, runs_qblock, runs_lds_pad, shared_lean, shared_waves, sorted1,
- * join_unique, zone_maps;
+ * join_unique, zone_maps, topk_bpc;
  * values are clamped to each
  * knob's range; an unknown name is QM_ERR_INVALID_ARG. */
 int qm_tune_set(qm_ctx *ctx, const char *name, double value);

@@ -771,7 +771,7 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
     sc.nqueries = 1;
     sc.cols[0] = c;
     const uint64_t tiles = (n + 2047) / 2048;
-    int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * 2);
+    int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * (uint64_t)ctx->tune.topk_bpc);
     if (grid < 1) grid = 1;
     DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
     DevBuf counts(ctx, (size_t)grid * 4 + 64);
@@ -1013,6 +1013,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
     {"zone_maps", "QM_TUNE_ZONE_MAPS", 0, 1, [](qm_ctx::Tune &t, double v) { t.zone_maps = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.zone_maps; }},
+    {"topk_bpc", "QM_TUNE_TOPK_BPC", 1, 16, [](qm_ctx::Tune &t, double v) { t.topk_bpc = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.topk_bpc; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_lds_pad; }},
 };
@@ -1637,7 +1639,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
       h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
       uint64_t k = (uint64_t)(pl.limit + pl.offset);
       const uint64_t tiles = std::max<uint64_t>(1, (T->rows + 2047) / 2048);
-      int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * 2);
+      int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * (uint64_t)ctx->tune.topk_bpc);
       uint64_t take = 0;
       if (k > 0) {
         DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
