@@ -71,9 +71,12 @@ __device__ uint32_t prune(TopKey *buf, uint32_t cap, uint32_t count, uint32_t k,
   return n;
 }
 
+// gthr: the smallest main order word of any workgroup's k-th candidate so far
+// (~0: none yet). A row whose word is above it is worse than k rows of that
+// workgroup, so no workgroup keeps it; each thread reads it one tile ahead.
 __global__ void __launch_bounds__(TK_BLOCK) topk_scan_kernel(DScan sc, const DQuery *__restrict__ qs, int sort_col,
                                                              int desc, uint32_t k, uint32_t cap, TopKey *cand,
-                                                             uint32_t *cand_counts) {
+                                                             uint32_t *cand_counts, uint64_t *gthr) {
   extern __shared__ uint64_t lds_raw[];
   TopKey *buf = reinterpret_cast<TopKey *>(lds_raw);
   __shared__ uint32_t s_count;
@@ -85,7 +88,10 @@ __global__ void __launch_bounds__(TK_BLOCK) topk_scan_kernel(DScan sc, const DQu
   const bool isf = dt_is_float(c.dtype);
   const uint64_t ntiles = (sc.nrows + TK_TILE - 1) / TK_TILE;
   uint32_t count = 0;
+  uint64_t ga_next = ~0ull;
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const uint64_t ga = ga_next;
+    ga_next = __hip_atomic_load(gthr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint64_t row0 = tile * TK_TILE + (uint64_t)threadIdx.x * RPT;
     uint32_t valid = 0;
     if (row0 < sc.nrows) {
@@ -95,6 +101,12 @@ __global__ void __launch_bounds__(TK_BLOCK) topk_scan_kernel(DScan sc, const DQu
     uint32_t m = (q.npreds > 0) ? eval_query8(q, sc.cols, row0, valid) : valid;
     if (m) {
       TopKey thr = s_thr;
+      if (ga != ~0ull) {
+        TopKey gt;
+        gt.a = ga + 1;
+        gt.b = 0;
+        if (key_less(gt, thr)) thr = gt;
+      }
       uint32_t nm = load8_nulls(c, row0);
       TopKey kk[RPT];
       if (isf) {
@@ -118,7 +130,10 @@ __global__ void __launch_bounds__(TK_BLOCK) topk_scan_kernel(DScan sc, const DQu
     count = s_count;
     if (count + TK_TILE > cap) {
       count = prune(buf, cap, count, k, &s_thr);
-      if (threadIdx.x == 0) s_count = count;
+      if (threadIdx.x == 0) {
+        s_count = count;
+        if (s_thr.a != ~0ull) __hip_atomic_fetch_min(gthr, s_thr.a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
       __syncthreads();
     }
   }
@@ -188,7 +203,8 @@ extern "C" uint32_t qmk_topk_cap(uint32_t k) {
 }
 
 extern "C" hipError_t qmk_topk_scan(const DScan *sc, const DQuery *d_q, int sort_col, int desc, uint32_t k,
-                                    int grid, TopKey *cand, uint32_t *cand_counts, hipStream_t st) {
+                                    int grid, TopKey *cand, uint32_t *cand_counts, uint64_t *gthr, hipStream_t st) {
+  if (!gthr) return hipErrorInvalidValue;
   uint32_t cap = qmk_topk_cap(k);
   if (!cap || k == 0) return hipErrorInvalidValue;
   if ((size_t)cap * sizeof(TopKey) > 65536) {
@@ -197,7 +213,7 @@ extern "C" hipError_t qmk_topk_scan(const DScan *sc, const DQuery *d_q, int sort
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(topk_scan_kernel, dim3(grid), dim3(TK_BLOCK), (size_t)cap * sizeof(TopKey), st, *sc, d_q,
-                     sort_col, desc, k, cap, cand, cand_counts);
+                     sort_col, desc, k, cap, cand, cand_counts, gthr);
   return hipGetLastError();
 }
 

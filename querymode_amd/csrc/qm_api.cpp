@@ -774,9 +774,10 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
     int grid = (int)std::min<uint64_t>(tiles, (uint64_t)ctx->num_cus * (uint64_t)ctx->tune.topk_bpc);
     if (grid < 1) grid = 1;
     DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
-    DevBuf counts(ctx, (size_t)grid * 4 + 64);
+    DevBuf counts(ctx, (size_t)grid * 4 + 64), gthr(ctx, 64);
+    HIPX(qmk_fill_u64(gthr.as<uint64_t>(), 1, ~0ull, ctx->stream));
     HIPX(qmk_topk_scan(&sc, d_noq, 0, desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(), counts.as<uint32_t>(),
-                       ctx->stream));
+                       gthr.as<uint64_t>(), ctx->stream));
     ++*launches;
     const TopKey *lists = cand.as<TopKey>();
     const uint32_t *lcounts = counts.as<uint32_t>();
@@ -1643,10 +1644,11 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
       uint64_t take = 0;
       if (k > 0) {
         DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
-        DevBuf ccount(ctx, (size_t)grid * 4 + 64);
+        DevBuf ccount(ctx, (size_t)grid * 4 + 64), gthr(ctx, 64);
+        HIPX(qmk_fill_u64(gthr.as<uint64_t>(), 1, ~0ull, ctx->stream));
         HIPX(hipEventRecord(ctx->evk0, ctx->stream));
         HIPX(qmk_topk_scan(&sc, dq.as<DQuery>(), sslot, pl.desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(),
-                           ccount.as<uint32_t>(), ctx->stream));
+                           ccount.as<uint32_t>(), gthr.as<uint64_t>(), ctx->stream));
         HIPX(hipEventRecord(ctx->evk1, ctx->stream));
         timed_scan = true;
         ++launches;

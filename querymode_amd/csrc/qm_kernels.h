@@ -82,8 +82,10 @@ hipError_t qmk_gather(const qm::DCol *c, const uint64_t *rows, const uint32_t *i
                       int32_t out_dt, void *out, uint32_t *nulls, hipStream_t st);
 
 // top-K: order keys from one sort column (optionally filtered by a query)
+// gthr: one word of device memory, set to ~0 by the caller before the launch
 hipError_t qmk_topk_scan(const qm::DScan *sc, const qm::DQuery *d_q, int sort_col, int desc,
-                         uint32_t k, int grid, qm::TopKey *cand, uint32_t *cand_counts, hipStream_t st);
+                         uint32_t k, int grid, qm::TopKey *cand, uint32_t *cand_counts, uint64_t *gthr,
+                         hipStream_t st);
 hipError_t qmk_topk_merge(const qm::TopKey *cand, const uint32_t *cand_counts, uint32_t nlists,
                           uint32_t lists_per_block, uint32_t k, qm::TopKey *out, uint32_t *out_count,
                           hipStream_t st);
