@@ -996,7 +996,7 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.runs_min_q; }},
     {"runs_min_len", "QM_TUNE_RUNS_MIN_LEN", 0, 1e12, [](qm_ctx::Tune &t, double v) { t.runs_min_len = v; },
      [](const qm_ctx::Tune &t) { return t.runs_min_len; }},
-    {"runs_chunks_per_cu", "QM_TUNE_RUNS_CHUNKS", 1, 1024,
+    {"runs_chunks_per_cu", "QM_TUNE_RUNS_CHUNKS_PER_CU", 1, 1024,
      [](qm_ctx::Tune &t, double v) { t.runs_chunks_per_cu = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.runs_chunks_per_cu; }},
     {"runs_win", "QM_TUNE_RUNS_WIN", 1024, 2048, [](qm_ctx::Tune &t, double v) { t.runs_win = v >= 2048 ? 2048 : 1024; },
