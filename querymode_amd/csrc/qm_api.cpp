@@ -2858,12 +2858,16 @@ bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint
   v.ctx = T->ctx;
   v.name = T->name;
   v.rows = r1 - r0;
-  v.cols = T->cols;
-  for (TCol &c : v.cols) {
-    c.d = static_cast<uint8_t *>(c.d) + r0 * (uint64_t)dt_size(c.dtype);
-    if (c.dnull) c.dnull += r0 / 8;
-    c.zmin.clear();
-    c.zmax.clear();
+  v.cols.clear();
+  v.cols.reserve(T->cols.size());
+  for (const TCol &t : T->cols) {  // the view needs no zone maps of its own
+    TCol c;
+    c.name = t.name;
+    c.dtype = t.dtype;
+    c.d = static_cast<uint8_t *>(t.d) + r0 * (uint64_t)dt_size(t.dtype);
+    c.dnull = t.dnull ? t.dnull + r0 / 8 : nullptr;
+    c.st = t.st;
+    v.cols.push_back(std::move(c));
   }
   *skipped = (uint32_t)(pages - (p1 - p0 + 1));
   return true;
