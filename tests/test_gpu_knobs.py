@@ -59,3 +59,43 @@ def test_knobs_keep_results(qm):
     finally:
         for k, v in defaults.items():
             dev.tune_set(k, v)
+
+
+def test_topk_blocks_keep_results(qm):
+    """Top-K with 1, 2 and 8 scan workgroups per CU (each keeps its own
+    candidates; the shared k-th threshold prunes the others): the same rows in
+    the same order, ties (repeated values) broken by row id, NULLs last."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from querymode_amd.plan import FilterOp
+
+    from ._compare import assert_rows_equal
+
+    n = 3_000_017
+    _say(f"top-K workgroups per CU: {n} rows")
+    rng = np.random.default_rng(23)
+    cols = {"v": rng.integers(0, 5000, n).astype(np.int64), "w": rng.random(n), "g": rng.integers(0, 9, n).astype(np.int32)}
+    nulls = {"w": rng.random(n) < 0.05}
+    qm.register_table("tk", cols, nulls)
+    ot = O.Table(cols, nulls)
+    descs = [QueryDescriptor("tk", projections=["v", "g"], sortColumn="v", sortDirection="desc", limit=k)
+             for k in (10, 100, 1000)]
+    descs.append(QueryDescriptor("tk", projections=["w"], sortColumn="w", sortDirection="asc", limit=500, offset=3))
+    descs.append(QueryDescriptor("tk", filters=[FilterOp("g", "eq", 4)], projections=["w", "v"], sortColumn="w",
+                                 sortDirection="desc", limit=100))
+    dev = qm.device
+    old = dev.tune_get("topk_bpc")
+    ex = qm.executor
+    try:
+        want = None
+        for bpc in (2, 1, 8):
+            dev.tune_set("topk_bpc", bpc)
+            got = [bytes(ex.execute(d).columnarData) for d in descs]
+            if want is None:
+                want = got
+                for d in descs:
+                    assert_rows_equal(ex.execute(d).rows, O.run(d, {"tk": ot}), ctx=f"top-K bpc {bpc}")
+            assert got == want, bpc
+    finally:
+        dev.tune_set("topk_bpc", old)
