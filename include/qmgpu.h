@@ -87,6 +87,8 @@ typedef struct qm_stats {
   uint32_t retries;           /* hash-table regrowths */
   uint32_t strategy;          /* group-by strategy picked by the tuner (QM_STRAT_*) */
   uint32_t pages_skipped;     /* zone-map pages (65,536 rows) not read (QueryResult.pagesSkipped) */
+  double   pass_ms[4];        /* device time of the strategy's passes (QM_STRAT_PART_RUNS: row scatter,
+                                 record scatter, group count, write), HIP events on the library stream */
 } qm_stats;
 
 #define QM_STRAT_NONE        0
@@ -98,6 +100,7 @@ typedef struct qm_stats {
 #define QM_STRAT_PART_DIRECT 6  /* partitioned: scatter by dense key slice, LDS slice per partition */
 #define QM_STRAT_SHARED_INTERVAL 7 /* batched: one range filter + small key, aggregated once per range interval */
 #define QM_STRAT_SORTED_RUNS 8  /* key column stored in order: segmented reduction over its runs */
+#define QM_STRAT_PART_RUNS   9  /* batched: key in any row order, scattered into key partitions, runs sorted in LDS */
 
 /* ---- lifecycle ----
  * Replaces the executor construction QueryMode.local() does

@@ -21,7 +21,7 @@ QM_F64, QM_I64, QM_UTF8, QM_BOOL, QM_F32VEC, QM_NULL, QM_I32, QM_F32 = 0, 1, 2, 
 QM_I8, QM_I16 = 16, 17
 
 STRATEGIES = {0: "none", 1: "direct_lds", 2: "hash_lds", 3: "direct_global", 4: "hash_global", 5: "part_hash",
-              6: "part_direct", 7: "shared_interval", 8: "sorted_runs"}
+              6: "part_direct", 7: "shared_interval", 8: "sorted_runs", 9: "part_runs"}
 
 
 class ColumnDesc(C.Structure):
@@ -34,11 +34,13 @@ class Stats(C.Structure):
         ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double), ("scan_ms", C.c_double),
         ("wall_ms", C.c_double),
         ("kernel_launches", C.c_uint32), ("retries", C.c_uint32), ("strategy", C.c_uint32), ("pages_skipped", C.c_uint32),
+        ("pass_ms", C.c_double * 4),
     ]
 
     def as_dict(self) -> dict:
         d = {f: getattr(self, f) for f, _ in self._fields_ if not f.startswith("_")}
         d["strategy"] = STRATEGIES.get(self.strategy, str(self.strategy))
+        d["pass_ms"] = list(self.pass_ms)
         return d
 
 

@@ -76,6 +76,13 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     if (sh_hll[r]) atomicMax(&hll[r], sh_hll[r]);
 }
 
+// key histogram: rows per bin of 2^g consecutive key values (bin = (v - kmin) >> g)
+__global__ void __launch_bounds__(256) key_hist_kernel(DCol c, uint64_t n, int64_t kmin, int g, uint32_t *hist) {
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += stride)
+    atomicAdd(&hist[(uint64_t)(load1_i(c, r) - kmin) >> g], 1u);
+}
+
 }  // namespace qm
 
 using namespace qm;
@@ -132,5 +139,13 @@ extern "C" hipError_t qmk_zone_maps(const DCol *c, uint64_t n, uint64_t *zmin, u
   if (n == 0) return hipSuccess;
   const uint64_t pages = (n + ZONE_ROWS - 1) / ZONE_ROWS;
   hipLaunchKernelGGL(zone_kernel, dim3((unsigned)pages), dim3(256), 0, st, *c, n, zmin, zmax);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_key_hist(const DCol *c, uint64_t n, int64_t kmin, int g, uint32_t *hist, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t grid = (n + 255) / 256;
+  if (grid > 65536) grid = 65536;
+  hipLaunchKernelGGL(key_hist_kernel, dim3((unsigned)grid), dim3(256), 0, st, *c, n, kmin, g, hist);
   return hipGetLastError();
 }

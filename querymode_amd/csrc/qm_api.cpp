@@ -1201,6 +1201,19 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
       s.sorted = s.nulls == 0 && sw[6] == 0;
     }
     s.distinct = s.nonnull ? std::max(1.0, hll_estimate(regs.data())) : 0.0;
+    // key histogram: at most 2^20 bins of 2^g values
+    if (s.has_int && s.nulls == 0 && (uint64_t)s.imax - (uint64_t)s.imin < (1ull << 32)) {
+      const uint64_t D = (uint64_t)s.imax - (uint64_t)s.imin + 1;
+      int kg = 0;
+      while (((D + (1ull << kg) - 1) >> kg) > (1ull << 20)) ++kg;
+      const uint64_t nbins = (D + (1ull << kg) - 1) >> kg;
+      DevBuf hb(ctx, nbins * 4 + 64);
+      zero(ctx, hb.p, nbins * 4);
+      HIPX(qmk_key_hist(&dc, nrows, s.imin, kg, hb.as<uint32_t>(), ctx->stream));
+      tc.khist.resize(nbins);
+      d2h(ctx, tc.khist.data(), hb.p, nbins);
+      tc.khist_g = kg;
+    }
     // skew probe on a strided host sample
     if (nrows >= 2) {
       const uint64_t ns = std::min<uint64_t>(nrows, 16384);
@@ -1741,7 +1754,9 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
 // BK_RUNS: grouped by a sorted key too sparse for a DIRECT table; only the
 // run-sorted kernels fuse such queries (they emit one group per run and need
 // no table), the rest run through the per-query path
-enum : int { BK_NONE = 0, BK_SHARED = 1, BK_SORTED = 2, BK_RUNS = 3 };
+// BK_PRUNS: grouped by an integer key in any row order with a key histogram;
+// the partitioned-runs kernels (k_pruns.hip) fuse such queries
+enum : int { BK_NONE = 0, BK_SHARED = 1, BK_SORTED = 2, BK_RUNS = 3, BK_PRUNS = 4 };
 
 // one query of a fused launch
 struct BItem {
@@ -1824,6 +1839,14 @@ void batch_classify(qm_ctx *ctx, const qm_table *T, const Plan &pl, BItem &it) {
     it.st_t[s] = q.states[s].col_a >= 0 ? cs.tcols[q.states[s].col_a] : nullptr;
   }
   const int nk = q.key.ncols;
+  // any row order: a key histogram sizes the key partitions
+  auto try_pruns = [&]() {
+    if (it.kind != BK_NONE || nk != 1 || it.nslots > 1 || q.nstates > PR_MAX_STATES) return;
+    const TCol *kt = cs.tcols[q.key.col[0]];
+    if (kt->st.nulls || dt_float(kt->dtype) || kt->khist_g < 0 || kt->khist.empty()) return;
+    it.key_t = kt;
+    it.kind = BK_PRUNS;
+  };
   if (q.gmode == GM_HASH && nk == 1 && it.nslots <= 1 && q.nstates <= R_MAX_STATES) {
     const TCol *kt = cs.tcols[q.key.col[0]];
     if (kt->st.sorted && !kt->st.nulls && !dt_float(kt->dtype)) {
@@ -1831,7 +1854,7 @@ void batch_classify(qm_ctx *ctx, const qm_table *T, const Plan &pl, BItem &it) {
       it.kind = BK_RUNS;
     }
   }
-  if (q.gmode != GM_DIRECT) return;
+  if (q.gmode != GM_DIRECT) { try_pruns(); return; }
   if (nk == 1) {
     const TCol *kt = cs.tcols[q.key.col[0]];
     if (kt->st.nulls || dt_float(kt->dtype)) return;
@@ -1841,9 +1864,7 @@ void batch_classify(qm_ctx *ctx, const qm_table *T, const Plan &pl, BItem &it) {
   } else if (nk == 0 && it.nslots == 1 && q.nstates <= B_MAX_SHARED_STATES) {
     it.kind = BK_SHARED;
   }
-  if (it.kind == BK_SHARED && it.slot_t[0]->st.nulls == 0 && it.lo[0] == 0 && it.hi[0] == ~0ull) {
-    // an IS NOT NULL-only filter on a column without NULLs is a full range: fine
-  }
+  try_pruns();
 }
 
 // a shared-interval group: queries with the same range column and key
@@ -2035,20 +2056,24 @@ void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_res
 // at most one range on one common column (k_runs.hip) ----
 struct RunsPlan {
   const TCol *pcol = nullptr;
-  int perm[R_MAX_STATES] = {};       // run state i = query state perm[i]
+  int perm[PR_MAX_STATES] = {};      // run state i = query state perm[i]
   int nsum = 0, nmax = 0;
   ROut outs[R_MAX_OUTS] = {};
   int nouts = 0;
   std::vector<uint64_t> bounds;      // sorted distinct bound codes
 };
 
-bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan &rp) {
+// any_order: the key may be stored in any row order (the partitioned-runs
+// launch); otherwise it must be sorted with runs of runs_min_len rows
+bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan &rp, bool any_order = false) {
   const size_t n = L.sorted.size();
-  if (n == 0 || n > (size_t)R_MAX_QUERIES || (int)n < ctx->tune.runs_min_q || !L.groups.empty()) return false;
+  if (n == 0 || n > (size_t)R_MAX_QUERIES || !L.groups.empty()) return false;
+  if (!any_order && (int)n < ctx->tune.runs_min_q) return false;
   if (T->rows >= (1ull << 32)) return false;  // 32-bit positions and per-chunk counters
   const TCol *key = L.sorted_key;
-  if (!key || !key->st.sorted || key->st.nulls || dt_float(key->dtype)) return false;
-  if ((double)T->rows / std::max(1.0, key->st.distinct) < ctx->tune.runs_min_len) return false;
+  if (!key || key->st.nulls || dt_float(key->dtype)) return false;
+  if (!any_order && !key->st.sorted) return false;
+  if (!any_order && (double)T->rows / std::max(1.0, key->st.distinct) < ctx->tune.runs_min_len) return false;
   for (const BItem *it : L.sorted) {
     if (it->nslots > 1) return false;
     if (it->nslots == 1 && !it->never) {
@@ -2059,7 +2084,7 @@ bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan 
   if (!rp.pcol) rp.pcol = key;
   const BItem &i0 = *L.sorted[0];
   const DQuery &q0 = i0.ab.q;
-  if (q0.nstates < 1 || q0.nstates > R_MAX_STATES) return false;
+  if (q0.nstates < 1 || q0.nstates > (any_order ? PR_MAX_STATES : R_MAX_STATES)) return false;
   if (q0.states[0].kind != SK_COUNT || q0.states[0].col_a >= 0) return false;
   std::vector<int> adds, maxs;
   for (int s = 1; s < q0.nstates; ++s) {
@@ -2078,7 +2103,7 @@ bool runs_plan(const qm_ctx *ctx, const qm_table *T, const BLaunch &L, RunsPlan 
   }
   rp.nsum = (int)adds.size();
   rp.nmax = (int)maxs.size();
-  if (!qmk_runs_supported(rp.nsum, rp.nmax)) return false;
+  if (!(any_order ? qmk_pr_supported(rp.nsum, rp.nmax) : qmk_runs_supported(rp.nsum, rp.nmax))) return false;
   int k = 0;
   rp.perm[k++] = 0;
   for (int s : adds) rp.perm[k++] = s;
@@ -2339,6 +2364,297 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
   }
 }
 
+// ---- partitioned-runs launches: queries grouped by one integer key stored in
+// any row order (k_pruns.hip) ----
+
+// Partition geometry for a launch of `nbk` buckets and `nsn` non-COUNT(*)
+// states, from the key histogram: final partitions of 2^s keys (the largest
+// s whose fullest partition fits the write pass's LDS), level-1 partitions of
+// 2^(sh1 - s) final ones. Cached on the key column; nullptr when no s fits.
+const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uint32_t nbk, int nsum, int nmax,
+                              uint32_t rw) {
+  TCol *kt = const_cast<TCol *>(kt_c);  // the cache; callers hold ctx->mu
+  const uint32_t nsn = (uint32_t)(nsum + nmax);
+  for (auto &g : kt->prgeo)
+    if (g.nbk == nbk && g.nsn == nsn && g.rw == rw) return g.s >= 0 ? &g : nullptr;
+  TCol::PRGeo g;
+  g.nbk = nbk;
+  g.nsn = nsn;
+  g.rw = rw;
+  g.tile = qmk_pr_tile((int)rw);
+  const std::vector<uint32_t> &h = kt->khist;
+  const int kg = kt->khist_g;
+  const uint64_t D = (uint64_t)kt->st.imax - (uint64_t)kt->st.imin + 1;
+  std::vector<uint64_t> caps;
+  for (int s = PR_MAX_S; s >= kg && g.tile; --s) {
+    const uint32_t cap = qmk_pr_max_cap(nbk << s, nsum, nmax);
+    if (!cap) continue;
+    const uint64_t per = 1ull << (s - kg);
+    const uint64_t np = (D + (1ull << s) - 1) >> s;
+    // sparse keys (few rows per partition of 64 keys): the per-query paths serve them better
+    if (np > std::max<uint64_t>(4096, rows / 32)) break;
+    caps.assign(np, 0);
+    uint64_t mx = 0;
+    for (size_t b = 0; b < h.size(); ++b) caps[b / per] += h[b];
+    for (uint64_t c : caps) mx = std::max(mx, c);
+    if (mx > cap) continue;
+    int lg = 0;
+    while ((1ull << lg) < np) ++lg;
+    const int b2max = std::min(11, 16 - s);
+    int b2 = std::min((lg + 1) / 2, b2max);
+    while (((D + (1ull << (s + b2)) - 1) >> (s + b2)) > (uint64_t)PR_MAX_P && b2 < b2max) ++b2;
+    const uint64_t np1 = (D + (1ull << (s + b2)) - 1) >> (s + b2);
+    if (np1 > (uint64_t)PR_MAX_P || np >= (1ull << 31)) continue;
+    g.s = s;
+    g.sh1 = s + b2;
+    g.np = (uint32_t)np;
+    g.np1 = (uint32_t)np1;
+    g.cap = cap;
+    break;
+  }
+  if (g.s >= 0) {
+    std::vector<uint64_t> st2(g.np + 1, 0), st1(g.np1 + 1, 0);
+    for (uint32_t P = 0; P < g.np; ++P) st2[P + 1] = st2[P] + caps[P];
+    const int b2 = g.sh1 - g.s;
+    std::vector<uint32_t> ch(g.np1 + 1, 0);
+    for (uint32_t p = 0; p <= g.np1; ++p) st1[p] = st2[std::min<uint64_t>((uint64_t)p << b2, g.np)];
+    for (uint32_t p = 0; p < g.np1; ++p) ch[p + 1] = ch[p] + (uint32_t)((st1[p + 1] - st1[p] + g.tile - 1) / g.tile);
+    g.total = st2[g.np];
+    g.nchunk2 = ch[g.np1];
+    g.start1 = std::make_shared<DevBuf>(ctx, st1.size() * 8 + 64);
+    g.start2 = std::make_shared<DevBuf>(ctx, st2.size() * 8 + 64);
+    g.chunk0 = std::make_shared<DevBuf>(ctx, ch.size() * 4 + 64);
+    h2d(ctx, g.start1->p, st1.data(), st1.size());
+    h2d(ctx, g.start2->p, st2.data(), st2.size());
+    h2d(ctx, g.chunk0->p, ch.data(), ch.size());
+  }
+  kt->prgeo.push_back(g);
+  return g.s >= 0 ? &kt->prgeo.back() : nullptr;
+}
+
+// One launch through the partitioned-runs passes; false (nothing run) when
+// its records or partitions do not fit the kernels.
+bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &rp, std::vector<qm_result *> &out,
+               float &scan_ms, double pass_ms[4]) {
+  const size_t n = L.sorted.size();
+  const std::vector<uint64_t> &b = rp.bounds;
+  const uint32_t nb = (uint32_t)b.size();
+  int steps = 0;
+  while (((1u << steps) - 1) < nb) ++steps;
+  if (steps == 0) steps = 1;  // one padded bound: the search loop runs once
+  std::vector<uint64_t> padded((1u << steps) - 1, ~0ull);
+  std::copy(b.begin(), b.end(), padded.begin());
+  std::vector<uint32_t> qe(n);
+  uint32_t on[R_MAX_BUCKETS / 32] = {};
+  for (size_t i = 0; i < n; ++i) {
+    const BItem &it = *L.sorted[i];
+    uint32_t lo, hi;
+    if (it.never) {
+      lo = hi = 0;
+    } else if (it.nslots == 0) {
+      lo = 0;
+      hi = nb + 2;  // every row, NULL filter values included
+    } else {
+      lo = (uint32_t)(std::upper_bound(b.begin(), b.end(), it.lo[0]) - b.begin());
+      hi = it.hi[0] == ~0ull ? nb + 1 : (uint32_t)(std::upper_bound(b.begin(), b.end(), it.hi[0] + 1) - b.begin());
+    }
+    qe[i] = lo | (hi << 16);
+    for (uint32_t e = lo; e < hi; ++e) on[e >> 5] |= 1u << (e & 31);
+  }
+  // record layout: key / bucket word, the distinct state columns, NULL bits
+  const BItem &i0 = *L.sorted[0];
+  const DQuery &q0 = i0.ab.q;
+  PRScan ps;
+  memset(&ps, 0, sizeof ps);
+  std::vector<const TCol *> vc;
+  int rw = 1;
+  bool nullable = false;
+  ps.nstates = 1 + rp.nsum + rp.nmax;
+  ps.nsum = rp.nsum;
+  for (int i = 0; i < ps.nstates; ++i) {
+    const int s = rp.perm[i];
+    ps.st[i] = q0.states[s];
+    const TCol *t = i0.st_t[s];
+    ps.st_v[i] = -1;
+    if (!t || i == 0) continue;
+    auto f = std::find(vc.begin(), vc.end(), t);
+    if (f == vc.end()) {
+      if (vc.size() >= (size_t)PR_MAX_V) return false;
+      vc.push_back(t);
+      f = vc.end() - 1;
+    }
+    ps.st_v[i] = (int8_t)(f - vc.begin());
+  }
+  ps.nv = (int)vc.size();
+  for (size_t j = 0; j < vc.size(); ++j) {
+    ps.vcol[j] = vc[j]->d;
+    ps.vnull[j] = vc[j]->st.nulls ? vc[j]->dnull : nullptr;
+    ps.v_dt[j] = vc[j]->dtype;
+    ps.v_word[j] = rw;
+    rw += (vc[j]->dtype == DT_I64 || vc[j]->dtype == DT_F64) ? 2 : 1;
+    nullable = nullable || vc[j]->st.nulls;
+  }
+  ps.null_word = -1;
+  if (nullable) ps.null_word = rw++;
+  rw = std::max(rw, 2);
+  if (rw > PR_MAX_RW) return false;
+  const TCol *kt = L.sorted_key;
+  const TCol::PRGeo *g = pr_geometry(ctx, kt, T->rows, nb + 2, rp.nsum, rp.nmax, (uint32_t)rw);
+  if (!g) return false;
+
+  const uint64_t rows = T->rows;
+  ps.nrows = rows;
+  ps.key = kt->d;
+  ps.key_dt = kt->dtype;
+  ps.pcol = rp.pcol->d;
+  ps.p_dt = rp.pcol->dtype;
+  ps.pnull = rp.pcol->st.nulls ? rp.pcol->dnull : nullptr;
+  ps.kmin = kt->st.imin;
+  ps.rw = rw;
+  ps.nbounds = (int)nb;
+  ps.bsteps = steps;
+  memcpy(ps.bucket_on, on, sizeof on);
+  ps.s = g->s;
+  ps.sh1 = g->sh1;
+  ps.np1 = g->np1;
+  ps.np = g->np;
+  ps.tile = g->tile;
+  ps.start1 = g->start1->as<uint64_t>();
+  ps.start2 = g->start2->as<uint64_t>();
+  ps.chunk0 = g->chunk0->as<uint32_t>();
+  ps.cap = g->cap;
+  ps.nq = (int)n;
+  ps.nouts = rp.nouts;
+  for (int j = 0; j < rp.nouts; ++j) ps.outs[j] = rp.outs[j];
+
+  DevBuf dbounds(ctx, padded.size() * 8 + 64), dqe(ctx, n * 4 + 64);
+  h2d(ctx, dbounds.p, padded.data(), padded.size());
+  h2d(ctx, dqe.p, qe.data(), n);
+  ps.bounds = dbounds.as<uint64_t>();
+  ps.qe = dqe.as<uint32_t>();
+  const uint64_t recb = g->total * (uint64_t)rw * 4 + 64;
+  DevBuf rec1(ctx, recb), rec2(ctx, recb);
+  DevBuf cur1(ctx, (uint64_t)g->np1 * 4 + 64), cur2(ctx, (uint64_t)g->np * 4 + 64);
+  DevBuf brows(ctx, R_MAX_BUCKETS * 8 + 64), status(ctx, 64);
+  const uint64_t ncnt = (uint64_t)n * g->np;
+  DevBuf gcount(ctx, ncnt * 8 + 64), gtotal(ctx, 64), stmp(ctx, qmk_exscan_tmp_words(ncnt) * 8 + 64);
+  zero(ctx, cur1.p, (uint64_t)g->np1 * 4 + 64);
+  zero(ctx, cur2.p, (uint64_t)g->np * 4 + 64);
+  zero(ctx, brows.p, R_MAX_BUCKETS * 8 + 64);
+  zero(ctx, status.p, 64);
+  ps.rec1 = rec1.as<uint32_t>();
+  ps.rec2 = rec2.as<uint32_t>();
+  ps.cur1 = cur1.as<uint32_t>();
+  ps.cur2 = cur2.as<uint32_t>();
+  ps.bucket_rows = brows.as<uint64_t>();
+  ps.status = status.as<uint32_t>();
+  ps.gcount = gcount.as<uint64_t>();
+
+  // persistent scatter grids: the workgroups that fit the device at once
+  const uint64_t tile = g->tile;
+  const uint64_t ntiles = (rows + tile - 1) / tile;
+  const uint64_t pgrid = (uint64_t)ctx->num_cus * 2;
+  hipEvent_t ev[6];
+  for (auto &e : ev) HIPX(hipEventCreate(&e));
+  struct EvFree { hipEvent_t *e; ~EvFree() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); } } evfree{ev};
+  HIPX(hipEventRecord(ev[0], ctx->stream));
+  if (ntiles) HIPX(qmk_pr_scatter(&ps, 0, (uint32_t)std::min(ntiles, pgrid), ctx->stream));
+  HIPX(hipEventRecord(ev[1], ctx->stream));
+  if (g->nchunk2) HIPX(qmk_pr_scatter(&ps, 1, (uint32_t)std::min<uint64_t>(g->nchunk2, pgrid), ctx->stream));
+  HIPX(hipEventRecord(ev[2], ctx->stream));
+  HIPX(qmk_pr_count(&ps, ctx->stream));
+  HIPX(qmk_exscan_u64(gcount.as<uint64_t>(), ncnt, gcount.as<uint64_t>(), gtotal.as<uint64_t>(),
+                      stmp.as<uint64_t>(), ctx->stream));
+  HIPX(hipEventRecord(ev[3], ctx->stream));
+  std::vector<uint64_t> base(n + 1, 0);
+  HIPX(hipMemcpy2DAsync(base.data(), 8, gcount.p, (size_t)g->np * 8, 8, n, hipMemcpyDeviceToHost, ctx->stream));
+  d2h(ctx, &base[n], gtotal.p, 1);
+  std::vector<uint64_t> brow(R_MAX_BUCKETS);
+  d2h(ctx, brow.data(), brows.p, R_MAX_BUCKETS);
+  uint32_t stv = 0;
+  d2h(ctx, &stv, status.p, 1);
+  if (stv) fail(QM_ERR_QUERY_FAILED, "partitioned-runs batch: a key partition exceeded its region (status %u)", stv);
+  std::vector<uint64_t> tot(n);
+  for (size_t i = 0; i < n; ++i) tot[i] = base[i + 1] - base[i];
+
+  // result arenas as in run_runs: query i's rows at a padded offset
+  std::vector<uint64_t> roff(n);
+  uint64_t arows = 0;
+  for (size_t i = 0; i < n; ++i) {
+    roff[i] = arows;
+    arows += pad_rows(tot[i]);
+  }
+  std::vector<std::shared_ptr<DevBuf>> arenas(rp.nouts);
+  for (int j = 0; j < rp.nouts; ++j)
+    arenas[j] = std::make_shared<DevBuf>(ctx, std::max<uint64_t>(arows, 1) * dt_size(rp.outs[j].out_dtype) + 64);
+  std::vector<uint32_t *> nbase(rp.nouts, nullptr);
+  for (int j = 0; j < rp.nouts; ++j) {
+    const ROut &r = rp.outs[j];
+    if (r.kind == OK_KEY || r.kind == OK_COUNT || r.cnt_state <= 0) continue;
+    const uint64_t nbb = std::max<uint64_t>(arows, 1) / 8 + 64;
+    arenas.push_back(std::make_shared<DevBuf>(ctx, nbb));
+    zero(ctx, arenas.back()->p, nbb);
+    nbase[j] = arenas.back()->as<uint32_t>();
+  }
+  for (size_t i = 0; i < n; ++i) {
+    const BItem &it = *L.sorted[i];
+    qm_result *res = new qm_result();
+    out[it.idx] = res;
+    res->ctx = ctx;
+    res->rows = tot[i];
+    res->arenas = arenas;
+    for (int j = 0; j < rp.nouts; ++j) {
+      const ROut &r = rp.outs[j];
+      OutCol oc;
+      oc.name = it.ab.outs[j].name;
+      oc.dtype = r.out_dtype;
+      oc.data.p = static_cast<uint8_t *>(arenas[j]->p) + roff[i] * dt_size(r.out_dtype);
+      oc.data.n = pad_rows(tot[i]) * dt_size(r.out_dtype);
+      if (nbase[j]) {
+        oc.nulls.p = reinterpret_cast<uint8_t *>(nbase[j]) + roff[i] / 8;
+        oc.nulls.n = pad_rows(tot[i]) / 8 + 64;
+      }
+      res->cols.push_back(std::move(oc));
+    }
+  }
+  DevBuf droff(ctx, n * 8 + 64);
+  h2d(ctx, droff.p, roff.data(), n);
+  for (int j = 0; j < rp.nouts; ++j) {
+    ps.out_base[j] = arenas[j]->p;
+    ps.out_nbase[j] = nbase[j];
+  }
+  ps.row_off = droff.as<uint64_t>();
+  HIPX(hipEventRecord(ev[4], ctx->stream));
+  if (base[n]) HIPX(qmk_pr_runs(&ps, ctx->stream));
+  HIPX(hipEventRecord(ev[5], ctx->stream));
+  HIPX(hipEventSynchronize(ev[5]));
+  d2h(ctx, &stv, status.p, 1);
+  if (stv) fail(QM_ERR_QUERY_FAILED, "partitioned-runs batch: a key partition exceeded the LDS capacity (status %u)", stv);
+  float ms[4] = {0, 0, 0, 0};
+  HIPX(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
+  HIPX(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
+  HIPX(hipEventElapsedTime(&ms[2], ev[2], ev[3]));
+  HIPX(hipEventElapsedTime(&ms[3], ev[4], ev[5]));
+  for (int i = 0; i < 4; ++i) pass_ms[i] = ms[i];
+  scan_ms = std::max(scan_ms, ms[0]);  // the pass reading the table
+  DQuery noq;
+  memset(&noq, 0, sizeof noq);
+  DevBuf dnoq(ctx, sizeof(DQuery));
+  h2d(ctx, dnoq.p, reinterpret_cast<const uint8_t *>(&noq), sizeof noq);
+  for (size_t i = 0; i < n; ++i) {
+    qm_result *res = out[L.sorted[i]->idx];
+    const uint32_t lo = qe[i] & 0xFFFFu, hi = qe[i] >> 16;
+    uint64_t sel = 0;
+    for (uint32_t e = lo; e < hi; ++e) sel += brow[e];
+    res->stats.rows_selected = sel;
+    res->stats.strategy = QM_STRAT_PART_RUNS;
+    for (int k = 0; k < 4; ++k) res->stats.pass_ms[k] = pass_ms[k];
+    uint32_t l = 0;
+    sort_limit_result(res, *L.sorted[i]->plan, dnoq.as<DQuery>(), &l);
+  }
+  return true;
+}
+
 // One shared-interval group through the lean pass: bounds padded for the
 // branch-free search, the group's distinct value columns (the range column
 // first), one workgroup range of rows per block.
@@ -2401,10 +2717,11 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
   Timer tm(ctx);
   // shared groups
   std::vector<std::unique_ptr<BGroup>> groups;
-  std::vector<BItem *> sorted, sparse;
+  std::vector<BItem *> sorted, sparse, pruns;
   for (BItem *it : items) {
     if (it->kind == BK_SORTED) { sorted.push_back(it); continue; }
     if (it->kind == BK_RUNS) { sparse.push_back(it); continue; }
+    if (it->kind == BK_PRUNS) { pruns.push_back(it); continue; }
     BGroup *g = nullptr;
     for (auto &gg : groups)
       if (gg->col == it->slot_t[0] && gg->key == it->key_t && gg->items.size() < 4 * B_MAX_BOUNDS / 8) g = gg.get();
@@ -2531,6 +2848,42 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     } else {
       if (!rejected) fail(QM_ERR_QUERY_FAILED, "sparse sorted-key query without a run-sorted launch");
       for (BItem *it : L.sorted) rejected->push_back(it->idx);
+    }
+  }
+  // keys in any row order: one partitioned-runs launch per (key, range
+  // column, state signature), at most R_MAX_QUERIES queries each
+  {
+    std::vector<BLaunch> pl;
+    std::vector<const TCol *> pl_pcol;
+    for (BItem *it : pruns) {
+      const TCol *pc = (it->nslots == 1 && !it->never) ? it->slot_t[0] : nullptr;
+      BLaunch *L = nullptr;
+      for (size_t l = 0; l < pl.size() && !L; ++l) {
+        BLaunch &c = pl[l];
+        if (c.sorted_key == it->key_t && c.sorted.size() < (size_t)R_MAX_QUERIES && same_signature(*c.sorted[0], *it) &&
+            (!pc || !pl_pcol[l] || pc == pl_pcol[l])) {
+          L = &c;
+          if (pc) pl_pcol[l] = pc;
+        }
+      }
+      if (!L) {
+        pl.emplace_back();
+        pl_pcol.push_back(pc);
+        L = &pl.back();
+        L->sorted_key = it->key_t;
+      }
+      L->sorted.push_back(it);
+    }
+    for (auto &L : pl) {
+      RunsPlan rp;
+      double pms[4] = {0, 0, 0, 0};
+      if (runs_plan(ctx, T, L, rp, true) && run_pruns(ctx, T, L, rp, out, scan_ms_total, pms)) {
+        for (BItem *it : L.sorted) runs_done.push_back(it);
+        ++nlaunches;
+      } else {
+        if (!rejected) fail(QM_ERR_QUERY_FAILED, "key-partitioned query without a launch");
+        for (BItem *it : L.sorted) rejected->push_back(it->idx);
+      }
     }
   }
   {

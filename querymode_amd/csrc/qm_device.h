@@ -373,4 +373,66 @@ struct S1Scan {
   uint64_t *selected;
 };
 
+// ---- many queries grouped by one integer key stored in any row order
+// (k_pruns.hip). Rows selected by at least one query become records
+// (u32 words: word 0 = key bits below the level-1 digit | bucket << 16,
+// then the raw words of the state columns, then a NULL-bit word when a state
+// column has NULLs), scattered by the key's high bits into level-1
+// partitions, then by the next bits into final partitions of 2^s keys; a
+// workgroup then sorts one final partition by (key, bucket) in LDS and
+// answers every query from prefix sums over that order (lanes = keys).
+static constexpr int PR_MAX_V = 4;       // distinct state columns copied into records
+static constexpr int PR_MAX_RW = 8;      // record words
+static constexpr int PR_MAX_P = 2048;    // digits per scatter pass
+static constexpr int PR_BLOCK = 512;     // scatter / partition workgroup
+static constexpr int PR_STAGE_WORDS = 16384;  // LDS staging of one scatter tile (64 KB)
+static constexpr int PR_MAX_S = 6;       // key bits per final partition (one key per lane)
+static constexpr int PR_MAX_STATES = 6;  // COUNT(*) + up to 5 more (COUNT(col) guards of nullable columns)
+struct PRScan {
+  uint64_t nrows;
+  const void *key;
+  int32_t key_dt;
+  int32_t p_dt;                  // filter column (order codes) and its NULL bitmap
+  const void *pcol;
+  const uint8_t *pnull;
+  int64_t kmin;
+  int32_t nv;                    // state columns copied into records
+  int32_t rw;                    // record words
+  const void *vcol[PR_MAX_V];
+  const uint8_t *vnull[PR_MAX_V];
+  int32_t v_dt[PR_MAX_V];
+  int32_t v_word[PR_MAX_V];      // first record word of column j (1 or 2 words)
+  int32_t null_word;             // record word with the column NULL bits (-1: none)
+  int32_t nbounds, bsteps;       // buckets: bucket(x) = #{bounds <= x}; NULL filter value: nbounds + 1
+  const uint64_t *bounds;        // 2^bsteps - 1 entries (padding all-ones)
+  uint32_t bucket_on[R_MAX_BUCKETS / 32];  // buckets some query covers (others are dropped)
+  uint64_t *bucket_rows;         // [R_MAX_BUCKETS] rows per bucket (selected-row counts)
+  int32_t s;                     // key bits per final partition
+  int32_t sh1;                   // level-1 digit = key code >> sh1 (sh1 = s + level-2 bits)
+  uint32_t np1, np;              // level-1 partitions, final partitions
+  uint32_t tile;                 // records per scatter tile (PR_STAGE_WORDS / rw)
+  uint32_t _p0;
+  const uint64_t *start1;        // [np1] first record of each level-1 region (capacity from the key histogram)
+  uint32_t *cur1;                // [np1] records written (zeroed before the batch)
+  const uint64_t *start2;        // [np]
+  uint32_t *cur2;                // [np]
+  uint32_t *rec1, *rec2;         // record buffers
+  const uint32_t *chunk0;        // [np1 + 1] first pass-2 workgroup of each level-1 partition
+  uint32_t cap;                  // records a final partition may hold (LDS)
+  int32_t nq;
+  const uint32_t *qe;            // per query: e_lo | e_hi << 16 (bucket range [e_lo, e_hi))
+  int32_t nstates, nsum;         // state 0 = COUNT(*); 1..nsum additive; the rest order codes
+  DState st[PR_MAX_STATES];
+  int8_t st_v[PR_MAX_STATES];    // record column of each state (-1: none)
+  int32_t nouts;
+  ROut outs[R_MAX_OUTS];
+  uint64_t *gcount;              // [nq][np] groups per (query, partition); the write pass reads its
+                                 // exclusive scan (in place): query q's rows of partition P start at
+                                 // gcount[q][P] - gcount[q][0]
+  void *out_base[R_MAX_OUTS];
+  uint32_t *out_nbase[R_MAX_OUTS];
+  const uint64_t *row_off;       // [nq] first arena row of each query
+  uint32_t *status;              // [0] a partition exceeded cap
+};
+
 }  // namespace qm

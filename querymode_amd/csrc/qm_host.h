@@ -178,6 +178,22 @@ struct TCol {
   // zone maps: per ZONE_ROWS-row page, min / max order code of the non-NULL
   // values (empty page: min ~0, max 0)
   std::vector<uint64_t> zmin, zmax;
+  // key histogram (integer columns without NULLs whose range spans < 2^32
+  // values): rows per bin of 2^khist_g consecutive values from st.imin; sizes
+  // the regions of the batched group-by's key partitions
+  std::vector<uint32_t> khist;
+  int khist_g = -1;
+  // partition geometries of the batched group-by derived from khist, by
+  // (buckets, states) of the launch (device copies of the region starts)
+  struct PRGeo {
+    uint32_t nbk = 0, nsn = 0, rw = 0;   // what the geometry was derived for
+    int s = -1, sh1 = 0;
+    uint32_t np1 = 0, np = 0, cap = 0, tile = 0;
+    uint64_t total = 0;                  // records the regions hold
+    std::shared_ptr<DevBuf> start1, start2, chunk0;
+    uint32_t nchunk2 = 0;                // pass-2 workgroup units
+  };
+  std::vector<PRGeo> prgeo;
 };
 
 }  // namespace qm
