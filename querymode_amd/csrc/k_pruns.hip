@@ -35,6 +35,7 @@
 // integer SUM, MIN / MAX by value, NaN never updates MIN / MAX); filters
 // those of src/decode.ts:477-578 (NULL never matches) through the bucket
 // ranges the host derives from each query's coerced range.
+#include <algorithm>
 #include "k_common.hip.h"
 #include "qm_kernels.h"
 
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
   __shared__ uint32_t s_sw[PR_WAVES + 1];
   const int t = threadIdx.x;
   const uint32_t ndig = SRC == 0 ? ps.np1 : (1u << (ps.sh1 - ps.s));
-  const uint32_t dpt = (ndig + PR_BLOCK - 1) / PR_BLOCK;  // digits per thread in the scan (<= 4)
+  const uint32_t dpt = (ndig + PR_BLOCK - 1) / PR_BLOCK;  // digits per thread in the scan (<= 2)
   const uint32_t nb = (uint32_t)ps.nbounds;
   if constexpr (SRC == 0) {
     const int nbp = (1 << ps.bsteps) - 1;
@@ -214,9 +215,11 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
     }
     __syncthreads();
     // digit starts in the tile, and one reservation per digit in its region
-    uint32_t cnt[4] = {0, 0, 0, 0}, mine = 0;
+    constexpr int DPT = PR_MAX_P / PR_BLOCK;
+    uint32_t cnt[DPT], mine = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < DPT; ++k) {
+      cnt[k] = 0;
       const uint32_t d = (uint32_t)t * dpt + k;
       if (k < (int)dpt && d < ndig) cnt[k] = s_hist[d];
       mine += cnt[k];
@@ -224,7 +227,7 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
     uint32_t total;
     uint32_t run = pr_block_exscan(mine, s_sw, &total);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < DPT; ++k) {
       const uint32_t d = (uint32_t)t * dpt + k;
       if (k >= (int)dpt || d >= ndig) continue;
       s_hist[d] = run;
@@ -301,8 +304,10 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
 __global__ void __launch_bounds__(PR_BLOCK) pr_count_kernel(PRScan ps) {
   constexpr int BW = R_MAX_BUCKETS / 32 + 1;  // bitmap words per key (+1: no bank conflicts across keys)
   __shared__ uint32_t s_bm[64 * BW];
+  __shared__ uint32_t s_qe[R_MAX_QUERIES];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const uint32_t P = blockIdx.x;
+  if (t < ps.nq) s_qe[t] = ps.qe[t];
   const uint64_t base = ps.start2[P];
   const uint64_t capr = ps.start2[P + 1] - base;
   const uint64_t n = ps.cur2[P] < capr ? ps.cur2[P] : capr;
@@ -318,7 +323,7 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_count_kernel(PRScan ps) {
   const bool kon = (uint32_t)lane <= km;
   const uint32_t *bm = s_bm + lane * BW;
   for (int q = wave; q < ps.nq; q += PR_WAVES) {
-    const uint32_t qe = ps.qe[q];
+    const uint32_t qe = s_qe[q];
     const uint32_t elo = qe & 0xFFFFu, ehi = qe >> 16;
     bool any = false;
     if (elo < ehi) {
@@ -342,225 +347,296 @@ __device__ __forceinline__ uint32_t pr_start(const uint32_t *bins, uint32_t x) {
   return (x & 1) ? (w >> 16) : (w & 0xFFFFu);
 }
 
-__device__ __forceinline__ void pr_store_i(void *out, int32_t dt, uint64_t i, int64_t v) {
-  switch (dt) {
-    case DT_I8: static_cast<int8_t *>(out)[i] = (int8_t)v; break;
-    case DT_BOOL: static_cast<uint8_t *>(out)[i] = (uint8_t)v; break;
-    case DT_I16: static_cast<int16_t *>(out)[i] = (int16_t)v; break;
-    case DT_I32: static_cast<int32_t *>(out)[i] = (int32_t)v; break;
-    default: static_cast<int64_t *>(out)[i] = v; break;
-  }
-}
-__device__ __forceinline__ void pr_store_f(void *out, int32_t dt, uint64_t i, double v) {
-  if (dt == DT_F32) static_cast<float *>(out)[i] = (float)v;
-  else static_cast<double *>(out)[i] = v;
-}
-
 // one output value of one group (the values finalize_slot derives: a group
-// with no counted value is NULL; an all-NaN MIN / MAX keeps the identity)
+// with no counted value is NULL; an all-NaN MIN / MAX keeps the identity),
+// formed as a bit pattern and stored with the output's width
 __device__ __forceinline__ void pr_emit(const ROut &ro, void *dst, uint32_t *nbm, uint64_t gg, int64_t key,
                                         uint64_t v, uint64_t cnt) {
+  uint64_t bits = 0;
   bool isnull = false;
+  int width = 8;
+  const int32_t dt = ro.out_dtype;
   switch (ro.kind) {
-    case OK_KEY: pr_store_i(dst, ro.out_dtype, gg, key); break;
-    case OK_COUNT: static_cast<int64_t *>(dst)[gg] = (int64_t)cnt; break;
-    case OK_SUM_I64:
-      isnull = cnt == 0;
-      static_cast<int64_t *>(dst)[gg] = isnull ? 0 : (int64_t)v;
-      break;
+    case OK_KEY: bits = (uint64_t)key; width = (dt == DT_I64) ? 8 : (dt == DT_I32 ? 4 : (dt == DT_I16 ? 2 : 1)); break;
+    case OK_COUNT: bits = cnt; break;
+    case OK_SUM_I64: isnull = cnt == 0; bits = isnull ? 0 : v; break;
     case OK_AVG_I64:
       isnull = cnt == 0;
-      static_cast<double *>(dst)[gg] = isnull ? 0.0 : (double)(int64_t)v / (double)cnt;
+      bits = f64_bits(isnull ? 0.0 : (double)(int64_t)v / (double)cnt);
       break;
     default: {  // OK_MIN / OK_MAX
       const bool mn = ro.kind == OK_MIN;
-      if (cnt == 0) {
-        isnull = true;
-        if (ro.src_is_float) pr_store_f(dst, ro.out_dtype, gg, 0.0);
-        else pr_store_i(dst, ro.out_dtype, gg, 0);
-      } else if (ro.src_is_float) {
-        double d;
-        if (v == 0) d = mn ? bits_f64(0x7FF0000000000000ull) : bits_f64(0xFFF0000000000000ull);
-        else d = bits_f64(decode_f64_code(mn ? ~v : v));
-        pr_store_f(dst, ro.out_dtype, gg, d);
+      isnull = cnt == 0;
+      if (ro.src_is_float) {
+        const uint64_t d = isnull ? 0ull
+                                  : (v == 0 ? (mn ? 0x7FF0000000000000ull : 0xFFF0000000000000ull)
+                                            : decode_f64_code(mn ? ~v : v));
+        if (dt == DT_F32) { bits = f32_bits((float)bits_f64(d)); width = 4; }
+        else bits = d;
       } else {
-        pr_store_i(dst, ro.out_dtype, gg, decode_i64(mn ? ~v : v));
+        bits = isnull ? 0ull : (uint64_t)decode_i64(mn ? ~v : v);
+        width = (dt == DT_I64) ? 8 : (dt == DT_I32 ? 4 : (dt == DT_I16 ? 2 : 1));
       }
       break;
     }
   }
+  uint8_t *p = static_cast<uint8_t *>(dst) + gg * (uint64_t)width;
+  if (width == 8) *reinterpret_cast<uint64_t *>(p) = bits;
+  else if (width == 4) *reinterpret_cast<uint32_t *>(p) = (uint32_t)bits;
+  else if (width == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)bits;
+  else *p = (uint8_t)bits;
   if (isnull && nbm) atomicOr(&nbm[gg >> 5], 1u << (gg & 31));
 }
 
-// state value si of a group: 0 = its row count, then the additive sums, then the max codes
-template <int NSUM, int NMAX>
-__device__ __forceinline__ uint64_t pr_state(int si, uint64_t cnt, const uint64_t (&sm)[NSUM > 0 ? NSUM : 1],
-                                             const uint64_t (&mx)[NMAX > 0 ? NMAX : 1]) {
-  uint64_t v = cnt;
+// contribution of one record to non-COUNT(*) state 1 + s2 (raw record words in w)
+__device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const uint32_t (&w)[PR_MAX_RW]) {
+  const DState &st = ps.st[1 + s2];
+  const int vj = ps.st_v[1 + s2];
+  if (vj < 0) return st.kind == SK_COUNT ? 1ull : 0ull;  // COUNT(*)
+  uint32_t nbits = 0, lo = 0, hi = 0;
+  // words by runtime index, selected over the unrolled record (no dynamic register indexing)
 #pragma unroll
-  for (int j = 0; j < NSUM; ++j) v = si == 1 + j ? sm[j] : v;
-#pragma unroll
-  for (int j = 0; j < NMAX; ++j) v = si == 1 + NSUM + j ? mx[j] : v;
-  return v;
+  for (int k = 0; k < PR_MAX_RW; ++k) {
+    nbits = k == ps.null_word ? w[k] : nbits;
+    lo = k == ps.v_word[vj] ? w[k] : lo;
+    hi = k == ps.v_word[vj] + 1 ? w[k] : hi;
+  }
+  if ((nbits >> vj) & 1u) return 0;
+  const int32_t dt = ps.v_dt[vj];
+  const uint64_t bits = pr_wide(dt) ? ((uint64_t)lo | ((uint64_t)hi << 32)) : (uint64_t)lo;
+  if (st.kind == SK_COUNT) return 1;
+  if (st.kind == SK_SUM_I64) return (uint64_t)pr_ival(bits, dt);
+  // SK_MAX_CODE: MAX keeps code, MIN ~code; NaN never updates either
+  if (dt_is_float(dt)) {
+    const double fv = pr_fval(bits, dt);
+    if (fv != fv) return 0;
+    const uint64_t code = code_f64_bits(f64_bits(fv));
+    return st.is_min ? ~code : code;
+  }
+  const uint64_t code = code_i64(pr_ival(bits, dt));
+  return st.is_min ? ~code : code;
 }
 
+// A workgroup takes a contiguous range of final partitions, so the rows it
+// writes for one query are consecutive across its partitions (each result
+// column is written as long runs, not as short pieces from many workgroups);
+// the next partition's records are loaded while the current one's queries
+// are answered.
 template <int NSUM, int NMAX>
 __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
   constexpr int NSA = NSUM > 0 ? NSUM : 1, NMA = NMAX > 0 ? NMAX : 1;
   extern __shared__ __align__(16) uint8_t s_dyn[];
   __shared__ uint32_t s_sw[PR_WAVES + 1];
   __shared__ uint64_t s_sw64[PR_WAVES];
+  __shared__ uint32_t s_qe[R_MAX_QUERIES];
+  __shared__ uint64_t s_qrow[R_MAX_QUERIES];  // first result row of (query, current partition)
+  __shared__ uint32_t s_qcnt[R_MAX_QUERIES], s_qbase[R_MAX_QUERIES];  // groups of (query, partition), flat start
+  __shared__ uint8_t s_gk[R_MAX_QUERIES * 64];  // keys (partition lanes) of each query's groups, ascending
+  __shared__ uint8_t s_gq[R_MAX_QUERIES * 64];  // query of each flat group
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const uint32_t P = blockIdx.x;
-  const uint64_t base = ps.start2[P];
-  const uint64_t capr = ps.start2[P + 1] - base;
-  const uint32_t n = (uint32_t)(ps.cur2[P] < capr ? ps.cur2[P] : capr);
-  if (n > ps.cap) {  // the host sizes partitions from the key histogram; never expected
-    if (t == 0) atomicOr(ps.status, 2u);
-    return;
-  }
+  const uint32_t per = (ps.np + gridDim.x - 1) / gridDim.x;
+  const uint32_t P0 = blockIdx.x * per;
+  const uint32_t P1 = P0 + per < ps.np ? P0 + per : ps.np;
+  if (P0 >= P1) return;
   const uint32_t NBK = (uint32_t)ps.nbounds + 2;
   const uint32_t nbin = (NBK << ps.s);
   const uint32_t nbw = nbin / 2 + 1;              // packed u16 entries 0 .. nbin
   const uint32_t cap = ps.cap;
+  const uint32_t rw = (uint32_t)ps.rw;
   uint32_t *s_bin = reinterpret_cast<uint32_t *>(s_dyn);
   uint64_t *s_add = reinterpret_cast<uint64_t *>(s_dyn + ((nbw * 4 + 15) & ~15u));
   uint64_t *s_max = s_add + (uint64_t)NSUM * cap;
-  for (uint32_t i = t; i < nbw; i += PR_BLOCK) s_bin[i] = 0;
-  __syncthreads();
-  // 1. records -> (key, bucket) bin and rank within it; state contributions
+  if (t < ps.nq) s_qe[t] = ps.qe[t];
+  // output descriptors, read once (scalar registers for the whole kernel)
+  const int nouts = ps.nouts;
+  ROut ro[R_MAX_OUTS];
+  void *ob[R_MAX_OUTS];
+  uint32_t *onb[R_MAX_OUTS];
+#pragma unroll
+  for (int o = 0; o < R_MAX_OUTS; ++o) {
+    ro[o] = ps.outs[o];
+    ob[o] = ps.out_base[o];
+    onb[o] = ps.out_nbase[o];
+  }
   const uint32_t km = (1u << ps.s) - 1u;
-  uint32_t bn[PR_CIPT], rk[PR_CIPT];
-  uint64_t ca[PR_CIPT][NSA], cm[PR_CIPT][NMA];
-#pragma unroll
-  for (int i = 0; i < PR_CIPT; ++i) {
-    const uint32_t j = (uint32_t)t + (uint32_t)i * PR_BLOCK;
-    rk[i] = ~0u;
-    bn[i] = 0;
-    if (j >= n) continue;
-    const uint32_t *r = ps.rec2 + (base + j) * (uint64_t)ps.rw;
-    const uint32_t kb = r[0];
-    const uint32_t nbits = ps.null_word >= 0 ? r[ps.null_word] : 0u;
-#pragma unroll
-    for (int s2 = 0; s2 < NSUM + NMAX; ++s2) {
-      const DState &st = ps.st[1 + s2];
-      const int vj = ps.st_v[1 + s2];
-      uint64_t c = 0;
-      if (vj < 0) {
-        c = st.kind == SK_COUNT ? 1ull : 0ull;  // COUNT(*)
-      } else if (!((nbits >> vj) & 1u)) {
-        const int32_t dt = ps.v_dt[vj];
-        const int wd = ps.v_word[vj];
-        const uint64_t bits = pr_wide(dt) ? ((uint64_t)r[wd] | ((uint64_t)r[wd + 1] << 32)) : (uint64_t)r[wd];
-        if (st.kind == SK_COUNT) {
-          c = 1;
-        } else if (st.kind == SK_SUM_I64) {
-          c = (uint64_t)pr_ival(bits, dt);
-        } else {  // SK_MAX_CODE: MAX keeps code, MIN ~code; NaN never updates either
-          bool nan = false;
-          uint64_t code;
-          if (dt_is_float(dt)) {
-            const double fv = pr_fval(bits, dt);
-            nan = fv != fv;
-            code = code_f64_bits(f64_bits(fv));
-          } else {
-            code = code_i64(pr_ival(bits, dt));
-          }
-          c = nan ? 0ull : (st.is_min ? ~code : code);
-        }
-      }
-      if (s2 < NSUM) ca[i][s2 < NSA ? s2 : 0] = c;
-      else cm[i][(s2 - NSUM) < NMA ? (s2 - NSUM) : 0] = c;
-    }
-    const uint32_t e = kb >> 16;
-    bn[i] = (kb & km) * NBK + e;
-    const uint32_t sh = (bn[i] & 1u) * 16u;
-    rk[i] = (atomicAdd(&s_bin[bn[i] >> 1], 1u << sh) >> sh) & 0xFFFFu;
-  }
-  __syncthreads();
-  // 2. exclusive scan of the bins (packed pairs; thread t owns a word range)
-  {
-    const uint32_t wpt = (nbw + PR_BLOCK - 1) / PR_BLOCK;
-    const uint32_t w0 = (uint32_t)t * wpt < nbw ? (uint32_t)t * wpt : nbw;
-    const uint32_t w1 = w0 + wpt < nbw ? w0 + wpt : nbw;
-    uint32_t sum = 0;
-    for (uint32_t w = w0; w < w1; ++w) { const uint32_t x = s_bin[w]; sum += (x & 0xFFFFu) + (x >> 16); }
-    uint32_t total;
-    uint32_t run = pr_block_exscan(sum, s_sw, &total);
-    for (uint32_t w = w0; w < w1; ++w) {
-      const uint32_t x = s_bin[w];
-      const uint32_t lo = run;
-      run += x & 0xFFFFu;
-      const uint32_t hi = run;
-      run += x >> 16;
-      s_bin[w] = lo | (hi << 16);
-    }
-  }
-  __syncthreads();
-  // 3. scatter the contributions into (key, bucket) order
-#pragma unroll
-  for (int i = 0; i < PR_CIPT; ++i) {
-    if (rk[i] == ~0u) continue;
-    const uint32_t slot = pr_start(s_bin, bn[i]) + rk[i];
-#pragma unroll
-    for (int j = 0; j < NSUM; ++j) s_add[(uint64_t)j * cap + slot] = ca[i][j];
-#pragma unroll
-    for (int j = 0; j < NMAX; ++j) s_max[(uint64_t)j * cap + slot] = cm[i][j];
-  }
-  __syncthreads();
-  // 4. inclusive prefix sums of the additive states over the sorted order
-#pragma unroll
-  for (int j = 0; j < NSUM; ++j) {
-    uint64_t *a = s_add + (uint64_t)j * cap;
-    const uint32_t per = (n + PR_BLOCK - 1) / PR_BLOCK;
-    const uint32_t i0 = (uint32_t)t * per < n ? (uint32_t)t * per : n;
-    const uint32_t i1 = i0 + per < n ? i0 + per : n;
-    uint64_t sum = 0;
-    for (uint32_t i = i0; i < i1; ++i) sum += a[i];
-    uint64_t run = pr_block_exscan64(sum, s_sw64);
-    for (uint32_t i = i0; i < i1; ++i) { run += a[i]; a[i] = run; }
-    __syncthreads();
-  }
-  // 5. lanes = keys, one query per wave step
   const bool kon = (uint32_t)lane <= km;
-  const int64_t key = ps.kmin + (int64_t)(((uint64_t)P << ps.s) + (uint64_t)lane);
-  const uint32_t kbin = (uint32_t)lane * NBK;
   const uint64_t lt = (1ull << lane) - 1ull;
-  for (int q = wave; q < ps.nq; q += PR_WAVES) {
-    const uint32_t qe = ps.qe[q];
-    const uint32_t elo = qe & 0xFFFFu, ehi = qe >> 16;
-    if (elo >= ehi) continue;
-    uint32_t a = 0, b = 0;
-    if (kon) {
-      a = pr_start(s_bin, kbin + elo);
-      b = pr_start(s_bin, kbin + ehi);
+  const uint32_t kbin = (uint32_t)lane * NBK;
+
+  // records of partition P into registers (raw words), its size, and (thread
+  // t < nq) the first result row of (query t, P)
+  uint32_t wr[PR_CIPT][PR_MAX_RW];
+  uint32_t n = 0;
+  uint64_t qrow = 0;
+  auto load_part = [&](uint32_t P) {
+    const uint64_t base = ps.start2[P];
+    const uint64_t capr = ps.start2[P + 1] - base;
+    const uint32_t c2 = ps.cur2[P];
+    n = (uint32_t)(c2 < capr ? c2 : capr);
+#pragma unroll
+    for (int i = 0; i < PR_CIPT; ++i) {
+      const uint32_t j = (uint32_t)t + (uint32_t)i * PR_BLOCK;
+      const bool in = j < n && j < cap;
+      const uint32_t *r = ps.rec2 + (base + (in ? j : 0)) * (uint64_t)rw;
+#pragma unroll
+      for (int k = 0; k < PR_MAX_RW; ++k) wr[i][k] = (in && (uint32_t)k < rw) ? r[k] : 0u;
     }
-    const bool has = b > a;
-    const uint64_t m = __ballot(has);
-    if (m == 0) continue;
-    if (!has) continue;
-    const uint64_t qrow = ps.gcount[(uint64_t)q * ps.np + P] - ps.gcount[(uint64_t)q * ps.np];
-    const uint64_t gg = ps.row_off[q] + qrow + (uint64_t)__builtin_popcountll(m & lt);
-    const uint64_t cnt = b - a;
-    uint64_t sm[NSA], mx[NMA];
+    if (t < ps.nq) {
+      const uint64_t *gq = ps.gcount + (uint64_t)t * ps.np;
+      qrow = ps.row_off[t] + (gq[P] - gq[0]);
+    }
+  };
+  load_part(P0);
+  for (uint32_t P = P0; P < P1; ++P) {
+    for (uint32_t i = t; i < nbw; i += PR_BLOCK) s_bin[i] = 0;
+    if (t < ps.nq) s_qrow[t] = qrow;
+    const uint32_t nP = n;
+    if (nP > cap && t == 0) atomicOr(ps.status, 2u);  // the host sizes partitions from the key histogram
+    __syncthreads();
+    // 1. records -> (key, bucket) bin and rank within it; state contributions
+    uint32_t bn[PR_CIPT], rk[PR_CIPT];
+    uint64_t ca[PR_CIPT][NSA], cm[PR_CIPT][NMA];
+#pragma unroll
+    for (int i = 0; i < PR_CIPT; ++i) {
+      const uint32_t j = (uint32_t)t + (uint32_t)i * PR_BLOCK;
+      rk[i] = ~0u;
+      bn[i] = 0;
+#pragma unroll
+      for (int k = 0; k < NSA; ++k) ca[i][k] = 0;
+#pragma unroll
+      for (int k = 0; k < NMA; ++k) cm[i][k] = 0;
+      if (j >= nP || j >= cap) continue;
+#pragma unroll
+      for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr[i]);
+#pragma unroll
+      for (int s2 = 0; s2 < NMAX; ++s2) cm[i][s2] = pr_contrib(ps, NSUM + s2, wr[i]);
+      const uint32_t kb = wr[i][0];
+      bn[i] = (kb & km) * NBK + (kb >> 16);
+      const uint32_t sh = (bn[i] & 1u) * 16u;
+      rk[i] = (atomicAdd(&s_bin[bn[i] >> 1], 1u << sh) >> sh) & 0xFFFFu;
+    }
+    __syncthreads();
+    // the next partition's records are in flight from here on
+    if (P + 1 < P1) load_part(P + 1);
+    // 2. exclusive scan of the bins (packed pairs; thread t owns a word range)
+    {
+      const uint32_t wpt = (nbw + PR_BLOCK - 1) / PR_BLOCK;
+      const uint32_t w0 = (uint32_t)t * wpt < nbw ? (uint32_t)t * wpt : nbw;
+      const uint32_t w1 = w0 + wpt < nbw ? w0 + wpt : nbw;
+      uint32_t sum = 0;
+      for (uint32_t w = w0; w < w1; ++w) { const uint32_t x = s_bin[w]; sum += (x & 0xFFFFu) + (x >> 16); }
+      uint32_t total;
+      uint32_t run = pr_block_exscan(sum, s_sw, &total);
+      for (uint32_t w = w0; w < w1; ++w) {
+        const uint32_t x = s_bin[w];
+        const uint32_t lo = run;
+        run += x & 0xFFFFu;
+        const uint32_t hi = run;
+        run += x >> 16;
+        s_bin[w] = lo | (hi << 16);
+      }
+    }
+    __syncthreads();
+    // 3. scatter the contributions into (key, bucket) order
+#pragma unroll
+    for (int i = 0; i < PR_CIPT; ++i) {
+      if (rk[i] == ~0u) continue;
+      const uint32_t slot = pr_start(s_bin, bn[i]) + rk[i];
+#pragma unroll
+      for (int j = 0; j < NSUM; ++j) s_add[(uint64_t)j * cap + slot] = ca[i][j];
+#pragma unroll
+      for (int j = 0; j < NMAX; ++j) s_max[(uint64_t)j * cap + slot] = cm[i][j];
+    }
+    __syncthreads();
+    // 4. inclusive prefix sums of the additive states over the sorted order
+    const uint32_t nn = nP < cap ? nP : cap;
 #pragma unroll
     for (int j = 0; j < NSUM; ++j) {
-      const uint64_t *ad = s_add + (uint64_t)j * cap;
-      sm[j] = ad[b - 1] - (a ? ad[a - 1] : 0ull);
+      uint64_t *a = s_add + (uint64_t)j * cap;
+      const uint32_t pe = (nn + PR_BLOCK - 1) / PR_BLOCK;
+      const uint32_t i0 = (uint32_t)t * pe < nn ? (uint32_t)t * pe : nn;
+      const uint32_t i1 = i0 + pe < nn ? i0 + pe : nn;
+      uint64_t sum = 0;
+      for (uint32_t i = i0; i < i1; ++i) sum += a[i];
+      uint64_t run = pr_block_exscan64(sum, s_sw64);
+      for (uint32_t i = i0; i < i1; ++i) { run += a[i]; a[i] = run; }
+      __syncthreads();
     }
+    // 5. the non-empty (query, key) groups: lanes = keys, one query per wave
+    // step, lists each query's keys in ascending order ...
+    for (int q = wave; q < ps.nq; q += PR_WAVES) {
+      const uint32_t qe = s_qe[q];
+      const uint32_t elo = qe & 0xFFFFu, ehi = qe >> 16;
+      bool has = false;
+      if (kon && elo < ehi) has = pr_start(s_bin, kbin + elo) < pr_start(s_bin, kbin + ehi);
+      const uint64_t m = __ballot(has);
+      if (has) s_gk[q * 64 + __builtin_popcountll(m & lt)] = (uint8_t)lane;
+      if (lane == 0) s_qcnt[q] = (uint32_t)__builtin_popcountll(m);
+    }
+    __syncthreads();
+    // ... the flat index of each query's first group, and each group's query ...
+    uint32_t ngroups;
+    {
+      const uint32_t c = t < ps.nq ? s_qcnt[t] : 0u;
+      const uint32_t b0 = pr_block_exscan(c, s_sw, &ngroups);
+      if (t < ps.nq) s_qbase[t] = b0;
+    }
+    __syncthreads();
+    for (int q = wave; q < ps.nq; q += PR_WAVES) {
+      const uint32_t c = s_qcnt[q];
+      if ((uint32_t)lane < c) s_gq[s_qbase[q] + lane] = (uint8_t)q;
+    }
+    __syncthreads();
+    // ... then lanes = groups: consecutive lanes take consecutive keys of one
+    // query, so each result column is written in runs
+    for (uint32_t g = t; g < ngroups; g += PR_BLOCK) {
+      const uint32_t q = s_gq[g];
+      const uint32_t j = g - s_qbase[q];
+      const uint32_t k = s_gk[q * 64 + j];
+      const uint32_t qe = s_qe[q];
+      const uint32_t a = pr_start(s_bin, k * NBK + (qe & 0xFFFFu));
+      const uint32_t b = pr_start(s_bin, k * NBK + (qe >> 16));
+      const uint64_t gg = s_qrow[q] + j;
+      const int64_t key = ps.kmin + (int64_t)(((uint64_t)P << ps.s) + (uint64_t)k);
+      uint64_t sv[1 + NSUM + NMAX];  // state values: COUNT(*), sums, max codes
+      sv[0] = b - a;
 #pragma unroll
-    for (int j = 0; j < NMAX; ++j) {
-      const uint64_t *md = s_max + (uint64_t)j * cap;
-      uint64_t x = 0;
-      for (uint32_t i = a; i < b; ++i) x = u64max(x, md[i]);
-      mx[j] = x;
+      for (int jj = 0; jj < NSUM; ++jj) {
+        const uint64_t *ad = s_add + (uint64_t)jj * cap;
+        sv[1 + jj] = ad[b - 1] - (a ? ad[a - 1] : 0ull);
+      }
+      if constexpr (NMAX > 0) {
+        uint64_t x0[NMA], x1[NMA];
+#pragma unroll
+        for (int jj = 0; jj < NMA; ++jj) { x0[jj] = 0; x1[jj] = 0; }
+        uint32_t i = a;
+        for (; i + 1 < b; i += 2) {
+#pragma unroll
+          for (int jj = 0; jj < NMAX; ++jj) {
+            const uint64_t *md = s_max + (uint64_t)jj * cap;
+            x0[jj] = u64max(x0[jj], md[i]);
+            x1[jj] = u64max(x1[jj], md[i + 1]);
+          }
+        }
+#pragma unroll
+        for (int jj = 0; jj < NMAX; ++jj) {
+          if (i < b) x0[jj] = u64max(x0[jj], s_max[(uint64_t)jj * cap + i]);
+          sv[1 + NSUM + jj] = u64max(x0[jj], x1[jj]);
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < R_MAX_OUTS; ++o) {
+        if (o >= nouts) break;
+        uint64_t v = 0, c = 0;
+#pragma unroll
+        for (int jj = 0; jj < 1 + NSUM + NMAX; ++jj) {
+          if (ro[o].state == jj) v = sv[jj];
+          if (ro[o].cnt_state == jj) c = sv[jj];
+        }
+        pr_emit(ro[o], ob[o], onb[o], gg, key, v, c);
+      }
     }
-    for (int o = 0; o < ps.nouts; ++o) {
-      const ROut ro = ps.outs[o];
-      const uint64_t v = ro.state >= 0 ? pr_state<NSUM, NMAX>(ro.state, cnt, sm, mx) : 0ull;
-      const uint64_t c = ro.cnt_state >= 0 ? pr_state<NSUM, NMAX>(ro.cnt_state, cnt, sm, mx) : 0ull;
-      pr_emit(ro, ps.out_base[o], ps.out_nbase[o], gg, key, v, c);
-    }
+    __syncthreads();  // LDS (bins, sorted states, first rows) is reused by the next partition
   }
 }
 
@@ -608,7 +684,7 @@ extern "C" size_t qmk_pr_runs_lds(uint32_t nbin, int nsum, int nmax, uint32_t ca
 }
 // largest partition capacity that fits the LDS of one workgroup
 extern "C" uint32_t qmk_pr_max_cap(uint32_t nbin, int nsum, int nmax) {
-  const size_t budget = 150 * 1024;
+  const size_t budget = 118 * 1024;  // of 160 KB: the rest is the kernel's static LDS (query and group lists)
   const size_t fixed = qmk_pr_runs_lds(nbin, 0, 0, 0);
   if (fixed >= budget) return 0;
   uint64_t cap = (uint64_t)PR_CIPT * PR_BLOCK;
@@ -643,7 +719,14 @@ extern "C" hipError_t qmk_pr_runs(const PRScan *ps, hipStream_t st) {
   const size_t lds = qmk_pr_runs_lds(nbin, ps->nsum, nmax, ps->cap);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
+  int bpc = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, PR_BLOCK, lds) != hipSuccess || bpc < 1) bpc = 1;
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * (uint64_t)bpc, ps->np);
   PRScan a = *ps;
   void *args[] = {&a};
-  return hipLaunchKernel(fn, dim3(ps->np), dim3(PR_BLOCK), args, lds, st);
+  return hipLaunchKernel(fn, dim3(grid), dim3(PR_BLOCK), args, lds, st);
 }

@@ -76,11 +76,28 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     if (sh_hll[r]) atomicMax(&hll[r], sh_hll[r]);
 }
 
-// key histogram: rows per bin of 2^g consecutive key values (bin = (v - kmin) >> g)
-__global__ void __launch_bounds__(256) key_hist_kernel(DCol c, uint64_t n, int64_t kmin, int g, uint32_t *hist) {
+// key histogram: rows per bin of 2^g consecutive key values (bin = (v - kmin) >> g);
+// up to KH_LDS_BINS bins are counted in LDS per workgroup first
+static constexpr uint32_t KH_LDS_BINS = 16384;
+template <bool LDS>
+__global__ void __launch_bounds__(256) key_hist_kernel(DCol c, uint64_t n, int64_t kmin, int g, uint32_t nbins,
+                                                       uint32_t *hist) {
+  __shared__ uint32_t s_h[LDS ? KH_LDS_BINS : 1];
+  if constexpr (LDS) {
+    for (uint32_t i = threadIdx.x; i < nbins; i += 256) s_h[i] = 0;
+    __syncthreads();
+  }
   const uint64_t stride = (uint64_t)gridDim.x * 256;
-  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += stride)
-    atomicAdd(&hist[(uint64_t)(load1_i(c, r) - kmin) >> g], 1u);
+  for (uint64_t r = (uint64_t)blockIdx.x * 256 + threadIdx.x; r < n; r += stride) {
+    const uint64_t b = (uint64_t)(load1_i(c, r) - kmin) >> g;
+    if constexpr (LDS) atomicAdd(&s_h[b], 1u);
+    else atomicAdd(&hist[b], 1u);
+  }
+  if constexpr (LDS) {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nbins; i += 256)
+      if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+  }
 }
 
 }  // namespace qm
@@ -142,10 +159,16 @@ extern "C" hipError_t qmk_zone_maps(const DCol *c, uint64_t n, uint64_t *zmin, u
   return hipGetLastError();
 }
 
-extern "C" hipError_t qmk_key_hist(const DCol *c, uint64_t n, int64_t kmin, int g, uint32_t *hist, hipStream_t st) {
+extern "C" hipError_t qmk_key_hist(const DCol *c, uint64_t n, int64_t kmin, int g, uint32_t nbins, uint32_t *hist,
+                                   hipStream_t st) {
   if (n == 0) return hipSuccess;
   uint64_t grid = (n + 255) / 256;
-  if (grid > 65536) grid = 65536;
-  hipLaunchKernelGGL(key_hist_kernel, dim3((unsigned)grid), dim3(256), 0, st, *c, n, kmin, g, hist);
+  if (nbins <= KH_LDS_BINS) {
+    if (grid > 2048) grid = 2048;
+    hipLaunchKernelGGL(key_hist_kernel<true>, dim3((unsigned)grid), dim3(256), 0, st, *c, n, kmin, g, nbins, hist);
+  } else {
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(key_hist_kernel<false>, dim3((unsigned)grid), dim3(256), 0, st, *c, n, kmin, g, nbins, hist);
+  }
   return hipGetLastError();
 }

@@ -1209,7 +1209,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
       const uint64_t nbins = (D + (1ull << kg) - 1) >> kg;
       DevBuf hb(ctx, nbins * 4 + 64);
       zero(ctx, hb.p, nbins * 4);
-      HIPX(qmk_key_hist(&dc, nrows, s.imin, kg, hb.as<uint32_t>(), ctx->stream));
+      HIPX(qmk_key_hist(&dc, nrows, s.imin, kg, (uint32_t)nbins, hb.as<uint32_t>(), ctx->stream));
       tc.khist.resize(nbins);
       d2h(ctx, tc.khist.data(), hb.p, nbins);
       tc.khist_g = kg;
@@ -2400,7 +2400,7 @@ const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uin
     if (mx > cap) continue;
     int lg = 0;
     while ((1ull << lg) < np) ++lg;
-    const int b2max = std::min(11, 16 - s);
+    const int b2max = std::min(10, 16 - s);  // 2^b2 <= PR_MAX_P digits in pass 2
     int b2 = std::min((lg + 1) / 2, b2max);
     while (((D + (1ull << (s + b2)) - 1) >> (s + b2)) > (uint64_t)PR_MAX_P && b2 < b2max) ++b2;
     const uint64_t np1 = (D + (1ull << (s + b2)) - 1) >> (s + b2);

@@ -383,9 +383,9 @@ struct S1Scan {
 // answers every query from prefix sums over that order (lanes = keys).
 static constexpr int PR_MAX_V = 4;       // distinct state columns copied into records
 static constexpr int PR_MAX_RW = 8;      // record words
-static constexpr int PR_MAX_P = 2048;    // digits per scatter pass
+static constexpr int PR_MAX_P = 1024;    // digits per scatter pass
 static constexpr int PR_BLOCK = 512;     // scatter / partition workgroup
-static constexpr int PR_STAGE_WORDS = 16384;  // LDS staging of one scatter tile (64 KB)
+static constexpr int PR_STAGE_WORDS = 12288;  // LDS staging of one scatter tile (48 KB: two workgroups per CU)
 static constexpr int PR_MAX_S = 6;       // key bits per final partition (one key per lane)
 static constexpr int PR_MAX_STATES = 6;  // COUNT(*) + up to 5 more (COUNT(col) guards of nullable columns)
 struct PRScan {

@@ -154,7 +154,8 @@ hipError_t qmk_col_stats(const qm::DCol *c, uint64_t n, uint64_t *stats /*8 word
 // zone maps: per ZONE_ROWS-row page, min / max order code of the non-NULL values
 hipError_t qmk_zone_maps(const qm::DCol *c, uint64_t n, uint64_t *zmin, uint64_t *zmax, hipStream_t st);
 // key histogram of an integer column: hist[(v - kmin) >> g] += 1 (hist zeroed by the caller)
-hipError_t qmk_key_hist(const qm::DCol *c, uint64_t n, int64_t kmin, int g, uint32_t *hist, hipStream_t st);
+hipError_t qmk_key_hist(const qm::DCol *c, uint64_t n, int64_t kmin, int g, uint32_t nbins, uint32_t *hist,
+                        hipStream_t st);
 // batched group-by over a key in any row order (k_pruns.hip): scatter passes
 // (src 0: table rows -> level-1 regions; src 1: level-1 -> final partitions),
 // per-(query, partition) group counts, and the write pass
