@@ -164,6 +164,22 @@ int qm_batch_wait(qm_batch *b, qm_result **results_out);
 /* Waits for the batch if it still runs; frees results not taken. */
 int qm_batch_close(qm_batch *b);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): qm_exec of one filter -> GROUP BY -> aggregate plan (no
+ * join, computed column, sort, limit / offset or IN list) that calls no
+ * device allocator. The group table, descriptor, counters and compaction
+ * temporaries live in a per-device static scratch array of 32 MiB; a plan
+ * needing more fails with QM_ERR_MEMORY_EXCEEDED. The result columns go to
+ * the caller's device buffers: out_data[k] holds `cap` values of output column
+ * k (the dtype qm_exec's result gives that column), out_nulls[k] (optional,
+ * 8-byte aligned, ceil(cap / 64) 64-bit words) receives its NULL bitmap
+ * (QMCB layout, bit set = null); *rows_out = groups written (> cap:
+ * QM_ERR_INVALID_ARG). Calls that overlap on one device race: they share the
+ * scratch array. */
+int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, uint32_t *const *out_nulls,
+                    uint64_t cap, uint64_t *rows_out);
+
 /* Device bytes one execution of `p` needs beyond the resident tables (no
  * reference counterpart: the reference checks its memoryBudget per operator,
  * src/operators.ts:2383-2462; the serving loop admits requests with this). */

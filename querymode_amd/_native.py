@@ -58,6 +58,8 @@ SIGNATURES = [
     ("qm_plan_compile", C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("qm_plan_close", C.c_int, [C.c_void_p]),
     ("qm_exec", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("qm_exec_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_uint64,
+                                  C.POINTER(C.c_uint64)]),
     ("qm_batch_exec", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("qm_batch_submit", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p,
                                   C.POINTER(C.c_void_p)]),

@@ -1027,6 +1027,14 @@ const KnobDef *find_knob(const char *name) {
 }
 }  // namespace
 
+// Scratch of qm_exec_noalloc: group table, descriptor, counters and
+// compaction temporaries of one call, in a file-scope device array (one copy
+// per device, held until the process ends). Host code never uses the array's
+// name as an address: qm_init records this device's copy's address
+// (hipGetSymbolAddress) in the context.
+static constexpr uint64_t QM_NOALLOC_SCRATCH_BYTES = 32ull << 20;
+static __device__ __align__(256) uint8_t scratch[QM_NOALLOC_SCRATCH_BYTES];
+
 extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   API_BEGIN
   if (!out) fail(QM_ERR_INVALID_ARG, "null out");
@@ -1055,6 +1063,12 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   HIPX(hipEventCreate(&c->evk0));
   HIPX(hipEventCreate(&c->evk1));
   HIPX(hipEventCreate(&c->evk2));
+  {
+    void *sp = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) fail(QM_ERR_QUERY_FAILED, "scratch array address: %s", hipGetErrorString(e));
+    c->scratch = sp;
+  }
   *out = c.release();
   API_END
 }
@@ -3310,6 +3324,125 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
 // (group tables, partition records, join index, result columns), from the
 // same binding the execution uses; the serving loop admits requests against
 // the HBM budget with it.
+// One filter -> GROUP BY -> aggregate plan without any device allocation:
+// the group table, the query descriptor, the counters and the compaction
+// temporaries live in the context's scratch array; the result columns go to
+// the caller's device buffers. Calls that overlap on one device race (they
+// share the array); every call clears what it needs.
+extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, uint32_t *const *out_nulls,
+                               uint64_t cap, uint64_t *rows_out) {
+  API_BEGIN
+  if (!p || !t || !out_data || !rows_out) fail(QM_ERR_INVALID_ARG, "null argument");
+  *rows_out = 0;
+  qm_ctx *ctx = t->ctx;
+  if (p->ctx != ctx) fail(QM_ERR_INVALID_ARG, "plan and table belong to different contexts");
+  const Plan &pl = p->plan;
+  if (pl.aggs.empty() || pl.has_join || !pl.computed.empty() || pl.has_sort || pl.limit >= 0 || pl.offset)
+    fail(QM_ERR_UNSUPPORTED, "qm_exec_noalloc runs filter -> GROUP BY -> aggregate plans (no join, computed "
+                             "column, sort, limit or offset)");
+  auto has_list = [](const std::vector<Filter> &fs) {
+    for (const Filter &f : fs) if (f.op == OP_IN || f.op == OP_NOT_IN) return true;
+    return false;
+  };
+  bool lists = has_list(pl.filters);
+  for (const auto &g : pl.groups) lists = lists || has_list(g);
+  if (lists) fail(QM_ERR_UNSUPPORTED, "qm_exec_noalloc takes no IN / NOT IN lists (their values need device memory)");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  if (!ctx->scratch) fail(QM_ERR_QUERY_FAILED, "no scratch array on this device");
+  uint8_t *const base = static_cast<uint8_t *>(ctx->scratch);
+  ColSpace cs;
+  cs.L = t;
+  cs.plan = &pl;
+  AggBinding ab;
+  ListStore ls;
+  bind_filters(ctx, cs, pl.filters, pl.groups, ab.q, ls, false);
+  const DQuery filt = ab.q;
+  auto a256 = [](uint64_t x) { return (x + 255) & ~255ull; };
+  for (uint32_t regrow = 0;; ++regrow) {
+    ab.q = filt;
+    ab.outs.clear();
+    ab.key_slots.clear();
+    bind_aggregate(ctx, cs, pl, t->rows, false, ab, regrow);
+    ab.part = false;  // one scan into the table
+    DQuery &q = ab.q;
+    const uint64_t nb = qmk_compact_blocks(ab.nslots);
+    const uint64_t tw = qmk_exscan_tmp_words(nb);
+    // scratch layout: table | status, selected | descriptor | block counts, offsets, scan temporaries
+    const uint64_t o_table = 0, tb = a256(ab.table_words * 8);
+    const uint64_t o_stat = o_table + tb, o_desc = o_stat + 256;
+    const uint64_t o_cnt = o_desc + a256(sizeof(DQuery)), o_offs = o_cnt + a256(nb * 8);
+    const uint64_t o_tmp = o_offs + a256(nb * 8), need = o_tmp + a256((tw + 1) * 8);
+    if (need > QM_NOALLOC_SCRATCH_BYTES)
+      fail(QM_ERR_MEMORY_EXCEEDED, "qm_exec_noalloc: the plan needs %llu bytes of scratch, the maximum is %llu",
+           (unsigned long long)need, (unsigned long long)QM_NOALLOC_SCRATCH_BYTES);
+    uint64_t *table = reinterpret_cast<uint64_t *>(base + o_table);
+    uint64_t *stat = reinterpret_cast<uint64_t *>(base + o_stat);  // [0] status words, [1] selected rows
+    // cleared with a fill kernel (no memset nodes for callers that capture)
+    HIPX(qmk_fill_u64(table, ab.table_words, 0, ctx->stream));
+    HIPX(qmk_fill_u64(stat, 4, 0, ctx->stream));
+    q.table = table;
+    q.status = reinterpret_cast<uint32_t *>(stat);
+    q.selected = stat + 1;
+    q.lds_offset = 0;
+    DQuery *dq = reinterpret_cast<DQuery *>(base + o_desc);
+    h2d(ctx, dq, reinterpret_cast<const uint8_t *>(&q), sizeof(DQuery));
+    DScan sc;
+    memset(&sc, 0, sizeof sc);
+    sc.nrows = t->rows;
+    sc.ncols = (int)cs.cols.size();
+    sc.nqueries = 1;
+    for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+    int grid, block;
+    size_t lds_bytes;
+    scan_launch_shape(ctx, t->rows, (uint64_t)q.lds_words, &grid, &block, &lds_bytes);
+    if (lds_bytes > 160 * 1024) fail(QM_ERR_UNSUPPORTED, "group table exceeds the workgroup LDS");
+    if (t->rows) HIPX(qmk_scan_agg(&sc, dq, grid, block, lds_bytes, ctx->stream));
+    uint32_t st[2] = {0, 0};
+    d2h(ctx, st, stat, 2);
+    if (st[0]) {
+      if (regrow >= 4) fail(QM_ERR_QUERY_FAILED, "group-by hash table kept overflowing");
+      continue;
+    }
+    DFinal f;
+    memset(&f, 0, sizeof f);
+    f.table = table;
+    f.nslots = ab.nslots;
+    f.entry_words = q.entry_words;
+    f.count_word = q.count_word;
+    f.gmode = q.gmode;
+    f.always_one_row = ab.always_one_row ? 1 : 0;
+    f.key = q.key;
+    f.status = q.status;
+    uint64_t *cnt = reinterpret_cast<uint64_t *>(base + o_cnt), *offs = reinterpret_cast<uint64_t *>(base + o_offs);
+    uint64_t *tmp = reinterpret_cast<uint64_t *>(base + o_tmp);
+    HIPX(qmk_compact_count(&f, cnt, ctx->stream));
+    HIPX(qmk_exscan_u64(cnt, nb, offs, tmp + tw, tmp, ctx->stream));
+    uint64_t total = 0;
+    d2h(ctx, &total, tmp + tw, 1);
+    if (total > cap)
+      fail(QM_ERR_INVALID_ARG, "qm_exec_noalloc: the result has %llu rows, the output columns hold %llu",
+           (unsigned long long)total, (unsigned long long)cap);
+    f.nouts = (int)ab.outs.size();
+    for (size_t k = 0; k < ab.outs.size(); ++k) {
+      f.outs[k] = ab.outs[k].o;
+      if (!out_data[k]) fail(QM_ERR_INVALID_ARG, "output column %zu has no buffer", k);
+      f.outs[k].data = out_data[k];
+      uint32_t *nbm = (out_nulls && ab.outs[k].o.kind != OK_COUNT) ? out_nulls[k] : nullptr;
+      if (nbm && (reinterpret_cast<uintptr_t>(nbm) & 7))
+        fail(QM_ERR_INVALID_ARG, "NULL bitmap of output column %zu is not 8-byte aligned", k);
+      f.outs[k].nulls = nbm;
+      if (nbm && total) HIPX(qmk_fill_u64(reinterpret_cast<uint64_t *>(nbm), (total + 63) / 64, 0, ctx->stream));
+    }
+    if (total) HIPX(qmk_compact_write(&f, offs, ctx->stream));
+    d2h(ctx, st, stat, 2);
+    if (st[1]) fail(QM_ERR_OVERFLOW, "SUM over an int64 column does not fit int64");
+    *rows_out = total;
+    break;
+  }
+  API_END
+}
+
 extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *bytes) {
   API_BEGIN
   if (!p || !left || !bytes) fail(QM_ERR_INVALID_ARG, "null argument");

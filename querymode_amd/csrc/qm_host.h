@@ -105,6 +105,7 @@ struct qm_ctx {
   uint64_t pooled = 0;
   int num_cus = 256;
   std::string name;
+  void *scratch = nullptr;  // device address of this device's copy of the no-allocation scratch array
   // tuning parameters of the partitioned group-by (results do not depend on
   // them); defaults, overridable with QM_TUNE_* environment variables
   struct Tune {
