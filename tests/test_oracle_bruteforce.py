@@ -7,6 +7,7 @@ groups) and numpy.lexsort (top-K) — not from the oracle's own code.
 from __future__ import annotations
 
 import math
+import zlib
 from fractions import Fraction
 
 import numpy as np
@@ -75,7 +76,7 @@ CASES = [("eq", 7), ("neq", 7), ("lt", -3), ("lte", -3), ("gt", 11), ("gte", 11)
 @pytest.mark.parametrize("dt", ALL_DTYPES, ids=lambda d: np.dtype(d).name)
 @pytest.mark.parametrize("op,lit", CASES)
 def test_filter_matches_numpy(dt, op, lit):
-    rng = np.random.default_rng(abs(hash((np.dtype(dt).name, op, str(lit)))) % 2 ** 32)
+    rng = np.random.default_rng(zlib.crc32(f"{np.dtype(dt).name}|{op}|{lit}".encode()))
     x = _column(rng, dt, N)
     isnull = rng.random(N) < 0.05
     t = O.Table({"x": x}, {"x": isnull})
@@ -131,7 +132,7 @@ def _norm_key(v):
 @pytest.mark.parametrize("kdt", [np.int32, np.int64, np.float64])
 @pytest.mark.parametrize("vdt", [np.int32, np.int64, np.float64, np.float32, np.int8])
 def test_groupby_matches_bruteforce(kdt, vdt):
-    rng = np.random.default_rng(hash((np.dtype(kdt).name, np.dtype(vdt).name)) % 2 ** 32)
+    rng = np.random.default_rng(zlib.crc32(f"{np.dtype(kdt).name}|{np.dtype(vdt).name}".encode()))
     n = 20_000
     if kdt == np.float64:
         k = rng.integers(-30, 30, n).astype(np.float64) / 4
@@ -241,3 +242,70 @@ def test_product_aggregate_exact():
         sel = g == row["g"]
         want = math.fsum((a[sel] * w[sel].astype(np.float64)).tolist())
         assert math.isclose(row["aw"], want, rel_tol=1e-12)
+
+
+# ---- the reference's heap TopK (src/operators.ts:1716-1802), the CPU
+# baseline's top-K: a binary heap of k rows whose root is the worst kept row,
+# replaced only by a strictly better one; NULL / NaN sort last ----
+@pytest.mark.parametrize("desc", [True, False])
+@pytest.mark.parametrize("k", [1, 10, 100, 1000, 5000])
+def test_topk_heap_value_multiset(desc, k):
+    rng = np.random.default_rng(100 + k + (1 if desc else 0))
+    n = 3000
+    v = rng.integers(-200, 200, n).astype(np.float64) / 8  # many ties
+    v[rng.random(n) < 0.02] = np.nan
+    t = O.Table({"v": v})
+    got = O.topk_heap_rows(t, "v", desc, k)
+    assert len(got) == min(k, n)
+    assert len(set(got.tolist())) == len(got)
+    gv = v[got.astype(np.int64)]
+    finite = v[~np.isnan(v)]
+    want = np.sort(finite)[::-1] if desc else np.sort(finite)
+    nk = min(k, n)
+    want = np.concatenate([want, np.full(max(0, nk - len(want)), np.nan)])[:nk]
+    # kept values = the k best values (a multiset), best first, NaN last
+    assert np.array_equal(gv, want, equal_nan=True)
+
+
+@pytest.mark.parametrize("k", [1, 7, 64])
+def test_topk_heap_ties_keep_first_seen(k):
+    # all rows equal: no later row is strictly better than the root, so the
+    # kept rows are the first k scanned
+    n = 500
+    for desc in (True, False):
+        t = O.Table({"v": np.full(n, 2.5)})
+        got = O.topk_heap_rows(t, "v", desc, k)
+        assert sorted(got.tolist()) == list(range(k))
+    # a strictly better row evicts a kept one; a tie with the root does not enter
+    v = np.array([1.0, 1.0, 1.0, 3.0, 1.0, 3.0])
+    got = O.topk_heap_rows(O.Table({"v": v}), "v", True, 3)
+    assert sorted(v[got.astype(np.int64)].tolist()) == [1.0, 3.0, 3.0]
+    assert {3, 5} <= set(got.tolist()) and 4 not in got.tolist()
+
+
+def to_int32(x: float) -> int:
+    """ECMAScript ToInt32: what a JS number becomes as a WASM i32 argument."""
+    if not math.isfinite(x):
+        return 0
+    v = int(math.trunc(x)) % 2 ** 32
+    return v - 2 ** 32 if v >= 2 ** 31 else v
+
+
+INT32_LITERALS = [2.5, -2.5, 3e9, -3e9, 2.0 ** 31 + 0.5, float("nan"), float("inf"), float("-inf")]
+
+
+@pytest.mark.parametrize("lit", INT32_LITERALS, ids=str)
+@pytest.mark.parametrize("op", ["eq", "neq", "lt", "lte", "gt", "gte", "between"])
+def test_int32_literal_coerced_by_toint32(op, lit):
+    # int32 columns take the WASM filter, whose i32 argument is ToInt32 of the
+    # literal (src/operators.ts:595-607, 622-666)
+    rng = np.random.default_rng(zlib.crc32(f"{op}|{lit}".encode()))
+    specials = np.array([to_int32(lit) + d for d in (-1, 0, 1)] + [0, -1, 1, 2, -2, -3, 3], dtype=np.int64)
+    x = np.concatenate([rng.integers(-(2 ** 31), 2 ** 31, N - 200), np.repeat(specials, 20)[:200]])
+    x = np.clip(x, -(2 ** 31), 2 ** 31 - 1).astype(np.int32)
+    t = O.Table({"x": x})
+    if op == "between":
+        fop, want = FilterOp("x", op, [-lit, lit]), (x >= to_int32(-lit)) & (x <= to_int32(lit))
+    else:
+        fop, want = FilterOp("x", op, lit), _expect(op, x, to_int32(lit), np.zeros(N, bool))
+    assert np.array_equal(O.filter_rows(t, [fop]), np.nonzero(want)[0])
