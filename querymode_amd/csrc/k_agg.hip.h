@@ -126,12 +126,14 @@ __device__ __forceinline__ void fetch_state_vals(const DState &s, const DCol *co
 
 // Per-value contribution of a state, as the words to atomically combine.
 struct Contrib {
-  uint64_t a, b, c;  // meaning depends on kind
+  uint64_t a, b, c;  // meaning depends on kind (SK_SUM_WIDE: the three signed digits)
+  uint64_t f;        // SK_SUM_WIDE: non-finite flags
+  int32_t wi;        // SK_SUM_WIDE: first digit word (-1: zero)
   bool skip;
 };
 
 __device__ __forceinline__ Contrib contrib_of(const DState &s, bool is_float, int64_t iv, double fv) {
-  Contrib k; k.a = k.b = k.c = 0; k.skip = false;
+  Contrib k; k.a = k.b = k.c = 0; k.f = 0; k.wi = -1; k.skip = false;
   switch (s.kind) {
     case SK_COUNT: k.a = 1; break;
     case SK_SUM_I64: k.a = (uint64_t)iv; break;
@@ -148,6 +150,17 @@ __device__ __forceinline__ Contrib contrib_of(const DState &s, bool is_float, in
         k.a = (uint64_t)iv; k.b = (iv < 0) ? ~0ull : 0ull;
       }
       break;
+    case SK_SUM_WIDE: {
+      const uint64_t bits = f64_bits(fv);
+      if (((bits >> 52) & 0x7FF) == 0x7FF) {
+        k.f = f64_is_nan_bits(bits) ? SUMF_NAN : ((bits >> 63) ? SUMF_NINF : SUMF_PINF);
+      } else {
+        int64_t d[3];
+        f64_wide_digits(fv, s.scale, &k.wi, d);
+        k.a = (uint64_t)d[0]; k.b = (uint64_t)d[1]; k.c = (uint64_t)d[2];
+      }
+      break;
+    }
     default: {  // SK_MAX_CODE
       uint64_t code;
       if (is_float) {
@@ -171,6 +184,14 @@ __device__ __forceinline__ void apply_glb(const DState &s, uint64_t *e, const Co
       g_add128(w, k.a, k.b);
       if (s.fixed && k.c) g_atomic_or(w + 2, k.c);
       break;
+    case SK_SUM_WIDE:
+      if (k.f) g_atomic_or(w, k.f);
+      if (k.wi >= 0) {
+        if (k.a) g_atomic_add(w + 1 + k.wi, k.a);
+        if (k.b) g_atomic_add(w + 2 + k.wi, k.b);
+        if (k.c) g_atomic_add(w + 3 + k.wi, k.c);
+      }
+      break;
     default: g_atomic_max(w, k.a); break;
   }
 }
@@ -182,6 +203,14 @@ __device__ __forceinline__ void apply_lds(const DState &s, lds_u64 *e, const Con
     case SK_SUM_128:
       l_add128(w, k.a, k.b);
       if (s.fixed && k.c) l_atomic_or(w + 2, k.c);
+      break;
+    case SK_SUM_WIDE:
+      if (k.f) l_atomic_or(w, k.f);
+      if (k.wi >= 0) {
+        if (k.a) l_atomic_add(w + 1 + k.wi, k.a);
+        if (k.b) l_atomic_add(w + 2 + k.wi, k.b);
+        if (k.c) l_atomic_add(w + 3 + k.wi, k.c);
+      }
       break;
     default: l_atomic_max(w, k.a); break;
   }
@@ -198,6 +227,10 @@ __device__ __forceinline__ void merge_entry_glb(const DQuery &q, uint64_t *dst, 
       case SK_SUM_128:
         g_add128(d, w[0], w[1]);
         if (s.fixed && w[2]) g_atomic_or(d + 2, w[2]);
+        break;
+      case SK_SUM_WIDE:
+        if (w[0]) g_atomic_or(d, w[0]);
+        for (int j = 1; j <= s.nw; ++j) if (w[j]) g_atomic_add(d + j, w[j]);
         break;
       default: if (w[0]) g_atomic_max(d, w[0]); break;
     }

@@ -221,6 +221,16 @@ __device__ void finalize_slot(const DFinal &f, uint64_t slot, uint64_t pos) {
         static_cast<double *>(o.data)[pos] = finish_f64_sum(q, e[o.word + 2], o.scale) / (double)cnt;
         break;
       }
+      case OK_SUM_WIDE: {
+        if (cnt == 0) { isnull = 1; static_cast<double *>(o.data)[pos] = 0; break; }
+        static_cast<double *>(o.data)[pos] = wide_to_f64(e + o.word + 1, o.nw, o.scale, e[o.word]);
+        break;
+      }
+      case OK_AVG_WIDE: {
+        if (cnt == 0) { isnull = 1; static_cast<double *>(o.data)[pos] = 0; break; }
+        static_cast<double *>(o.data)[pos] = wide_to_f64(e + o.word + 1, o.nw, o.scale, e[o.word]) / (double)cnt;
+        break;
+      }
       case OK_AVG_I64: {
         if (cnt == 0) { isnull = 1; static_cast<double *>(o.data)[pos] = 0; break; }
         static_cast<double *>(o.data)[pos] = (double)(int64_t)e[o.word] / (double)cnt;

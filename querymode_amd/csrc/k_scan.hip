@@ -144,6 +144,10 @@ __device__ void flush_lds(const DQuery &q, lds_u64 *lds) {
               if (s.fixed) acc[w + 2] |= e[w + 2];
               break;
             }
+            case SK_SUM_WIDE:  // flags word, then independent digit words
+              acc[w] |= e[w];
+              for (int j = 1; j <= s.nw; ++j) acc[w + j] += e[w + j];
+              break;
             default: if (e[w] > acc[w]) acc[w] = e[w]; break;
           }
         }

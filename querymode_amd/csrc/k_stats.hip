@@ -10,12 +10,13 @@ namespace qm {
 
 // stats words: [0] max(~code) (MIN), [1] max(code) (MAX), [2] max finite |x| bits,
 // [3] NaN count, [4] NULL count, [5] non-null count, [6] descents of an integer
-// column without NULLs (rows i with x[i] > x[i+1]; 0 = sorted ascending)
+// column without NULLs (rows i with x[i] > x[i+1]; 0 = sorted ascending), [7]
+// max(~bits) of the nonzero finite |x| (the smallest one is ~stats[7]; 0 = none)
 __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *hll) {
   __shared__ uint32_t sh_hll[4096];
   for (int i = threadIdx.x; i < 4096; i += blockDim.x) sh_hll[i] = 0;
   __syncthreads();
-  uint64_t mn = 0, mx = 0, mabs = 0, nan = 0, nul = 0, nn = 0, desc = 0;
+  uint64_t mn = 0, mx = 0, mabs = 0, mnz = 0, nan = 0, nul = 0, nn = 0, desc = 0;
   const bool isf = dt_is_float(c.dtype);
   const bool order_check = !isf && c.nulls == nullptr;
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -36,6 +37,7 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
         if (!f64_is_inf_bits(b)) {
           uint64_t a = b & 0x7FFFFFFFFFFFFFFFull;
           if (a > mabs) mabs = a;
+          if (a && ~a > mnz) mnz = ~a;
         }
       }
     } else {
@@ -57,6 +59,7 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     t = __shfl_xor(mn, off); if (t > mn) mn = t;
     t = __shfl_xor(mx, off); if (t > mx) mx = t;
     t = __shfl_xor(mabs, off); if (t > mabs) mabs = t;
+    t = __shfl_xor(mnz, off); if (t > mnz) mnz = t;
     nan += __shfl_xor(nan, off);
     nul += __shfl_xor(nul, off);
     nn += __shfl_xor(nn, off);
@@ -66,6 +69,7 @@ __global__ void col_stats_kernel(DCol c, uint64_t n, uint64_t *stats, uint32_t *
     if (mn) g_atomic_max(&stats[0], mn);
     if (mx) g_atomic_max(&stats[1], mx);
     if (mabs) g_atomic_max(&stats[2], mabs);
+    if (mnz) g_atomic_max(&stats[7], mnz);
     if (nan) g_atomic_add(&stats[3], nan);
     if (nul) g_atomic_add(&stats[4], nul);
     if (nn) g_atomic_add(&stats[5], nn);

@@ -487,6 +487,7 @@ struct StateReq {
   uint8_t kind, is_min, fixed, product;
   int8_t a, b;
   int scale;
+  int nw = 0;  // SK_SUM_WIDE digit words
 };
 
 void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, bool fused_join, AggBinding &ab,
@@ -574,8 +575,26 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
           if (!std::isfinite(bound)) bound = 1.7976931348623157e308;
         }
         scale = fixed_scale_for(bound);
-        sr.kind = SK_SUM_128; sr.fixed = 1; sr.scale = scale;
-        ok = (a.fn == AGG_SUM) ? OK_SUM_F64 : OK_AVG_F64;
+        // the smallest nonzero |x| (a product's: of the operands' smallest,
+        // halved for its rounding; integers: 1)
+        auto low_of = [&](int slot) {
+          const ColStats &st = cs.st(slot);
+          return dt_float(cs.cols[slot].dtype) ? st.min_abs : 1.0;
+        };
+        double low = low_of(r.slot_a);
+        if (r.product()) low = low * low_of(r.slot_b) * 0.5;
+        int wbase = 0, wnw = 0;
+        if (low > 0 && bound > 0) wide_layout(low, bound, &wbase, &wnw);
+        if (low > 0 && bound > 0 && wbase < -scale) {
+          // some value has bits below the 128-bit fixed point's resolution:
+          // digit words over the column's whole exponent range instead
+          sr.kind = SK_SUM_WIDE; sr.scale = wbase; sr.nw = wnw;
+          scale = wbase;
+          ok = (a.fn == AGG_SUM) ? OK_SUM_WIDE : OK_AVG_WIDE;
+        } else {
+          sr.kind = SK_SUM_128; sr.fixed = 1; sr.scale = scale;
+          ok = (a.fn == AGG_SUM) ? OK_SUM_F64 : OK_AVG_F64;
+        }
         odt = DT_F64;
       } else if (dta == DT_I64) {
         sr.kind = SK_SUM_128;
@@ -588,6 +607,7 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
       }
       int ri = want(sr);
       reqs[ri].scale = scale;
+      reqs[ri].nw = sr.nw;
       pend.push_back(Pending{ri, cnt, ok, odt, isf, scale, a.alias});
       continue;
     }
@@ -617,10 +637,13 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
     s.col_a = reqs[i].a;
     s.col_b = reqs[i].b;
     s.scale = reqs[i].scale;
+    s.nw = reqs[i].nw;
     s.word = (int16_t)w;
-    w += (s.kind == SK_SUM_128) ? (s.fixed ? 3 : 2) : 1;
+    w += (s.kind == SK_SUM_128) ? (s.fixed ? 3 : 2) : (s.kind == SK_SUM_WIDE ? 1 + s.nw : 1);
   }
-  if (w > 24) fail(QM_ERR_UNSUPPORTED, "aggregate state too wide (%d words)", w);
+  bool wide = false;
+  for (int i = 0; i < q.nstates; ++i) wide = wide || q.states[i].kind == SK_SUM_WIDE;
+  if (w > (wide ? 160 : 24)) fail(QM_ERR_UNSUPPORTED, "aggregate state too wide (%d words)", w);
   q.entry_words = w;
   q.count_word = q.states[0].word;
 
@@ -681,8 +704,12 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
     }
   }
   if (fused_join && q.lmode == LM_HASH) q.lmode = LM_NONE;
+  if (wide && w > 24) {  // entries wider than the LDS merge buffer update the global table directly
+    q.lmode = LM_NONE;
+    q.lds_words = 0;
+  }
 
-  plan_partitioned(ctx, cs, nrows, ab.est_groups, fused_join, ab);
+  if (!wide) plan_partitioned(ctx, cs, nrows, ab.est_groups, fused_join, ab);
 
   // ---- outputs ----
   for (int i = 0; i < nk; ++i) {
@@ -704,6 +731,7 @@ void bind_aggregate(qm_ctx *ctx, ColSpace &cs, const Plan &pl, uint64_t nrows, b
     ao.o.out_dtype = p.out_dt;
     ao.o.src_is_float = p.src_float ? 1 : 0;
     ao.o.scale = p.scale;
+    ao.o.nw = (p.req >= 0 && q.states[p.req].kind == SK_SUM_WIDE) ? (uint8_t)q.states[p.req].nw : 0;
     ao.o.word = (p.req >= 0) ? q.states[p.req].word : 0;
     ao.o.cnt_word = q.states[p.cnt_req].word;
     ab.outs.push_back(ao);
@@ -1207,6 +1235,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
         s.fmax = bits_f64(decode_f64_code(sw[1]));
       }
       s.max_abs = bits_f64(sw[2]);
+      s.min_abs = sw[7] ? bits_f64(~sw[7]) : 0.0;
     } else if (s.nonnull) {
       s.has_int = true;
       s.imin = decode_i64(~sw[0]);
@@ -1849,7 +1878,7 @@ void batch_classify(qm_ctx *ctx, const qm_table *T, const Plan &pl, BItem &it) {
     it.slot_t[s] = cs.tcols[q.slot_col[s]];
   }
   for (int s = 0; s < q.nstates; ++s) {
-    if (q.states[s].product) return;
+    if (q.states[s].product || q.states[s].kind == SK_SUM_WIDE) return;
     it.st_t[s] = q.states[s].col_a >= 0 ? cs.tcols[q.states[s].col_a] : nullptr;
   }
   const int nk = q.key.ncols;

@@ -215,4 +215,84 @@ QM_HD double bigint_avg(u128 s, uint64_t n) {
   return qd + rd / (double)n;
 }
 
+// ---------------------------------------------------------------------------
+// Exact FP64 summation over any exponent range (SK_SUM_WIDE): signed 32-bit
+// digits held in 64-bit words, value = sum_i w_i * 2^(base + 32 i). A finite
+// x = +-m * 2^e (m < 2^53) adds +-(m << o) split into three digits starting
+// at word (e - base) >> 5, o = (e - base) & 31 (base <= the exponent of the
+// lowest bit of every value of the column). Words never carry while summing
+// (|w_i| < 2^63 for up to 2^31 terms), so the sum is independent of the
+// order of addition; carries are propagated once at the end, and the exact
+// integer is rounded once (to nearest, ties to even).
+// ---------------------------------------------------------------------------
+// *idx = first word (-1: x == 0), d[3] = signed digits
+QM_HD void f64_wide_digits(double x, int base, int *idx, int64_t d[3]) {
+  const uint64_t b = f64_bits(x);
+  const int neg = (int)(b >> 63);
+  const int ex = (int)((b >> 52) & 0x7FF);
+  uint64_t m = b & 0xFFFFFFFFFFFFFull;
+  int e;
+  if (ex == 0) e = -1074; else { m |= 0x10000000000000ull; e = ex - 1075; }
+  if (m == 0) { *idx = -1; d[0] = d[1] = d[2] = 0; return; }
+  const int p = e - base;
+  const int o = p & 31;
+  *idx = p >> 5;
+  const uint64_t lo = m << o;
+  const uint64_t hi = o ? (m >> (64 - o)) : 0ull;
+  d[0] = (int64_t)(lo & 0xFFFFFFFFull);
+  d[1] = (int64_t)(lo >> 32);
+  d[2] = (int64_t)hi;
+  if (neg) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
+}
+
+// Correctly rounded double of the digit words (flags: SUMF_* of the
+// non-finite values).
+QM_HD double wide_to_f64(const uint64_t *w, int nw, int base, uint64_t flags) {
+  if ((flags & SUMF_NAN) || ((flags & SUMF_PINF) && (flags & SUMF_NINF))) return bits_f64(0x7FF8000000000000ull);
+  if (flags & SUMF_PINF) return bits_f64(0x7FF0000000000000ull);
+  if (flags & SUMF_NINF) return bits_f64(0xFFF0000000000000ull);
+  // sign: the carry out of the top word
+  int64_t c = 0;
+  for (int i = 0; i < nw; ++i) c = ((int64_t)w[i] + c) >> 32;
+  const bool neg = c < 0;
+  // magnitude digits (of -sum when negative): highest nonzero digit
+  int top = -1;
+  c = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int64_t t = (neg ? -(int64_t)w[i] : (int64_t)w[i]) + c;
+    if ((uint32_t)t) top = i;
+    c = t >> 32;
+  }
+  if (top < 0) return 0.0;
+  // digits top, top - 1, top - 2 (96 bits, >= 65 significant) and a sticky bit for the rest
+  uint64_t d2 = 0, d1 = 0, d0 = 0;
+  bool sticky = false;
+  c = 0;
+  for (int i = 0; i <= top; ++i) {
+    const int64_t t = (neg ? -(int64_t)w[i] : (int64_t)w[i]) + c;
+    const uint64_t dig = (uint32_t)t;
+    c = t >> 32;
+    if (i == top) d2 = dig;
+    else if (i == top - 1) d1 = dig;
+    else if (i == top - 2) d0 = dig;
+    else if (dig) sticky = true;
+  }
+  u128 q;
+  q.hi = d2;
+  q.lo = (d1 << 32) | d0 | (sticky ? 1ull : 0ull);
+  const double v = fixed_to_f64(q, -(base + 32 * (top - 2)));
+  return neg ? -v : v;
+}
+
+// Digit words for a column whose nonzero finite |x| lie in [min_abs, max_abs]:
+// base = exponent of the lowest bit min_abs can have; room for the largest
+// value's top digit, 2^31 terms and the sign.
+QM_HD void wide_layout(double min_abs, double max_abs, int *base, int *nw) {
+  const uint64_t bl = f64_bits(min_abs), bh = f64_bits(max_abs);
+  const int el = (int)((bl >> 52) & 0x7FF), eh = (int)((bh >> 52) & 0x7FF);
+  *base = (el == 0 ? -1074 : el - 1075);
+  const int top = (eh == 0 ? -1074 : eh - 1075) + 53;  // |x| < 2^top
+  *nw = ((top - *base) >> 5) + 3;                      // value digits + 2^31 terms + sign
+}
+
 }  // namespace qm

@@ -69,6 +69,7 @@ enum : uint8_t {
   SK_SUM_I64 = 2,     // exact int64 sum of an i8/i16/i32/bool column
   SK_SUM_128 = 3,     // 128-bit sum: int64 column, or fixed-point f64 (+flags word)
   SK_MAX_CODE = 4,    // unsigned max of an order code (MAX, or MIN via ~code)
+  SK_SUM_WIDE = 5,    // f64 sum over a wide exponent range: flags word + nw signed 32-bit digits in 64-bit words
 };
 
 struct DState {
@@ -78,8 +79,8 @@ struct DState {
   uint8_t product;     // value = col_a * col_b in f64
   int8_t col_a, col_b;
   int16_t word;        // first word of this state in a table entry
-  int32_t scale;       // fixed-point scale
-  int32_t _pad;
+  int32_t scale;       // fixed-point scale; SK_SUM_WIDE: exponent of digit 0's lowest bit
+  int32_t nw;          // SK_SUM_WIDE: digit words
 };
 
 // Group-by addressing.

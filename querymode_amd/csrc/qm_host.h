@@ -161,6 +161,7 @@ struct ColStats {
   bool has_f = false;        // fmin/fmax valid (non-NaN values exist)
   double fmin = 0, fmax = 0;
   double max_abs = 0;        // largest finite |x|
+  double min_abs = 0;        // smallest nonzero finite |x| (floats; 0 = none)
   uint64_t nan = 0, nulls = 0, nonnull = 0;
   double distinct = 0;       // HyperLogLog estimate
   double dup_frac = 0;       // share of a strided host sample whose value repeats in it

@@ -8,14 +8,14 @@ namespace qm {
 
 enum : uint8_t {
   OK_KEY = 1, OK_COUNT, OK_SUM_I64, OK_SUM_I128, OK_SUM_F64, OK_AVG_F64, OK_AVG_I64, OK_AVG_BIG,
-  OK_MIN, OK_MAX,
+  OK_MIN, OK_MAX, OK_SUM_WIDE, OK_AVG_WIDE,
 };
 
 struct DOut {
   uint8_t kind;
   uint8_t key_idx;
   uint8_t src_is_float;
-  uint8_t _p;
+  uint8_t nw;         // OK_SUM_WIDE / OK_AVG_WIDE: digit words (scale: exponent of digit 0)
   int32_t out_dtype;
   int16_t word;       // state word
   int16_t cnt_word;   // count word deciding null-ness / divisor (-1 = none)
