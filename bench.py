@@ -134,6 +134,55 @@ def bench_zipf(args) -> dict:
     return out
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_batch_worker(job):
+    """One CPU worker: the batch table's rows [row0, row0 + n) regenerated
+    (same bytes as the device table), every query of the mix over them through
+    the C restatement of the reference CPU path. Returns its seconds."""
+    n, row0, nq, k2_order = job
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as O
+    from querymode_amd import datagen
+    cols = datagen.batch_table(n=n, seed=0, row0=row0) if not k2_order else None
+    descs = datagen.batch_queries(nq, seed=0)
+    t = O.Table(cols)
+    t1 = time.perf_counter()
+    for d in descs:
+        O.run(d, {"batch": t})
+    return time.perf_counter() - t1
+
+
+def cpu_batch_baseline(nq: int, sample_rows: int, workers: int) -> dict:
+    """The reference CPU path (C restatement, `oracle/`) over `sample_rows`
+    rows per worker, at one thread and at `workers` processes on disjoint row
+    ranges (partial results are not merged: the merge of per-worker groups is
+    left out of the multi-core figure)."""
+    import multiprocessing as mp
+    one = _cpu_batch_worker((sample_rows, 0, nq, False))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        t0 = time.perf_counter()
+        pool.map(_cpu_batch_worker, [(sample_rows, i * sample_rows, nq, False) for i in range(workers)])
+        wall = time.perf_counter() - t0
+    return {"value": nq * sample_rows / one, "unit": "rows/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "sample": f"all {nq} queries over rows [0, {sample_rows}) of the same table: C restatement of the "
+                      f"reference CPU path (per-page index-list filter + hash aggregate per query), one thread, "
+                      f"{one:.2f} s",
+            "multi_core": {"value": nq * sample_rows * workers / wall, "unit": "rows/s", "cores": workers,
+                           "sample": f"{workers} processes, each all {nq} queries over its own {sample_rows} rows "
+                                     f"of the table ({wall:.2f} s wall; per-worker partial groups not merged)"}}
+
+
 def bench_batch(args) -> dict:
     import numpy as np
     from querymode_amd import _native as N
@@ -157,19 +206,18 @@ def bench_batch(args) -> dict:
         tuned = Autotuner(dev).tune_batch(table, descs).as_dict()
     plans = [_Plan(dev, d) for d in descs]
     arr = (C.c_void_p * nq)(*[p.handle for p in plans])
+    k2 = [i for i, d in enumerate(descs) if d.groupBy == ["k2"]]
 
     def batch_step():
         outs = (C.c_void_p * nq)()
         N.check(lib.qm_batch_exec(dev.handle, arr, nq, table.handle, outs))
-        st = N.Stats()
-        N.check(lib.qm_result_stats(C.c_void_p(outs[0]), C.byref(st)))
-        groups = 0
+        per = []
         for i in range(nq):
             s2 = N.Stats()
             lib.qm_result_stats(C.c_void_p(outs[i]), C.byref(s2))
-            groups += s2.groups
+            per.append(s2)
             lib.qm_result_close(C.c_void_p(outs[i]))
-        return st, groups
+        return per
 
     for _ in range(args.warmup):
         batch_step()
@@ -180,14 +228,38 @@ def bench_batch(args) -> dict:
     wall = time.perf_counter() - t0
     ms_step = wall / args.steps * 1e3
     batched_rows_s = nq * rows / (wall / args.steps)
-    scan_ms = float(np.mean([s.scan_ms for s, _ in stats]))  # the batch's longest launch reading the table
-    dev_ms = float(np.mean([s.device_ms for s, _ in stats]))
+    last = stats[-1]
+    groups = int(sum(s.groups for s in last))
+    dev_ms = float(np.mean([st[0].device_ms for st in stats]))
     batch_bytes = rows * sum(np.dtype(c.dtype).itemsize for c in cols.values())  # every column once per pass
-    # the dominant launch is the run-sorted write pass over the k2 queries: it
-    # reads the key (k2), the filter column (v2) and the aggregated column (v1)
-    kname = "runs_kernel<1, 1, 2, 2>"
-    alg_bytes = rows * sum(np.dtype(cols[c].dtype).itemsize for c in ("k2", "v2", "v1"))
+    strat = N.STRATEGIES.get(last[k2[0]].strategy) if k2 else None
+    kernels = {}
+    if strat == "part_runs":
+        # the k2 queries' launch: row scatter (reads k2, v1, v2 once), record
+        # scatter, group count, write pass (records in, result columns out)
+        names = ["pr_scatter_kernel<0, 4>", "pr_scatter_kernel<1, 4>", "pr_count_kernel", "pr_runs_kernel<1, 2>"]
+        pm = [float(np.mean([st[k2[0]].pass_ms[j] for st in stats])) for j in range(4)]
+        records = int(last[k2[0]].pass_records)
+        k2_groups = int(sum(last[i].groups for i in k2))
+        rec_bytes = 16  # key code / bucket word + v1 (4 B) + v2 (8 B)
+        out_bytes = 8 + 8 + 8 + 4 + 8  # k2, SUM(v1), COUNT(*), MIN(v1), MAX(v2)
+        alg = [rows * 20, records * rec_bytes, records * rec_bytes, records * rec_bytes + k2_groups * out_bytes]
+        for j in range(4):
+            kernels[names[j]] = {"avg_ms": pm[j], "algorithmic_bytes": alg[j],
+                                 "achieved_gbs": alg[j] / (pm[j] * 1e-3) / 1e9}
+        dom = max(range(4), key=lambda j: pm[j])
+        kname, scan_ms, alg_bytes = names[dom], pm[dom], alg[dom]
+    else:
+        kname = "runs_kernel<1, 1, 2, 2>"
+        scan_ms = float(np.mean([st[0].scan_ms for st in stats]))
+        alg_bytes = rows * sum(np.dtype(cols[c].dtype).itemsize for c in ("k2", "v2", "v1"))
+        records = None
     achieved = alg_bytes / (scan_ms * 1e-3) / 1e9
+    read_ceiling = None
+    if not args.no_read_ceiling:
+        g = C.c_double()
+        N.check(lib.qm_measure_read_bw(dev.handle, 8 << 30, 10, C.byref(g)))
+        read_ceiling = g.value
     # the per-query drop-in: the same queries one at a time (qm_exec)
     for p in plans[:4]:
         _exec_stats(lib, N, p, table)
@@ -202,19 +274,15 @@ def bench_batch(args) -> dict:
     served = serve_mix(args, dev, cols, rows, descs) if args.serve_seconds > 0 else None
     if args.copy_calibration:
         copy_calibration(1 << 30)
+    for p in plans:
+        p.close()
+    dev.close()
+    del cols
     cpu = None
     if not args.no_cpu_baseline:
-        from oracle import oracle as O
-        n = min(args.cpu_sample_rows or 2_000_000, rows)
-        t = O.Table({k: v[:n] for k, v in cols.items()})
-        t1 = time.perf_counter()
-        for d in descs:
-            O.run(d, {"batch": t})
-        cpu_s = time.perf_counter() - t1
-        cpu = {"value": nq * n / cpu_s, "unit": "rows/s", "cores": 1, "kind": "port",
-               "sample": f"all {nq} queries over the first {n} rows of the same table: C restatement of the "
-                         f"reference CPU path (per-page index-list filter + hash aggregate per query), one "
-                         f"thread, {cpu_s:.2f} s"}
+        workers = args.cpu_workers or min(16, os.cpu_count() or 1)
+        cpu = cpu_batch_baseline(nq, min(args.cpu_sample_rows or 1_000_000, rows), workers)
+    tfile = "traffic_batch_part_runs.json" if strat == "part_runs" else "traffic_batch_mix.json"
     out = {
         "metric": METRIC, "value": batched_rows_s, "unit": "rows/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
@@ -222,19 +290,23 @@ def bench_batch(args) -> dict:
         "data": "synthetic (deterministic splitmix64 generator, seed 0)" + ("; table stored in k2 order" if args.k2_order else "; rows in generated order"),
         "config": {"workload": f"batched {nq}-query mix over a resident {rows // 1_000_000}M-row table "
                                "(range filter on v2, GROUP BY k1 or k2, SUM/COUNT/MIN/MAX)",
-                   "rows": rows, "queries": nq, "groups_per_batch": int(stats[-1][1]),
+                   "rows": rows, "queries": nq, "groups_per_batch": groups,
                    "k2_order": bool(args.k2_order), "parallelism": "single GPU"},
         "batched_rows_per_s": batched_rows_s,
         "per_query_rows_per_s": per_query_rows_s,
         "batched_vs_per_query": batched_rows_s / per_query_rows_s,
         "per_query_ms_total": pq_wall * 1e3,
         "per_query_strategies": strategies,
+        "batch_strategy_k2": strat,
         "served": served,
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _traffic(kname, rows, "traffic_batch_mix.json")},
-        "kernel": {"name": kname, "avg_ms": scan_ms, "algorithmic_bytes": alg_bytes},
+                     "frac_of_measured_read": (achieved / read_ceiling) if read_ceiling else None,
+                     "traffic": _traffic(kname, rows, tfile)},
+        "kernel": {"name": kname, "avg_ms": scan_ms, "algorithmic_bytes": alg_bytes, "records": records},
+        "kernels": kernels,
+        "read_ceiling_gbs": read_ceiling,
         # the whole fused batch: every referenced column read once per pass
         "batch_hbm": {"device_ms": dev_ms, "algorithmic_bytes": batch_bytes,
                       "achieved": batch_bytes / (dev_ms * 1e-3) / 1e9,
@@ -242,9 +314,6 @@ def bench_batch(args) -> dict:
         "cpu_baseline": cpu,
         "setup": {"generate_s": gen_s, "upload_s": upload_s},
     }
-    for p in plans:
-        p.close()
-    dev.close()
     return out
 
 
@@ -460,6 +529,8 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="served phase length (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's default knobs")
+    ap.add_argument("--no-read-ceiling", action="store_true", help="skip the streaming-read ceiling probe")
+    ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (default: min(16, nproc))")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, run one 1 GiB device-to-device copy (the known-size dispatch the "
                          "FETCH_SIZE / WRITE_SIZE counters are scaled by)")

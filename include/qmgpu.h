@@ -89,6 +89,8 @@ typedef struct qm_stats {
   uint32_t pages_skipped;     /* zone-map pages (65,536 rows) not read (QueryResult.pagesSkipped) */
   double   pass_ms[4];        /* device time of the strategy's passes (QM_STRAT_PART_RUNS: row scatter,
                                  record scatter, group count, write), HIP events on the library stream */
+  uint64_t pass_records;      /* QM_STRAT_PART_RUNS: rows the first pass kept as records (selected by any
+                                 query of the launch) */
 } qm_stats;
 
 #define QM_STRAT_NONE        0
@@ -184,6 +186,12 @@ int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, uint32_t *co
  * reference counterpart: the reference checks its memoryBudget per operator,
  * src/operators.ts:2383-2462; the serving loop admits requests with this). */
 int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right, uint64_t *bytes);
+
+/* Measurement probe (no reference counterpart): a plain streaming read of
+ * `bytes` of device memory (16-byte loads, every byte once, `reps` times);
+ * *gbps = bytes / (average launch time). The achievable read ceiling beside
+ * the 8 TB/s nominal peak. */
+int qm_measure_read_bw(qm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbps);
 
 /* Tuning knobs of the device planner, set at run time by an empirical tuner
  * (querymode_amd/autotune.py) or read at qm_init from QM_TUNE_<NAME>. No

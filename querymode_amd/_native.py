@@ -34,7 +34,7 @@ class Stats(C.Structure):
         ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double), ("scan_ms", C.c_double),
         ("wall_ms", C.c_double),
         ("kernel_launches", C.c_uint32), ("retries", C.c_uint32), ("strategy", C.c_uint32), ("pages_skipped", C.c_uint32),
-        ("pass_ms", C.c_double * 4),
+        ("pass_ms", C.c_double * 4), ("pass_records", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -58,6 +58,7 @@ SIGNATURES = [
     ("qm_plan_compile", C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
     ("qm_plan_close", C.c_int, [C.c_void_p]),
     ("qm_exec", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("qm_measure_read_bw", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     ("qm_exec_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_uint64,
                                   C.POINTER(C.c_uint64)]),
     ("qm_batch_exec", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p, C.POINTER(C.c_void_p)]),

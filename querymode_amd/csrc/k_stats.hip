@@ -104,6 +104,26 @@ __global__ void __launch_bounds__(256) key_hist_kernel(DCol c, uint64_t n, int64
   }
 }
 
+// streaming read: every 16-byte word once (eight loads in flight per
+// thread), summed so the loads are not dropped
+__global__ void __launch_bounds__(256) stream_read_kernel(const uint4 *p, uint64_t n16, uint64_t *sink) {
+  uint32_t acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * 256;
+  uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 7 * stride < n16; i += 8 * stride) {
+    gv4u v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = gld<gv4u>(p, (i + k * stride) * 16);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc ^= v[k].x ^ v[k].y ^ v[k].z ^ v[k].w;
+  }
+  for (; i < n16; i += stride) {
+    const gv4u a = gld<gv4u>(p, i * 16);
+    acc ^= a.x ^ a.y ^ a.z ^ a.w;
+  }
+  if (acc == 0x9E3779B9u) sink[0] = acc;  // practically never: keeps the loads
+}
+
 }  // namespace qm
 
 using namespace qm;
@@ -174,5 +194,10 @@ extern "C" hipError_t qmk_key_hist(const DCol *c, uint64_t n, int64_t kmin, int 
     if (grid > 65536) grid = 65536;
     hipLaunchKernelGGL(key_hist_kernel<false>, dim3((unsigned)grid), dim3(256), 0, st, *c, n, kmin, g, nbins, hist);
   }
+  return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_stream_read(const void *p, uint64_t bytes, uint32_t grid, uint64_t *sink, hipStream_t st) {
+  hipLaunchKernelGGL(stream_read_kernel, dim3(grid), dim3(256), 0, st, static_cast<const uint4 *>(p), bytes / 16, sink);
   return hipGetLastError();
 }

@@ -2619,6 +2619,8 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
   if (stv) fail(QM_ERR_QUERY_FAILED, "partitioned-runs batch: a key partition exceeded its region (status %u)", stv);
   std::vector<uint64_t> tot(n);
   for (size_t i = 0; i < n; ++i) tot[i] = base[i + 1] - base[i];
+  uint64_t records = 0;
+  for (uint32_t e = 0; e < R_MAX_BUCKETS; ++e) records += brow[e];
 
   // result arenas as in run_runs: query i's rows at a padded offset
   std::vector<uint64_t> roff(n);
@@ -2692,6 +2694,7 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
     res->stats.rows_selected = sel;
     res->stats.strategy = QM_STRAT_PART_RUNS;
     for (int k = 0; k < 4; ++k) res->stats.pass_ms[k] = pass_ms[k];
+    res->stats.pass_records = records;
     uint32_t l = 0;
     sort_limit_result(res, *L.sorted[i]->plan, dnoq.as<DQuery>(), &l);
   }
@@ -3514,6 +3517,26 @@ extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right,
     if (pl.has_sort && pl.limit < 0) b += n * 24;  // radix sort keys + values
   }
   *bytes = b;
+  API_END
+}
+
+extern "C" int qm_measure_read_bw(qm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbps) {
+  API_BEGIN
+  if (!ctx || !gbps || bytes < (1u << 20) || reps == 0) fail(QM_ERR_INVALID_ARG, "bad argument");
+  std::lock_guard<std::mutex> g(ctx->mu);
+  HIPX(hipSetDevice(ctx->device));
+  bytes &= ~(uint64_t)15;
+  DevBuf buf(ctx, bytes), sink(ctx, 64);
+  HIPX(qmk_fill_u64(buf.as<uint64_t>(), bytes / 8, 0x0123456789ABCDEFull, ctx->stream));
+  const uint32_t grid = (uint32_t)ctx->num_cus * 8;
+  HIPX(qmk_stream_read(buf.p, bytes, grid, sink.as<uint64_t>(), ctx->stream));  // warm
+  HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+  for (uint32_t r = 0; r < reps; ++r) HIPX(qmk_stream_read(buf.p, bytes, grid, sink.as<uint64_t>(), ctx->stream));
+  HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+  HIPX(hipEventSynchronize(ctx->evk1));
+  float ms = 0;
+  HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+  *gbps = (double)bytes * reps / (ms * 1e-3) / 1e9;
   API_END
 }
 

@@ -159,6 +159,7 @@ hipError_t qmk_key_hist(const qm::DCol *c, uint64_t n, int64_t kmin, int g, uint
 // batched group-by over a key in any row order (k_pruns.hip): scatter passes
 // (src 0: table rows -> level-1 regions; src 1: level-1 -> final partitions),
 // per-(query, partition) group counts, and the write pass
+hipError_t qmk_stream_read(const void *p, uint64_t bytes, uint32_t grid, uint64_t *sink, hipStream_t st);
 uint32_t qmk_pr_tile(int rw);
 int qmk_pr_supported(int nsum, int nmax);
 size_t qmk_pr_runs_lds(uint32_t nbin, int nsum, int nmax, uint32_t cap);
