@@ -44,7 +44,11 @@ namespace qm {
 namespace {
 
 static constexpr int PR_WAVES = PR_BLOCK / 64;
-static constexpr int PR_CIPT = 8;  // records per thread in the write pass (cap <= PR_CIPT * PR_BLOCK)
+// write pass: 1024 threads (16 waves per CU to hide the LDS latency of the
+// per-group chains), 4 records per thread
+static constexpr int PR_RBLOCK = 1024;
+static constexpr int PR_RWAVES = PR_RBLOCK / 64;
+static constexpr int PR_RIPT = 4;  // records per thread in the write pass (cap <= PR_RIPT * PR_RBLOCK)
 
 __device__ __forceinline__ bool pr_null(const uint8_t *nb, uint64_t row) {
   return nb != nullptr && ((gld<uint8_t>(nb, row >> 3) >> (row & 7)) & 1);
@@ -81,6 +85,12 @@ __device__ __forceinline__ uint32_t pr_bucket(const uint64_t *sb, int bsteps, ui
 }
 
 __device__ __forceinline__ uint64_t u64max(uint64_t a, uint64_t b) { return a > b ? a : b; }
+// a value every lane holds the same copy of, as a scalar
+__device__ __forceinline__ uint64_t rfl64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int32_t)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
 
 // block-wide exclusive scan of one u32 per thread (PR_BLOCK threads)
 __device__ __forceinline__ uint32_t pr_block_exscan(uint32_t v, uint32_t *sw /*[PR_WAVES + 1]*/, uint32_t *total) {
@@ -117,6 +127,46 @@ __device__ __forceinline__ uint64_t pr_block_exscan64(uint64_t v, uint64_t *sw /
   uint64_t base = 0;
 #pragma unroll
   for (int w = 0; w < PR_WAVES; ++w) base += w < wave ? sw[w] : 0ull;
+  __syncthreads();
+  return base + x - v;
+}
+
+template <int WAVES>
+__device__ __forceinline__ uint32_t pr_block_exscan_n(uint32_t v, uint32_t *sw, uint32_t *total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sw[wave] = x;
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) {
+    const uint32_t c = sw[w];
+    base += w < wave ? c : 0u;
+    all += c;
+  }
+  *total = all;
+  __syncthreads();
+  return base + x - v;
+}
+template <int WAVES>
+__device__ __forceinline__ uint64_t pr_block_exscan64_n(uint64_t v, uint64_t *sw) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) sw[wave] = x;
+  __syncthreads();
+  uint64_t base = 0;
+#pragma unroll
+  for (int w = 0; w < WAVES; ++w) base += w < wave ? sw[w] : 0ull;
   __syncthreads();
   return base + x - v;
 }
@@ -423,11 +473,11 @@ __device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const u
 // the next partition's records are loaded while the current one's queries
 // are answered.
 template <int NSUM, int NMAX>
-__global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
+__global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   constexpr int NSA = NSUM > 0 ? NSUM : 1, NMA = NMAX > 0 ? NMAX : 1;
   extern __shared__ __align__(16) uint8_t s_dyn[];
-  __shared__ uint32_t s_sw[PR_WAVES + 1];
-  __shared__ uint64_t s_sw64[PR_WAVES];
+  __shared__ uint32_t s_sw[PR_RWAVES + 1];
+  __shared__ uint64_t s_sw64[PR_RWAVES];
   __shared__ uint32_t s_qe[R_MAX_QUERIES];
   __shared__ uint64_t s_qrow[R_MAX_QUERIES];  // first result row of (query, current partition)
   __shared__ uint32_t s_qcnt[R_MAX_QUERIES], s_qbase[R_MAX_QUERIES];  // groups of (query, partition), flat start
@@ -447,58 +497,48 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
   uint64_t *s_add = reinterpret_cast<uint64_t *>(s_dyn + ((nbw * 4 + 15) & ~15u));
   uint64_t *s_max = s_add + (uint64_t)NSUM * cap;
   if (t < ps.nq) s_qe[t] = ps.qe[t];
-  // output descriptors, read once (scalar registers for the whole kernel)
+  // output descriptors in LDS (read back as uniform values per output)
+  __shared__ uint64_t s_out[R_MAX_OUTS][3];  // ROut bits, column base, NULL bitmap base
   const int nouts = ps.nouts;
-  ROut ro[R_MAX_OUTS];
-  void *ob[R_MAX_OUTS];
-  uint32_t *onb[R_MAX_OUTS];
-#pragma unroll
-  for (int o = 0; o < R_MAX_OUTS; ++o) {
-    ro[o] = ps.outs[o];
-    ob[o] = ps.out_base[o];
-    onb[o] = ps.out_nbase[o];
+  if (t < R_MAX_OUTS) {
+    ROut r = ps.outs[t < nouts ? t : 0];
+    uint64_t rb = 0;
+    memcpy(&rb, &r, sizeof r);
+    s_out[t][0] = rb;
+    s_out[t][1] = reinterpret_cast<uint64_t>(ps.out_base[t < nouts ? t : 0]);
+    s_out[t][2] = reinterpret_cast<uint64_t>(ps.out_nbase[t < nouts ? t : 0]);
   }
   const uint32_t km = (1u << ps.s) - 1u;
   const bool kon = (uint32_t)lane <= km;
   const uint64_t lt = (1ull << lane) - 1ull;
   const uint32_t kbin = (uint32_t)lane * NBK;
 
-  // records of partition P into registers (raw words), its size, and (thread
-  // t < nq) the first result row of (query t, P)
-  uint32_t wr[PR_CIPT][PR_MAX_RW];
+  // partition P's size and (thread t < nq) the first result row of (query t, P)
   uint32_t n = 0;
-  uint64_t qrow = 0;
+  uint64_t qrow = 0, pbase = 0;
   auto load_part = [&](uint32_t P) {
-    const uint64_t base = ps.start2[P];
-    const uint64_t capr = ps.start2[P + 1] - base;
+    pbase = ps.start2[P];
+    const uint64_t capr = ps.start2[P + 1] - pbase;
     const uint32_t c2 = ps.cur2[P];
     n = (uint32_t)(c2 < capr ? c2 : capr);
-#pragma unroll
-    for (int i = 0; i < PR_CIPT; ++i) {
-      const uint32_t j = (uint32_t)t + (uint32_t)i * PR_BLOCK;
-      const bool in = j < n && j < cap;
-      const uint32_t *r = ps.rec2 + (base + (in ? j : 0)) * (uint64_t)rw;
-#pragma unroll
-      for (int k = 0; k < PR_MAX_RW; ++k) wr[i][k] = (in && (uint32_t)k < rw) ? r[k] : 0u;
-    }
     if (t < ps.nq) {
       const uint64_t *gq = ps.gcount + (uint64_t)t * ps.np;
       qrow = ps.row_off[t] + (gq[P] - gq[0]);
     }
   };
-  load_part(P0);
   for (uint32_t P = P0; P < P1; ++P) {
-    for (uint32_t i = t; i < nbw; i += PR_BLOCK) s_bin[i] = 0;
+    load_part(P);
+    for (uint32_t i = t; i < nbw; i += PR_RBLOCK) s_bin[i] = 0;
     if (t < ps.nq) s_qrow[t] = qrow;
     const uint32_t nP = n;
     if (nP > cap && t == 0) atomicOr(ps.status, 2u);  // the host sizes partitions from the key histogram
     __syncthreads();
     // 1. records -> (key, bucket) bin and rank within it; state contributions
-    uint32_t bn[PR_CIPT], rk[PR_CIPT];
-    uint64_t ca[PR_CIPT][NSA], cm[PR_CIPT][NMA];
+    uint32_t bn[PR_RIPT], rk[PR_RIPT];
+    uint64_t ca[PR_RIPT][NSA], cm[PR_RIPT][NMA];
 #pragma unroll
-    for (int i = 0; i < PR_CIPT; ++i) {
-      const uint32_t j = (uint32_t)t + (uint32_t)i * PR_BLOCK;
+    for (int i = 0; i < PR_RIPT; ++i) {
+      const uint32_t j = (uint32_t)t + (uint32_t)i * PR_RBLOCK;
       rk[i] = ~0u;
       bn[i] = 0;
 #pragma unroll
@@ -506,27 +546,29 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
 #pragma unroll
       for (int k = 0; k < NMA; ++k) cm[i][k] = 0;
       if (j >= nP || j >= cap) continue;
+      uint32_t wr[PR_MAX_RW];
+      const uint32_t *r = ps.rec2 + (pbase + j) * (uint64_t)rw;
 #pragma unroll
-      for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr[i]);
+      for (int k = 0; k < PR_MAX_RW; ++k) wr[k] = (uint32_t)k < rw ? r[k] : 0u;
 #pragma unroll
-      for (int s2 = 0; s2 < NMAX; ++s2) cm[i][s2] = pr_contrib(ps, NSUM + s2, wr[i]);
-      const uint32_t kb = wr[i][0];
+      for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr);
+#pragma unroll
+      for (int s2 = 0; s2 < NMAX; ++s2) cm[i][s2] = pr_contrib(ps, NSUM + s2, wr);
+      const uint32_t kb = wr[0];
       bn[i] = (kb & km) * NBK + (kb >> 16);
       const uint32_t sh = (bn[i] & 1u) * 16u;
       rk[i] = (atomicAdd(&s_bin[bn[i] >> 1], 1u << sh) >> sh) & 0xFFFFu;
     }
     __syncthreads();
-    // the next partition's records are in flight from here on
-    if (P + 1 < P1) load_part(P + 1);
     // 2. exclusive scan of the bins (packed pairs; thread t owns a word range)
     {
-      const uint32_t wpt = (nbw + PR_BLOCK - 1) / PR_BLOCK;
+      const uint32_t wpt = (nbw + PR_RBLOCK - 1) / PR_RBLOCK;
       const uint32_t w0 = (uint32_t)t * wpt < nbw ? (uint32_t)t * wpt : nbw;
       const uint32_t w1 = w0 + wpt < nbw ? w0 + wpt : nbw;
       uint32_t sum = 0;
       for (uint32_t w = w0; w < w1; ++w) { const uint32_t x = s_bin[w]; sum += (x & 0xFFFFu) + (x >> 16); }
       uint32_t total;
-      uint32_t run = pr_block_exscan(sum, s_sw, &total);
+      uint32_t run = pr_block_exscan_n<PR_RWAVES>(sum, s_sw, &total);
       for (uint32_t w = w0; w < w1; ++w) {
         const uint32_t x = s_bin[w];
         const uint32_t lo = run;
@@ -539,7 +581,7 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
     __syncthreads();
     // 3. scatter the contributions into (key, bucket) order
 #pragma unroll
-    for (int i = 0; i < PR_CIPT; ++i) {
+    for (int i = 0; i < PR_RIPT; ++i) {
       if (rk[i] == ~0u) continue;
       const uint32_t slot = pr_start(s_bin, bn[i]) + rk[i];
 #pragma unroll
@@ -553,18 +595,18 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
 #pragma unroll
     for (int j = 0; j < NSUM; ++j) {
       uint64_t *a = s_add + (uint64_t)j * cap;
-      const uint32_t pe = (nn + PR_BLOCK - 1) / PR_BLOCK;
+      const uint32_t pe = (nn + PR_RBLOCK - 1) / PR_RBLOCK;
       const uint32_t i0 = (uint32_t)t * pe < nn ? (uint32_t)t * pe : nn;
       const uint32_t i1 = i0 + pe < nn ? i0 + pe : nn;
       uint64_t sum = 0;
       for (uint32_t i = i0; i < i1; ++i) sum += a[i];
-      uint64_t run = pr_block_exscan64(sum, s_sw64);
+      uint64_t run = pr_block_exscan64_n<PR_RWAVES>(sum, s_sw64);
       for (uint32_t i = i0; i < i1; ++i) { run += a[i]; a[i] = run; }
       __syncthreads();
     }
     // 5. the non-empty (query, key) groups: lanes = keys, one query per wave
     // step, lists each query's keys in ascending order ...
-    for (int q = wave; q < ps.nq; q += PR_WAVES) {
+    for (int q = wave; q < ps.nq; q += PR_RWAVES) {
       const uint32_t qe = s_qe[q];
       const uint32_t elo = qe & 0xFFFFu, ehi = qe >> 16;
       bool has = false;
@@ -578,18 +620,18 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
     uint32_t ngroups;
     {
       const uint32_t c = t < ps.nq ? s_qcnt[t] : 0u;
-      const uint32_t b0 = pr_block_exscan(c, s_sw, &ngroups);
+      const uint32_t b0 = pr_block_exscan_n<PR_RWAVES>(c, s_sw, &ngroups);
       if (t < ps.nq) s_qbase[t] = b0;
     }
     __syncthreads();
-    for (int q = wave; q < ps.nq; q += PR_WAVES) {
+    for (int q = wave; q < ps.nq; q += PR_RWAVES) {
       const uint32_t c = s_qcnt[q];
       if ((uint32_t)lane < c) s_gq[s_qbase[q] + lane] = (uint8_t)q;
     }
     __syncthreads();
     // ... then lanes = groups: consecutive lanes take consecutive keys of one
     // query, so each result column is written in runs
-    for (uint32_t g = t; g < ngroups; g += PR_BLOCK) {
+    for (uint32_t g = t; g < ngroups; g += PR_RBLOCK) {
       const uint32_t q = s_gq[g];
       const uint32_t j = g - s_qbase[q];
       const uint32_t k = s_gk[q * 64 + j];
@@ -624,16 +666,20 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_runs_kernel(PRScan ps) {
           sv[1 + NSUM + jj] = u64max(x0[jj], x1[jj]);
         }
       }
-#pragma unroll
-      for (int o = 0; o < R_MAX_OUTS; ++o) {
-        if (o >= nouts) break;
+#pragma unroll 1
+      for (int o = 0; o < nouts; ++o) {
+        const uint64_t rb = rfl64(s_out[o][0]);
+        ROut ro;
+        memcpy(&ro, &rb, sizeof ro);
+        void *ob = reinterpret_cast<void *>(rfl64(s_out[o][1]));
+        uint32_t *onb = reinterpret_cast<uint32_t *>(rfl64(s_out[o][2]));
         uint64_t v = 0, c = 0;
 #pragma unroll
         for (int jj = 0; jj < 1 + NSUM + NMAX; ++jj) {
-          if (ro[o].state == jj) v = sv[jj];
-          if (ro[o].cnt_state == jj) c = sv[jj];
+          if (ro.state == jj) v = sv[jj];
+          if (ro.cnt_state == jj) c = sv[jj];
         }
-        pr_emit(ro[o], ob[o], onb[o], gg, key, v, c);
+        pr_emit(ro, ob, onb, gg, key, v, c);
       }
     }
     __syncthreads();  // LDS (bins, sorted states, first rows) is reused by the next partition
@@ -687,7 +733,7 @@ extern "C" uint32_t qmk_pr_max_cap(uint32_t nbin, int nsum, int nmax) {
   const size_t budget = 118 * 1024;  // of 160 KB: the rest is the kernel's static LDS (query and group lists)
   const size_t fixed = qmk_pr_runs_lds(nbin, 0, 0, 0);
   if (fixed >= budget) return 0;
-  uint64_t cap = (uint64_t)PR_CIPT * PR_BLOCK;
+  uint64_t cap = (uint64_t)PR_RIPT * PR_RBLOCK;
   if (nsum + nmax > 0) {
     const uint64_t c2 = (budget - fixed) / (8ull * (uint64_t)(nsum + nmax));
     cap = c2 < cap ? c2 : cap;
@@ -720,7 +766,7 @@ extern "C" hipError_t qmk_pr_runs(const PRScan *ps, hipStream_t st) {
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   int bpc = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, PR_BLOCK, lds) != hipSuccess || bpc < 1) bpc = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, PR_RBLOCK, lds) != hipSuccess || bpc < 1) bpc = 1;
   int dev = 0, cus = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus < 1)
@@ -728,5 +774,5 @@ extern "C" hipError_t qmk_pr_runs(const PRScan *ps, hipStream_t st) {
   const uint32_t grid = (uint32_t)std::min<uint64_t>((uint64_t)cus * (uint64_t)bpc, ps->np);
   PRScan a = *ps;
   void *args[] = {&a};
-  return hipLaunchKernel(fn, dim3(grid), dim3(PR_BLOCK), args, lds, st);
+  return hipLaunchKernel(fn, dim3(grid), dim3(PR_RBLOCK), args, lds, st);
 }
