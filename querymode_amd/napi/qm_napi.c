@@ -6,7 +6,8 @@
  * over linear memory (src/wasm-engine.ts:21-188), and its native build wraps
  * opaque handles for JS (wasm/src/nodejs.zig:101-742). Handles cross as
  * napi_external values; a handle closed explicitly is not closed again by its
- * finaliser. Failures throw an Error whose `code` is the QueryModeError code
+ * finaliser, a handle a pending batch uses closes when the batch settles, and
+ * a context shuts down only after its tables, plans and results. Failures throw an Error whose `code` is the QueryModeError code
  * (src/errors.ts:7-32). Results cross as QMCB ArrayBuffers (src/columnar.ts).
  *
  * Exports: init, shutdown, lastError, deviceInfo, tableCreate, tableDrop,
@@ -27,7 +28,11 @@ enum { H_CTX = 1, H_TABLE, H_PLAN, H_RESULT };
 
 typedef struct handle {
   int kind;
-  void *p;  /* NULL once closed */
+  void *p;             /* NULL once closed */
+  int busy;            /* batches in flight that use it; for a context also its open tables, plans, results */
+  int close_pending;   /* closed by the caller while busy: closed when busy drops to 0 */
+  int orphaned;        /* finalised by JS while busy: the struct is freed when busy drops to 0 */
+  struct handle *parent; /* the context a table, plan or result belongs to (holds one of its busy counts) */
 } handle;
 
 static const char *code_name(int rc) {
@@ -68,42 +73,97 @@ static napi_value throw_rc(napi_env env, int rc) {
     if ((call) != napi_ok) return throw_code(env, "INVALID_ARGUMENT", #call); \
   } while (0)
 
+static void close_now(handle *h);
+
+/* Drop one busy count; a close asked for meanwhile happens at 0, and a
+ * struct JS has already finalised is freed then. */
+static void unref(handle *h) {
+  if (--h->busy > 0) return;
+  if (h->close_pending) close_now(h);
+  if (h->orphaned) free(h);
+}
+
+/* Close what a handle holds (the library object is gone afterwards), then
+ * release the context it belonged to: a context closes only after its
+ * tables, plans and results, whose close paths use it. */
+static void close_now(handle *h) {
+  if (!h->p) return;
+  switch (h->kind) {
+    case H_TABLE: qm_table_drop((qm_table *)h->p); break;
+    case H_PLAN: qm_plan_close((qm_plan *)h->p); break;
+    case H_RESULT: qm_result_close((qm_result *)h->p); break;
+    case H_CTX: qm_shutdown((qm_ctx *)h->p); break;
+  }
+  h->p = NULL;
+  if (h->parent) {
+    handle *par = h->parent;
+    h->parent = NULL;
+    unref(par);
+  }
+}
+
+/* A handle a running batch (or, for a context, a live table, plan or result)
+ * still uses is closed when the last user lets go. */
+static void close_or_defer(handle *h) {
+  if (h->busy > 0) h->close_pending = 1;
+  else close_now(h);
+}
+
 static void finalize_handle(napi_env env, void *data, void *hint) {
   (void)env;
   (void)hint;
   handle *h = (handle *)data;
-  if (h->p) {
-    switch (h->kind) {
-      case H_RESULT: qm_result_close((qm_result *)h->p); break;
-      case H_PLAN: qm_plan_close((qm_plan *)h->p); break;
-      case H_TABLE: qm_table_drop((qm_table *)h->p); break;
-      case H_CTX: break; /* a context is shut down explicitly only */
-    }
-  }
-  free(h);
-}
-
-static napi_value wrap(napi_env env, int kind, void *p) {
-  handle *h = (handle *)malloc(sizeof *h);
-  napi_value v;
-  if (!h) return throw_code(env, "MEMORY_EXCEEDED", "out of host memory");
-  h->kind = kind;
-  h->p = p;
-  if (napi_create_external(env, h, finalize_handle, NULL, &v) != napi_ok) {
+  if (h->kind != H_CTX) {
+    close_now(h); /* a batch holds references to what it uses: never busy here */
     free(h);
-    return throw_code(env, "QUERY_FAILED", "napi_create_external failed");
+  } else if (h->busy > 0) {
+    h->orphaned = 1; /* a context is shut down explicitly only */
+  } else {
+    free(h);
   }
-  return v;
 }
 
-static handle *unwrap(napi_env env, napi_value v, int kind) {
+/* Wrap a library object; `parent` (a context handle, or NULL) stays open
+ * while the new handle is. On failure the object is closed. */
+static napi_value wrap(napi_env env, int kind, void *p, handle *parent) {
+  handle *h = (handle *)calloc(1, sizeof *h);
+  napi_value v;
+  if (h) {
+    h->kind = kind;
+    h->p = p;
+    if (napi_create_external(env, h, finalize_handle, NULL, &v) == napi_ok) {
+      if (parent) {
+        h->parent = parent;
+        parent->busy++;
+      }
+      return v;
+    }
+    h->p = NULL;
+    free(h);
+  }
+  switch (kind) {
+    case H_TABLE: qm_table_drop((qm_table *)p); break;
+    case H_PLAN: qm_plan_close((qm_plan *)p); break;
+    case H_RESULT: qm_result_close((qm_result *)p); break;
+    case H_CTX: qm_shutdown((qm_ctx *)p); break;
+  }
+  return throw_code(env, "MEMORY_EXCEEDED", "could not wrap a handle");
+}
+
+/* The open handle of `kind` in v, or NULL. `use` also rejects a table or plan
+ * whose context the caller has shut down: no new work starts on a closing
+ * context, while closing them, and reading results, stays allowed. */
+static handle *unwrap_as(napi_env env, napi_value v, int kind, int use) {
   napi_valuetype t;
   void *data = NULL;
   if (napi_typeof(env, v, &t) != napi_ok || t != napi_external) return NULL;
   if (napi_get_value_external(env, v, &data) != napi_ok || !data) return NULL;
   handle *h = (handle *)data;
-  return (h->kind == kind && h->p) ? h : NULL;
+  if (h->kind != kind || !h->p || h->close_pending) return NULL;
+  if (use && kind != H_RESULT && h->parent && h->parent->close_pending) return NULL;
+  return h;
 }
+static handle *unwrap(napi_env env, napi_value v, int kind) { return unwrap_as(env, v, kind, 1); }
 
 static int get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
   size_t argc = want;
@@ -163,17 +223,14 @@ static napi_value js_init(napi_env env, napi_callback_info info) {
   qm_ctx *ctx = NULL;
   int rc = qm_init(dev, budget, &ctx);
   if (rc != QM_OK) return throw_rc(env, rc);
-  return wrap(env, H_CTX, ctx);
+  return wrap(env, H_CTX, ctx, NULL);
 }
 
 static napi_value js_shutdown(napi_env env, napi_callback_info info) {
   napi_value argv[1], undef;
   get_args(env, info, 1, argv);
   handle *h = unwrap(env, argv[0], H_CTX);
-  if (h) {
-    qm_shutdown((qm_ctx *)h->p);
-    h->p = NULL;
-  }
+  if (h) close_or_defer(h);
   napi_get_undefined(env, &undef);
   return undef;
 }
@@ -267,7 +324,7 @@ static napi_value js_table_create(napi_env env, napi_callback_info info) {
     qm_table *t = NULL;
     int rc = qm_table_create((qm_ctx *)h->p, tname, nrows, ncols, cols, &t);
     if (rc != QM_OK) throw_rc(env, rc);
-    else result = wrap(env, H_TABLE, t);
+    else result = wrap(env, H_TABLE, t, h);
   }
 done:
   free(cols);
@@ -278,13 +335,8 @@ done:
 static napi_value close_kind(napi_env env, napi_callback_info info, int kind) {
   napi_value argv[1], undef;
   get_args(env, info, 1, argv);
-  handle *h = unwrap(env, argv[0], kind);
-  if (h) {
-    if (kind == H_TABLE) qm_table_drop((qm_table *)h->p);
-    else if (kind == H_PLAN) qm_plan_close((qm_plan *)h->p);
-    else if (kind == H_RESULT) qm_result_close((qm_result *)h->p);
-    h->p = NULL;
-  }
+  handle *h = unwrap_as(env, argv[0], kind, 0);
+  if (h) close_or_defer(h);
   napi_get_undefined(env, &undef);
   return undef;
 }
@@ -312,7 +364,7 @@ static napi_value js_plan_compile(napi_env env, napi_callback_info info) {
   qm_plan *p = NULL;
   int rc = qm_plan_compile((qm_ctx *)h->p, (const uint8_t *)data, len, &p);
   if (rc != QM_OK) return throw_rc(env, rc);
-  return wrap(env, H_PLAN, p);
+  return wrap(env, H_PLAN, p, h);
 }
 
 static napi_value js_exec(napi_env env, napi_callback_info info) {
@@ -325,7 +377,7 @@ static napi_value js_exec(napi_env env, napi_callback_info info) {
   qm_result *res = NULL;
   int rc = qm_exec((qm_plan *)p->p, (qm_table *)l->p, r ? (qm_table *)r->p : NULL, &res);
   if (rc != QM_OK) return throw_rc(env, rc);
-  return wrap(env, H_RESULT, res);
+  return wrap(env, H_RESULT, res, p->parent);
 }
 
 /* planDeviceBytes(plan, table[, rightTable]) -> device bytes beyond the resident tables */
@@ -376,7 +428,7 @@ static napi_value js_batch_exec(napi_env env, napi_callback_info info) {
     }
   }
   napi_create_array_with_length(env, n, &arr);
-  for (uint32_t i = 0; i < n; ++i) napi_set_element(env, arr, i, wrap(env, H_RESULT, res[i]));
+  for (uint32_t i = 0; i < n; ++i) napi_set_element(env, arr, i, wrap(env, H_RESULT, res[i], c));
   out = arr;
 done:
   free(plans);
@@ -391,7 +443,10 @@ done:
 typedef struct batch_work {
   napi_async_work work;
   napi_deferred deferred;
-  napi_ref keep_plans, keep_table;
+  napi_ref keep_ctx, keep_table;
+  napi_ref *keep_plan;          /* each plan element (not only the array) */
+  handle *hctx, *htable;
+  handle **hplan;
   qm_batch *batch;
   uint32_t n;
   qm_result **res;
@@ -413,7 +468,7 @@ static void batch_wait_done(napi_env env, napi_status status, void *data) {
   if (status == napi_ok && w->rc == QM_OK) {
     napi_value arr;
     napi_create_array_with_length(env, w->n, &arr);
-    for (uint32_t i = 0; i < w->n; ++i) napi_set_element(env, arr, i, wrap(env, H_RESULT, w->res[i]));
+    for (uint32_t i = 0; i < w->n; ++i) napi_set_element(env, arr, i, wrap(env, H_RESULT, w->res[i], w->hctx));
     napi_resolve_deferred(env, w->deferred, arr);
   } else {
     napi_value ecode, emsg, err;
@@ -424,9 +479,19 @@ static void batch_wait_done(napi_env env, napi_status status, void *data) {
     napi_set_named_property(env, err, "code", ecode);
     napi_reject_deferred(env, w->deferred, err);
   }
-  napi_delete_reference(env, w->keep_plans);
+  /* the batch is done with its handles: closes the caller asked for meanwhile happen now
+     (plans and the table before the context, which closes after them) */
+  for (uint32_t i = 0; i < w->n; ++i) {
+    unref(w->hplan[i]);
+    napi_delete_reference(env, w->keep_plan[i]);
+  }
+  unref(w->htable);
+  unref(w->hctx); /* the results just wrapped hold the context open until they close */
   napi_delete_reference(env, w->keep_table);
+  napi_delete_reference(env, w->keep_ctx);
   napi_delete_async_work(env, w->work);
+  free(w->keep_plan);
+  free(w->hplan);
   free(w->res);
   free(w);
 }
@@ -442,33 +507,43 @@ static napi_value js_batch_submit(napi_env env, napi_callback_info info) {
   qm_plan **plans = (qm_plan **)calloc(n, sizeof *plans);
   batch_work *w = (batch_work *)calloc(1, sizeof *w);
   qm_result **res = (qm_result **)calloc(n, sizeof *res);
-  if (!plans || !w || !res) {
-    free(plans);
-    free(w);
-    free(res);
+  handle **hplan = (handle **)calloc(n, sizeof *hplan);
+  napi_value *pv = (napi_value *)calloc(n, sizeof *pv);
+  napi_ref *keep_plan = (napi_ref *)calloc(n, sizeof *keep_plan);
+  if (!plans || !w || !res || !hplan || !pv || !keep_plan) {
+    free(plans); free(w); free(res); free(hplan); free(pv); free(keep_plan);
     return throw_code(env, "MEMORY_EXCEEDED", "out of host memory");
   }
   for (uint32_t i = 0; i < n; ++i) {
-    napi_value v;
     handle *ph;
-    if (napi_get_element(env, argv[1], i, &v) != napi_ok || !(ph = unwrap(env, v, H_PLAN))) {
-      free(plans);
-      free(w);
-      free(res);
+    if (napi_get_element(env, argv[1], i, &pv[i]) != napi_ok || !(ph = unwrap(env, pv[i], H_PLAN))) {
+      free(plans); free(w); free(res); free(hplan); free(pv); free(keep_plan);
       return throw_code(env, "INVALID_ARGUMENT", "plans[] must hold plan handles");
     }
     plans[i] = (qm_plan *)ph->p;
+    hplan[i] = ph;
   }
   int rc = qm_batch_submit((qm_ctx *)c->p, plans, n, (qm_table *)t->p, &w->batch);
   free(plans);
   if (rc != QM_OK) {
-    free(w);
-    free(res);
+    free(w); free(res); free(hplan); free(pv); free(keep_plan);
     return throw_rc(env, rc);
   }
   w->n = n;
   w->res = res;
-  napi_create_reference(env, argv[1], 1, &w->keep_plans);
+  /* the handles stay open (and referenced, so not finalised) until the batch settles */
+  w->hctx = c;
+  w->htable = t;
+  w->hplan = hplan;
+  w->keep_plan = keep_plan;
+  c->busy++;
+  t->busy++;
+  for (uint32_t i = 0; i < n; ++i) {
+    hplan[i]->busy++;
+    napi_create_reference(env, pv[i], 1, &keep_plan[i]);
+  }
+  free(pv);
+  napi_create_reference(env, argv[0], 1, &w->keep_ctx);
   napi_create_reference(env, argv[2], 1, &w->keep_table);
   napi_create_promise(env, &w->deferred, &promise);
   napi_create_string_utf8(env, "qm_batch", NAPI_AUTO_LENGTH, &name);

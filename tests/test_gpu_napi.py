@@ -48,7 +48,7 @@ out.join = qm.table('rows').join(qm.table('right'), { left: 'id', right: 'id' })
   out.solo = frames.map((f) => f.exec().rows);
   qm.close();
   console.log(JSON.stringify(out, (k, v) => typeof v === 'bigint' ? v.toString() : v));
-})();
+})().catch((e) => { console.error(e); process.exit(1); });
 """
 
 
@@ -73,3 +73,41 @@ def test_js_host_end_to_end():
     key = lambda r: int(r["region"])  # noqa: E731
     assert [sorted(b, key=key) for b in out["batch"]] == [sorted(x, key=key) for x in out["solo"]]
     assert all(len(b) == 4 for b in out["batch"])
+
+
+SCRIPT_CLOSE = r"""
+const q = require(%s);
+const N = 2000000;
+const region = new Int32Array(N), amount = new Int32Array(N);
+for (let i = 0; i < N; i++) { region[i] = i %% 4; amount[i] = (i * 7 + 13) %% 1000; }
+const qm = q.QueryMode.gpu(0);
+qm.registerTable('rows', { region, amount });
+const frames = [0, 250, 500, 750].map((lo) => qm.table('rows').filter('amount', 'gte', lo)
+  .filter('amount', 'lt', lo + 300).groupBy('region').aggregate('sum', 'amount').aggregate('count', '*'));
+(async () => {
+  const pending = qm.executor.executeBatch(frames.map((f) => f.desc));
+  // while the batch runs: the table is registered again under its name (the old one is
+  // dropped) and the executor is closed; both take effect once the batch settles
+  qm.registerTable('rows', { region: region.slice(0, 1000), amount: amount.slice(0, 1000) });
+  qm.close();
+  const got = (await pending).map((r) => r.rows);
+  console.log(JSON.stringify(got, (k, v) => typeof v === 'bigint' ? v.toString() : v));
+})().catch((e) => { console.error(e); process.exit(1); });
+"""
+
+
+def test_js_close_while_batch_runs():
+    """Dropping the table, re-registering its name and closing the executor
+    while executeBatch is pending must not free what the running batch uses:
+    the batch's results are whole, and the context closes after them."""
+    print("node: close while a batch runs", flush=True)
+    res = subprocess.run([NODE, "-e", SCRIPT_CLOSE % json.dumps(str(JS))], capture_output=True, text=True,
+                         timeout=120, cwd=ROOT)
+    assert res.returncode == 0, res.stderr
+    got = json.loads(res.stdout.strip().splitlines()[-1])
+    assert len(got) == 4
+    for lo, rows in zip([0, 250, 500, 750], got):
+        assert len(rows) == 4
+        total = sum(int(r["count_*"]) for r in rows)
+        # amount = (7 i + 13) mod 1000 takes each value 2000 times over 2M rows
+        assert total == 2000 * len(range(lo, min(lo + 300, 1000)))
