@@ -356,7 +356,6 @@ def _arg_key(e) -> str:
 
 _FLIP = {"lt": "gt", "le": "ge", "gt": "lt", "ge": "le", "eq": "eq", "ne": "ne"}
 _OPNAME = {"eq": "eq", "ne": "neq", "lt": "lt", "le": "lte", "gt": "gt", "ge": "gte"}
-_NEG = {"eq": "ne", "ne": "eq", "lt": "ge", "le": "gt", "gt": "le", "ge": "lt"}
 
 
 def _lit_value(e):
@@ -369,8 +368,9 @@ def _lit_value(e):
     raise _err("SQL: predicate operand must be a literal", "UNSUPPORTED")
 
 
-def _leaf(e, neg: bool) -> FilterOp:
-    """One simple predicate (compiler.ts:365-466, 501-531); `neg` applies NOT."""
+def _leaf(e, neg: bool = False) -> FilterOp:
+    """One simple predicate (compiler.ts:372-451, 501-531); `neg` is set only
+    for NOT BETWEEN, the one NOT the filter path takes."""
     k = e[0]
     if k == "cmp":
         op, a, b = e[1], e[2], e[3]
@@ -380,8 +380,6 @@ def _leaf(e, neg: bool) -> FilterOp:
             col, val, op = b[1], _lit_value(a), _FLIP[op]
         else:
             raise _err("SQL: column-to-column comparisons are not on the device path", "UNSUPPORTED")
-        if neg:
-            op = _NEG[op]
         return FilterOp(col, _OPNAME[op], val)
     if k == "between":
         if e[1][0] != "col":
@@ -391,59 +389,74 @@ def _leaf(e, neg: bool) -> FilterOp:
     if k == "in":
         if e[1][0] != "col":
             raise _err("SQL: IN needs a column", "UNSUPPORTED")
-        negated = e[3] != neg
-        return FilterOp(e[1][1], "not_in" if negated else "in", [_lit_value(v) for v in e[2]])
+        return FilterOp(e[1][1], "not_in" if e[3] else "in", [_lit_value(v) for v in e[2]])
     if k == "isnull":
         if e[1][0] != "col":
             raise _err("SQL: IS NULL needs a column", "UNSUPPORTED")
-        negated = e[2] != neg
-        return FilterOp(e[1][1], "is_not_null" if negated else "is_null", 0)
+        return FilterOp(e[1][1], "is_not_null" if e[2] else "is_null", 0)
     if k == "like":
         raise _err("SQL: LIKE needs string columns, which are not on the device path", "UNSUPPORTED")
     raise _err(f"SQL: unsupported predicate {k}", "UNSUPPORTED")
 
 
-def _dnf(e, neg: bool = False) -> list[list[FilterOp]]:
-    """Disjunctive normal form: OR of AND-groups, NOT pushed to the leaves."""
+def _flatten(e, out: list[FilterOp]) -> bool:
+    """tryFlattenFilters (compiler.ts:365-466): an AND tree of simple
+    predicates, where the only NOT taken is NOT BETWEEN (NOT LIKE is a string
+    predicate). False for anything else: OR, NOT over a comparison, IN or IS
+    NULL, NOT over AND/OR, a doubled NOT."""
     k = e[0]
-    if k == "not":
-        return _dnf(e[1], not neg)
-    if (k == "and" and not neg) or (k == "or" and neg):
-        left, right = _dnf(e[1], neg), _dnf(e[2], neg)
-        out = [a + b for a in left for b in right]
-        if len(out) > MAX_DEVICE_PREDS:
-            raise _err("SQL: WHERE clause expands past the device predicate limit", "UNSUPPORTED")
-        return out
-    if (k == "or" and not neg) or (k == "and" and neg):
-        return _dnf(e[1], neg) + _dnf(e[2], neg)
-    return [[_leaf(e, neg)]]
+    if k == "and":
+        left = _flatten(e[1], out)
+        right = _flatten(e[2], out)
+        return left and right
+    if k in ("cmp", "between", "in", "isnull", "like"):
+        out.append(_leaf(e))
+        return True
+    if k == "not" and e[1][0] == "between" and not e[1][4]:
+        out.append(_leaf(e[1], True))
+        return True
+    if k == "not" and e[1][0] == "like":
+        out.append(_leaf(e[1]))
+    return False
 
 
-def _conjuncts(e, neg: bool = False) -> list[tuple]:
-    """Top-level AND factors (with NOT pushed through OR by De Morgan)."""
-    if e[0] == "not":
-        return _conjuncts(e[1], not neg)
-    if (e[0] == "and" and not neg) or (e[0] == "or" and neg):
-        return _conjuncts(e[1], neg) + _conjuncts(e[2], neg)
-    return [(e, neg)]
+def _or_branches(e, out: list) -> list:
+    """collectOrBranches (compiler.ts:487-495)."""
+    if e[0] == "or":
+        _or_branches(e[1], out)
+        _or_branches(e[2], out)
+    else:
+        out.append(e)
+    return out
+
+
+_ROW_BY_ROW = ("SQL: this WHERE clause is evaluated row by row by the SQL evaluator, not on the filter path "
+               "(NOT over a comparison, IN or IS NULL, NOT over AND/OR, or an AND that holds an OR); "
+               "it is not on the device path")
 
 
 def lower_where(where) -> tuple[list[FilterOp], list[list[FilterOp]]]:
-    """WHERE -> (filters, filterGroups): AND-ed simple factors, plus one OR
-    factor (or the product of several) as AND-groups."""
+    """WHERE -> (filters, filterGroups) exactly where the reference's SQL
+    compiler pushes the clause to the filter path (compiler.ts:90-109): an AND
+    tree of simple predicates becomes `filters`; failing that, a top-level OR
+    whose every branch is such a tree becomes `filterGroups`. Every other
+    clause goes to the reference's row-by-row evaluator (executor.ts:89-91),
+    whose comparisons differ from the filter path's (NaN sorts greatest,
+    integer columns compare exactly against fractional literals,
+    evaluator.ts:169-200), so it raises UNSUPPORTED here rather than run with
+    the filter path's meaning."""
     filters: list[FilterOp] = []
-    ors: list[list[list[FilterOp]]] = []
-    for factor, neg in _conjuncts(where):
-        groups = _dnf(factor, neg)
-        if len(groups) == 1:
-            filters.extend(groups[0])
-        else:
-            ors.append(groups)
     groups: list[list[FilterOp]] = []
-    if ors:
-        groups = ors[0]
-        for more in ors[1:]:
-            groups = [a + b for a in groups for b in more]
+    if not _flatten(where, filters):
+        filters = []
+        branches = _or_branches(where, [])
+        if len(branches) < 2:
+            raise _err(_ROW_BY_ROW, "UNSUPPORTED")
+        for br in branches:
+            g: list[FilterOp] = []
+            if not _flatten(br, g) or not g:
+                raise _err(_ROW_BY_ROW, "UNSUPPORTED")
+            groups.append(g)
     if len(filters) + sum(len(g) for g in groups) > MAX_DEVICE_PREDS:
         raise _err("SQL: WHERE clause has more predicates than the device path takes", "UNSUPPORTED")
     return filters, groups

@@ -95,3 +95,34 @@ def test_and_or_groups_on_device(qm):
                  [FilterOp("c", "eq", 4)], [FilterOp("d", "eq", -2), FilterOp("e", "lt", -0.25)]])
     m = (a >= 10) & (((b > 0.5) & (c < 0)) | np.isin(a, [11, 12, 13]) | (c == 4) | ((d == -2) & (e < np.float32(-0.25))))
     assert np.array_equal(got, np.nonzero(m)[0])
+
+
+def test_sql_pushed_clauses_with_nan_and_fractions_on_device(qm):
+    """SQL clauses the reference pushes to the filter path (a top-level OR of
+    AND trees, NOT BETWEEN, NOT IN), over a float column holding NaN and an
+    int32 column compared with fractional literals: device rows equal the
+    oracle's filter (src/sql/compiler.ts:90-109, src/decode.ts:477-578). The
+    row-by-row shapes (NOT over a comparison, AND over OR) are refused before
+    anything runs."""
+    from querymode_amd.errors import QueryModeError
+    from querymode_amd.sql import sql_to_descriptor
+
+    rng = np.random.default_rng(13)
+    x = rng.normal(0, 1, N)
+    x[rng.random(N) < 0.1] = np.nan
+    i = rng.integers(-4, 5, N).astype(np.int32)
+    nulls = {"x": rng.random(N) < 0.03, "i": rng.random(N) < 0.03}
+    cols = {"x": x, "i": i, "rid": np.arange(N, dtype=np.int64)}
+    qm.register_table("sq", cols, nulls)
+    ot = O.Table(cols, nulls)
+    for where in ["x > 0.5 OR i >= 2.5", "x NOT BETWEEN -0.5 AND 0.5", "i NOT IN (1.5, 2, -3)",
+                  "x < -1 AND i <= -2.5 OR x >= 1 AND i > 1.5 OR x IS NULL", "NOT x BETWEEN -1 AND 1 OR i = 2.5",
+                  "i BETWEEN -2.5 AND 2.5 AND x <> 0.25"]:
+        _say(f"sql on device: {where}")
+        d = sql_to_descriptor("SELECT rid FROM sq WHERE " + where)
+        got = _rows(qm, "sq", d.filters, d.filterGroups)
+        assert np.array_equal(got, O.filter_rows(ot, d.filters, d.filterGroups)), where
+    for where in ["NOT (x < 0.5)", "NOT (i >= 2.5)", "x > 0 AND (i = 1 OR x < -1)"]:
+        with pytest.raises(QueryModeError) as e:
+            sql_to_descriptor("SELECT rid FROM sq WHERE " + where)
+        assert e.value.code == "UNSUPPORTED"

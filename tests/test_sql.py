@@ -40,19 +40,36 @@ def test_or_groups_and_precedence():
     d = S("SELECT * FROM t WHERE a > 2 AND b < 50 OR a = 1")
     assert d.filters == [] and d.filterGroups == [[FilterOp("a", "gt", 2), FilterOp("b", "lt", 50)],
                                                   [FilterOp("a", "eq", 1)]]
-    d = S("SELECT * FROM t WHERE a > 2 AND (b < 50 OR a = 1)")
-    assert d.filters == [FilterOp("a", "gt", 2)]
-    assert d.filterGroups == [[FilterOp("b", "lt", 50)], [FilterOp("a", "eq", 1)]]
+    d = S("SELECT * FROM t WHERE a = 1 OR (b < 2 OR a BETWEEN 1 AND 3)")
+    assert d.filters == [] and d.filterGroups == [[FilterOp("a", "eq", 1)], [FilterOp("b", "lt", 2)],
+                                                  [FilterOp("a", "between", [1, 3])]]
 
 
-def test_not_pushdown():
-    assert S("SELECT * FROM t WHERE NOT (NOT (a > 5))").filters == [FilterOp("a", "gt", 5)]
-    assert S("SELECT * FROM t WHERE NOT a > 5").filters == [FilterOp("a", "lte", 5)]
-    d = S("SELECT * FROM t WHERE NOT (a > 5 AND b < 50)")
-    assert d.filterGroups == [[FilterOp("a", "lte", 5)], [FilterOp("b", "gte", 50)]]
+def test_not_forms_on_the_filter_path():
+    # the NOT forms the reference's compiler pushes down (compiler.ts:383-451)
     assert S("SELECT * FROM t WHERE a NOT BETWEEN 2 AND 5").filters == [FilterOp("a", "not_between", [2, 5])]
-    assert S("SELECT * FROM t WHERE NOT a IN (1, 2)").filters == [FilterOp("a", "not_in", [1, 2])]
-    assert S("SELECT * FROM t WHERE NOT a IS NULL").filters == [FilterOp("a", "is_not_null", 0)]
+    assert S("SELECT * FROM t WHERE NOT a BETWEEN 2 AND 5").filters == [FilterOp("a", "not_between", [2, 5])]
+    assert S("SELECT * FROM t WHERE a NOT IN (1, 2)").filters == [FilterOp("a", "not_in", [1, 2])]
+    assert S("SELECT * FROM t WHERE a IS NOT NULL").filters == [FilterOp("a", "is_not_null", 0)]
+    d = S("SELECT * FROM t WHERE a NOT BETWEEN 1 AND 2 OR b IS NULL")
+    assert d.filterGroups == [[FilterOp("a", "not_between", [1, 2])], [FilterOp("b", "is_null", 0)]]
+
+
+@pytest.mark.parametrize("where", [
+    "NOT (NOT (a > 5))", "NOT a > 5", "NOT (a < 5)", "NOT (i >= 2.5)", "NOT (a > 5 AND b < 50)",
+    "NOT (a > 5 OR b < 50)", "NOT a IN (1, 2)", "NOT a IS NULL", "NOT a NOT BETWEEN 2 AND 5",
+    "a > 2 AND (b < 50 OR a = 1)", "(a = 1 OR b > 50) AND a < 10", "a > 1 OR NOT (b < 2)",
+    "(a > 1 OR b < 2) AND (a < 0 OR b > 9)",
+])
+def test_row_by_row_shapes_are_unsupported(where):
+    """Clauses the reference's compiler leaves to its row-by-row evaluator
+    (compiler.ts:90-109, executor.ts:89-91), whose comparisons put NaN last and
+    compare integer columns exactly against fractions (evaluator.ts:169-200):
+    lowering them to filter-path predicates would change their rows (NOT (x < 5)
+    keeps NaN there; NOT (i >= 2.5) keeps i = 2), so they are refused."""
+    with pytest.raises(QueryModeError) as e:
+        S("SELECT * FROM t WHERE " + where)
+    assert e.value.code == "UNSUPPORTED"
 
 
 def test_literals():
