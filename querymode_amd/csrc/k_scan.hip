@@ -335,6 +335,149 @@ __device__ __forceinline__ void join_apply8(const DScan &sc, const DQuery &q, ui
   }
 }
 
+// ---- fused probe through the join payload (DScan::join_pay) ----
+__device__ __forceinline__ uint64_t pay_bits(const gv4u &p, int off) {
+  const uint64_t b0 = p.x | ((uint64_t)p.y << 32), b1 = p.z | ((uint64_t)p.w << 32);
+  return off < 8 ? (b0 >> (8 * off)) : (b1 >> (8 * (off - 8)));
+}
+__device__ __forceinline__ int64_t pay_i(const gv4u &p, int off, int32_t dt) {
+  const uint64_t v = pay_bits(p, off);
+  switch (dt) {
+    case DT_I8: return (int8_t)v;
+    case DT_BOOL: return (uint8_t)v;
+    case DT_I16: return (int16_t)v;
+    case DT_I32: return (int32_t)v;
+    default: return (int64_t)v;
+  }
+}
+__device__ __forceinline__ double pay_f(const gv4u &p, int off, int32_t dt) {
+  const uint64_t v = pay_bits(p, off);
+  return dt == DT_F32 ? (double)__uint_as_float((uint32_t)v) : __longlong_as_double((long long)v);
+}
+
+// The 8 values (and NULL bits) of column ci for one lane's probe rows: a
+// probe-side column by vector loads, a build-side one out of the payload.
+__device__ __forceinline__ void join_vals8(const DScan &sc, int ci, uint64_t row0, const gv4u p[RPT], bool as_f,
+                                           double f[RPT], int64_t iv[RPT], uint32_t *nm) {
+  const DCol &c = sc.cols[ci];
+  uint32_t n = 0;
+  if (c.side) {
+    const int off = sc.pay_off[ci], nb = sc.pay_null[ci];
+    const bool isf = dt_is_float(c.dtype);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      if (nb) n |= ((p[r].w >> nb) & 1u) << r;
+      if (as_f) f[r] = isf ? pay_f(p[r], off, c.dtype) : (double)pay_i(p[r], off, c.dtype);
+      else iv[r] = pay_i(p[r], off, c.dtype);
+    }
+  } else {
+    n = load8_nulls(c, row0);
+    if (as_f && dt_is_float(c.dtype)) {
+      load8_f(c, row0, f);
+    } else {
+      int64_t t[RPT];
+      load8_i(c, row0, t);
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) {
+        if (as_f) f[r] = (double)t[r];
+        else iv[r] = t[r];
+      }
+    }
+  }
+  *nm = n;
+}
+
+// join_apply8 with the build rows' columns taken from their payload slots
+// (p[r].w bit 0: probe row r has a build row). Same values and updates.
+__device__ __forceinline__ void join_apply8p(const DScan &sc, const DQuery &q, uint64_t row0, const gv4u p[RPT],
+                                             lds_u64 *lds) {
+  const int W = q.entry_words;
+  uint64_t ent[RPT];
+  uint32_t has = 0;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) has |= (p[r].w & 1u) << r;
+  if (!has) return;
+  {
+    int64_t v[MAX_KEYS][RPT];
+    uint32_t n[MAX_KEYS];
+#pragma unroll
+    for (int i = 0; i < MAX_KEYS; ++i) {
+      n[i] = 0;
+#pragma unroll
+      for (int r = 0; r < RPT; ++r) v[i][r] = 0;
+      if (i >= q.key.ncols) continue;
+      const int ci = q.key.col[i];
+      const bool isf = q.key.is_float[i] != 0;
+      double f[RPT];
+      join_vals8(sc, ci, row0, p, isf, f, v[i], &n[i]);
+      if (isf) {
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) v[i][r] = (int64_t)canon_f64_key_bits(f64_bits(f[r]));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      ent[r] = ~0ull;
+      if (!((has >> r) & 1)) continue;
+      int64_t kv[MAX_KEYS];
+      uint32_t kn[MAX_KEYS];
+#pragma unroll
+      for (int i = 0; i < MAX_KEYS; ++i) { kv[i] = v[i][r]; kn[i] = (n[i] >> r) & 1; }
+      if (q.key.ncols == 0) {
+        ent[r] = (q.lmode == LM_NONE) ? (1ull << 63) : 0;
+      } else if (q.gmode == GM_DIRECT) {
+        const uint64_t idx = key_index_direct(q.key, kv, kn);
+        ent[r] = (q.lmode == LM_DIRECT) ? idx : (idx | (1ull << 63));
+      } else {
+        int side;
+        const uint64_t stored = key_code_hash(q.key, kv, kn, &side);
+        if (side >= 0) {
+          ent[r] = (q.gcap + (uint64_t)side) | (1ull << 63);
+        } else {
+          const uint64_t g = glb_find(q, stored);
+          ent[r] = (g == ~0ull) ? ~0ull : (g | (1ull << 63));
+        }
+      }
+    }
+  }
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t rep_base = (q.lmode == LM_DIRECT)
+      ? (uint64_t)(lane & (uint32_t)(q.lds_replicas - 1)) * q.lds_stride : 0;
+  for (int si = 0; si < q.nstates; ++si) {
+    const DState &s = q.states[si];
+    int64_t iv[RPT];
+    double fv[RPT];
+    uint32_t nm = 0;
+    bool isf = false;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) { iv[r] = 0; fv[r] = 0.0; }
+    if (s.col_a >= 0) {
+      if (s.product) {
+        double x[RPT], y[RPT];
+        int64_t unused[RPT];
+        uint32_t na, nb;
+        join_vals8(sc, s.col_a, row0, p, true, x, unused, &na);
+        join_vals8(sc, s.col_b, row0, p, true, y, unused, &nb);
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) fv[r] = x[r] * y[r];
+        nm = na | nb;
+        isf = true;
+      } else {
+        isf = dt_is_float(sc.cols[s.col_a].dtype);
+        join_vals8(sc, s.col_a, row0, p, isf, fv, iv, &nm);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      if (ent[r] == ~0ull || ((nm >> r) & 1)) continue;  // no match / this state skips a NULL value
+      const Contrib k = contrib_of(s, isf, iv[r], fv[r]);
+      if (k.skip) continue;
+      if (ent[r] >> 63) apply_glb(s, q.table + (ent[r] & ~(1ull << 63)) * W, k);
+      else apply_lds(s, lds + q.lds_offset + rep_base + ent[r] * W, k);
+    }
+  }
+}
+
 // JOIN: the fused join probe (one query); else up to MAX_QUERIES queries over
 // one table. Two instantiations, so each keeps only its own registers.
 template <bool JOIN>
@@ -387,7 +530,21 @@ __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc_arg, const DQuer
         load8_i(kc, row0, kv);
       }
       m &= ~load8_nulls(kc, row0);  // NULL never matches in joins
-      if (sc.join_unique) {
+      if (sc.join_pay) {
+        // unique build keys, build columns packed per key slot: one 16-byte gather per probe row
+        gv4u p[RPT];
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) {
+          p[r] = gv4u{0u, 0u, 0u, 0u};
+          if ((m >> r) & 1) {
+            const uint64_t idx = (uint64_t)(kv[r] - sc.join_min);
+            if (idx < sc.join_cap) p[r] = gld<gv4u>(sc.join_pay, idx * 16);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < RPT; ++r) nmatch += p[r].w & 1u;
+        join_apply8p(sc, q, row0, p, lds);
+      } else if (sc.join_unique) {
         // unique build keys: the index holds the build row; all 8 lookups in flight at once
         uint32_t b1[RPT];
 #pragma unroll

@@ -946,7 +946,7 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
   ji.rows = DevBuf(ctx, std::max<uint64_t>(nbuild, 1) * 4);
   ji.cursor = DevBuf(ctx, nent * 4 + 64);
   zero(ctx, ji.cursor.p, nent * 4 + 64);
-  ji.tmp = DevBuf(ctx, (2 * nent + qmk_exscan_tmp_words(nent)) * 8);
+  ji.tmp = DevBuf(ctx, qmk_join_build_tmp_words(nent, nbuild) * 8);
   DevBuf status(ctx, 64);
   zero(ctx, status.p, 64);
   HIPX(qmk_join_build(&key, nbuild, sel, ji.mode, ji.key_float, ji.kmin, ji.cap, ji.table.as<uint64_t>(),
@@ -963,6 +963,41 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
   d2h(ctx, st, status.p, 2);
   if (st[0]) fail(QM_ERR_QUERY_FAILED, "join hash table overflow");
   if (st[1]) ji.unique.reset();  // duplicate build keys: probe through the row lists
+}
+
+// Pack the build-side columns a fused probe reads (at most JOIN_PAY_BYTES)
+// into 16 bytes per DIRECT key slot, widest first, with a flags word
+// (DScan::join_pay). Leaves sc.join_pay null when they do not fit.
+void attach_join_payload(qm_ctx *ctx, const ColSpace &cs, const JoinIndex &ji, DScan &sc, DevBuf &jpay,
+                         uint32_t *launches) {
+  std::vector<int> side1;
+  int bytes = 0;
+  for (size_t i = 0; i < cs.cols.size(); ++i)
+    if (cs.cols[i].side == 1) {
+      side1.push_back((int)i);
+      bytes += (int)dt_size(cs.cols[i].dtype);
+    }
+  if (side1.empty() || bytes > JOIN_PAY_BYTES) return;
+  std::stable_sort(side1.begin(), side1.end(),
+                   [&](int a, int b) { return dt_size(cs.cols[a].dtype) > dt_size(cs.cols[b].dtype); });
+  DJoinPay jp;
+  memset(&jp, 0, sizeof jp);
+  int off = 0;
+  for (int ci : side1) {
+    const int j = jp.n++;
+    jp.cols[j] = cs.cols[ci];
+    jp.off[j] = (int8_t)off;
+    jp.nbit[j] = cs.cols[ci].nulls ? (int8_t)(j + 1) : 0;
+    sc.pay_off[ci] = jp.off[j];
+    sc.pay_null[ci] = jp.nbit[j];
+    off += (int)dt_size(cs.cols[ci].dtype);
+  }
+  if (!jpay.p) {
+    jpay = DevBuf(ctx, ji.cap * 16);
+    HIPX(qmk_join_payload(ji.unique.as<uint32_t>(), ji.cap, &jp, jpay.as<uint32_t>(), ctx->stream));
+    *launches += 1;
+  }
+  sc.join_pay = jpay.as<uint32_t>();
 }
 
 void attach_join(DScan &sc, const JoinIndex &ji, int probe_slot) {
@@ -1040,6 +1075,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.sorted1; }},
     {"join_unique", "QM_TUNE_JOIN_UNIQUE", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_unique = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
+    {"join_payload", "QM_TUNE_JOIN_PAYLOAD", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_payload = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.join_payload; }},
     {"zone_maps", "QM_TUNE_ZONE_MAPS", 0, 1, [](qm_ctx::Tune &t, double v) { t.zone_maps = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.zone_maps; }},
     {"topk_bpc", "QM_TUNE_TOPK_BPC", 1, 16, [](qm_ctx::Tune &t, double v) { t.topk_bpc = (int)v; },
@@ -1474,6 +1511,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
   std::vector<AggBinding> ab(nq);
   std::vector<ListStore> ls(nq);
   std::vector<uint32_t> regrow(nq, 0);
+  DevBuf jpay;  // fused probe: the build columns packed per DIRECT key slot
   for (uint32_t attempt = 0;; ++attempt) {
     Timer tm(ctx);
     uint32_t launches = 0;
@@ -1514,6 +1552,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
       attach_join(sc, *ji, ps);
       sc.join_unique = ji->unique.as<uint32_t>();  // fused aggregate: one gather per probe row
+      if (ji->unique.p && ctx->tune.join_payload) attach_join_payload(ctx, cs, *ji, sc, jpay, &launches);
     }
     std::vector<DQuery> hq(nq);
     for (size_t i = 0; i < nq; ++i) hq[i] = ab[i].q;

@@ -178,6 +178,24 @@ struct DScan {
   // DIRECT with unique build keys (fused aggregates only): build row + 1 per
   // key slot (0 = no build row), one gather per probe row
   const uint32_t *join_unique; // nullptr: use join_table / join_rows
+  // DIRECT with unique build keys and at most JOIN_PAY_BYTES of build-side
+  // columns: per key slot 16 bytes, the columns' values at pay_off[c] and a
+  // flags word at bytes 12..15 (bit 0: a build row exists, bit pay_null[c]:
+  // side-1 column c is NULL; pay_null 0: never NULL). One 16-byte gather per
+  // probe row then replaces the row-id gather and the per-column gathers.
+  const uint32_t *join_pay;    // nullptr: no payload
+  int8_t pay_off[MAX_COLS];
+  int8_t pay_null[MAX_COLS];
+};
+static constexpr int JOIN_PAY_BYTES = 12;
+
+// Build-side columns packed into the join payload (host -> kernel).
+struct DJoinPay {
+  int32_t n;
+  int32_t _p;
+  DCol cols[MAX_COLS];
+  int8_t off[MAX_COLS];
+  int8_t nbit[MAX_COLS];
 };
 
 // ---- batched execution: many aggregate queries over one table in one pass.

@@ -99,13 +99,18 @@ hipError_t qmk_sort_keys(const qm::DCol *c, uint64_t n, int desc, uint64_t *rows
 size_t qmk_sort_tmp_bytes(uint64_t n);
 
 // join build
-// sel: optional ascending build-row ids (rows passing the right side's filters)
+// sel: optional ascending build-row ids (rows passing the right side's filters);
+// tmp: qmk_join_build_tmp_words(nent, nrows) u64 words; cursor: nent + 1 u32, zero
+uint64_t qmk_join_build_tmp_words(uint64_t nent, uint64_t nrows);
 hipError_t qmk_join_build(const qm::DCol *key, uint64_t nrows, const uint64_t *sel, int mode, int key_float,
                           int64_t kmin, uint64_t cap, uint64_t *table, uint32_t *rows, uint32_t *cursor,
                           uint64_t *tmp, uint32_t *status, hipStream_t st);
 // unique-key DIRECT index (build row + 1 per slot); status[1] <- duplicate keys
 hipError_t qmk_join_unique(const uint64_t *table, uint64_t nent, const uint32_t *rows, uint32_t *uniq,
                            uint32_t *status, hipStream_t st);
+// join payload (DScan::join_pay) of `cap` DIRECT slots from the unique index: 4 u32 per slot
+hipError_t qmk_join_payload(const uint32_t *uniq, uint64_t cap, const qm::DJoinPay *jp, uint32_t *out,
+                            hipStream_t st);
 // materialised inner-join pairs (probe row, build row) in probe order
 hipError_t qmk_join_count(const qm::DScan *sc, const qm::DQuery *d_q, uint64_t nblocks, uint64_t tpb,
                           uint64_t *block_counts, hipStream_t st);

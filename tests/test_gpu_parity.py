@@ -97,6 +97,63 @@ def test_join_materialised_order(qm):
     assert_rows_equal(got.rows, want, float_cols={"right_x"}, ctx="join rows")
 
 
+@pytest.mark.parametrize("payload", [1, 0])
+def test_join_aggregate_build_columns_with_nulls(qm, payload):
+    """Fused probe over unique build keys whose group key and values come from
+    the build side (int16 / int8 / f32 with NULLs) and the probe side, with
+    the build columns gathered from the packed per-key payload (payload=1) or
+    per column (payload=0); both equal the oracle (src/operators.ts:2673-2713,
+    src/partial-agg.ts:62-80)."""
+    _say(f"case join aggregate build columns with nulls payload={payload}")
+    rng = np.random.default_rng(23)
+    nb, npr = 50_000, 400_000
+    build = {"id": rng.permutation(nb).astype(np.int64) * 3 - 1000,
+             "g": rng.integers(-3, 4, nb).astype(np.int8),
+             "h": rng.integers(-200, 200, nb).astype(np.int16),
+             "w": rng.random(nb).astype(np.float32)}
+    bnull = {"g": rng.random(nb) < 0.05, "h": rng.random(nb) < 0.05, "w": rng.random(nb) < 0.05}
+    probe = {"fk": (rng.integers(0, nb + nb // 9, npr) * 3 - 1000).astype(np.int64),
+             "pg": rng.integers(0, 3, npr).astype(np.int32),
+             "amount": rng.normal(10, 3, npr)}
+    pnull = {"amount": rng.random(npr) < 0.05}
+    d = QueryDescriptor("p", join=JoinDescriptor(QueryDescriptor("b"), "fk", "id"),
+                        computedColumns=[("aw", "amount", "w")], groupBy=["g", "pg"],
+                        aggregates=[AggregateOp("sum", "aw"), AggregateOp("count", "*"), AggregateOp("min", "h"),
+                                    AggregateOp("max", "w"), AggregateOp("sum", "h"), AggregateOp("count", "w"),
+                                    AggregateOp("avg", "amount")])
+    qm.device.tune_set("join_payload", payload)
+    try:
+        got, want = run_both(qm, d, {"p": probe, "b": build}, {"p": pnull, "b": bnull})
+    finally:
+        qm.device.tune_set("join_payload", 1)
+    assert_groups_equal(got.rows, want, ["g", "pg"], float_cols={"sum_aw", "max_w", "avg_amount"}, ctx="join nulls")
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_join_heavily_duplicated_build_keys(qm, sparse):
+    """Build keys repeated 100K and 40K times (row ids spread over more than
+    one 262144-value window) and ~200 times each: every key's build rows come
+    out ascending, i.e. in the reference's insertion order
+    (src/operators.ts:2349-2436, 2673-2713); DIRECT (dense) and HASH (sparse) index."""
+    _say(f"case join heavy duplicates sparse={sparse}")
+    rng = np.random.default_rng(17)
+    nb = 300_000
+    bk = rng.integers(0, 1000, nb).astype(np.int64)
+    pos = rng.permutation(nb)
+    bk[pos[:100_000]] = 7
+    bk[pos[100_000:140_000]] = 9
+    pk = rng.integers(0, 1000, 2000).astype(np.int64)
+    pk[[5, 700, 1999]] = 7
+    if sparse:
+        bk, pk = bk * 1_000_003 + 11, pk * 1_000_003 + 11
+    left = {"k": pk, "x": np.arange(2000, dtype=np.int32)}
+    right = {"k": bk, "y": np.arange(nb, dtype=np.int32)}
+    d = QueryDescriptor("l", join=JoinDescriptor(QueryDescriptor("r"), "k", "k"), projections=["x", "y"])
+    got, want = run_both(qm, d, {"l": left, "r": right})
+    assert len(got.rows) > 300_000
+    assert_rows_equal(got.rows, want, ctx="join heavy duplicates")
+
+
 @pytest.mark.parametrize("k", [10, 100, 1000])
 def test_topk(qm, k):
     _say(f"case topk k={k}")
