@@ -380,6 +380,29 @@ extern "C" hipError_t qmk_join_payload(const uint32_t *uniq, uint64_t cap, const
   return hipGetLastError();
 }
 
+// DIRECT index straight from the build rows when every key is unique: each
+// row claims its slot (build row + 1); a slot claimed twice sets status[1]
+// and the caller falls back to the counted build.
+__global__ void join_claim_kernel(DCol c, uint64_t n, const uint64_t *sel, int64_t kmin, uint64_t cap,
+                                  uint32_t *uniq, uint32_t *status) {
+  uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; j < n; j += stride) {
+    const uint64_t i = sel ? sel[j] : j;
+    if (load1_null(c, i)) continue;  // NULL never matches (src/operators.ts:2424-2428)
+    const uint64_t idx = (uint64_t)(load1_i(c, i) - kmin);
+    if (idx >= cap) continue;  // outside the key statistics' domain: cannot happen for this table
+    if (atomicCAS(&uniq[idx], 0u, (uint32_t)(i + 1)) != 0u) atomicOr(&status[1], 1u);
+  }
+}
+
+extern "C" hipError_t qmk_join_claim(const DCol *key, uint64_t nrows, const uint64_t *sel, int64_t kmin, uint64_t cap,
+                                     uint32_t *uniq, uint32_t *status, hipStream_t st) {
+  hipLaunchKernelGGL(join_claim_kernel, dim3(gridn(nrows)), dim3(256), 0, st, *key, nrows, sel, kmin, cap, uniq,
+                     status);
+  return hipGetLastError();
+}
+
 extern "C" hipError_t qmk_join_unique(const uint64_t *table, uint64_t nent, const uint32_t *rows, uint32_t *uniq,
                                       uint32_t *status, hipStream_t st) {
   hipLaunchKernelGGL(join_unique_kernel, dim3(gridn(nent)), dim3(256), 0, st, table, nent, rows, uniq, status);

@@ -888,7 +888,10 @@ struct JoinIndex {
 DevBuf select_rows(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, uint64_t limit_cap, uint64_t *n_out,
                    uint32_t *launches);
 
-void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32_t *launches, DevBuf *keep) {
+// fused: the index feeds a fused aggregate, which needs only the unique-key
+// DIRECT index when the build keys are unique.
+void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32_t *launches, DevBuf *keep,
+                bool fused) {
   const qm_table *R = cs.R;
   int rk = R->find(pl.rkey);
   if (rk < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "join key not found: %s", pl.rkey.c_str());
@@ -938,6 +941,19 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
   } else {
     ji.mode = 2;
     ji.cap = pow2ceil(std::max<uint64_t>(1024, 2 * nbuild));
+  }
+  if (fused && ji.mode == 1 && ctx->tune.join_unique && ctx->tune.join_claim) {
+    ji.unique = DevBuf(ctx, ji.cap * 4 + 64);
+    zero(ctx, ji.unique.p, ji.cap * 4 + 64);
+    DevBuf status(ctx, 64);
+    zero(ctx, status.p, 64);
+    HIPX(qmk_join_claim(&key, nbuild, sel, ji.kmin, ji.cap, ji.unique.as<uint32_t>(), status.as<uint32_t>(),
+                        ctx->stream));
+    *launches += 1;
+    uint32_t st[2];
+    d2h(ctx, st, status.p, 2);
+    if (!st[1]) return;  // unique keys: the claimed slots are the whole index
+    ji.unique.reset();   // duplicate build keys: the counted build below
   }
   uint64_t nent = (ji.mode == 1) ? ji.cap : ji.cap + 1;
   uint64_t twords = (ji.mode == 1) ? ji.cap : 2 * (ji.cap + 1);
@@ -1075,6 +1091,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.sorted1; }},
     {"join_unique", "QM_TUNE_JOIN_UNIQUE", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_unique = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
+    {"join_claim", "QM_TUNE_JOIN_CLAIM", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_claim = v != 0; },
+     [](const qm_ctx::Tune &t) { return (double)t.join_claim; }},
     {"join_payload", "QM_TUNE_JOIN_PAYLOAD", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_payload = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_payload; }},
     {"zone_maps", "QM_TUNE_ZONE_MAPS", 0, 1, [](qm_ctx::Tune &t, double v) { t.zone_maps = v != 0; },
@@ -1695,7 +1713,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   DevBuf dq(ctx, sizeof(DQuery));
   h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
   if (pl.has_join) {
-    build_join(ctx, cs, pl, ji, &launches, &keep);
+    build_join(ctx, cs, pl, ji, &launches, &keep, false);
     int ps = cs.plain(pl.lkey);
     sc.ncols = (int)cs.cols.size();
     for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
@@ -3340,7 +3358,7 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       cs.plan = &pl;
       uint32_t l = 0;
       Timer tb(ctx);
-      build_join(ctx, cs, pl, ji, &l, &keep);
+      build_join(ctx, cs, pl, ji, &l, &keep, true);
       double bms = tb.stop();
       try {
         run_aggregates(ctx, left, right, plans, results, true, &ji);

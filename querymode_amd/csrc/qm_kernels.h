@@ -105,6 +105,10 @@ uint64_t qmk_join_build_tmp_words(uint64_t nent, uint64_t nrows);
 hipError_t qmk_join_build(const qm::DCol *key, uint64_t nrows, const uint64_t *sel, int mode, int key_float,
                           int64_t kmin, uint64_t cap, uint64_t *table, uint32_t *rows, uint32_t *cursor,
                           uint64_t *tmp, uint32_t *status, hipStream_t st);
+// unique-key DIRECT index straight from the build rows (uniq zeroed, cap + 16 u32);
+// status[1] <- duplicate keys (the index is then incomplete)
+hipError_t qmk_join_claim(const qm::DCol *key, uint64_t nrows, const uint64_t *sel, int64_t kmin, uint64_t cap,
+                          uint32_t *uniq, uint32_t *status, hipStream_t st);
 // unique-key DIRECT index (build row + 1 per slot); status[1] <- duplicate keys
 hipError_t qmk_join_unique(const uint64_t *table, uint64_t nent, const uint32_t *rows, uint32_t *uniq,
                            uint32_t *status, hipStream_t st);

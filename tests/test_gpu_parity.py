@@ -152,6 +152,12 @@ def test_join_heavily_duplicated_build_keys(qm, sparse):
     got, want = run_both(qm, d, {"l": left, "r": right})
     assert len(got.rows) > 300_000
     assert_rows_equal(got.rows, want, ctx="join heavy duplicates")
+    # the fused aggregate over the same duplicated keys (the unique-key index is refused)
+    right["g"] = (np.arange(nb) % 5).astype(np.int32)
+    d = QueryDescriptor("l", join=JoinDescriptor(QueryDescriptor("r"), "k", "k"), groupBy=["g"],
+                        aggregates=[AggregateOp("count", "*"), AggregateOp("sum", "y"), AggregateOp("min", "x")])
+    got, want = run_both(qm, d, {"l": left, "r": right})
+    assert_groups_equal(got.rows, want, ["g"], ctx="join heavy duplicates aggregate")
 
 
 @pytest.mark.parametrize("k", [10, 100, 1000])
