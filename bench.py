@@ -255,6 +255,38 @@ def bench_batch(args) -> dict:
         alg_bytes = rows * sum(np.dtype(cols[c].dtype).itemsize for c in ("k2", "v2", "v1"))
         records = None
     achieved = alg_bytes / (scan_ms * 1e-3) / 1e9
+    # the same batch captured into a HIP graph (qm_batch_prepare) and replayed:
+    # every pass runs again with one host launch and no host planning
+    replay = None
+    if not args.no_replay:
+        from querymode_amd.client import PreparedBatch
+        t0 = time.perf_counter()
+        ph = C.c_void_p()
+        N.check(lib.qm_batch_prepare(dev.handle, arr, nq, table.handle, C.byref(ph)))
+        prep_s = time.perf_counter() - t0
+        prep = PreparedBatch(dev, ph, table, nq)
+
+        def replay_step():
+            outs = (C.c_void_p * nq)()
+            prep.run_handles(outs)
+            s2 = N.Stats()
+            lib.qm_result_stats(C.c_void_p(outs[0]), C.byref(s2))
+            for i in range(nq):
+                lib.qm_result_close(C.c_void_p(outs[i]))
+            return s2.device_ms
+
+        for _ in range(args.warmup):
+            replay_step()
+        t0 = time.perf_counter()
+        rdev = [replay_step() for _ in range(args.steps)]
+        rwall = (time.perf_counter() - t0) / args.steps
+        info = prep.info()
+        prep.close()
+        replay = {"ms_per_step": rwall * 1e3, "device_ms": float(np.mean(rdev)), "rows_per_s": nq * rows / rwall,
+                  "vs_batch_exec": ms_step / (rwall * 1e3), "prepare_s": prep_s,
+                  "graph_nodes": info["graph_nodes"], "device_bytes": info["device_bytes"]}
+        print(f"replay: {rwall * 1e3:.2f} ms/step (qm_batch_exec {ms_step:.2f}), {info['graph_nodes']} graph nodes",
+              file=sys.stderr, flush=True)
     read_ceiling = None
     if not args.no_read_ceiling:
         g = C.c_double()
@@ -271,7 +303,9 @@ def bench_batch(args) -> dict:
     for s in pq:
         k = N.STRATEGIES.get(s.strategy)
         strategies[k] = strategies.get(k, 0) + 1
-    served = serve_mix(args, dev, cols, rows, descs) if args.serve_seconds > 0 else None
+    served = None
+    if args.serve_seconds > 0:
+        served = serve_mix(args, dev, cols, rows, descs, [args.qps, args.qps_high])
     if args.copy_calibration:
         copy_calibration(1 << 30)
     for p in plans:
@@ -299,6 +333,7 @@ def bench_batch(args) -> dict:
         "per_query_strategies": strategies,
         "batch_strategy_k2": strat,
         "served": served,
+        "graph_replay": replay,
         "autotune": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -317,17 +352,18 @@ def bench_batch(args) -> dict:
     return out
 
 
-def serve_mix(args, dev, cols, rows, descs) -> dict:
-    """Poisson arrivals at a fixed QPS through the continuous-batching server:
+def serve_mix(args, dev, cols, rows, descs, rates) -> dict:
+    """Poisson arrivals at fixed rates through the continuous-batching server:
     40% group-by queries of the batched mix, 30% the 200M x 10M join +
-    aggregate, 30% sort(v2 desc).limit(k), k in {10, 100, 1000}."""
+    aggregate, 30% sort(v2 desc).limit(k), k in {10, 100, 1000}. One phase of
+    `serve_seconds` per rate; the first rate's figures are the top-level ones."""
     import random
     import threading
 
     from querymode_amd import datagen
     from querymode_amd.client import QueryMode
     from querymode_amd.plan import AggregateOp, JoinDescriptor, QueryDescriptor
-    from querymode_amd.serve import Server, percentile
+    from querymode_amd.serve import LANES, Server, percentile
 
     qm = QueryMode(dev)
     nbuild, nprobe = 10_000_000, 200_000_000
@@ -336,57 +372,71 @@ def serve_mix(args, dev, cols, rows, descs) -> dict:
     join = QueryDescriptor("probe", join=JoinDescriptor(QueryDescriptor("build"), "fk", "id"), groupBy=["region"],
                            computedColumns=[("aw", "amount", "weight")],
                            aggregates=[AggregateOp("sum", "aw"), AggregateOp("count", "*")])
-    rng = random.Random(11)
-    mix, nrows = [], []
-    t = 0.0
-    while True:
-        t += rng.expovariate(args.qps)
-        if t > args.serve_seconds:
-            break
-        u = rng.random()
-        if u < 0.4:
-            mix.append((t, descs[rng.randrange(len(descs))]))
-            nrows.append(rows)
-        elif u < 0.7:
-            mix.append((t, join))
-            nrows.append(nprobe + nbuild)
-        else:
-            mix.append((t, QueryDescriptor("batch", sortColumn="v2", sortDirection="desc",
-                                           limit=rng.choice([10, 100, 1000]))))
-            nrows.append(rows)
     # warm the join index, top-K and group tables once
-    for d in (join, mix[0][1], QueryDescriptor("batch", sortColumn="v2", sortDirection="desc", limit=10)):
+    for d in (join, descs[0], QueryDescriptor("batch", sortColumn="v2", sortDirection="desc", limit=10)):
         qm.executor.execute(d)
-    srv = Server(qm)
-    reqs = [None] * len(mix)
-    t0 = time.perf_counter()
 
-    def producer():
-        for i, (at, d) in enumerate(mix):
-            delay = t0 + at - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            reqs[i] = srv.submit(d)
+    def phase(qps: float, seed: int) -> dict:
+        rng = random.Random(seed)
+        mix, nrows = [], []
+        t = 0.0
+        while True:
+            t += rng.expovariate(qps)
+            if t > args.serve_seconds:
+                break
+            u = rng.random()
+            if u < 0.4:
+                mix.append((t, descs[rng.randrange(len(descs))]))
+                nrows.append(rows)
+            elif u < 0.7:
+                mix.append((t, join))
+                nrows.append(nprobe + nbuild)
+            else:
+                mix.append((t, QueryDescriptor("batch", sortColumn="v2", sortDirection="desc",
+                                               limit=rng.choice([10, 100, 1000]))))
+                nrows.append(rows)
+        srv = Server(qm)
+        reqs = [None] * len(mix)
+        t0 = time.perf_counter()
 
-    th = threading.Thread(target=producer)
-    th.start()
-    th.join()
-    for r in reqs:
-        r.wait(600)
-    elapsed = time.perf_counter() - t0
-    srv.close()
-    lat = [r.latency_ms for r in reqs]
-    kinds = {"group-by": [], "join": [], "top-k": []}
-    for r in reqs:
-        k = "join" if r.desc.join is not None else ("group-by" if r.desc.aggregates else "top-k")
-        kinds[k].append(r.latency_ms)
-    return {"qps": args.qps, "seconds": args.serve_seconds, "requests": len(reqs),
-            "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
-            "rows_per_s": sum(nrows) / elapsed, "achieved_qps": len(reqs) / elapsed,
-            "cycles": srv.cycles, "mean_requests_per_cycle": len(reqs) / max(1, srv.cycles),
-            "p50_ms_by_kind": {k: (percentile(v, 50) if v else None) for k, v in kinds.items()},
-            "p99_ms_by_kind": {k: (percentile(v, 99) if v else None) for k, v in kinds.items()},
-            "mix": "40% group-by (batched mix), 30% join 200M x 10M + aggregate, 30% top-K v2 desc k in {10,100,1000}"}
+        def producer():
+            for i, (at, d) in enumerate(mix):
+                delay = t0 + at - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                reqs[i] = srv.submit(d)
+
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        for r in reqs:
+            r.wait(600)
+        elapsed = time.perf_counter() - t0
+        srv.close()
+        lat = [r.latency_ms for r in reqs]
+        kinds = {"group-by": [], "join": [], "top-k": []}
+        for r in reqs:
+            k = "join" if r.desc.join is not None else ("group-by" if r.desc.aggregates else "top-k")
+            kinds[k].append(r.latency_ms)
+        gb = [r for r in reqs if r.fused]
+        print(f"served {qps:.0f} QPS: {len(reqs)} requests, p50 {percentile(lat, 50):.1f} ms, "
+              f"p99 {percentile(lat, 99):.1f} ms, {srv.launches} launches, overlap {srv.overlap_share:.2f}",
+              file=sys.stderr, flush=True)
+        return {"qps": qps, "seconds": args.serve_seconds, "requests": len(reqs),
+                "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                "rows_per_s": sum(nrows) / elapsed, "achieved_qps": len(reqs) / elapsed,
+                "launches": srv.launches, "mean_requests_per_launch": len(reqs) / max(1, srv.launches),
+                "mean_group_by_batch": (sum(r.batch_size for r in gb) / len(gb)) if gb else None,
+                "lanes": LANES, "max_concurrent_launches": srv.max_overlap,
+                "overlap_share_of_busy_time": srv.overlap_share,
+                "p50_ms_by_kind": {k: (percentile(v, 50) if v else None) for k, v in kinds.items()},
+                "p99_ms_by_kind": {k: (percentile(v, 99) if v else None) for k, v in kinds.items()}}
+
+    phases = [phase(q, 11 + i) for i, q in enumerate(rates)]
+    out = dict(phases[0])
+    out["mix"] = "40% group-by (batched mix), 30% join 200M x 10M + aggregate, 30% top-K v2 desc k in {10,100,1000}"
+    out["phases"] = phases
+    return out
 
 
 def bench_join(args) -> dict:
@@ -526,9 +576,11 @@ def main() -> int:
                     help="store the batch table clustered by k2 (a named variant; the default keeps generated row order)")
     ap.add_argument("--cpu-sample-rows", type=int, default=0)
     ap.add_argument("--qps", type=float, default=40.0, help="served phase: Poisson arrival rate")
+    ap.add_argument("--qps-high", type=float, default=120.0, help="served phase: second rate, near saturation")
     ap.add_argument("--serve-seconds", type=float, default=6.0, help="served phase length (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the planner's default knobs")
+    ap.add_argument("--no-replay", action="store_true", help="skip the prepared-batch (HIP graph) replay phase")
     ap.add_argument("--no-read-ceiling", action="store_true", help="skip the streaming-read ceiling probe")
     ap.add_argument("--cpu-workers", type=int, default=0, help="CPU baseline processes (default: min(16, nproc))")
     ap.add_argument("--copy-calibration", action="store_true",

@@ -15,8 +15,11 @@
  *     no C++ exception crosses the ABI;
  *   - output buffers are caller-owned with an explicit capacity;
  *   - a failing call leaves a message readable through qm_last_error().
- * Handles may be used from several host threads; device submission is
- * serialised per context on one HIP stream.
+ * Handles may be used from several host threads. A context has four
+ * execution lanes, each a HIP stream of its own: every call that runs device
+ * work holds one lane for its duration, so up to four calls (from four host
+ * threads) run on the device at once; a fifth waits for a free lane. Table
+ * upload / drop and knob changes wait until no call runs.
  */
 #ifndef QMGPU_H
 #define QMGPU_H
@@ -62,6 +65,7 @@ typedef struct qm_table  qm_table;
 typedef struct qm_plan   qm_plan;
 typedef struct qm_result qm_result;
 typedef struct qm_batch  qm_batch;
+typedef struct qm_prepared qm_prepared;
 
 /* One host column handed to qm_table_create. `null_bitmap` follows QMCB:
  * ceil(nrows/8) bytes, bit (row & 7) of byte (row >> 3) set = row is null;
@@ -123,6 +127,7 @@ int qm_device_info(qm_ctx *ctx, char *name, size_t cap, uint32_t *num_cus,
  * being decoded into JS arrays page by page. */
 int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows,
                     uint32_t ncols, const qm_column_desc *cols, qm_table **out);
+/* Fails with QM_ERR_INVALID_ARG while prepared batches over the table are open. */
 int qm_table_drop(qm_table *t);
 uint64_t qm_table_rows(const qm_table *t);
 
@@ -149,9 +154,30 @@ int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **out);
 int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
                   qm_table *table, qm_result **results_out);
 
+/* Prepared batches (no reference counterpart; the nearest is the
+ * reference's per-query assemblePipeline, src/operators.ts:3073-3150, rebuilt
+ * for every execution): the qm_batch_exec of `plans` over `table` is run once
+ * to size every buffer (the host plans with the result sizes it reads back),
+ * then its whole launch sequence is captured into a HIP graph. Each
+ * qm_prepared_run replays that graph with one host launch: every pass over
+ * the table runs again, with no host planning, allocation or readback. Table
+ * columns are immutable once uploaded, so a replay's results are the batch's
+ * results. The nplans result handles qm_prepared_run returns are views of
+ * buffers the prepared batch owns: valid until its next run or its close
+ * (close them with qm_result_close as usual). Runs of one prepared batch are
+ * serialised; the plans may be closed after qm_batch_prepare returns; the
+ * table stays open until the prepared batch is closed. Joins are rejected
+ * (QM_ERR_UNSUPPORTED) as in qm_batch_exec. */
+int qm_batch_prepare(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans,
+                     qm_table *table, qm_prepared **out);
+int qm_prepared_run(qm_prepared *p, qm_result **results_out);
+/* Plans, graph nodes and device bytes held by the prepared batch. */
+int qm_prepared_info(qm_prepared *p, uint32_t *nplans, uint64_t *graph_nodes, uint64_t *device_bytes);
+int qm_prepared_close(qm_prepared *p);
+
 /* Asynchronous form of qm_batch_exec, for callers that keep working (or
  * queue the next batch) while one runs: the batch executes on a worker
- * thread of the library; batches of one context run one after another. The
+ * thread of the library, on a lane of its own. The
  * plans and the table must stay open until the batch is done. Replaces the
  * reference's Promise-returning execute (src/client.ts:1280-1304) for a set
  * of queries. */

@@ -67,6 +67,12 @@ SIGNATURES = [
     ("qm_batch_poll", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("qm_batch_wait", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("qm_batch_close", C.c_int, [C.c_void_p]),
+    ("qm_batch_prepare", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.c_void_p,
+                                   C.POINTER(C.c_void_p)]),
+    ("qm_prepared_run", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("qm_prepared_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_uint64)]),
+    ("qm_prepared_close", C.c_int, [C.c_void_p]),
     ("qm_plan_device_bytes", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_tune_set", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
     ("qm_tune_get", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_double)]),

@@ -260,6 +260,49 @@ class PendingBatch:
             pass
 
 
+class PreparedBatch:
+    """A batch captured into a HIP graph (qm_batch_prepare): ``run()`` replays
+    every pass over the table with one host launch. The results of a run are
+    read out (QMCB) before the next run reuses the prepared buffers."""
+
+    def __init__(self, dev: "Device", handle, table: "GpuTable", size: int):
+        self._dev, self._h, self._table, self.size = dev, handle, table, size
+
+    def info(self) -> dict:
+        n, nodes, b = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        N.check(self._dev.lib.qm_prepared_info(self._h, C.byref(n), C.byref(nodes), C.byref(b)))
+        return {"plans": n.value, "graph_nodes": nodes.value, "device_bytes": b.value}
+
+    def run_handles(self, outs) -> None:
+        """Replay into `outs` (nplans result handles; the caller closes them)."""
+        if not self._h:
+            raise QueryModeError("INVALID_ARGUMENT", "the prepared batch is closed")
+        N.check(self._dev.lib.qm_prepared_run(self._h, outs))
+
+    def run(self) -> list[QueryResult]:
+        outs = (C.c_void_p * self.size)()
+        t0 = time.perf_counter()
+        self.run_handles(outs)
+        return _batch_results(self._dev.lib, outs, self.size, t0)
+
+    def close(self) -> None:
+        if self._h:
+            N.check(self._dev.lib.qm_prepared_close(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
 class GpuExecutor:
     """QueryExecutor (src/client.ts:1280-1304) backed by the device library."""
 
@@ -312,6 +355,22 @@ class GpuExecutor:
             for p in plans:
                 p.close()
         return _batch_results(self.dev.lib, outs, len(descs), t0)
+
+    def prepare_batch(self, descs: list[QueryDescriptor]) -> PreparedBatch:
+        """Run the batch once and capture its launches into a HIP graph
+        (qm_batch_prepare); ``PreparedBatch.run()`` replays it."""
+        if not descs:
+            raise QueryModeError("INVALID_ARGUMENT", "an empty batch")
+        table = self._batch_table(descs)
+        plans = [_Plan(self.dev, d) for d in descs]
+        try:
+            arr = (C.c_void_p * len(plans))(*[p.handle for p in plans])
+            h = C.c_void_p()
+            N.check(self.dev.lib.qm_batch_prepare(self.dev.handle, arr, len(plans), table.handle, C.byref(h)))
+        finally:
+            for p in plans:
+                p.close()
+        return PreparedBatch(self.dev, h, table, len(descs))
 
     def submit_batch(self, descs: list[QueryDescriptor]) -> "PendingBatch":
         """Start a batch on the library's worker thread (qm_batch_submit) and

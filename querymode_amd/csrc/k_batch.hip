@@ -927,7 +927,7 @@ extern "C" int qmk_batch_waves_per_cu(int nc, int wide, int mode, size_t lds_byt
   int nb = 0;
   const void *fn = batch_fn(nc, wide, mode);
   if (lds_bytes > 64 * 1024)
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    (void)qmk_func_lds(fn, lds_bytes);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, B_BLOCK, lds_bytes) != hipSuccess) nb = 1;
   return (nb < 1 ? 1 : nb) * (B_BLOCK / 64);
 }
@@ -938,7 +938,7 @@ extern "C" hipError_t qmk_batch_scan(const BScan *bs, const BQuery *d_q, uint32_
   if ((bs->nshared > 0) == (bs->nq > 0)) return hipErrorInvalidValue;  // one kind of query per launch
   const void *fn = batch_fn(bs->ncols, wide, bs->nshared > 0 ? 1 : 2);
   if (lds_bytes > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipError_t e = qmk_func_lds(fn, lds_bytes);
     if (e != hipSuccess) return e;
   }
   BScan a = *bs;

@@ -593,7 +593,7 @@ extern "C" hipError_t qmk_part_scan(const DScan *sc, const DQuery *d_q, const DP
   const void *fn = fast ? fast_fn(fast) : (const void *)part_scan_kernel;
   if (!fn) return hipErrorInvalidValue;
   if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = qmk_func_lds(fn, (size_t)lds);
     if (e != hipSuccess) return e;
   }
   if (fast) {
@@ -608,8 +608,7 @@ extern "C" hipError_t qmk_part_scatter(const DQuery *d_q, const DPart *pp, uint3
                                        int entry_words, hipStream_t st) {
   size_t lds = (size_t)(pp->nparts + 1) * 8 + (pp->hot_n ? (size_t)HOT_SLOTS * entry_words * 8 : 0);
   if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)part_scatter_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = qmk_func_lds((const void *)part_scatter_kernel, (size_t)lds);
     if (e != hipSuccess) return e;
   }
   uint32_t g2 = (g1 + rpb - 1) / rpb;
@@ -628,8 +627,7 @@ extern "C" hipError_t qmk_part_agg(const DQuery *d_q, const DPart *pp, uint32_t 
                                    hipStream_t st) {
   size_t lds = (size_t)pp->lds_entries * entry_words * 8;
   if (lds > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)part_agg_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)lds);
+    hipError_t e = qmk_func_lds((const void *)part_agg_kernel, (size_t)lds);
     if (e != hipSuccess) return e;
   }
   if (nseg == 0) return hipSuccess;

@@ -763,7 +763,7 @@ extern "C" hipError_t qmk_pr_runs(const PRScan *ps, hipStream_t st) {
   if (ps->np == 0) return hipSuccess;
   const uint32_t nbin = (uint32_t)(ps->nbounds + 2) << ps->s;
   const size_t lds = qmk_pr_runs_lds(nbin, ps->nsum, nmax, ps->cap);
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = qmk_func_lds(fn, (size_t)lds);
   if (e != hipSuccess) return e;
   int bpc = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, fn, PR_RBLOCK, lds) != hipSuccess || bpc < 1) bpc = 1;

@@ -1042,7 +1042,7 @@ extern "C" hipError_t qmk_runs_write(const RScan *rs, uint32_t lds_pad, hipStrea
   const void *fn = runs_write_fn(rs->nsum, rs->nstates - 1 - rs->nsum, runs_qpl(rs->nq));
   if (!fn) return hipErrorInvalidValue;
   if (lds_pad) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pad);
+    hipError_t e = qmk_func_lds(fn, (size_t)lds_pad);
     if (e != hipSuccess) return e;
   }
   RScan a = *rs;
@@ -1060,7 +1060,7 @@ extern "C" hipError_t qmk_runs_place(const RScan *rs, int win, hipStream_t st) {
   const void *fn = runs_place_fn(ns, win);
   if (!fn) return hipErrorInvalidValue;
   const size_t lds = runs_place_lds(ns, win);
-  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipError_t e = qmk_func_lds(fn, (size_t)lds);
   if (e != hipSuccess) return e;
   RScan a = *rs;
   void *args[] = {&a};
@@ -1076,7 +1076,7 @@ extern "C" int qmk_shared_blocks_per_cu(const SScan *ss) {
   const void *fn = shared_fn(ss->nv);
   if (!fn) return 0;
   const size_t lds = qmk_shared_lds_bytes(ss);
-  if (lds > 64 * 1024) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (lds > 64 * 1024) (void)qmk_func_lds(fn, (size_t)lds);
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, S_BLOCK, lds) != hipSuccess) nb = 1;
   return nb < 1 ? 1 : nb;
@@ -1093,7 +1093,7 @@ extern "C" hipError_t qmk_shared_scan(const SScan *ss, uint32_t grid, hipStream_
   const size_t lds = qmk_shared_lds_bytes(ss);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   if (lds > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = qmk_func_lds(fn, (size_t)lds);
     if (e != hipSuccess) return e;
   }
   SScan a = *ss;

@@ -17,7 +17,10 @@
 // or an open-addressing cache for large ones) that is merged into the global
 // table with atomics when the workgroup finishes; rows whose key misses the
 // LDS cache go straight to the global table.
+#include <map>
+#include <mutex>
 #include "k_agg.hip.h"
+#include "qm_kernels.h"
 
 namespace qm {
 
@@ -589,6 +592,22 @@ __global__ void fill_u64_kernel(uint64_t *p, uint64_t n, uint64_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// Byte-exact zero fill of [p, p + n) at any alignment (graph-safe replacement
+// for a memset node): 8-byte stores over the aligned middle, byte stores for
+// the < 8 bytes before and after it.
+__global__ void zero_bytes_kernel(uint8_t *p, uint64_t n) {
+  const uint64_t mis = (uint64_t)((8 - ((uintptr_t)p & 7)) & 7);
+  const uint64_t head = mis < n ? mis : n;
+  const uint64_t words = (n - head) / 8;
+  const uint64_t tail0 = head + words * 8;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint64_t *w = reinterpret_cast<uint64_t *>(p + head);
+  for (uint64_t k = i; k < words; k += stride) w[k] = 0;
+  if (i < head) p[i] = 0;
+  if (i < n - tail0) p[tail0 + i] = 0;
+}
+
 }  // namespace qm
 
 // ---- host launchers ----
@@ -596,7 +615,7 @@ extern "C" hipError_t qmk_scan_agg(const qm::DScan *sc, const qm::DQuery *d_quer
                                    int block, size_t lds_bytes, hipStream_t st) {
   const void *fn = sc->join_mode ? (const void *)qm::scan_agg_kernel<true> : (const void *)qm::scan_agg_kernel<false>;
   if (lds_bytes > 65536) {
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    hipError_t e = qmk_func_lds(fn, lds_bytes);
     if (e != hipSuccess) return e;
   }
   qm::DScan a = *sc;
@@ -611,4 +630,24 @@ extern "C" hipError_t qmk_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStrea
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(qm::fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_zero_bytes(void *p, uint64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n / 8 + 255) / 256;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(qm::zero_bytes_kernel, dim3((unsigned)blocks), dim3(256), 0, st, static_cast<uint8_t *>(p), n);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_func_lds(const void *fn, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void *, size_t> set;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = set.find(fn);
+  if (it != set.end() && it->second >= bytes) return hipSuccess;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) set[fn] = bytes;
+  return e;
 }

@@ -208,8 +208,7 @@ extern "C" hipError_t qmk_topk_scan(const DScan *sc, const DQuery *d_q, int sort
   uint32_t cap = qmk_topk_cap(k);
   if (!cap || k == 0) return hipErrorInvalidValue;
   if ((size_t)cap * sizeof(TopKey) > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)topk_scan_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(cap * sizeof(TopKey)));
+    hipError_t e = qmk_func_lds((const void *)topk_scan_kernel, (size_t)cap * sizeof(TopKey));
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(topk_scan_kernel, dim3(grid), dim3(TK_BLOCK), (size_t)cap * sizeof(TopKey), st, *sc, d_q,
@@ -226,8 +225,7 @@ extern "C" hipError_t qmk_topk_merge(const TopKey *cand, const uint32_t *cand_co
   uint32_t grid = (nlists + lists_per_block - 1) / lists_per_block;
   if (grid == 0) grid = 1;
   if ((size_t)cap * sizeof(TopKey) > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void *)topk_merge_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(cap * sizeof(TopKey)));
+    hipError_t e = qmk_func_lds((const void *)topk_merge_kernel, (size_t)cap * sizeof(TopKey));
     if (e != hipSuccess) return e;
   }
   hipLaunchKernelGGL(topk_merge_kernel, dim3(grid), dim3(TK_BLOCK), (size_t)cap * sizeof(TopKey), st, cand,

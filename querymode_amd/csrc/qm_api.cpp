@@ -52,9 +52,51 @@ size_t pool_round(size_t bytes) {
 }
 }  // namespace qm
 
+namespace qm {
+// the lane the calling thread holds, per context (set by LaneHold)
+thread_local qm_ctx *tl_ctx = nullptr;
+thread_local qm_lane *tl_lane = nullptr;
+}  // namespace qm
+
+qm_lane *qm_ctx::held_lane() { return qm::tl_ctx == this ? qm::tl_lane : nullptr; }
+qm_lane &qm_ctx::lane() {
+  qm_lane *l = held_lane();
+  return l ? *l : lanes[0];
+}
+
 void *qm_ctx::alloc(size_t bytes) {
+  qm_lane *L = held_lane();
+  if (L && L->cap && L->cap->mode == LANE_CAPTURE) {  // no allocator call inside a stream capture
+    auto it = L->cap->free.find(bytes);
+    if (it == L->cap->free.end())
+      fail(QM_ERR_QUERY_FAILED, "prepared batch: the captured run needs a %zu-byte buffer the recorded run did not",
+           bytes);
+    void *p = it->second;
+    L->cap->free.erase(it);
+    return p;
+  }
+  void *p = alloc_shared(bytes, L);
+  if (L && L->cap && L->cap->mode == LANE_RECORD) {
+    qm_capture &k = *L->cap;
+    k.rec_live[p] = bytes;
+    k.peak[bytes] = std::max(k.peak[bytes], ++k.live[bytes]);
+  }
+  return p;
+}
+
+void *qm_ctx::alloc_shared(size_t bytes, qm_lane *L) {
   {
     std::lock_guard<std::mutex> g(pool_mu);
+    if (L) {  // a buffer this call released: its stream orders the reuse
+      auto it = L->held.find(bytes);
+      if (it != L->held.end()) {
+        void *p = it->second;
+        L->held.erase(it);
+        L->held_bytes -= bytes;
+        pooled -= bytes;
+        return p;
+      }
+    }
     auto it = pool.find(bytes);
     if (it != pool.end()) {
       void *p = it->second;
@@ -83,20 +125,105 @@ void *qm_ctx::alloc(size_t bytes) {
 }
 
 void qm_ctx::release(void *p, size_t bytes) {
+  qm_lane *L = held_lane();
+  if (L && L->cap && L->cap->mode == LANE_CAPTURE) {  // back to the graph's private pool
+    L->cap->free.emplace(bytes, p);
+    return;
+  }
+  if (L && L->cap && L->cap->mode == LANE_RECORD) {
+    auto it = L->cap->rec_live.find(p);
+    if (it != L->cap->rec_live.end()) {
+      --L->cap->live[it->second];
+      L->cap->rec_live.erase(it);
+    }
+  }
   std::lock_guard<std::mutex> g(pool_mu);
-  pool.emplace(bytes, p);
+  if (L) {  // kernels of this call may still use it: other lanes only get it once the stream drained
+    L->held.emplace(bytes, p);
+    L->held_bytes += bytes;
+  } else {
+    pool.emplace(bytes, p);
+  }
   pooled += bytes;
 }
 
+// Frees the shared pool and, when the caller holds a lane, that lane's own
+// released buffers once its stream has drained (hipFree of memory a pending
+// kernel uses is never issued).
 void qm_ctx::trim_pool() {
+  qm_lane *L = held_lane();
+  if (L && L->cap && L->cap->mode == LANE_CAPTURE) return;  // no frees (or waits) inside a stream capture
+  if (L && !L->held.empty()) (void)hipStreamSynchronize(L->stream);
   std::lock_guard<std::mutex> g(pool_mu);
   for (auto &kv : pool) {
     (void)hipFree(kv.second);
     used -= kv.first;
+    pooled -= kv.first;
   }
   pool.clear();
-  pooled = 0;
+  if (L) {
+    for (auto &kv : L->held) {
+      (void)hipFree(kv.second);
+      used -= kv.first;
+    }
+    pooled -= L->held_bytes;
+    L->held.clear();
+    L->held_bytes = 0;
+  }
 }
+
+namespace qm {
+// Holds one lane of `ctx` for the calling thread until destroyed; executions
+// hold the table lock shared, table upload / drop and knob changes exclusive
+// (then no other call runs). On release the lane's stream is drained and the
+// buffers the call released go back to the context pool. A call made while
+// the thread already holds a lane of the same context keeps that lane.
+struct LaneHold {
+  qm_ctx *ctx;
+  qm_lane *lane = nullptr;
+  qm_ctx *prev_ctx;
+  qm_lane *prev_lane;
+  std::shared_lock<std::shared_mutex> rd;
+  std::unique_lock<std::shared_mutex> wr;
+  LaneHold(qm_ctx *c, bool exclusive = false) : ctx(c), prev_ctx(tl_ctx), prev_lane(tl_lane) {
+    if (tl_ctx == c) return;  // nested: the outer hold covers it
+    if (exclusive) wr = std::unique_lock<std::shared_mutex>(c->tables_mu);
+    else rd = std::shared_lock<std::shared_mutex>(c->tables_mu);
+    {
+      std::unique_lock<std::mutex> g(c->lanes_mu);
+      for (;;) {
+        for (qm_lane &l : c->lanes)
+          if (!l.busy) { lane = &l; break; }
+        if (lane) break;
+        c->lanes_cv.wait(g);
+      }
+      lane->busy = true;
+    }
+    tl_ctx = c;
+    tl_lane = lane;
+    if (hipSetDevice(c->device) != hipSuccess) (void)hipGetLastError();
+  }
+  ~LaneHold() {
+    if (!lane) return;
+    (void)hipStreamSynchronize(lane->stream);
+    {
+      std::lock_guard<std::mutex> g(ctx->pool_mu);
+      for (auto &kv : lane->held) ctx->pool.emplace(kv.first, kv.second);
+      lane->held.clear();
+      lane->held_bytes = 0;
+    }
+    tl_ctx = prev_ctx;
+    tl_lane = prev_lane;
+    {
+      std::lock_guard<std::mutex> g(ctx->lanes_mu);
+      lane->busy = false;
+    }
+    ctx->lanes_cv.notify_one();
+  }
+  LaneHold(const LaneHold &) = delete;
+  LaneHold &operator=(const LaneHold &) = delete;
+};
+}  // namespace qm
 
 // ===========================================================================
 // helpers
@@ -129,30 +256,90 @@ double hll_estimate(const uint32_t *reg) {
   return e;
 }
 
-// Device-time bracket on the context stream.
+qm_capture *lane_cap(qm_ctx *c) { return c->lane().cap; }
+bool capturing(qm_ctx *c) {
+  const qm_capture *k = lane_cap(c);
+  return k && k->mode == LANE_CAPTURE;
+}
+
+// Stream waits and event timing; no-ops while the lane's stream is being
+// captured (a graph has no host to wait for).
+void stream_sync(qm_ctx *c) {
+  if (!capturing(c)) HIPX(hipStreamSynchronize(c->lane().stream));
+}
+void ev_record(qm_ctx *c, hipEvent_t e) {
+  if (!capturing(c)) HIPX(hipEventRecord(e, c->lane().stream));
+}
+void ev_sync(qm_ctx *c, hipEvent_t e) {
+  if (!capturing(c)) HIPX(hipEventSynchronize(e));
+}
+void ev_ms(qm_ctx *c, float *ms, hipEvent_t a, hipEvent_t b) {
+  *ms = 0;
+  if (!capturing(c)) HIPX(hipEventElapsedTime(ms, a, b));
+}
+
+// Device-time bracket on the lane's stream.
 struct Timer {
   qm_ctx *c;
-  explicit Timer(qm_ctx *ctx) : c(ctx) { HIPX(hipEventRecord(c->ev0, c->stream)); }
+  explicit Timer(qm_ctx *ctx) : c(ctx) { ev_record(c, c->lane().ev0); }
   double stop() {
-    HIPX(hipEventRecord(c->ev1, c->stream));
-    HIPX(hipEventSynchronize(c->ev1));
+    if (capturing(c)) return 0;
+    HIPX(hipEventRecord(c->lane().ev1, c->lane().stream));
+    HIPX(hipEventSynchronize(c->lane().ev1));
     float ms = 0;
-    HIPX(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    HIPX(hipEventElapsedTime(&ms, c->lane().ev0, c->lane().ev1));
     return ms;
   }
 };
 
-template <class T> void h2d(qm_ctx *c, void *dst, const T *src, size_t n) {
-  if (n) HIPX(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+void h2d_bytes(qm_ctx *c, void *dst, const void *src, size_t bytes) {
+  if (!bytes) return;
+  qm_capture *k = lane_cap(c);
+  if (k && k->mode == LANE_CAPTURE) {
+    // the graph's copy node reads pinned memory that lives as long as the graph
+    const size_t off = (k->staging_used + 255) & ~(size_t)255;
+    if (off + bytes > k->staging_cap) fail(QM_ERR_QUERY_FAILED, "prepared batch: upload staging exhausted");
+    memcpy(k->staging + off, src, bytes);
+    k->staging_used = off + bytes;
+    HIPX(hipMemcpyAsync(dst, k->staging + off, bytes, hipMemcpyHostToDevice, c->lane().stream));
+    return;
+  }
+  if (k) k->h2d_bytes += ((bytes + 255) & ~(size_t)255) + 256;
+  HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->lane().stream));
 }
-template <class T> void d2h(qm_ctx *c, T *dst, const void *src, size_t n) {
-  if (n) {
-    HIPX(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-    HIPX(hipStreamSynchronize(c->stream));
+template <class T> void h2d(qm_ctx *c, void *dst, const T *src, size_t n) { h2d_bytes(c, dst, src, n * sizeof(T)); }
+
+// Device-to-host readback of `rows` rows of `width` bytes, `pitch` apart on the
+// device (pitch == width: one contiguous block). Recorded on the tape when
+// recording, taken from it when capturing.
+void d2h_rows(qm_ctx *c, void *dst, const void *src, size_t width, size_t pitch, size_t rows) {
+  const size_t bytes = width * rows;
+  if (!bytes) return;
+  qm_capture *k = lane_cap(c);
+  if (k && k->mode == LANE_CAPTURE) {
+    if (k->pos >= k->tape.size() || k->tape[k->pos].size() != bytes)
+      fail(QM_ERR_QUERY_FAILED, "prepared batch: the captured run diverged from the recorded one");
+    memcpy(dst, k->tape[k->pos].data(), bytes);
+    ++k->pos;
+    return;
+  }
+  if (pitch == width)
+    HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->lane().stream));
+  else
+    HIPX(hipMemcpy2DAsync(dst, width, src, pitch, width, rows, hipMemcpyDeviceToHost, c->lane().stream));
+  HIPX(hipStreamSynchronize(c->lane().stream));
+  if (k && k->mode == LANE_RECORD) {
+    const uint8_t *b = static_cast<const uint8_t *>(dst);
+    k->tape.emplace_back(b, b + bytes);
   }
 }
+template <class T> void d2h(qm_ctx *c, T *dst, const void *src, size_t n) {
+  d2h_rows(c, dst, src, n * sizeof(T), n * sizeof(T), 1);
+}
 void zero(qm_ctx *c, void *p, size_t bytes) {
-  if (bytes) HIPX(hipMemsetAsync(p, 0, bytes, c->stream));
+  if (!bytes) return;
+  if (capturing(c)) HIPX(qmk_zero_bytes(p, bytes, c->lane().stream));  // no memset nodes in graphs
+  else HIPX(hipMemsetAsync(p, 0, bytes, c->lane().stream));
 }
 
 int qmcb_tag(int dt) {
@@ -772,7 +959,7 @@ void permute_result(qm_result *res, const uint64_t *d_rows, uint64_t n) {
   for (OutCol &o : res->cols) {
     OutCol x = make_out(ctx, o.name, o.dtype, n, o.nulls.p != nullptr);
     DCol c = dcol_of(o);
-    HIPX(qmk_gather(&c, d_rows, nullptr, n, o.dtype, x.data.p, x.nulls.as<uint32_t>(), ctx->stream));
+    HIPX(qmk_gather(&c, d_rows, nullptr, n, o.dtype, x.data.p, x.nulls.as<uint32_t>(), ctx->lane().stream));
     nc.push_back(std::move(x));
   }
   res->cols = std::move(nc);
@@ -803,9 +990,9 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
     if (grid < 1) grid = 1;
     DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
     DevBuf counts(ctx, (size_t)grid * 4 + 64), gthr(ctx, 64);
-    HIPX(qmk_fill_u64(gthr.as<uint64_t>(), 1, ~0ull, ctx->stream));
+    HIPX(qmk_fill_u64(gthr.as<uint64_t>(), 1, ~0ull, ctx->lane().stream));
     HIPX(qmk_topk_scan(&sc, d_noq, 0, desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(), counts.as<uint32_t>(),
-                       gthr.as<uint64_t>(), ctx->stream));
+                       gthr.as<uint64_t>(), ctx->lane().stream));
     ++*launches;
     const TopKey *lists = cand.as<TopKey>();
     const uint32_t *lcounts = counts.as<uint32_t>();
@@ -815,7 +1002,7 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
       uint32_t g2 = (nlists + 15) / 16;
       m1 = DevBuf(ctx, (size_t)g2 * k * sizeof(TopKey));
       m1c = DevBuf(ctx, (size_t)g2 * 4 + 64);
-      HIPX(qmk_topk_merge(lists, lcounts, nlists, 16, (uint32_t)k, m1.as<TopKey>(), m1c.as<uint32_t>(), ctx->stream));
+      HIPX(qmk_topk_merge(lists, lcounts, nlists, 16, (uint32_t)k, m1.as<TopKey>(), m1c.as<uint32_t>(), ctx->lane().stream));
       ++*launches;
       lists = m1.as<TopKey>();
       lcounts = m1c.as<uint32_t>();
@@ -824,9 +1011,9 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
     DevBuf fin(ctx, (size_t)k * sizeof(TopKey));
     DevBuf finc(ctx, 64);
     HIPX(qmk_topk_merge(lists, lcounts, nlists, nlists, (uint32_t)k, fin.as<TopKey>(), finc.as<uint32_t>(),
-                        ctx->stream));
+                        ctx->lane().stream));
     HIPX(qmk_topk_rows(fin.as<TopKey>(), finc.as<uint32_t>(), (uint32_t)start, rows.as<uint64_t>(), (uint32_t)take,
-                       ctx->stream));
+                       ctx->lane().stream));
     *launches += 2;
     return rows;
   }
@@ -834,9 +1021,9 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
   size_t tb = qmk_sort_tmp_bytes(n);
   DevBuf tmp(ctx, tb);
   DevBuf all(ctx, n * 8);
-  HIPX(qmk_sort_keys(&c, n, desc ? 1 : 0, all.as<uint64_t>(), tmp.p, tmp.n, ctx->stream));
+  HIPX(qmk_sort_keys(&c, n, desc ? 1 : 0, all.as<uint64_t>(), tmp.p, tmp.n, ctx->lane().stream));
   *launches += 4;
-  HIPX(hipMemcpyAsync(rows.p, all.as<uint64_t>() + start, take * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPX(hipMemcpyAsync(rows.p, all.as<uint64_t>() + start, take * 8, hipMemcpyDeviceToDevice, ctx->lane().stream));
   return rows;
 }
 
@@ -923,7 +1110,7 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
     for (size_t i = 0; i < rcs.cols.size(); ++i) sc.cols[i] = rcs.cols[i];
     *keep = select_rows(ctx, sc, dq.as<DQuery>(), ~0ull, &nbuild, launches);
     sel = keep->as<uint64_t>();
-    HIPX(hipStreamSynchronize(ctx->stream));
+    stream_sync(ctx);
   }
   ji.build_rows = nbuild;
   DCol key;
@@ -948,7 +1135,7 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
     DevBuf status(ctx, 64);
     zero(ctx, status.p, 64);
     HIPX(qmk_join_claim(&key, nbuild, sel, ji.kmin, ji.cap, ji.unique.as<uint32_t>(), status.as<uint32_t>(),
-                        ctx->stream));
+                        ctx->lane().stream));
     *launches += 1;
     uint32_t st[2];
     d2h(ctx, st, status.p, 2);
@@ -967,12 +1154,12 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
   zero(ctx, status.p, 64);
   HIPX(qmk_join_build(&key, nbuild, sel, ji.mode, ji.key_float, ji.kmin, ji.cap, ji.table.as<uint64_t>(),
                       ji.rows.as<uint32_t>(), ji.cursor.as<uint32_t>(), ji.tmp.as<uint64_t>(), status.as<uint32_t>(),
-                      ctx->stream));
+                      ctx->lane().stream));
   *launches += 8;
   if (ji.mode == 1 && ctx->tune.join_unique) {
     ji.unique = DevBuf(ctx, ji.cap * 4 + 64);
     HIPX(qmk_join_unique(ji.table.as<uint64_t>(), ji.cap, ji.rows.as<uint32_t>(), ji.unique.as<uint32_t>(),
-                         status.as<uint32_t>(), ctx->stream));
+                         status.as<uint32_t>(), ctx->lane().stream));
     *launches += 1;
   }
   uint32_t st[2];
@@ -1010,7 +1197,7 @@ void attach_join_payload(qm_ctx *ctx, const ColSpace &cs, const JoinIndex &ji, D
   }
   if (!jpay.p) {
     jpay = DevBuf(ctx, ji.cap * 16);
-    HIPX(qmk_join_payload(ji.unique.as<uint32_t>(), ji.cap, &jp, jpay.as<uint32_t>(), ctx->stream));
+    HIPX(qmk_join_payload(ji.unique.as<uint32_t>(), ji.cap, &jp, jpay.as<uint32_t>(), ctx->lane().stream));
     *launches += 1;
   }
   sc.join_pay = jpay.as<uint32_t>();
@@ -1035,14 +1222,14 @@ DevBuf select_rows(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, uint64_t lim
   uint64_t tpb = (ntiles + nblocks - 1) / nblocks;
   nblocks = (ntiles + tpb - 1) / tpb;
   DevBuf counts(ctx, nblocks * 8), offs(ctx, nblocks * 8), stmp(ctx, qmk_exscan_tmp_words(nblocks) * 8 + 64);
-  HIPX(qmk_select_count(&sc, d_q, nblocks, tpb, counts.as<uint64_t>(), ctx->stream));
+  HIPX(qmk_select_count(&sc, d_q, nblocks, tpb, counts.as<uint64_t>(), ctx->lane().stream));
   uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nblocks);
-  HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(), ctx->stream));
+  HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(), ctx->lane().stream));
   uint64_t total = 0;
   d2h(ctx, &total, total_d, 1);
   uint64_t n = std::min(total, limit_cap);
   DevBuf rows(ctx, std::max<uint64_t>(n, 1) * 8 + 64);
-  if (n) HIPX(qmk_select_write(&sc, d_q, nblocks, tpb, offs.as<uint64_t>(), n, rows.as<uint64_t>(), ctx->stream));
+  if (n) HIPX(qmk_select_write(&sc, d_q, nblocks, tpb, offs.as<uint64_t>(), n, rows.as<uint64_t>(), ctx->lane().stream));
   *launches += 5;
   *n_out = n;
   return rows;
@@ -1140,12 +1327,14 @@ extern "C" int qm_init(int device, uint64_t hbm_budget_bytes, qm_ctx **out) {
   c->name = prop.gcnArchName;
   for (const KnobDef &k : kKnobs)
     if (const char *e = getenv(k.env)) k.set(c->tune, std::min(k.hi, std::max(k.lo, atof(e))));
-  HIPX(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPX(hipEventCreate(&c->ev0));
-  HIPX(hipEventCreate(&c->ev1));
-  HIPX(hipEventCreate(&c->evk0));
-  HIPX(hipEventCreate(&c->evk1));
-  HIPX(hipEventCreate(&c->evk2));
+  for (qm_lane &l : c->lanes) {  // every stream at the default priority
+    HIPX(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+    HIPX(hipEventCreate(&l.ev0));
+    HIPX(hipEventCreate(&l.ev1));
+    HIPX(hipEventCreate(&l.evk0));
+    HIPX(hipEventCreate(&l.evk1));
+    HIPX(hipEventCreate(&l.evk2));
+  }
   {
     void *sp = nullptr;
     const hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
@@ -1160,14 +1349,19 @@ extern "C" int qm_shutdown(qm_ctx *ctx) {
   API_BEGIN
   if (!ctx) return QM_OK;
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
-  ctx->trim_pool();
-  (void)hipEventDestroy(ctx->ev0);
-  (void)hipEventDestroy(ctx->ev1);
-  (void)hipEventDestroy(ctx->evk0);
-  (void)hipEventDestroy(ctx->evk1);
-  (void)hipEventDestroy(ctx->evk2);
-  (void)hipStreamDestroy(ctx->stream);
+  {
+    std::unique_lock<std::shared_mutex> wr(ctx->tables_mu);  // no call runs past this point
+    for (qm_lane &l : ctx->lanes) (void)hipStreamSynchronize(l.stream);
+    ctx->trim_pool();
+  }
+  for (qm_lane &l : ctx->lanes) {
+    (void)hipEventDestroy(l.ev0);
+    (void)hipEventDestroy(l.ev1);
+    (void)hipEventDestroy(l.evk0);
+    (void)hipEventDestroy(l.evk1);
+    (void)hipEventDestroy(l.evk2);
+    (void)hipStreamDestroy(l.stream);
+  }
   delete ctx;
   API_END
 }
@@ -1203,7 +1397,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
   if (!ctx || !out || (ncols && !cols)) fail(QM_ERR_INVALID_ARG, "null argument");
   *out = nullptr;
   if (nrows >= (1ull << 40)) fail(QM_ERR_INVALID_ARG, "table too large");
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx, true);
   HIPX(hipSetDevice(ctx->device));
   std::unique_ptr<qm_table> t(new qm_table());
   t->ctx = ctx;
@@ -1239,7 +1433,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
     t->cols.push_back(c);
     TCol &tc = t->cols.back();
     zero(ctx, tc.d, pr * es);
-    if (nrows) HIPX(hipMemcpyAsync(tc.d, cd.data, nrows * es, hipMemcpyHostToDevice, ctx->stream));
+    if (nrows) HIPX(hipMemcpyAsync(tc.d, cd.data, nrows * es, hipMemcpyHostToDevice, ctx->lane().stream));
     bool any_null = false;
     if (cd.null_bitmap) {
       for (uint64_t b = 0; b < (nrows + 7) / 8 && !any_null; ++b) {
@@ -1254,8 +1448,8 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
       zero(ctx, tc.dnull, pr / 8 + 64);
       std::vector<uint8_t> nb(cd.null_bitmap, cd.null_bitmap + (nrows + 7) / 8);
       if (nrows & 7) nb.back() &= (uint8_t)((1u << (nrows & 7)) - 1);
-      HIPX(hipMemcpyAsync(tc.dnull, nb.data(), nb.size(), hipMemcpyHostToDevice, ctx->stream));
-      HIPX(hipStreamSynchronize(ctx->stream));
+      HIPX(hipMemcpyAsync(tc.dnull, nb.data(), nb.size(), hipMemcpyHostToDevice, ctx->lane().stream));
+      stream_sync(ctx);
     }
     // statistics (min/max, max |x|, NaN/NULL counts, HLL)
     zero(ctx, stats.p, 64);
@@ -1265,7 +1459,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
     dc.nulls = tc.dnull;
     dc.dtype = dt;
     dc.side = 0;
-    HIPX(qmk_col_stats(&dc, nrows, stats.as<uint64_t>(), hll.as<uint32_t>(), ctx->stream));
+    HIPX(qmk_col_stats(&dc, nrows, stats.as<uint64_t>(), hll.as<uint32_t>(), ctx->lane().stream));
     uint64_t sw[8];
     std::vector<uint32_t> regs(4096);
     d2h(ctx, sw, stats.p, 8);
@@ -1273,7 +1467,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
     if (nrows) {
       const uint64_t pages = (nrows + ZONE_ROWS - 1) / ZONE_ROWS;
       DevBuf zb(ctx, pages * 16 + 64);
-      HIPX(qmk_zone_maps(&dc, nrows, zb.as<uint64_t>(), zb.as<uint64_t>() + pages, ctx->stream));
+      HIPX(qmk_zone_maps(&dc, nrows, zb.as<uint64_t>(), zb.as<uint64_t>() + pages, ctx->lane().stream));
       tc.zmin.resize(pages);
       tc.zmax.resize(pages);
       d2h(ctx, tc.zmin.data(), zb.p, pages);
@@ -1307,7 +1501,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
       const uint64_t nbins = (D + (1ull << kg) - 1) >> kg;
       DevBuf hb(ctx, nbins * 4 + 64);
       zero(ctx, hb.p, nbins * 4);
-      HIPX(qmk_key_hist(&dc, nrows, s.imin, kg, (uint32_t)nbins, hb.as<uint32_t>(), ctx->stream));
+      HIPX(qmk_key_hist(&dc, nrows, s.imin, kg, (uint32_t)nbins, hb.as<uint32_t>(), ctx->lane().stream));
       tc.khist.resize(nbins);
       d2h(ctx, tc.khist.data(), hb.p, nbins);
       tc.khist_g = kg;
@@ -1351,7 +1545,7 @@ extern "C" int qm_table_create(qm_ctx *ctx, const char *name, uint64_t nrows, ui
       }
     }
   }
-  HIPX(hipStreamSynchronize(ctx->stream));
+  stream_sync(ctx);
   cleanup.armed = false;
   *out = t.release();
   API_END
@@ -1361,8 +1555,9 @@ extern "C" int qm_table_drop(qm_table *t) {
   API_BEGIN
   if (!t) return QM_OK;
   qm_ctx *ctx = t->ctx;
-  std::lock_guard<std::mutex> g(ctx->mu);
-  HIPX(hipStreamSynchronize(ctx->stream));
+  LaneHold hold(ctx, true);
+  if (t->prepared.load()) fail(QM_ERR_INVALID_ARG, "table %s: close its prepared batches first", t->name.c_str());
+  stream_sync(ctx);
   for (auto &c : t->cols) {
     if (c.d) ctx->release(c.d, c.bytes);
     if (c.dnull) ctx->release(c.dnull, c.null_bytes);
@@ -1471,13 +1666,13 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   pp.blk_cap = blk_cap;
   pp.blk_count = bcount.as<uint64_t>();
   pp.hist = hist.as<uint64_t>();
-  HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-  HIPX(qmk_part_scan(&sc, d_q, &pp, (uint32_t)grid, tpb, fast_code, 1, ctx->stream));
-  HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+  ev_record(ctx, ctx->lane().evk0);
+  HIPX(qmk_part_scan(&sc, d_q, &pp, (uint32_t)grid, tpb, fast_code, 1, ctx->lane().stream));
+  ev_record(ctx, ctx->lane().evk1);
   DevBuf poff(ctx, P1 * grid * 8), stmp(ctx, qmk_exscan_tmp_words(P1 * grid) * 8 + 64);
   uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(P1 * grid);
   HIPX(qmk_exscan_u64(hist.as<uint64_t>(), P1 * grid, poff.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
-                      ctx->stream));
+                      ctx->lane().stream));
   uint64_t total = 0;
   d2h(ctx, &total, total_d, 1);
   h2d(ctx, d_selected, &total, 1);
@@ -1486,9 +1681,9 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   pp.precs = precs.as<uint64_t>();
   // one pass-1 region per scatter workgroup by default (every CU busy)
   const uint32_t rpb = ctx->tune.scatter_rpb ? (uint32_t)ctx->tune.scatter_rpb : 1u;
-  HIPX(qmk_part_scatter(d_q, &pp, (uint32_t)grid, rpb, q.entry_words, ctx->stream));
+  HIPX(qmk_part_scatter(d_q, &pp, (uint32_t)grid, rpb, q.entry_words, ctx->lane().stream));
   DevBuf starts(ctx, (P1 + 1) * 8);
-  HIPX(qmk_part_starts(poff.as<uint64_t>(), (uint32_t)P1, (uint32_t)grid, total, starts.as<uint64_t>(), ctx->stream));
+  HIPX(qmk_part_starts(poff.as<uint64_t>(), (uint32_t)P1, (uint32_t)grid, total, starts.as<uint64_t>(), ctx->lane().stream));
   std::vector<uint64_t> st(P1 + 1);
   d2h(ctx, st.data(), starts.p, P1 + 1);
   // segments: split partitions larger than seg_max across workgroups
@@ -1511,8 +1706,8 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
   h2d(ctx, segbuf.p, seg.data(), seg.size());
   pp.seg = segbuf.as<uint64_t>();
   pp.inserted = q.selected + 1;
-  HIPX(qmk_part_agg(d_q, &pp, (uint32_t)nseg, (uint32_t)ctx->num_cus, q.entry_words, ctx->stream));
-  HIPX(hipStreamSynchronize(ctx->stream));
+  HIPX(qmk_part_agg(d_q, &pp, (uint32_t)nseg, (uint32_t)ctx->num_cus, q.entry_words, ctx->lane().stream));
+  stream_sync(ctx);
   return 6;
 }
 
@@ -1583,9 +1778,9 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
     if (T->rows && nq == 1 && ab[0].part) {
       launches += run_partitioned(ctx, sc, dq.as<DQuery>(), ab[0], T->rows, selected[0].as<uint64_t>());
     } else if (T->rows) {
-      HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-      HIPX(qmk_scan_agg(&sc, dq.as<DQuery>(), grid, block, lds_bytes, ctx->stream));
-      HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+      ev_record(ctx, ctx->lane().evk0);
+      HIPX(qmk_scan_agg(&sc, dq.as<DQuery>(), grid, block, lds_bytes, ctx->lane().stream));
+      ev_record(ctx, ctx->lane().evk1);
       ++launches;
     }
     // overflow check
@@ -1614,9 +1809,9 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       f.status = status[i].as<uint32_t>();
       uint64_t nb = qmk_compact_blocks(f.nslots);
       DevBuf cnt(ctx, nb * 8), offs(ctx, nb * 8), stmp(ctx, qmk_exscan_tmp_words(nb) * 8 + 64);
-      HIPX(qmk_compact_count(&f, cnt.as<uint64_t>(), ctx->stream));
+      HIPX(qmk_compact_count(&f, cnt.as<uint64_t>(), ctx->lane().stream));
       uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nb);
-      HIPX(qmk_exscan_u64(cnt.as<uint64_t>(), nb, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(), ctx->stream));
+      HIPX(qmk_exscan_u64(cnt.as<uint64_t>(), nb, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(), ctx->lane().stream));
       uint64_t total = 0;
       d2h(ctx, &total, total_d, 1);
       qm_result *res = new qm_result();
@@ -1633,7 +1828,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
         f.outs[k].nulls = oc.nulls.as<uint32_t>();
         res->cols.push_back(std::move(oc));
       }
-      if (total) HIPX(qmk_compact_write(&f, offs.as<uint64_t>(), ctx->stream));
+      if (total) HIPX(qmk_compact_write(&f, offs.as<uint64_t>(), ctx->lane().stream));
       launches += 4;
       uint32_t st[2];
       d2h(ctx, st, status[i].p, 2);
@@ -1653,7 +1848,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
     }
     double ms = tm.stop();
     float scan_ms = 0;
-    if (T->rows) HIPX(hipEventElapsedTime(&scan_ms, ctx->evk0, ctx->evk1));
+    if (T->rows) ev_ms(ctx, &scan_ms, ctx->lane().evk0, ctx->lane().evk1);
     uint64_t rrows = fused_join ? ji->build_rows : 0;
     for (size_t i = 0; i < nq; ++i) {
       qm_result *res = results[i];
@@ -1724,10 +1919,10 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
     uint64_t tpb = (ntiles + nblocks - 1) / nblocks;
     nblocks = (ntiles + tpb - 1) / tpb;
     DevBuf counts(ctx, nblocks * 8), offs(ctx, nblocks * 8), stmp(ctx, qmk_exscan_tmp_words(nblocks) * 8 + 64);
-    HIPX(qmk_join_count(&sc, dq.as<DQuery>(), nblocks, tpb, counts.as<uint64_t>(), ctx->stream));
+    HIPX(qmk_join_count(&sc, dq.as<DQuery>(), nblocks, tpb, counts.as<uint64_t>(), ctx->lane().stream));
     uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nblocks);
     HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
-                        ctx->stream));
+                        ctx->lane().stream));
     uint64_t total = 0;
     d2h(ctx, &total, total_d, 1);
     n = total;
@@ -1736,7 +1931,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
     right_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
     if (n)
       HIPX(qmk_join_write(&sc, dq.as<DQuery>(), nblocks, tpb, offs.as<uint64_t>(), n, left_rows.as<uint64_t>(),
-                          right_rows.as<uint64_t>(), ctx->stream));
+                          right_rows.as<uint64_t>(), ctx->lane().stream));
     launches += 5;
     res->stats.rows_scanned = T->rows + R->rows;
   } else {
@@ -1758,11 +1953,11 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
       if (k > 0) {
         DevBuf cand(ctx, (size_t)grid * k * sizeof(TopKey));
         DevBuf ccount(ctx, (size_t)grid * 4 + 64), gthr(ctx, 64);
-        HIPX(qmk_fill_u64(gthr.as<uint64_t>(), 1, ~0ull, ctx->stream));
-        HIPX(hipEventRecord(ctx->evk0, ctx->stream));
+        HIPX(qmk_fill_u64(gthr.as<uint64_t>(), 1, ~0ull, ctx->lane().stream));
+        ev_record(ctx, ctx->lane().evk0);
         HIPX(qmk_topk_scan(&sc, dq.as<DQuery>(), sslot, pl.desc ? 1 : 0, (uint32_t)k, grid, cand.as<TopKey>(),
-                           ccount.as<uint32_t>(), gthr.as<uint64_t>(), ctx->stream));
-        HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+                           ccount.as<uint32_t>(), gthr.as<uint64_t>(), ctx->lane().stream));
+        ev_record(ctx, ctx->lane().evk1);
         timed_scan = true;
         ++launches;
         const TopKey *lists = cand.as<TopKey>();
@@ -1773,20 +1968,20 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
           uint32_t g2 = (nl + 15) / 16;
           m1 = DevBuf(ctx, (size_t)g2 * k * sizeof(TopKey));
           m1c = DevBuf(ctx, (size_t)g2 * 4 + 64);
-          HIPX(qmk_topk_merge(lists, lc, nl, 16, (uint32_t)k, m1.as<TopKey>(), m1c.as<uint32_t>(), ctx->stream));
+          HIPX(qmk_topk_merge(lists, lc, nl, 16, (uint32_t)k, m1.as<TopKey>(), m1c.as<uint32_t>(), ctx->lane().stream));
           ++launches;
           lists = m1.as<TopKey>();
           lc = m1c.as<uint32_t>();
           nl = g2;
         }
         DevBuf fin(ctx, (size_t)k * sizeof(TopKey)), finc(ctx, 64);
-        HIPX(qmk_topk_merge(lists, lc, nl, nl, (uint32_t)k, fin.as<TopKey>(), finc.as<uint32_t>(), ctx->stream));
+        HIPX(qmk_topk_merge(lists, lc, nl, nl, (uint32_t)k, fin.as<TopKey>(), finc.as<uint32_t>(), ctx->lane().stream));
         uint32_t fc = 0;
         d2h(ctx, &fc, finc.p, 1);
         take = fc > (uint64_t)pl.offset ? fc - (uint64_t)pl.offset : 0;
         left_rows = DevBuf(ctx, std::max<uint64_t>(take, 1) * 8);
         HIPX(qmk_topk_rows(fin.as<TopKey>(), finc.as<uint32_t>(), (uint32_t)pl.offset, left_rows.as<uint64_t>(),
-                           (uint32_t)take, ctx->stream));
+                           (uint32_t)take, ctx->lane().stream));
         launches += 2;
       } else {
         left_rows = DevBuf(ctx, 8);
@@ -1805,7 +2000,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
     int odt = out_dt_of(c.dtype);
     OutCol oc = make_out(ctx, pc.first, odt, n, c.nulls != nullptr);
     const uint64_t *rows = c.side ? right_rows.as<uint64_t>() : left_rows.as<uint64_t>();
-    HIPX(qmk_gather(&c, rows, nullptr, n, odt, oc.data.p, oc.nulls.as<uint32_t>(), ctx->stream));
+    HIPX(qmk_gather(&c, rows, nullptr, n, odt, oc.data.p, oc.nulls.as<uint32_t>(), ctx->lane().stream));
     ++launches;
     res->cols.push_back(std::move(oc));
   }
@@ -1827,7 +2022,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
       const DCol &c = cs.cols[s];
       OutCol oc = make_out(ctx, "\x01sort", out_dt_of(c.dtype), n, c.nulls != nullptr);
       const uint64_t *rows = c.side ? right_rows.as<uint64_t>() : left_rows.as<uint64_t>();
-      HIPX(qmk_gather(&c, rows, nullptr, n, oc.dtype, oc.data.p, oc.nulls.as<uint32_t>(), ctx->stream));
+      HIPX(qmk_gather(&c, rows, nullptr, n, oc.dtype, oc.data.p, oc.nulls.as<uint32_t>(), ctx->lane().stream));
       res->cols.push_back(std::move(oc));
       p2.sort_col = "\x01sort";
       sort_limit_result(res, p2, dnoq.as<DQuery>(), &launches);
@@ -1839,7 +2034,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   res->stats.device_ms = tm.stop();
   if (timed_scan) {
     float sm = 0;
-    HIPX(hipEventElapsedTime(&sm, ctx->evk0, ctx->evk1));
+    ev_ms(ctx, &sm, ctx->lane().evk0, ctx->lane().evk1);
     res->stats.scan_ms = sm;
   }
   res->stats.groups = res->rows;
@@ -2082,9 +2277,9 @@ void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_res
     uint64_t nb = qmk_compact_blocks(f[i].nslots);
     cnt[i] = DevBuf(ctx, nb * 8);
     offs[i] = DevBuf(ctx, nb * 8);
-    HIPX(qmk_compact_count(&f[i], cnt[i].as<uint64_t>(), ctx->stream));
+    HIPX(qmk_compact_count(&f[i], cnt[i].as<uint64_t>(), ctx->lane().stream));
     HIPX(qmk_exscan_u64(cnt[i].as<uint64_t>(), nb, offs[i].as<uint64_t>(), totals.as<uint64_t>() + i,
-                        stmp.as<uint64_t>(), ctx->stream));
+                        stmp.as<uint64_t>(), ctx->lane().stream));
   }
   // small tables: outputs sized for every slot, one compaction launch
   for (size_t i : small) {
@@ -2108,7 +2303,7 @@ void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_res
     for (size_t i : small) fs.push_back(f[i]);
     h2d(ctx, dsmall.p, reinterpret_cast<const uint8_t *>(fs.data()), fs.size() * sizeof(DFinal));
     DevBuf scount(ctx, small.size() * 8 + 64);
-    HIPX(qmk_compact_small(dsmall.as<DFinal>(), (uint32_t)small.size(), scount.as<uint64_t>(), ctx->stream));
+    HIPX(qmk_compact_small(dsmall.as<DFinal>(), (uint32_t)small.size(), scount.as<uint64_t>(), ctx->lane().stream));
     std::vector<uint64_t> sc(small.size());
     d2h(ctx, sc.data(), scount.p, small.size());
     for (size_t j = 0; j < small.size(); ++j) out[items[small[j]]->idx]->rows = sc[j];
@@ -2130,7 +2325,7 @@ void finalize_batch(qm_ctx *ctx, std::vector<BItem *> &items, std::vector<qm_res
       f[i].outs[k].nulls = oc.nulls.as<uint32_t>();
       res->cols.push_back(std::move(oc));
     }
-    if (tot[i]) HIPX(qmk_compact_write(&f[i], offs[i].as<uint64_t>(), ctx->stream));
+    if (tot[i]) HIPX(qmk_compact_write(&f[i], offs[i].as<uint64_t>(), ctx->lane().stream));
   }
   std::vector<uint32_t> st(2 * n);
   d2h(ctx, st.data(), status.p, 2 * n);
@@ -2336,15 +2531,15 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
 
   std::vector<uint64_t> tot(n + 1, 0);  // per query, then all staged records
   if (rows) {
-    HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-    HIPX(qmk_runs_count(&rs, ctx->stream));
-    HIPX(qmk_runs_offsets(rs.counts, nchunks, (uint32_t)n, offs.as<uint64_t>(), totals.as<uint64_t>(), ctx->stream));
+    ev_record(ctx, ctx->lane().evk0);
+    HIPX(qmk_runs_count(&rs, ctx->lane().stream));
+    HIPX(qmk_runs_offsets(rs.counts, nchunks, (uint32_t)n, offs.as<uint64_t>(), totals.as<uint64_t>(), ctx->lane().stream));
     HIPX(qmk_exscan_u64(cgroups.as<uint64_t>(), nseg, sbase.as<uint64_t>(), totals.as<uint64_t>() + n,
-                        stmp.as<uint64_t>(), ctx->stream));
-    HIPX(hipEventRecord(ctx->evk1, ctx->stream));
+                        stmp.as<uint64_t>(), ctx->lane().stream));
+    ev_record(ctx, ctx->lane().evk1);
     d2h(ctx, tot.data(), totals.p, n + 1);
     float ms = 0;
-    HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    ev_ms(ctx, &ms, ctx->lane().evk0, ctx->lane().evk1);
     scan_ms = std::max(scan_ms, ms);
   }
   // stg_base[nseg] = all records (the exclusive scan leaves it to the total)
@@ -2441,14 +2636,14 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
     rs.stg_pos = spos.as<uint32_t>();
     rs.stg_total = all;
     for (int j = 0; j < rs.nstates; ++j) rs.stg_val[j] = sval[j].p;
-    HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-    HIPX(qmk_runs_write(&rs, (uint32_t)ctx->tune.runs_lds_pad, ctx->stream));
-    HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-    HIPX(qmk_runs_place(&rs, ctx->tune.runs_win, ctx->stream));
-    HIPX(hipEventRecord(ctx->evk2, ctx->stream));
+    ev_record(ctx, ctx->lane().evk0);
+    HIPX(qmk_runs_write(&rs, (uint32_t)ctx->tune.runs_lds_pad, ctx->lane().stream));
+    ev_record(ctx, ctx->lane().evk1);
+    HIPX(qmk_runs_place(&rs, ctx->tune.runs_win, ctx->lane().stream));
+    ev_record(ctx, ctx->lane().evk2);
     d2h(ctx, sel.data(), selected.p, n);
     float mw = 0;
-    HIPX(hipEventElapsedTime(&mw, ctx->evk0, ctx->evk1));
+    ev_ms(ctx, &mw, ctx->lane().evk0, ctx->lane().evk1);
     scan_ms = std::max(scan_ms, mw);  // the place pass reads no table column
   }
   DQuery noq;
@@ -2473,7 +2668,8 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
 // 2^(sh1 - s) final ones. Cached on the key column; nullptr when no s fits.
 const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uint32_t nbk, int nsum, int nmax,
                               uint32_t rw) {
-  TCol *kt = const_cast<TCol *>(kt_c);  // the cache; callers hold ctx->mu
+  TCol *kt = const_cast<TCol *>(kt_c);  // the cache, guarded by ctx->cache_mu
+  std::lock_guard<std::mutex> cg(ctx->cache_mu);
   const uint32_t nsn = (uint32_t)(nsum + nmax);
   for (auto &g : kt->prgeo)
     if (g.nbk == nbk && g.nsn == nsn && g.rw == rw) return g.s >= 0 ? &g : nullptr;
@@ -2527,6 +2723,8 @@ const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uin
     h2d(ctx, g.start1->p, st1.data(), st1.size());
     h2d(ctx, g.start2->p, st2.data(), st2.size());
     h2d(ctx, g.chunk0->p, ch.data(), ch.size());
+    // calls on other lanes read these buffers once the entry is published
+    stream_sync(ctx);
   }
   kt->prgeo.push_back(g);
   return g.s >= 0 ? &kt->prgeo.back() : nullptr;
@@ -2657,17 +2855,17 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
   hipEvent_t ev[6];
   for (auto &e : ev) HIPX(hipEventCreate(&e));
   struct EvFree { hipEvent_t *e; ~EvFree() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); } } evfree{ev};
-  HIPX(hipEventRecord(ev[0], ctx->stream));
-  if (ntiles) HIPX(qmk_pr_scatter(&ps, 0, (uint32_t)std::min(ntiles, pgrid), ctx->stream));
-  HIPX(hipEventRecord(ev[1], ctx->stream));
-  if (g->nchunk2) HIPX(qmk_pr_scatter(&ps, 1, (uint32_t)std::min<uint64_t>(g->nchunk2, pgrid), ctx->stream));
-  HIPX(hipEventRecord(ev[2], ctx->stream));
-  HIPX(qmk_pr_count(&ps, ctx->stream));
+  ev_record(ctx, ev[0]);
+  if (ntiles) HIPX(qmk_pr_scatter(&ps, 0, (uint32_t)std::min(ntiles, pgrid), ctx->lane().stream));
+  ev_record(ctx, ev[1]);
+  if (g->nchunk2) HIPX(qmk_pr_scatter(&ps, 1, (uint32_t)std::min<uint64_t>(g->nchunk2, pgrid), ctx->lane().stream));
+  ev_record(ctx, ev[2]);
+  HIPX(qmk_pr_count(&ps, ctx->lane().stream));
   HIPX(qmk_exscan_u64(gcount.as<uint64_t>(), ncnt, gcount.as<uint64_t>(), gtotal.as<uint64_t>(),
-                      stmp.as<uint64_t>(), ctx->stream));
-  HIPX(hipEventRecord(ev[3], ctx->stream));
+                      stmp.as<uint64_t>(), ctx->lane().stream));
+  ev_record(ctx, ev[3]);
   std::vector<uint64_t> base(n + 1, 0);
-  HIPX(hipMemcpy2DAsync(base.data(), 8, gcount.p, (size_t)g->np * 8, 8, n, hipMemcpyDeviceToHost, ctx->stream));
+  d2h_rows(ctx, base.data(), gcount.p, 8, (size_t)g->np * 8, n);
   d2h(ctx, &base[n], gtotal.p, 1);
   std::vector<uint64_t> brow(R_MAX_BUCKETS);
   d2h(ctx, brow.data(), brows.p, R_MAX_BUCKETS);
@@ -2726,17 +2924,17 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
     ps.out_nbase[j] = nbase[j];
   }
   ps.row_off = droff.as<uint64_t>();
-  HIPX(hipEventRecord(ev[4], ctx->stream));
-  if (base[n]) HIPX(qmk_pr_runs(&ps, ctx->stream));
-  HIPX(hipEventRecord(ev[5], ctx->stream));
-  HIPX(hipEventSynchronize(ev[5]));
+  ev_record(ctx, ev[4]);
+  if (base[n]) HIPX(qmk_pr_runs(&ps, ctx->lane().stream));
+  ev_record(ctx, ev[5]);
+  ev_sync(ctx, ev[5]);
   d2h(ctx, &stv, status.p, 1);
   if (stv) fail(QM_ERR_QUERY_FAILED, "partitioned-runs batch: a key partition exceeded the LDS capacity (status %u)", stv);
   float ms[4] = {0, 0, 0, 0};
-  HIPX(hipEventElapsedTime(&ms[0], ev[0], ev[1]));
-  HIPX(hipEventElapsedTime(&ms[1], ev[1], ev[2]));
-  HIPX(hipEventElapsedTime(&ms[2], ev[2], ev[3]));
-  HIPX(hipEventElapsedTime(&ms[3], ev[4], ev[5]));
+  ev_ms(ctx, &ms[0], ev[0], ev[1]);
+  ev_ms(ctx, &ms[1], ev[1], ev[2]);
+  ev_ms(ctx, &ms[2], ev[2], ev[3]);
+  ev_ms(ctx, &ms[3], ev[4], ev[5]);
   for (int i = 0; i < 4; ++i) pass_ms[i] = ms[i];
   scan_ms = std::max(scan_ms, ms[0]);  // the pass reading the table
   DQuery noq;
@@ -2803,12 +3001,12 @@ void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms) 
   rpb = std::max<uint64_t>(S_BLOCK, (rpb + S_BLOCK - 1) / S_BLOCK * S_BLOCK);
   grid = (T->rows + rpb - 1) / rpb;
   ss.rows_per_block = rpb;
-  HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-  HIPX(qmk_shared_scan(&ss, (uint32_t)grid, ctx->stream));
-  HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-  HIPX(hipEventSynchronize(ctx->evk1));
+  ev_record(ctx, ctx->lane().evk0);
+  HIPX(qmk_shared_scan(&ss, (uint32_t)grid, ctx->lane().stream));
+  ev_record(ctx, ctx->lane().evk1);
+  ev_sync(ctx, ctx->lane().evk1);
   float ms = 0;
-  HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+  ev_ms(ctx, &ms, ctx->lane().evk0, ctx->lane().evk1);
   scan_ms = std::max(scan_ms, ms);
 }
 
@@ -3145,12 +3343,12 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
         size_t oi = std::find(order.begin(), order.end(), &it) - order.begin();
         q1.selected = selected.as<uint64_t>() + oi;
         if (T->rows) {
-          HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-          HIPX(qmk_sorted1_scan(&q1, ctx->stream));
-          HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-          HIPX(hipEventSynchronize(ctx->evk1));
+          ev_record(ctx, ctx->lane().evk0);
+          HIPX(qmk_sorted1_scan(&q1, ctx->lane().stream));
+          ev_record(ctx, ctx->lane().evk1);
+          ev_sync(ctx, ctx->lane().evk1);
           float ms = 0;
-          HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+          ev_ms(ctx, &ms, ctx->lane().evk0, ctx->lane().evk1);
           scan_ms_total = std::max(scan_ms_total, ms);
         }
         qbufs.push_back(std::move(dq));
@@ -3197,12 +3395,12 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
         const uint64_t waves = grid * B_WAVES;
         bs.tiles_per_wave = (ntiles + waves - 1) / waves;
         grid = ((ntiles + bs.tiles_per_wave - 1) / bs.tiles_per_wave + B_WAVES - 1) / B_WAVES;
-        HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-        HIPX(qmk_batch_qpf_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, nadd, nmax, qpl, ctx->stream));
-        HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-        HIPX(hipEventSynchronize(ctx->evk1));
+        ev_record(ctx, ctx->lane().evk0);
+        HIPX(qmk_batch_qpf_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, nadd, nmax, qpl, ctx->lane().stream));
+        ev_record(ctx, ctx->lane().evk1);
+        ev_sync(ctx, ctx->lane().evk1);
         float ms = 0;
-        HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+        ev_ms(ctx, &ms, ctx->lane().evk0, ctx->lane().evk1);
         scan_ms_total = std::max(scan_ms_total, ms);
         qbufs.push_back(std::move(dq));
         continue;
@@ -3214,12 +3412,12 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     const uint64_t waves = grid * B_WAVES;
     bs.tiles_per_wave = (ntiles + waves - 1) / waves;
     grid = ((ntiles + bs.tiles_per_wave - 1) / bs.tiles_per_wave + B_WAVES - 1) / B_WAVES;
-    HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-    HIPX(qmk_batch_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, lds_bytes, wide, ctx->stream));
-    HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-    HIPX(hipEventSynchronize(ctx->evk1));
+    ev_record(ctx, ctx->lane().evk0);
+    HIPX(qmk_batch_scan(&bs, dq.as<BQuery>(), (uint32_t)grid, lds_bytes, wide, ctx->lane().stream));
+    ev_record(ctx, ctx->lane().evk1);
+    ev_sync(ctx, ctx->lane().evk1);
     float ms = 0;
-    HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+    ev_ms(ctx, &ms, ctx->lane().evk0, ctx->lane().evk1);
     scan_ms_total = std::max(scan_ms_total, ms);
     qbufs.push_back(std::move(dq));
   }
@@ -3255,7 +3453,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
   DevBuf dcb(ctx, std::max<size_t>(cb.size(), 1) * sizeof(BCombine));
   if (!cb.empty()) {
     h2d(ctx, dcb.p, reinterpret_cast<const uint8_t *>(cb.data()), cb.size() * sizeof(BCombine));
-    HIPX(qmk_batch_combine(dcb.as<BCombine>(), (uint32_t)cb.size(), max_dom, ctx->stream));
+    HIPX(qmk_batch_combine(dcb.as<BCombine>(), (uint32_t)cb.size(), max_dom, ctx->lane().stream));
   }
   finalize_batch(ctx, order, out, status, selected);
   double ms = tm.stop();
@@ -3339,7 +3537,7 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
   if (left->ctx != ctx || (right && right->ctx != ctx)) fail(QM_ERR_INVALID_ARG, "handles from different contexts");
   const Plan &pl = p->plan;
   if (pl.has_join && !right) fail(QM_ERR_TABLE_NOT_FOUND, "join needs the right (build) table");
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx);
   HIPX(hipSetDevice(ctx->device));
   auto t0 = std::chrono::steady_clock::now();
   std::unique_ptr<qm_result> res;
@@ -3402,7 +3600,7 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
     run_rows_query(ctx, T, right, pl, res.get());
     res->stats.pages_skipped = skipped;
   }
-  HIPX(hipStreamSynchronize(ctx->stream));
+  stream_sync(ctx);
   res->stats.wall_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   *out = res.release();
@@ -3436,7 +3634,8 @@ extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, u
   bool lists = has_list(pl.filters);
   for (const auto &g : pl.groups) lists = lists || has_list(g);
   if (lists) fail(QM_ERR_UNSUPPORTED, "qm_exec_noalloc takes no IN / NOT IN lists (their values need device memory)");
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx);
+  std::lock_guard<std::mutex> sg(ctx->scratch_mu);
   HIPX(hipSetDevice(ctx->device));
   if (!ctx->scratch) fail(QM_ERR_QUERY_FAILED, "no scratch array on this device");
   uint8_t *const base = static_cast<uint8_t *>(ctx->scratch);
@@ -3468,8 +3667,8 @@ extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, u
     uint64_t *table = reinterpret_cast<uint64_t *>(base + o_table);
     uint64_t *stat = reinterpret_cast<uint64_t *>(base + o_stat);  // [0] status words, [1] selected rows
     // cleared with a fill kernel (no memset nodes for callers that capture)
-    HIPX(qmk_fill_u64(table, ab.table_words, 0, ctx->stream));
-    HIPX(qmk_fill_u64(stat, 4, 0, ctx->stream));
+    HIPX(qmk_fill_u64(table, ab.table_words, 0, ctx->lane().stream));
+    HIPX(qmk_fill_u64(stat, 4, 0, ctx->lane().stream));
     q.table = table;
     q.status = reinterpret_cast<uint32_t *>(stat);
     q.selected = stat + 1;
@@ -3486,7 +3685,7 @@ extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, u
     size_t lds_bytes;
     scan_launch_shape(ctx, t->rows, (uint64_t)q.lds_words, &grid, &block, &lds_bytes);
     if (lds_bytes > 160 * 1024) fail(QM_ERR_UNSUPPORTED, "group table exceeds the workgroup LDS");
-    if (t->rows) HIPX(qmk_scan_agg(&sc, dq, grid, block, lds_bytes, ctx->stream));
+    if (t->rows) HIPX(qmk_scan_agg(&sc, dq, grid, block, lds_bytes, ctx->lane().stream));
     uint32_t st[2] = {0, 0};
     d2h(ctx, st, stat, 2);
     if (st[0]) {
@@ -3505,8 +3704,8 @@ extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, u
     f.status = q.status;
     uint64_t *cnt = reinterpret_cast<uint64_t *>(base + o_cnt), *offs = reinterpret_cast<uint64_t *>(base + o_offs);
     uint64_t *tmp = reinterpret_cast<uint64_t *>(base + o_tmp);
-    HIPX(qmk_compact_count(&f, cnt, ctx->stream));
-    HIPX(qmk_exscan_u64(cnt, nb, offs, tmp + tw, tmp, ctx->stream));
+    HIPX(qmk_compact_count(&f, cnt, ctx->lane().stream));
+    HIPX(qmk_exscan_u64(cnt, nb, offs, tmp + tw, tmp, ctx->lane().stream));
     uint64_t total = 0;
     d2h(ctx, &total, tmp + tw, 1);
     if (total > cap)
@@ -3521,9 +3720,9 @@ extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, u
       if (nbm && (reinterpret_cast<uintptr_t>(nbm) & 7))
         fail(QM_ERR_INVALID_ARG, "NULL bitmap of output column %zu is not 8-byte aligned", k);
       f.outs[k].nulls = nbm;
-      if (nbm && total) HIPX(qmk_fill_u64(reinterpret_cast<uint64_t *>(nbm), (total + 63) / 64, 0, ctx->stream));
+      if (nbm && total) HIPX(qmk_fill_u64(reinterpret_cast<uint64_t *>(nbm), (total + 63) / 64, 0, ctx->lane().stream));
     }
-    if (total) HIPX(qmk_compact_write(&f, offs, ctx->stream));
+    if (total) HIPX(qmk_compact_write(&f, offs, ctx->lane().stream));
     d2h(ctx, st, stat, 2);
     if (st[1]) fail(QM_ERR_OVERFLOW, "SUM over an int64 column does not fit int64");
     *rows_out = total;
@@ -3539,7 +3738,7 @@ extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right,
   qm_ctx *ctx = p->ctx;
   const Plan &pl = p->plan;
   if (pl.has_join && !right) fail(QM_ERR_TABLE_NOT_FOUND, "join needs the right (build) table");
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx);
   HIPX(hipSetDevice(ctx->device));
   uint64_t b = 0;
   const uint64_t n = left->rows;
@@ -3580,19 +3779,19 @@ extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right,
 extern "C" int qm_measure_read_bw(qm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbps) {
   API_BEGIN
   if (!ctx || !gbps || bytes < (1u << 20) || reps == 0) fail(QM_ERR_INVALID_ARG, "bad argument");
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx);
   HIPX(hipSetDevice(ctx->device));
   bytes &= ~(uint64_t)15;
   DevBuf buf(ctx, bytes), sink(ctx, 64);
-  HIPX(qmk_fill_u64(buf.as<uint64_t>(), bytes / 8, 0x0123456789ABCDEFull, ctx->stream));
+  HIPX(qmk_fill_u64(buf.as<uint64_t>(), bytes / 8, 0x0123456789ABCDEFull, ctx->lane().stream));
   const uint32_t grid = (uint32_t)ctx->num_cus * 8;
-  HIPX(qmk_stream_read(buf.p, bytes, grid, sink.as<uint64_t>(), ctx->stream));  // warm
-  HIPX(hipEventRecord(ctx->evk0, ctx->stream));
-  for (uint32_t r = 0; r < reps; ++r) HIPX(qmk_stream_read(buf.p, bytes, grid, sink.as<uint64_t>(), ctx->stream));
-  HIPX(hipEventRecord(ctx->evk1, ctx->stream));
-  HIPX(hipEventSynchronize(ctx->evk1));
+  HIPX(qmk_stream_read(buf.p, bytes, grid, sink.as<uint64_t>(), ctx->lane().stream));  // warm
+  ev_record(ctx, ctx->lane().evk0);
+  for (uint32_t r = 0; r < reps; ++r) HIPX(qmk_stream_read(buf.p, bytes, grid, sink.as<uint64_t>(), ctx->lane().stream));
+  ev_record(ctx, ctx->lane().evk1);
+  ev_sync(ctx, ctx->lane().evk1);
   float ms = 0;
-  HIPX(hipEventElapsedTime(&ms, ctx->evk0, ctx->evk1));
+  ev_ms(ctx, &ms, ctx->lane().evk0, ctx->lane().evk1);
   *gbps = (double)bytes * reps / (ms * 1e-3) / 1e9;
   API_END
 }
@@ -3603,7 +3802,7 @@ extern "C" int qm_tune_set(qm_ctx *ctx, const char *name, double value) {
   const KnobDef *k = find_knob(name);
   if (!k) fail(QM_ERR_INVALID_ARG, "unknown tuning knob: %s", name ? name : "(null)");
   if (std::isnan(value)) fail(QM_ERR_INVALID_ARG, "knob %s: NaN", name);
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx, true);
   k->set(ctx->tune, std::min(k->hi, std::max(k->lo, value)));
   API_END
 }
@@ -3613,19 +3812,19 @@ extern "C" int qm_tune_get(qm_ctx *ctx, const char *name, double *value) {
   if (!ctx || !value) fail(QM_ERR_INVALID_ARG, "null argument");
   const KnobDef *k = find_knob(name);
   if (!k) fail(QM_ERR_INVALID_ARG, "unknown tuning knob: %s", name ? name : "(null)");
-  std::lock_guard<std::mutex> g(ctx->mu);
+  std::shared_lock<std::shared_mutex> g(ctx->tables_mu);
   *value = k->get(ctx->tune);
   API_END
 }
 
-extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_table *table,
-                             qm_result **results_out) {
-  API_BEGIN
-  if (!ctx || !table || (nplans && (!plans || !results_out))) fail(QM_ERR_INVALID_ARG, "null argument");
-  for (uint32_t i = 0; i < nplans; ++i) results_out[i] = nullptr;
-  std::lock_guard<std::mutex> g(ctx->mu);
-  HIPX(hipSetDevice(ctx->device));
-  auto t0 = std::chrono::steady_clock::now();
+namespace {
+// The batch itself, on the lane the caller holds: fused launches for the
+// aggregate plans that fit them, multi-query scans for the other aggregates,
+// one pass per row query. On return every result is complete (the lane's
+// stream has drained) or, on error, none is left allocated.
+void batch_exec_body(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_table *table,
+                     std::vector<qm_result *> &all) {
+  all.assign(nplans, nullptr);
   std::vector<uint32_t> agg_idx, other_idx;
   std::vector<std::unique_ptr<BItem>> fused;
   for (uint32_t i = 0; i < nplans; ++i) {
@@ -3643,7 +3842,6 @@ extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans
       other_idx.push_back(i);
     }
   }
-  std::vector<qm_result *> all(nplans, nullptr);
   try {
     if (!fused.empty()) {
       std::vector<BItem *> items;
@@ -3687,15 +3885,206 @@ extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans
       run_rows_query(ctx, table, nullptr, plans[i]->plan, r);
     }
   } catch (...) {
-    for (auto *r : all) delete r;
+    for (auto *&r : all) { delete r; r = nullptr; }
     throw;
   }
-  HIPX(hipStreamSynchronize(ctx->stream));
+  stream_sync(ctx);
+}
+}  // namespace
+
+extern "C" int qm_batch_exec(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_table *table,
+                             qm_result **results_out) {
+  API_BEGIN
+  if (!ctx || !table || (nplans && (!plans || !results_out))) fail(QM_ERR_INVALID_ARG, "null argument");
+  for (uint32_t i = 0; i < nplans; ++i) results_out[i] = nullptr;
+  LaneHold hold(ctx);
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<qm_result *> all;
+  batch_exec_body(ctx, plans, nplans, table, all);
   double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (uint32_t i = 0; i < nplans; ++i) {
     all[i]->stats.wall_ms = wall;
     results_out[i] = all[i];
   }
+  API_END
+}
+
+// ---- prepared batches: one batch captured into a HIP graph, replayed ----
+struct qm_prepared {
+  qm_ctx *ctx = nullptr;
+  qm_table *table = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  qm_capture cap;                    // tape, upload staging, the graph's private buffer pool
+  std::vector<qm_result *> tmpl;     // the captured run's results: the buffers every replay writes
+  uint64_t nodes = 0;
+  std::mutex mu;                     // one replay at a time (they share the buffers)
+  void free_all() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    for (qm_result *r : tmpl) delete r;
+    tmpl.clear();
+    for (auto &kv : cap.free) ctx->release(kv.second, kv.first);
+    cap.free.clear();
+    if (cap.staging) (void)hipHostFree(cap.staging);
+    cap.staging = nullptr;
+  }
+};
+
+namespace {
+// Ends a failed capture without ever waiting on a stream that may still be
+// capturing: if the capture cannot be ended the lane gets a new stream.
+void abandon_capture(qm_lane &L) {
+  hipGraph_t g = nullptr;
+  (void)hipStreamEndCapture(L.stream, &g);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(L.stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    hipStream_t ns = nullptr;
+    if (hipStreamCreateWithFlags(&ns, hipStreamNonBlocking) == hipSuccess) L.stream = ns;
+  }
+}
+}  // namespace
+
+extern "C" int qm_batch_prepare(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_table *table,
+                                qm_prepared **out) {
+  API_BEGIN
+  if (!out) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  if (!ctx || !table || (nplans && !plans)) fail(QM_ERR_INVALID_ARG, "null argument");
+  if (table->ctx != ctx) fail(QM_ERR_INVALID_ARG, "handles from different contexts");
+  for (uint32_t i = 0; i < nplans; ++i) {
+    if (!plans[i]) fail(QM_ERR_INVALID_ARG, "null plan %u", i);
+    if (plans[i]->plan.has_join) fail(QM_ERR_UNSUPPORTED, "joins are not batched; run them with qm_exec");
+  }
+  LaneHold hold(ctx);
+  qm_lane &L = ctx->lane();
+  std::unique_ptr<qm_prepared> P(new qm_prepared());
+  P->ctx = ctx;
+  P->table = table;
+  struct Detach {
+    qm_lane &L;
+    qm_prepared *P;
+    bool armed = true;
+    ~Detach() {
+      L.cap = nullptr;
+      if (armed) P->free_all();
+    }
+  } detach{L, P.get()};
+  // 1. an ordinary run that records every readback the host plans with and
+  //    the buffers it needs
+  P->cap.mode = LANE_RECORD;
+  L.cap = &P->cap;
+  {
+    std::vector<qm_result *> rec;
+    batch_exec_body(ctx, plans, nplans, table, rec);
+    for (qm_result *r : rec) delete r;
+  }
+  // 2. the graph's private pool and the pinned upload staging, allocated now
+  P->cap.mode = LANE_NORMAL;
+  for (const auto &kv : P->cap.peak)
+    for (int c = 0; c < kv.second; ++c) {
+      P->cap.free.emplace(kv.first, ctx->alloc(kv.first));
+      P->cap.private_bytes += kv.first;
+    }
+  P->cap.staging_cap = P->cap.h2d_bytes + 4096;
+  HIPX(hipHostMalloc(reinterpret_cast<void **>(&P->cap.staging), P->cap.staging_cap, hipHostMallocDefault));
+  stream_sync(ctx);  // every buffer above is idle before the graph takes it
+  // 3. the same host logic again while the stream is captured
+  hipStreamCaptureMode tmode = hipStreamCaptureModeRelaxed;
+  HIPX(hipThreadExchangeStreamCaptureMode(&tmode));
+  struct ModeBack {
+    hipStreamCaptureMode m;
+    ~ModeBack() { (void)hipThreadExchangeStreamCaptureMode(&m); }
+  } mode_back{tmode};
+  P->cap.mode = LANE_CAPTURE;
+  P->cap.pos = 0;
+  HIPX(hipStreamBeginCapture(L.stream, hipStreamCaptureModeRelaxed));
+  try {
+    batch_exec_body(ctx, plans, nplans, table, P->tmpl);
+  } catch (...) {
+    P->cap.mode = LANE_NORMAL;
+    abandon_capture(L);
+    throw;
+  }
+  P->cap.mode = LANE_NORMAL;
+  const hipError_t ee = hipStreamEndCapture(L.stream, &P->graph);
+  if (ee != hipSuccess) {
+    abandon_capture(L);
+    fail(QM_ERR_QUERY_FAILED, "prepared batch: ending the capture: %s", hipGetErrorString(ee));
+  }
+  if (P->cap.pos != P->cap.tape.size())
+    fail(QM_ERR_QUERY_FAILED, "prepared batch: the captured run read back %zu of %zu recorded values", P->cap.pos,
+         P->cap.tape.size());
+  size_t nn = 0;
+  HIPX(hipGraphGetNodes(P->graph, nullptr, &nn));
+  P->nodes = nn;
+  HIPX(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+  table->prepared.fetch_add(1);
+  detach.armed = false;
+  *out = P.release();
+  API_END
+}
+
+extern "C" int qm_prepared_run(qm_prepared *p, qm_result **results_out) {
+  API_BEGIN
+  if (!p || (!results_out && !p->tmpl.empty())) fail(QM_ERR_INVALID_ARG, "null argument");
+  for (size_t i = 0; i < p->tmpl.size(); ++i) results_out[i] = nullptr;
+  qm_ctx *ctx = p->ctx;
+  LaneHold hold(ctx);
+  std::lock_guard<std::mutex> g(p->mu);
+  auto t0 = std::chrono::steady_clock::now();
+  Timer tm(ctx);
+  HIPX(hipGraphLaunch(p->exec, ctx->lane().stream));
+  const double ms = tm.stop();
+  const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  std::vector<std::unique_ptr<qm_result>> views;
+  for (const qm_result *t : p->tmpl) {
+    std::unique_ptr<qm_result> r(new qm_result());
+    r->ctx = ctx;
+    r->rows = t->rows;
+    r->stats = t->stats;
+    r->stats.device_ms = ms;
+    r->stats.wall_ms = wall;
+    r->stats.kernel_launches = 1;
+    for (const OutCol &c : t->cols) {  // views: the prepared batch owns the memory
+      OutCol v;
+      v.name = c.name;
+      v.dtype = c.dtype;
+      v.data.p = c.data.p;
+      v.data.n = c.data.n;
+      v.nulls.p = c.nulls.p;
+      v.nulls.n = c.nulls.n;
+      r->cols.push_back(std::move(v));
+    }
+    views.push_back(std::move(r));
+  }
+  for (size_t i = 0; i < views.size(); ++i) results_out[i] = views[i].release();
+  API_END
+}
+
+extern "C" int qm_prepared_info(qm_prepared *p, uint32_t *nplans, uint64_t *graph_nodes, uint64_t *device_bytes) {
+  API_BEGIN
+  if (!p) fail(QM_ERR_INVALID_ARG, "null argument");
+  if (nplans) *nplans = (uint32_t)p->tmpl.size();
+  if (graph_nodes) *graph_nodes = p->nodes;
+  if (device_bytes) *device_bytes = p->cap.private_bytes;
+  API_END
+}
+
+extern "C" int qm_prepared_close(qm_prepared *p) {
+  API_BEGIN
+  if (!p) return QM_OK;
+  {
+    std::lock_guard<std::mutex> g(p->mu);  // no replay is in flight past this point
+    p->free_all();
+  }
+  p->table->prepared.fetch_sub(1);
+  delete p;
   API_END
 }
 
@@ -3748,7 +4137,7 @@ extern "C" int qm_result_qmcb_size(qm_result *r, uint64_t *size) {
   API_BEGIN
   if (!r || !size) fail(QM_ERR_INVALID_ARG, "null argument");
   if (r->rows > 0xFFFFFFFFull) fail(QM_ERR_UNSUPPORTED, "QMCB row count is 32-bit");
-  std::lock_guard<std::mutex> g(r->ctx->mu);
+  LaneHold hold(r->ctx);
   *size = qmcb_size(r, scan_nulls(r));
   API_END
 }
@@ -3759,7 +4148,7 @@ extern "C" int qm_result_qmcb(qm_result *r, uint8_t *out, uint64_t cap, uint64_t
   if (r->rows > 0xFFFFFFFFull) fail(QM_ERR_UNSUPPORTED, "QMCB row count is 32-bit");
   if (r->cols.size() > 0xFFFF) fail(QM_ERR_UNSUPPORTED, "too many columns");
   qm_ctx *ctx = r->ctx;
-  std::lock_guard<std::mutex> g(ctx->mu);
+  LaneHold hold(ctx);
   HIPX(hipSetDevice(ctx->device));
   auto nc = scan_nulls(r);
   uint64_t need = qmcb_size(r, nc);
@@ -3796,12 +4185,12 @@ extern "C" int qm_result_qmcb(qm_result *r, uint8_t *out, uint64_t cap, uint64_t
     } else {
       size_t b = qmcb_data_bytes(c.dtype, r->rows);
       if (b) {
-        HIPX(hipMemcpyAsync(w, c.data.p, b, hipMemcpyDeviceToHost, ctx->stream));
+        HIPX(hipMemcpyAsync(w, c.data.p, b, hipMemcpyDeviceToHost, ctx->lane().stream));
         w += b;
       }
     }
   }
-  HIPX(hipStreamSynchronize(ctx->stream));
+  stream_sync(ctx);
   if (written) *written = (uint64_t)(w - out);
   API_END
 }
@@ -3815,14 +4204,9 @@ extern "C" int qm_result_stats(qm_result *r, qm_stats *out) {
 
 extern "C" int qm_result_close(qm_result *r) {
   if (!r) return QM_OK;
-  qm_ctx *ctx = r->ctx;
-  if (ctx) {
-    std::lock_guard<std::mutex> g(ctx->mu);
-    (void)hipStreamSynchronize(ctx->stream);
-    delete r;
-  } else {
-    delete r;
-  }
+  // a result is complete when handed out (its call drained its lane), so its
+  // buffers go straight back to the pool
+  delete r;
   return QM_OK;
 }
 

@@ -5,9 +5,12 @@
 #include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <condition_variable>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 #include "../../include/qmgpu.h"
@@ -92,17 +95,61 @@ BoundPred coerce_filter(const Filter &f, int dt);
 struct qm_ctx_impl;
 }  // namespace qm
 
+// One execution lane of a context: a stream of its own and the events that
+// time work on it. Each call that runs device work holds one lane for its
+// whole duration, so up to QM_LANES calls overlap on the device. Buffers a
+// call releases stay with its lane (reusable by the same call, whose stream
+// orders the uses) until the lane's stream has drained; then they go back to
+// the context pool that every lane draws from.
+constexpr int QM_LANES = 4;
+
+// How a lane runs the host side of a call. RECORD executes normally and keeps
+// every device-to-host readback (the sizes and counts the host plans with) on
+// a tape, plus how many buffers of each size were live at once; CAPTURE runs
+// the same host logic while the lane's stream is being captured into a HIP
+// graph: readbacks come from the tape instead of the device, waits and timing
+// are skipped, zeroing is a fill kernel, host-to-device uploads are staged in
+// pinned memory the graph keeps, and buffers come from a private pool
+// reserved before the capture from the recorded sizes (no device allocator
+// call while the stream captures); the graph owns that pool.
+enum : int { LANE_NORMAL = 0, LANE_RECORD = 1, LANE_CAPTURE = 2 };
+struct qm_capture {
+  int mode = LANE_NORMAL;
+  std::vector<std::vector<uint8_t>> tape;   // readbacks, in call order
+  size_t pos = 0;                           // next tape entry (CAPTURE)
+  uint64_t h2d_bytes = 0;                   // uploaded bytes (RECORD: sizes the staging area)
+  uint8_t *staging = nullptr;               // pinned host memory (CAPTURE)
+  size_t staging_cap = 0, staging_used = 0;
+  std::map<void *, size_t> rec_live;        // RECORD: buffers the run allocated, still live
+  std::map<size_t, int> live, peak;         // RECORD: live / most buffers of each size at once
+  std::multimap<size_t, void *> free;       // CAPTURE: the private pool's free buffers
+  uint64_t private_bytes = 0;               // the private pool's size
+};
+
+struct qm_lane {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk0 = nullptr, evk1 = nullptr, evk2 = nullptr;  // around the dominant scan kernel(s)
+  bool busy = false;
+  std::multimap<size_t, void *> held;  // released during the current call
+  uint64_t held_bytes = 0;
+  qm_capture *cap = nullptr;           // set while recording / capturing a prepared batch
+};
+
 struct qm_ctx {
   int device = 0;
   uint64_t budget = 0;
   std::atomic<uint64_t> used{0};
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t evk0 = nullptr, evk1 = nullptr, evk2 = nullptr;  // around the dominant scan kernel(s)
-  std::mutex mu;           // one query on the device at a time per context
+  qm_lane lanes[QM_LANES];
+  std::mutex lanes_mu;
+  std::condition_variable lanes_cv;
+  // shared by executions, exclusive for table upload / drop and knob changes
+  std::shared_mutex tables_mu;
+  std::mutex cache_mu;     // lazily built per-column caches (partition geometries)
+  std::mutex scratch_mu;   // the no-allocation scratch array
   std::mutex pool_mu;
   std::multimap<size_t, void *> pool;  // free device buffers by size
-  uint64_t pooled = 0;
+  uint64_t pooled = 0;     // bytes in `pool` and in the lanes' held lists
   int num_cus = 256;
   std::string name;
   void *scratch = nullptr;  // device address of this device's copy of the no-allocation scratch array
@@ -130,8 +177,11 @@ struct qm_ctx {
   } tune;
 
   void *alloc(size_t bytes);
+  void *alloc_shared(size_t bytes, qm_lane *L);  // the lane's released buffers, the pool, or the allocator
   void release(void *p, size_t bytes);
   void trim_pool();
+  qm_lane &lane();        // the lane the calling thread holds (lane 0 outside any call)
+  qm_lane *held_lane();   // ... or nullptr
 };
 
 namespace qm {
@@ -197,7 +247,7 @@ struct TCol {
     std::shared_ptr<DevBuf> start1, start2, chunk0;
     uint32_t nchunk2 = 0;                // pass-2 workgroup units
   };
-  std::vector<PRGeo> prgeo;
+  std::deque<PRGeo> prgeo;  // stable addresses: a running call keeps a pointer
 };
 
 }  // namespace qm
@@ -206,6 +256,7 @@ struct qm_table {
   qm_ctx *ctx = nullptr;
   std::string name;
   uint64_t rows = 0;
+  std::atomic<int> prepared{0};  // prepared batches whose graphs read this table's columns
   std::vector<qm::TCol> cols;
   int find(const std::string &n) const {
     for (size_t i = 0; i < cols.size(); ++i) if (cols[i].name == n) return (int)i;

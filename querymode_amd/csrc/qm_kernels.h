@@ -60,6 +60,11 @@ extern "C" {
 hipError_t qmk_scan_agg(const qm::DScan *sc, const qm::DQuery *d_queries, int grid, int block,
                         size_t lds_bytes, hipStream_t st);
 hipError_t qmk_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
+hipError_t qmk_zero_bytes(void *p, uint64_t n, hipStream_t st);
+// Raises a kernel's dynamic-LDS limit to at least `bytes` (the attribute is
+// set once per kernel and size: launches captured into a graph then make no
+// attribute call).
+hipError_t qmk_func_lds(const void *fn, size_t bytes);
 
 hipError_t qmk_exscan_u64(const uint64_t *in, uint64_t n, uint64_t *out, uint64_t *total,
                           uint64_t *tmp_sums, hipStream_t st);

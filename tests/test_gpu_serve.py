@@ -1,6 +1,7 @@
 """The serving loop on the device: a concurrent mix of group-by, join and
-top-K requests arriving while others run; each result must equal the same
-request run alone (bit for bit; FP64 sums are exact and order-independent)."""
+top-K requests arriving while others run, several launches on the device at
+once (one per execution lane); each result must equal the same request run
+alone (bit for bit; FP64 sums are exact and order-independent)."""
 from __future__ import annotations
 
 import random
@@ -58,8 +59,9 @@ def test_mix_alone_equals_in_mix(qm):
     for r in reqs:
         r.wait(300)
     srv.close()
-    assert srv.cycles < len(reqs)  # requests were batched together
+    assert srv.launches < len(reqs)  # requests were batched together
     assert srv.batched > 0
+    assert srv.max_overlap >= 2      # launches ran beside each other
     for i, r in enumerate(reqs):
         alone = qm.executor.execute(r.desc)
         got = r.result
@@ -68,4 +70,5 @@ def test_mix_alone_equals_in_mix(qm):
         else:
             assert_rows_equal(got.rows, alone.rows, ctx=f"request {i}")
     lat = [r.latency_ms for r in reqs]
-    print(f"p50 {percentile(lat, 50):.2f} ms  p99 {percentile(lat, 99):.2f} ms  cycles {srv.cycles}", flush=True)
+    print(f"p50 {percentile(lat, 50):.2f} ms  p99 {percentile(lat, 99):.2f} ms  launches {srv.launches}  "
+          f"max concurrent {srv.max_overlap}  overlap share {srv.overlap_share:.2f}", flush=True)
