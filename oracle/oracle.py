@@ -19,6 +19,11 @@ from pathlib import Path
 
 import numpy as np
 
+try:
+    from . import sql_post as SP
+except ImportError:  # loaded as a top-level module
+    import sql_post as SP
+
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "_build" / "libqmoracle.so"
 
@@ -234,10 +239,42 @@ def run(desc, tables: dict[str, Table]) -> list[dict]:
                 else:
                     row[a.out_name()] = float(of[o])
             rows.append(row)
-        return _sort_limit_rows(rows, desc)
+        return _post_aggregate(rows, desc)
 
     # row-returning query
     proj = list(desc.projections) if desc.projections else names_out
+
+    def row_at(p_: int, names) -> dict:
+        row = {}
+        for name in names:
+            ci = space.index(name)
+            r_ = int(lrows[p_]) if side[ci] == 0 else int(rrows[p_])
+            if side[ci] == 0:
+                row[name] = L.value(ci, r_)
+            else:
+                base = name[6:] if name.startswith("right_") and name[6:] in R.names else name
+                row[name] = R.value(R.col(base), r_)
+        return row
+
+    dist = getattr(desc, "distinct", None)
+    order = list(getattr(desc, "orderBy", None) or [])
+    if dist is not None or order:
+        # SQL post-stages over every selected row in scan order, then OFFSET /
+        # LIMIT and the projection (src/sql/executor.ts:63-84, 114-170)
+        names = list(proj)
+        for c in list(dist or []) + [k for k, _ in order] + ([desc.sortColumn] if desc.sortColumn else []):
+            if c not in names:
+                names.append(c)
+        rows = [row_at(p_, names) for p_ in range(len(lrows))]
+        if dist is not None:
+            rows = SP.distinct(rows, list(dist) or list(proj))
+        if order:
+            rows = SP.order_by(rows, order)
+            rows = _slice(rows, desc)
+        else:
+            rows = _sort_limit_rows(rows, desc)
+        return [{k: r[k] for k in proj} for r in rows]
+
     idx = np.arange(len(lrows))
     if desc.sortColumn:
         ci = space.index(desc.sortColumn)
@@ -282,13 +319,34 @@ def _sort_key(v, desc):
     return (0, -v if desc else v)
 
 
+def _slice(rows, desc):
+    off = int(desc.offset or 0)
+    end = None if desc.limit is None else off + int(desc.limit)
+    return rows[off:end]
+
+
 def _sort_limit_rows(rows, desc):
     if desc.sortColumn:
         d = desc.sortDirection == "desc"
         rows = sorted(rows, key=lambda r: _sort_key(r[desc.sortColumn], d))  # stable
-    off = int(desc.offset or 0)
-    end = None if desc.limit is None else off + int(desc.limit)
-    return rows[off:end]
+    return _slice(rows, desc)
+
+
+def _post_aggregate(rows, desc):
+    """Group rows -> HAVING -> ORDER BY (one column: the descriptor's sort;
+    several: the SQL wrapper's comparator) -> OFFSET / LIMIT -> the SELECT
+    columns when HAVING added aggregates (src/sql/executor.ts:108-170)."""
+    hv = getattr(desc, "having", None)
+    if hv is not None:
+        rows = SP.having(rows, hv)
+    order = list(getattr(desc, "orderBy", None) or [])
+    if order:
+        rows = _slice(SP.order_by(rows, order), desc)
+    else:
+        rows = _sort_limit_rows(rows, desc)
+    if hv is not None and desc.projections:
+        rows = [{k: r[k] for k in desc.projections if k in r} for r in rows]
+    return rows
 
 
 def topk_heap_rows(t: Table, col: str, desc: bool, k: int, rows=None) -> np.ndarray:

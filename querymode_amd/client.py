@@ -473,6 +473,20 @@ class DataFrame:
 
     orderBy = sort
 
+    def sortBy(self, *keys) -> "DataFrame":
+        """Multi-column ORDER BY: keys are column names or (column, "asc" | "desc")
+        pairs; stable, NULLs last (the SQL wrapper's order, src/sql/executor.ts:114-141)."""
+        ks = [(k, "asc") if isinstance(k, str) else (k[0], k[1]) for k in keys]
+        return self._derive(orderBy=ks, sortColumn=None, sortDirection=None)
+
+    def distinct(self, *columns: str) -> "DataFrame":
+        """DISTINCT over `columns` (none: every output column), src/client.ts:410-413."""
+        return self._derive(distinct=list(columns))
+
+    def having(self, expr: tuple) -> "DataFrame":
+        """HAVING over the aggregate output: a tree of querymode_amd.plan.HAVING_NODES."""
+        return self._derive(having=expr)
+
     def limit(self, n: int) -> "DataFrame":
         return self._derive(limit=n)
 

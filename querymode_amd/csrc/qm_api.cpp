@@ -1032,9 +1032,219 @@ int find_out(const qm_result *r, const std::string &name) {
   return -1;
 }
 
+// ---- SQL post-stages over a materialised result (k_post.hip) ----
+
+// Ascending ids of the rows whose keep byte is set.
+DevBuf select_kept(qm_ctx *ctx, const uint8_t *keep, uint64_t n, uint64_t *n_out) {
+  const uint64_t nb = std::max<uint64_t>(1, qmk_mask_blocks(n));
+  DevBuf counts(ctx, nb * 8), offs(ctx, nb * 8), stmp(ctx, qmk_exscan_tmp_words(nb) * 8 + 64);
+  uint64_t total = 0;
+  if (n) {
+    HIPX(qmk_mask_count(keep, n, counts.as<uint64_t>(), ctx->lane().stream));
+    uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nb);
+    HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nb, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
+                        ctx->lane().stream));
+    d2h(ctx, &total, total_d, 1);
+  }
+  DevBuf ids(ctx, std::max<uint64_t>(total, 1) * 8);
+  if (total) HIPX(qmk_mask_write(keep, n, offs.as<uint64_t>(), ids.as<uint64_t>(), ctx->lane().stream));
+  *n_out = total;
+  return ids;
+}
+
+// HAVING (src/sql/executor.ts:108-112): the rows whose program value is true,
+// in their order.
+void having_stage(qm_result *res, const Plan &pl) {
+  qm_ctx *ctx = res->ctx;
+  HProg prog;
+  memset(&prog, 0, sizeof prog);
+  if (pl.having.size() > (size_t)HP_MAX_OPS) fail(QM_ERR_UNSUPPORTED, "HAVING has more than %d terms", HP_MAX_OPS);
+  std::vector<int> slot_of(res->cols.size(), -1);
+  auto col = [&](const std::string &name) -> uint8_t {
+    const int ci = find_out(res, name);
+    if (ci < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "HAVING column not found: %s", name.c_str());
+    if (slot_of[ci] < 0) {
+      if (prog.ncols >= HP_MAX_COLS) fail(QM_ERR_UNSUPPORTED, "HAVING references more than %d columns", HP_MAX_COLS);
+      slot_of[ci] = prog.ncols;
+      prog.cols[prog.ncols++] = dcol_of(res->cols[ci]);
+    }
+    return (uint8_t)slot_of[ci];
+  };
+  auto lit = [](const Lit &l, uint8_t &kind, double &f, int64_t &i) {
+    kind = l.kind == LIT_I64 ? HL_I64 : (l.kind == LIT_F64 ? HL_F64 : HL_NULL);
+    f = l.f;
+    i = l.i;
+  };
+  std::vector<uint8_t> lk;
+  std::vector<double> lf;
+  std::vector<int64_t> li;
+  int depth = 0, max_depth = 0;
+  for (const HavingOp &h : pl.having) {
+    HPOp &o = prog.ops[prog.nops++];
+    o.kind = (uint8_t)h.kind;  // HK_* and HP_* share their values
+    o.cmp = (uint8_t)h.cmp;
+    o.neg = h.negated ? 1 : 0;
+    switch (h.kind) {
+      case HK_CMP: o.col = col(h.col); lit(h.a, o.lk, o.f, o.i); ++depth; break;
+      case HK_CMP_COL: o.col = col(h.col); o.col2 = col(h.col2); ++depth; break;
+      case HK_BETWEEN: o.col = col(h.col); lit(h.a, o.lk, o.f, o.i); lit(h.b, o.lk2, o.f2, o.i2); ++depth; break;
+      case HK_IN:
+        o.col = col(h.col);
+        o.list_off = (uint32_t)lk.size();
+        o.list_n = (uint32_t)h.list.size();
+        for (const Lit &l : h.list) {
+          uint8_t k;
+          double f;
+          int64_t i;
+          lit(l, k, f, i);
+          lk.push_back(k);
+          lf.push_back(f);
+          li.push_back(i);
+        }
+        ++depth;
+        break;
+      case HK_ISNULL: o.col = col(h.col); ++depth; break;
+      case HK_AND: case HK_OR: --depth; break;
+      default: break;
+    }
+    max_depth = std::max(max_depth, depth);
+  }
+  if (max_depth > HP_MAX_DEPTH) fail(QM_ERR_UNSUPPORTED, "HAVING nests deeper than %d", HP_MAX_DEPTH);
+  const size_t nl = std::max<size_t>(lk.size(), 1);
+  DevBuf dk(ctx, nl), df(ctx, nl * 8), di(ctx, nl * 8);
+  if (!lk.empty()) {
+    h2d(ctx, dk.p, lk.data(), lk.size());
+    h2d(ctx, df.p, lf.data(), lf.size());
+    h2d(ctx, di.p, li.data(), li.size());
+  }
+  prog.list_kind = dk.as<uint8_t>();
+  prog.list_f = df.as<double>();
+  prog.list_i = di.as<int64_t>();
+  const uint64_t n = res->rows;
+  DevBuf keep(ctx, std::max<uint64_t>(n, 1));
+  if (n) HIPX(qmk_having(&prog, n, keep.as<uint8_t>(), ctx->lane().stream));
+  uint64_t kept = 0;
+  DevBuf ids = select_kept(ctx, keep.as<uint8_t>(), n, &kept);
+  permute_result(res, ids.as<uint64_t>(), kept);
+}
+
+// Stable LSD passes that order `perm` (the identity on entry) by the keys:
+// per key, from the last to the first, its value code and then its NULL flag.
+void stable_order(qm_ctx *ctx, const std::vector<std::pair<DCol, bool>> &keys, uint64_t n, DevBuf &perm) {
+  if (n < 2 || keys.empty()) return;
+  DevBuf ka(ctx, n * 8), kb(ctx, n * 8), pb(ctx, n * 8);
+  const size_t tb = std::max<size_t>(qmk_radix_pairs_tmp_bytes(n), 256);
+  DevBuf tmp(ctx, tb);
+  for (size_t j = keys.size(); j-- > 0;) {
+    const DCol &c = keys[j].first;
+    for (int pass = 0; pass < (c.nulls ? 2 : 1); ++pass) {
+      HIPX(qmk_order_codes(&c, perm.as<uint64_t>(), n, keys[j].second ? 1 : 0, pass, ka.as<uint64_t>(),
+                           ctx->lane().stream));
+      HIPX(qmk_radix_pairs(ka.as<uint64_t>(), kb.as<uint64_t>(), perm.as<uint64_t>(), pb.as<uint64_t>(), n,
+                           pass ? 1 : 64, tmp.p, tmp.n, ctx->lane().stream));
+      std::swap(perm, pb);
+    }
+  }
+}
+
+// Multi-column ORDER BY (src/sql/executor.ts:114-141: stable, NULLs last in
+// either direction, NaN above every number), then OFFSET / LIMIT.
+void order_by_stage(qm_result *res, const std::vector<OrderKey> &order, int64_t limit, int64_t offset) {
+  qm_ctx *ctx = res->ctx;
+  std::vector<std::pair<DCol, bool>> keys;
+  for (const OrderKey &k : order) {
+    const int ci = find_out(res, k.col);
+    if (ci < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "ORDER BY column not found: %s", k.col.c_str());
+    keys.push_back({dcol_of(res->cols[ci]), k.desc});
+  }
+  const uint64_t n = res->rows;
+  DevBuf perm(ctx, std::max<uint64_t>(n, 1) * 8);
+  HIPX(qmk_iota(perm.as<uint64_t>(), n, ctx->lane().stream));
+  stable_order(ctx, keys, n, perm);
+  const uint64_t start = std::min<uint64_t>((uint64_t)offset, n);
+  uint64_t take = n - start;
+  if (limit >= 0) take = std::min<uint64_t>(take, (uint64_t)limit);
+  permute_result(res, perm.as<uint64_t>() + start, take);
+}
+
+// DISTINCT (src/operators.ts:1188-1227): the first row of every distinct key
+// tuple, in row order; NULLs are one value, every NaN one value, -0 equals 0
+// (the reference keys rows by String(value)). `names` empty: every column.
+void distinct_stage(qm_result *res, const std::vector<std::string> &names) {
+  qm_ctx *ctx = res->ctx;
+  std::vector<std::pair<DCol, bool>> keys;
+  std::vector<DCol> cols;
+  if (names.empty()) {
+    for (const OutCol &o : res->cols) cols.push_back(dcol_of(o));
+  } else {
+    for (const std::string &nm : names) {
+      const int ci = find_out(res, nm);
+      if (ci < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "DISTINCT column not found: %s", nm.c_str());
+      cols.push_back(dcol_of(res->cols[ci]));
+    }
+  }
+  for (const DCol &c : cols) keys.push_back({c, false});
+  const uint64_t n = res->rows;
+  if (n < 2 || cols.empty()) return;
+  DevBuf perm(ctx, n * 8);
+  HIPX(qmk_iota(perm.as<uint64_t>(), n, ctx->lane().stream));
+  stable_order(ctx, keys, n, perm);  // equal tuples side by side, each run in row order
+  DevBuf dcols(ctx, cols.size() * sizeof(DCol));
+  h2d(ctx, dcols.p, reinterpret_cast<const uint8_t *>(cols.data()), cols.size() * sizeof(DCol));
+  DevBuf keep(ctx, n);
+  HIPX(qmk_distinct_marks(dcols.as<DCol>(), (int)cols.size(), perm.as<uint64_t>(), n, keep.as<uint8_t>(),
+                          ctx->lane().stream));
+  uint64_t m = 0;
+  DevBuf pos = select_kept(ctx, keep.as<uint8_t>(), n, &m);
+  DevBuf first(ctx, std::max<uint64_t>(m, 1) * 8), sorted(ctx, std::max<uint64_t>(m, 1) * 8),
+      sv(ctx, std::max<uint64_t>(m, 1) * 8);
+  HIPX(qmk_gather_u64(perm.as<uint64_t>(), pos.as<uint64_t>(), m, first.as<uint64_t>(), ctx->lane().stream));
+  if (m > 1) {  // back to row order
+    DevBuf tmp(ctx, std::max<size_t>(qmk_radix_pairs_tmp_bytes(m), 256));
+    HIPX(qmk_radix_pairs(first.as<uint64_t>(), sorted.as<uint64_t>(), first.as<uint64_t>(), sv.as<uint64_t>(), m, 64,
+                         tmp.p, tmp.n, ctx->lane().stream));
+    permute_result(res, sorted.as<uint64_t>(), m);
+  } else {
+    permute_result(res, first.as<uint64_t>(), m);
+  }
+}
+
+// The output columns of a plan whose projections name a subset of the
+// result (HAVING-only aggregates, hidden sort / DISTINCT columns): kept in
+// projection order, names not in the result skipped (src/sql/executor.ts:146-160).
+void project_result(qm_result *res, const std::vector<std::string> &proj) {
+  if (proj.empty()) return;
+  std::vector<OutCol> keep;
+  for (const std::string &nm : proj) {
+    const int ci = find_out(res, nm);
+    if (ci >= 0 && !res->cols[ci].name.empty()) {
+      keep.push_back(std::move(res->cols[ci]));
+      res->cols[ci].name.clear();  // a name listed twice is taken once
+    }
+  }
+  res->cols = std::move(keep);
+}
+
 // ORDER BY / LIMIT / OFFSET over a materialised result
 void sort_limit_result(qm_result *res, const Plan &pl, const DQuery *d_noq, uint32_t *launches) {
   qm_ctx *ctx = res->ctx;
+  if (!pl.having.empty()) {
+    having_stage(res, pl);
+    *launches += 4;
+  }
+  if (!pl.order_by.empty()) {
+    order_by_stage(res, pl.order_by, pl.limit, pl.offset);
+    *launches += 4 * (uint32_t)pl.order_by.size();
+    if (!pl.having.empty() && !pl.aggs.empty()) project_result(res, pl.proj);
+    return;
+  }
+  if (!pl.having.empty() && !pl.aggs.empty()) {
+    Plan rest = pl;  // the sort / limit below, then the projection
+    rest.having.clear();
+    sort_limit_result(res, rest, d_noq, launches);
+    project_result(res, pl.proj);
+    return;
+  }
   if (pl.has_sort) {
     int ci = find_out(res, pl.sort_col);
     if (ci < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "sort column not found: %s", pl.sort_col.c_str());
@@ -1926,7 +2136,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
     uint64_t total = 0;
     d2h(ctx, &total, total_d, 1);
     n = total;
-    if (!pl.has_sort && pl.limit >= 0) n = std::min<uint64_t>(n, (uint64_t)pl.limit + (uint64_t)pl.offset);
+    if (!pl.has_sort && pl.limit >= 0 && !pl.post()) n = std::min<uint64_t>(n, (uint64_t)pl.limit + (uint64_t)pl.offset);
     left_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
     right_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
     if (n)
@@ -1937,7 +2147,8 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   } else {
     sc.ncols = (int)cs.cols.size();
     for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
-    const bool topk = pl.has_sort && pl.limit >= 0 && qmk_topk_cap((uint32_t)std::min<int64_t>(pl.limit + pl.offset, 1 << 30)) &&
+    const bool topk = !pl.post() && pl.has_sort && pl.limit >= 0 &&
+                      qmk_topk_cap((uint32_t)std::min<int64_t>(pl.limit + pl.offset, 1 << 30)) &&
                       pl.limit + pl.offset < (1 << 30);
     res->stats.rows_scanned = T->rows;
     if (topk) {
@@ -1989,12 +2200,62 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
       n = take;
     } else {
       uint64_t cap = ~0ull;
-      if (!pl.has_sort && pl.limit >= 0) cap = (uint64_t)pl.limit + (uint64_t)pl.offset;
+      if (!pl.has_sort && pl.limit >= 0 && !pl.post()) cap = (uint64_t)pl.limit + (uint64_t)pl.offset;
       left_rows = select_rows(ctx, sc, dq.as<DQuery>(), cap, &n, &launches);
     }
   }
-  // gather the projected columns
   res->rows = n;
+  if (pl.post()) {
+    // DISTINCT / multi-column ORDER BY over every selected row, in row order:
+    // the projected columns plus hidden key columns, then the stages, OFFSET /
+    // LIMIT, and the projection
+    std::vector<std::pair<std::string, int>> cols = proj;
+    auto want = [&](const std::string &nm) {
+      for (auto &pc : cols) if (pc.first == nm) return;
+      cols.push_back({nm, cs.plain(nm)});
+    };
+    for (const std::string &d : pl.distinct) want(d);
+    for (const OrderKey &k : pl.order_by) want(k.col);
+    if (pl.has_sort) want(pl.sort_col);
+    for (auto &pc : cols) {
+      const DCol &c = cs.cols[pc.second];
+      const int odt = out_dt_of(c.dtype);
+      OutCol oc = make_out(ctx, pc.first, odt, n, c.nulls != nullptr);
+      const uint64_t *rows = c.side ? right_rows.as<uint64_t>() : left_rows.as<uint64_t>();
+      HIPX(qmk_gather(&c, rows, nullptr, n, odt, oc.data.p, oc.nulls.as<uint32_t>(), ctx->lane().stream));
+      ++launches;
+      res->cols.push_back(std::move(oc));
+    }
+    if (pl.has_distinct) {
+      std::vector<std::string> dn = pl.distinct;
+      if (dn.empty()) for (auto &pc : proj) dn.push_back(pc.first);
+      distinct_stage(res, dn);
+      launches += 8;
+    }
+    if (!pl.order_by.empty()) {
+      order_by_stage(res, pl.order_by, pl.limit, pl.offset);
+      launches += 4 * (uint32_t)pl.order_by.size();
+    } else if (pl.has_sort || pl.limit >= 0 || pl.offset > 0) {
+      Plan p2 = pl;
+      p2.has_distinct = false;
+      p2.distinct.clear();
+      DQuery noq;
+      memset(&noq, 0, sizeof noq);
+      DevBuf dnoq(ctx, sizeof(DQuery));
+      h2d(ctx, dnoq.p, reinterpret_cast<const uint8_t *>(&noq), sizeof noq);
+      sort_limit_result(res, p2, dnoq.as<DQuery>(), &launches);
+    }
+    std::vector<std::string> names;
+    for (auto &pc : proj) names.push_back(pc.first);
+    project_result(res, names);
+    res->stats.device_ms = tm.stop();
+    res->stats.groups = res->rows;
+    res->stats.rows_selected = n;
+    res->stats.kernel_launches = launches;
+    res->stats.bytes_algorithmic = referenced_bytes(cs, T->rows, pl.has_join ? R->rows : 0);
+    return;
+  }
+  // gather the projected columns
   for (auto &pc : proj) {
     const DCol &c = cs.cols[pc.second];
     int odt = out_dt_of(c.dtype);
@@ -3624,9 +3885,10 @@ extern "C" int qm_exec_noalloc(qm_plan *p, qm_table *t, void *const *out_data, u
   qm_ctx *ctx = t->ctx;
   if (p->ctx != ctx) fail(QM_ERR_INVALID_ARG, "plan and table belong to different contexts");
   const Plan &pl = p->plan;
-  if (pl.aggs.empty() || pl.has_join || !pl.computed.empty() || pl.has_sort || pl.limit >= 0 || pl.offset)
+  if (pl.aggs.empty() || pl.has_join || !pl.computed.empty() || pl.has_sort || pl.limit >= 0 || pl.offset ||
+      pl.post())
     fail(QM_ERR_UNSUPPORTED, "qm_exec_noalloc runs filter -> GROUP BY -> aggregate plans (no join, computed "
-                             "column, sort, limit or offset)");
+                             "column, sort, limit, offset, HAVING or multi-column ORDER BY)");
   auto has_list = [](const std::vector<Filter> &fs) {
     for (const Filter &f : fs) if (f.op == OP_IN || f.op == OP_NOT_IN) return true;
     return false;
@@ -3960,6 +4222,8 @@ extern "C" int qm_batch_prepare(qm_ctx *ctx, qm_plan *const *plans, uint32_t npl
   for (uint32_t i = 0; i < nplans; ++i) {
     if (!plans[i]) fail(QM_ERR_INVALID_ARG, "null plan %u", i);
     if (plans[i]->plan.has_join) fail(QM_ERR_UNSUPPORTED, "joins are not batched; run them with qm_exec");
+    if (plans[i]->plan.post())
+      fail(QM_ERR_UNSUPPORTED, "prepared batches take no HAVING, DISTINCT or multi-column ORDER BY (plan %u)", i);
   }
   LaneHold hold(ctx);
   qm_lane &L = ctx->lane();

@@ -61,6 +61,23 @@ struct Computed {
   std::string a, b;
 };
 
+// HAVING: a postfix program over result columns, evaluated per row with the
+// reference SQL evaluator's three-valued logic (src/sql/evaluator.ts:7-200).
+enum : int { HK_CMP = 1, HK_CMP_COL, HK_BETWEEN, HK_IN, HK_ISNULL, HK_AND, HK_OR, HK_NOT };
+struct HavingOp {
+  int kind = 0;
+  std::string col, col2;   // HK_CMP_COL compares col with col2
+  int cmp = 0;             // 0 eq, 1 ne, 2 lt, 3 le, 4 gt, 5 ge
+  bool negated = false;    // BETWEEN / IN / IS NULL
+  Lit a, b;                // literal; BETWEEN low / high
+  std::vector<Lit> list;   // IN values (kind LIT_NONE = NULL)
+};
+
+struct OrderKey {
+  std::string col;
+  bool desc = false;
+};
+
 struct Plan {
   std::vector<Filter> filters;
   std::vector<std::vector<Filter>> groups;
@@ -77,6 +94,12 @@ struct Plan {
   bool desc = false;
   int64_t limit = -1, offset = 0;
   std::vector<std::string> proj;
+  // SQL post-stages (src/sql/executor.ts:30-170, src/operators.ts:1177-1227)
+  bool has_distinct = false;
+  std::vector<std::string> distinct;   // empty with has_distinct: every column
+  std::vector<OrderKey> order_by;      // multi-column ORDER BY (replaces sort_col)
+  std::vector<HavingOp> having;
+  bool post() const { return has_distinct || !order_by.empty() || !having.empty(); }
 };
 
 Plan parse_plan(const uint8_t *p, size_t n);

@@ -54,6 +54,26 @@ struct BCombine {
 // top-K candidate (composite order key, lexicographic on (a, b))
 struct TopKey { uint64_t a, b; };
 
+// HAVING program (k_post.hip): postfix ops over up to HP_MAX_COLS result
+// columns; literals are JS numbers (HL_F64), bigints (HL_I64) or NULL.
+enum : uint8_t { HL_NULL = 0, HL_F64 = 1, HL_I64 = 2 };
+enum : uint8_t { HP_CMP = 1, HP_CMP_COL, HP_BETWEEN, HP_IN, HP_ISNULL, HP_AND, HP_OR, HP_NOT };
+constexpr int HP_MAX_OPS = 32, HP_MAX_COLS = 8, HP_MAX_DEPTH = 16;
+struct HPOp {
+  uint8_t kind, cmp, col, col2, neg, lk, lk2, pad;
+  uint32_t list_off, list_n;
+  double f, f2;
+  int64_t i, i2;
+};
+struct HProg {
+  int32_t nops, ncols;
+  DCol cols[HP_MAX_COLS];
+  HPOp ops[HP_MAX_OPS];
+  const uint8_t *list_kind;  // IN values of every HP_IN op
+  const double *list_f;
+  const int64_t *list_i;
+};
+
 }  // namespace qm
 
 extern "C" {
@@ -61,6 +81,21 @@ hipError_t qmk_scan_agg(const qm::DScan *sc, const qm::DQuery *d_queries, int gr
                         size_t lds_bytes, hipStream_t st);
 hipError_t qmk_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t st);
 hipError_t qmk_zero_bytes(void *p, uint64_t n, hipStream_t st);
+
+// SQL post-stages (k_post.hip)
+hipError_t qmk_having(const qm::HProg *prog, uint64_t n, uint8_t *keep, hipStream_t st);
+uint64_t qmk_mask_blocks(uint64_t n);
+hipError_t qmk_mask_count(const uint8_t *keep, uint64_t n, uint64_t *counts, hipStream_t st);
+hipError_t qmk_mask_write(const uint8_t *keep, uint64_t n, const uint64_t *offs, uint64_t *ids, hipStream_t st);
+hipError_t qmk_iota(uint64_t *p, uint64_t n, hipStream_t st);
+hipError_t qmk_order_codes(const qm::DCol *c, const uint64_t *perm, uint64_t n, int desc, int null_pass,
+                           uint64_t *keys, hipStream_t st);
+hipError_t qmk_distinct_marks(const qm::DCol *d_cols, int ncols, const uint64_t *perm, uint64_t n, uint8_t *keep,
+                              hipStream_t st);
+hipError_t qmk_gather_u64(const uint64_t *src, const uint64_t *idx, uint64_t n, uint64_t *dst, hipStream_t st);
+size_t qmk_radix_pairs_tmp_bytes(uint64_t n);
+hipError_t qmk_radix_pairs(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout, uint64_t n,
+                           int end_bit, void *tmp, size_t tmp_bytes, hipStream_t st);
 // Raises a kernel's dynamic-LDS limit to at least `bytes` (the attribute is
 // set once per kernel and size: launches captured into a graph then make no
 // attribute call).

@@ -1,7 +1,7 @@
 // qm_plan.cpp — the binary plan reader and the filter-literal coercions.
 //
 // Plan layout (little-endian; str = u16 length + UTF-8 bytes):
-//   u32 magic 0x4C504D51 ("QMPL"), u16 version = 1
+//   u32 magic 0x4C504D51 ("QMPL"), u16 version (1, or 2 with the SQL post-stages)
 //   u16 n, n x Filter                         filters (AND)
 //   u16 g, g x { u16 n, n x Filter }          filterGroups (OR of ANDs)
 //   u16 c, c x { str alias, u8 kind(1=multiply), str a, str b }
@@ -11,6 +11,13 @@
 //   u8 has_sort [str column, u8 desc]
 //   i64 limit (-1 = none), i64 offset
 //   u16 p, p x str                            projections
+//   version 2 only (src/sql/executor.ts:30-170):
+//   u8 has_distinct [u16 d, d x str]          DISTINCT columns (d = 0: every output column)
+//   u16 o, o x { str column, u8 desc }        multi-column ORDER BY
+//   u16 h, h x HavingOp                       HAVING, postfix:
+//     u8 1 cmp: str col, u8 cmp, lit   2 cmp_col: str col, u8 cmp, str col2
+//     3 between: str col, u8 negated, lit lo, lit hi   4 in: str col, u8 negated, u16 n, n x (u8 0 = NULL | lit)
+//     5 is_null: str col, u8 negated   6 and   7 or   8 not      (cmp: 0 eq 1 ne 2 lt 3 le 4 gt 5 ge)
 //   Filter = str column, u8 op, u8 vkind, value:
 //     vkind 1: f64 | 2: i64 | 3: u32 n, n x (u8 kind, 8 bytes) | 4: 2 x (u8 kind, 8 bytes)
 #include <cmath>
@@ -89,7 +96,8 @@ struct Reader {
 Plan parse_plan(const uint8_t *p, size_t n) {
   Reader r{p, n};
   if (r.u32() != 0x4C504D51u) fail(QM_ERR_INVALID_FORMAT, "bad plan magic");
-  if (r.u16() != 1) fail(QM_ERR_INVALID_FORMAT, "unsupported plan version");
+  const uint16_t version = r.u16();
+  if (version != 1 && version != 2) fail(QM_ERR_INVALID_FORMAT, "unsupported plan version %u", version);
   Plan pl;
   pl.filters = r.filters();
   pl.groups = r.groups();
@@ -135,6 +143,56 @@ Plan parse_plan(const uint8_t *p, size_t n) {
   pl.offset = r.i64();
   uint16_t np = r.u16();
   for (uint16_t i = 0; i < np; ++i) pl.proj.push_back(r.str());
+  if (version >= 2) {
+    // DISTINCT columns, multi-column ORDER BY, HAVING program
+    pl.has_distinct = r.u8() != 0;
+    if (pl.has_distinct) {
+      uint16_t nd = r.u16();
+      for (uint16_t i = 0; i < nd; ++i) pl.distinct.push_back(r.str());
+    }
+    uint16_t no = r.u16();
+    for (uint16_t i = 0; i < no; ++i) {
+      OrderKey k;
+      k.col = r.str();
+      k.desc = r.u8() != 0;
+      pl.order_by.push_back(k);
+    }
+    uint16_t nh = r.u16();
+    int depth = 0;
+    for (uint16_t i = 0; i < nh; ++i) {
+      HavingOp h;
+      h.kind = r.u8();
+      switch (h.kind) {
+        case HK_CMP: h.col = r.str(); h.cmp = r.u8(); h.a = r.lit(); ++depth; break;
+        case HK_CMP_COL: h.col = r.str(); h.cmp = r.u8(); h.col2 = r.str(); ++depth; break;
+        case HK_BETWEEN: h.col = r.str(); h.negated = r.u8() != 0; h.a = r.lit(); h.b = r.lit(); ++depth; break;
+        case HK_IN: {
+          h.col = r.str();
+          h.negated = r.u8() != 0;
+          uint16_t k = r.u16();
+          for (uint16_t j = 0; j < k; ++j) {
+            r.need(1);
+            if (r.p[r.pos] == LIT_NONE) { ++r.pos; h.list.push_back(Lit()); }  // a NULL in the list
+            else h.list.push_back(r.lit());
+          }
+          ++depth;
+          break;
+        }
+        case HK_ISNULL: h.col = r.str(); h.negated = r.u8() != 0; ++depth; break;
+        case HK_AND: case HK_OR: if (depth < 2) fail(QM_ERR_INVALID_FORMAT, "HAVING program underflow"); --depth; break;
+        case HK_NOT: if (depth < 1) fail(QM_ERR_INVALID_FORMAT, "HAVING program underflow"); break;
+        default: fail(QM_ERR_INVALID_FORMAT, "unknown HAVING op %d", h.kind);
+      }
+      if (h.cmp > 5) fail(QM_ERR_INVALID_FORMAT, "unknown HAVING comparison %d", h.cmp);
+      pl.having.push_back(h);
+    }
+    if (nh && depth != 1) fail(QM_ERR_INVALID_FORMAT, "HAVING program leaves %d values", depth);
+    if (!pl.having.empty() && pl.aggs.empty()) fail(QM_ERR_INVALID_AGGREGATE, "HAVING without aggregates");
+    if (pl.has_distinct && !pl.aggs.empty())
+      fail(QM_ERR_UNSUPPORTED, "DISTINCT together with aggregates is not on the device path");
+    if (!pl.order_by.empty() && pl.has_sort)
+      fail(QM_ERR_INVALID_FORMAT, "a plan has either sortColumn or a multi-column ORDER BY");
+  }
   if (r.pos != n) fail(QM_ERR_INVALID_FORMAT, "trailing bytes in plan");
   if (pl.offset < 0) fail(QM_ERR_INVALID_FORMAT, "negative offset");
   return pl;

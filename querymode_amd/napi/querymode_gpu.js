@@ -74,10 +74,32 @@ function writeFilters(w, fs) { fs = fs || []; w.u16(fs.length); fs.forEach((f) =
 
 function aggName(a) { return a.alias !== undefined && a.alias !== null ? a.alias : a.fn + '_' + a.column; }
 
+// HAVING tree -> postfix ops (the nodes of querymode_amd/plan.py HAVING_NODES)
+const HCMP = { eq: 0, ne: 1, lt: 2, le: 3, gt: 4, ge: 5 };
+const HK = { cmp: 1, cmp_col: 2, between: 3, in: 4, isnull: 5, and: 6, or: 7, not: 8 };
+function havingOps(e, out) {
+  const k = e[0];
+  if (!(k in HK)) throw qmError('INVALID_FORMAT', 'unknown HAVING node ' + k);
+  const w = new Writer();
+  if (k === 'and' || k === 'or') { havingOps(e[1], out); havingOps(e[2], out); w.u8(HK[k]); }
+  else if (k === 'not') { havingOps(e[1], out); w.u8(HK[k]); }
+  else if (k === 'cmp') { w.u8(HK[k]); w.str(e[2]); w.u8(HCMP[e[1]]); writeLit(w, e[3]); }
+  else if (k === 'cmp_col') { w.u8(HK[k]); w.str(e[2]); w.u8(HCMP[e[1]]); w.str(e[3]); }
+  else if (k === 'between') { w.u8(HK[k]); w.str(e[1]); w.u8(e[4] ? 1 : 0); writeLit(w, e[2]); writeLit(w, e[3]); }
+  else if (k === 'in') {
+    w.u8(HK[k]); w.str(e[1]); w.u8(e[3] ? 1 : 0); w.u16(e[2].length);
+    e[2].forEach((v) => { if (v === null) w.u8(0); else writeLit(w, v); });
+  } else { w.u8(HK[k]); w.str(e[1]); w.u8(e[2] ? 1 : 0); }
+  out.push(w.bytes());
+}
+
 /** QueryDescriptor (src/client.ts:1254-1277) -> plan bytes. */
 function encodePlan(d) {
   const w = new Writer();
-  w.u32(PLAN_MAGIC); w.u16(1);
+  const distinct = d.distinct === undefined ? null : d.distinct;
+  const orderBy = d.orderBy || [];
+  const v2 = distinct !== null || orderBy.length > 0 || (d.having !== undefined && d.having !== null);
+  w.u32(PLAN_MAGIC); w.u16(v2 ? 2 : 1);
   writeFilters(w, d.filters);
   const groups = d.filterGroups || [];
   w.u16(groups.length); groups.forEach((g) => writeFilters(w, g));
@@ -104,6 +126,14 @@ function encodePlan(d) {
   w.i64(d.offset || 0);
   const proj = d.projections || [];
   w.u16(proj.length); proj.forEach((p) => w.str(p));
+  if (v2) {
+    // SQL post-stages: DISTINCT columns, multi-column ORDER BY, HAVING program
+    if (distinct !== null) { w.u8(1); w.u16(distinct.length); distinct.forEach((c) => w.str(c)); } else { w.u8(0); }
+    w.u16(orderBy.length); orderBy.forEach((o) => { w.str(o[0]); w.u8(o[1] === 'desc' ? 1 : 0); });
+    const ops = [];
+    if (d.having !== undefined && d.having !== null) havingOps(d.having, ops);
+    w.u16(ops.length); ops.forEach((b) => w.parts.push(b));
+  }
   return w.bytes();
 }
 

@@ -100,6 +100,51 @@ class QueryDescriptor:
     join: JoinDescriptor | None = None
     # product expressions only: alias -> (left column, right column)
     computedColumns: list[tuple[str, str, str]] = field(default_factory=list)
+    # SQL post-stages (src/sql/executor.ts:30-170, src/operators.ts:1177-1227):
+    # DISTINCT columns (None: no DISTINCT; []: every output column), a
+    # multi-column ORDER BY [(column, "asc" | "desc")] in place of sortColumn,
+    # and a HAVING tree over the aggregate output (see HAVING_NODES)
+    distinct: list[str] | None = None
+    orderBy: list[tuple[str, str]] = field(default_factory=list)
+    having: tuple | None = None
+
+
+# HAVING tree nodes (three-valued, src/sql/evaluator.ts:7-155):
+#   ("cmp", op, column, literal)        op: eq ne lt le gt ge
+#   ("cmp_col", op, column, column)
+#   ("between", column, low, high, negated)
+#   ("in", column, [literal or None, ...], negated)
+#   ("isnull", column, negated)         negated: IS NOT NULL
+#   ("and", a, b)  ("or", a, b)  ("not", a)
+HAVING_NODES = ("cmp", "cmp_col", "between", "in", "isnull", "and", "or", "not")
+_HCMP = {"eq": 0, "ne": 1, "lt": 2, "le": 3, "gt": 4, "ge": 5}
+_HK = {"cmp": 1, "cmp_col": 2, "between": 3, "in": 4, "isnull": 5, "and": 6, "or": 7, "not": 8}
+
+
+def _having_ops(e, out: list) -> None:
+    """Postfix program of a HAVING tree (qm_plan.cpp reads it back)."""
+    k = e[0]
+    if k not in _HK:
+        raise ValueError(f"unknown HAVING node {k!r}")
+    if k in ("and", "or"):
+        _having_ops(e[1], out)
+        _having_ops(e[2], out)
+        out.append(bytes([_HK[k]]))
+    elif k == "not":
+        _having_ops(e[1], out)
+        out.append(bytes([_HK[k]]))
+    elif k == "cmp":
+        out.append(bytes([_HK[k]]) + _str(e[2]) + bytes([_HCMP[e[1]]]) + _lit(e[3]))
+    elif k == "cmp_col":
+        out.append(bytes([_HK[k]]) + _str(e[2]) + bytes([_HCMP[e[1]]]) + _str(e[3]))
+    elif k == "between":
+        out.append(bytes([_HK[k]]) + _str(e[1]) + bytes([1 if e[4] else 0]) + _lit(e[2]) + _lit(e[3]))
+    elif k == "in":
+        vals = list(e[2])
+        body = b"".join(bytes([0]) if v is None else _lit(v) for v in vals)
+        out.append(bytes([_HK[k]]) + _str(e[1]) + bytes([1 if e[3] else 0]) + struct.pack("<H", len(vals)) + body)
+    else:  # isnull
+        out.append(bytes([_HK[k]]) + _str(e[1]) + bytes([1 if e[2] else 0]))
 
 
 JOIN_TYPES = {"inner": 0, "left": 1, "right": 2, "full": 3, "cross": 4}
@@ -115,7 +160,8 @@ def _groups(gs: list[list[FilterOp]]) -> bytes:
 
 def encode_plan(d: QueryDescriptor) -> bytes:
     """QueryDescriptor -> plan bytes read by qm_plan_compile."""
-    out = bytearray(struct.pack("<IH", 0x4C504D51, 1))
+    v2 = d.distinct is not None or bool(d.orderBy) or d.having is not None
+    out = bytearray(struct.pack("<IH", 0x4C504D51, 2 if v2 else 1))
     out += _filters(d.filters)
     out += _groups(d.filterGroups)
     out += struct.pack("<H", len(d.computedColumns))
@@ -139,4 +185,16 @@ def encode_plan(d: QueryDescriptor) -> bytes:
         out += bytes([0])
     out += struct.pack("<qq", -1 if d.limit is None else int(d.limit), int(d.offset or 0))
     out += struct.pack("<H", len(d.projections)) + b"".join(_str(p) for p in d.projections)
+    if v2:
+        if d.distinct is not None:
+            out += bytes([1]) + struct.pack("<H", len(d.distinct)) + b"".join(_str(c) for c in d.distinct)
+        else:
+            out += bytes([0])
+        out += struct.pack("<H", len(d.orderBy))
+        for col, direction in d.orderBy:
+            out += _str(col) + bytes([1 if direction == "desc" else 0])
+        ops: list[bytes] = []
+        if d.having is not None:
+            _having_ops(d.having, ops)
+        out += struct.pack("<H", len(ops)) + b"".join(ops)
     return bytes(out)

@@ -6,8 +6,16 @@ executes, with the reference compiler's lowering rules
 
   SELECT <*, columns, aggregates [AS alias]>
   FROM t [alias] [[INNER] JOIN r [alias] ON x.k = y.k | USING (k)]
-  [WHERE <predicate>] [GROUP BY cols] [ORDER BY col [ASC|DESC]]
-  [LIMIT n] [OFFSET m]
+  [WHERE <predicate>] [GROUP BY cols [HAVING <predicate>]]
+  [ORDER BY col [ASC|DESC], ...] [LIMIT n] [OFFSET m]
+
+* SELECT DISTINCT, HAVING and multi-column ORDER BY run as device stages
+  after the aggregate / row selection, with the reference wrapper's
+  semantics (src/sql/executor.ts:30-170): HAVING is evaluated with the SQL
+  evaluator's comparisons and three-valued logic, and aggregates only HAVING
+  names are computed but not returned (compiler.ts:159-164, executor.ts:143-160);
+  a multi-column ORDER BY is a stable sort with NULLs last, then OFFSET /
+  LIMIT; DISTINCT keeps the first row of each key (compiler.ts:154-157).
 
 * Aggregates: COUNT(*), COUNT(col), SUM/AVG/MIN/MAX(col); an argument
   ``a * b`` is keyed ``multiply(a,b)`` (src/sql/ast.ts:129-149) and runs as a
@@ -38,7 +46,7 @@ from .plan import AggregateOp, FilterOp, JoinDescriptor, QueryDescriptor
 _KEYWORDS = {
     "SELECT", "FROM", "WHERE", "GROUP", "BY", "ORDER", "ASC", "DESC", "LIMIT", "OFFSET", "AND", "OR", "NOT",
     "BETWEEN", "IN", "IS", "NULL", "AS", "JOIN", "INNER", "LEFT", "RIGHT", "FULL", "OUTER", "ON", "USING",
-    "LIKE", "TRUE", "FALSE", "DISTINCT", "CROSS",
+    "LIKE", "TRUE", "FALSE", "DISTINCT", "CROSS", "HAVING",
 }
 _AGG_FNS = {"COUNT", "SUM", "AVG", "MIN", "MAX", "COUNT_DISTINCT", "STDDEV", "VARIANCE", "MEDIAN", "PERCENTILE"}
 _TOKEN = re.compile(r"""
@@ -133,6 +141,7 @@ class _Parser:
     # ---- statement ----
     def select(self) -> dict:
         self.expect("kw", "SELECT")
+        distinct = bool(self.accept("kw", "DISTINCT"))
         items = [self.select_item()]
         while self.accept("op", ","):
             items.append(self.select_item())
@@ -150,6 +159,9 @@ class _Parser:
             group.append(self.column()[1])
             while self.accept("op", ","):
                 group.append(self.column()[1])
+        having = None
+        if self.accept("kw", "HAVING"):
+            having = self.expr()
         order = []
         if self.accept("kw", "ORDER"):
             self.expect("kw", "BY")
@@ -165,7 +177,7 @@ class _Parser:
         if self.peek().kind != "end":
             raise _err(f"SQL: unexpected {self.peek().text!r} at {self.peek().pos}")
         return {"items": items, "from": frm, "join": join, "where": where, "group": group, "order": order,
-                "limit": limit, "offset": offset}
+                "limit": limit, "offset": offset, "distinct": distinct, "having": having}
 
     def select_item(self):
         if self.accept("op", "*"):
@@ -462,6 +474,65 @@ def lower_where(where) -> tuple[list[FilterOp], list[list[FilterOp]]]:
     return filters, groups
 
 
+def _agg_of(e) -> tuple[str, str]:
+    """(fn, column key) of an aggregate call (compiler.ts:287-310)."""
+    fn = e[1].lower()
+    if fn == "count" and e[3]:
+        fn = "count_distinct"
+    col = "*" if not e[2] or e[2][0][0] == "star" else _arg_key(e[2][0])
+    return fn, col
+
+
+def _having_operand(e, aggs: list[AggregateOp]):
+    """extractAggregatesFromExpr + rewriteAggregatesAsColumns
+    (compiler.ts:254-285, evaluator.ts:263-275): an aggregate call becomes the
+    column of its output (the SELECT alias when one matches), and an
+    aggregate that only HAVING uses is added to the query's aggregates."""
+    if e[0] == "call" and e[1] in _AGG_FNS:
+        fn, col = _agg_of(e)
+        match = next((a for a in aggs if a.fn == fn and a.column == col), None)
+        if match is None:
+            match = AggregateOp(fn, col)
+            aggs.append(match)
+        return ("col", match.out_name())
+    return e
+
+
+def lower_having(e, aggs: list[AggregateOp]) -> tuple:
+    """HAVING -> the device's HAVING tree (plan.HAVING_NODES), evaluated with the
+    SQL evaluator's semantics (src/sql/executor.ts:108-112)."""
+    k = e[0]
+    if k in ("and", "or"):
+        return (k, lower_having(e[1], aggs), lower_having(e[2], aggs))
+    if k == "not":
+        return ("not", lower_having(e[1], aggs))
+    if k == "cmp":
+        op, a, b = e[1], _having_operand(e[2], aggs), _having_operand(e[3], aggs)
+        if a[0] == "col" and b[0] == "col":
+            return ("cmp_col", op, a[1], b[1])
+        if a[0] == "col":
+            return ("cmp", op, a[1], _lit_value(b))
+        if b[0] == "col":
+            return ("cmp", _FLIP[op], b[1], _lit_value(a))
+        raise _err("SQL: HAVING comparison needs a column or aggregate", "UNSUPPORTED")
+    if k == "between":
+        a = _having_operand(e[1], aggs)
+        if a[0] != "col":
+            raise _err("SQL: HAVING BETWEEN needs a column or aggregate", "UNSUPPORTED")
+        return ("between", a[1], _lit_value(e[2]), _lit_value(e[3]), bool(e[4]))
+    if k == "in":
+        a = _having_operand(e[1], aggs)
+        if a[0] != "col":
+            raise _err("SQL: HAVING IN needs a column or aggregate", "UNSUPPORTED")
+        return ("in", a[1], [None if v[0] == "null" else _lit_value(v) for v in e[2]], bool(e[3]))
+    if k == "isnull":
+        a = _having_operand(e[1], aggs)
+        if a[0] != "col":
+            raise _err("SQL: HAVING IS NULL needs a column or aggregate", "UNSUPPORTED")
+        return ("isnull", a[1], bool(e[2]))
+    raise _err(f"SQL: unsupported HAVING term {k}", "UNSUPPORTED")
+
+
 def sql_to_descriptor(sql: str) -> QueryDescriptor:
     st = _Parser(sql).select()
     table, talias = st["from"]
@@ -508,10 +579,17 @@ def sql_to_descriptor(sql: str) -> QueryDescriptor:
         desc.join = JoinDescriptor(QueryDescriptor(table=rname), lk, rk, "inner")
     if st["where"] is not None:
         desc.filters, desc.filterGroups = lower_where(st["where"])
+    if st["having"] is not None:
+        if not st["group"]:
+            raise _err("SQL: HAVING needs GROUP BY", "UNSUPPORTED")
+        desc.having = lower_having(st["having"], desc.aggregates)
+    if st["distinct"]:
+        desc.distinct = list(proj)  # compiler.ts:154-157
     if st["order"]:
         if len(st["order"]) > 1:
-            raise _err("SQL: multi-column ORDER BY is not on the device path", "UNSUPPORTED")
-        desc.sortColumn, desc.sortDirection = st["order"][0]
+            desc.orderBy = list(st["order"])  # the wrapper's multi-column sort (compiler.ts:124-131)
+        else:
+            desc.sortColumn, desc.sortDirection = st["order"][0]
     desc.limit = st["limit"]
     desc.offset = st["offset"]
     return desc

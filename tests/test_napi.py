@@ -66,6 +66,14 @@ CASES = [
                                 "rightKey": "id"},
      "computedColumns": [["aw", "amount", "weight"]], "groupBy": ["region"],
      "aggregates": [["sum", "aw", "total"], ["count", "*", None]]},
+    {"table": "t", "groupBy": ["k"], "aggregates": [["count", "*", "n"], ["max", "v", None]],
+     "having": ["or", ["and", ["cmp", "gt", "n", 5], ["not", ["isnull", "max_v", True]]],
+                ["in", "max_v", [1.5, None, {"$bigint": "7"}], False]],
+     "orderBy": [["n", "desc"], ["k", "asc"]], "limit": 4, "projections": ["k", "n"]},
+    {"table": "t", "groupBy": ["k"], "aggregates": [["sum", "a", None], ["sum", "b", None]],
+     "having": ["and", ["cmp_col", "ge", "sum_a", "sum_b"], ["between", "sum_a", -2.5, {"$bigint": "9"}, True]]},
+    {"table": "t", "projections": ["a", "b"], "distinct": ["a"], "orderBy": [["b", "asc"], ["a", "desc"]]},
+    {"table": "t", "projections": ["a"], "distinct": [], "sortColumn": "a", "limit": 3},
 ]
 
 
@@ -76,7 +84,9 @@ def _py_desc(c) -> QueryDescriptor:
                         groupBy=c.get("groupBy", []), aggregates=[AggregateOp(*a) for a in c.get("aggregates", [])],
                         sortColumn=c.get("sortColumn"), sortDirection=c.get("sortDirection"), limit=c.get("limit"),
                         offset=c.get("offset"), projections=c.get("projections", []),
-                        computedColumns=[tuple(x) for x in c.get("computedColumns", [])])
+                        computedColumns=[tuple(x) for x in c.get("computedColumns", [])],
+                        distinct=c.get("distinct"), orderBy=[tuple(o) for o in c.get("orderBy", [])],
+                        having=_revive_py(c["having"]) if "having" in c else None)
     if "join" in c:
         j = c["join"]
         d.join = JoinDescriptor(QueryDescriptor(table=j["right"]["table"], filters=f(j["right"].get("filters", []))),
@@ -93,7 +103,8 @@ def test_js_plan_encoding_matches_python(case):
       const d={{table:c.table, filters:F(c.filters), filterGroups:(c.filterGroups||[]).map(F), groupBy:c.groupBy,
         aggregates:(c.aggregates||[]).map(([fn,column,alias])=>({{fn,column,alias}})), sortColumn:c.sortColumn,
         sortDirection:c.sortDirection, limit:c.limit, offset:c.offset, projections:c.projections,
-        computedColumns:c.computedColumns}};
+        computedColumns:c.computedColumns, distinct:c.distinct, orderBy:c.orderBy,
+        having:c.having===undefined?undefined:rev(c.having)}};
       if (c.join) d.join={{right:{{table:c.join.right.table, filters:F(c.join.right.filters)}}, leftKey:c.join.leftKey,
         rightKey:c.join.rightKey, type:'inner'}};
       console.log(q.encodePlan(d).toString('hex'));"""

@@ -84,7 +84,9 @@ def test_literals():
     "SELECT * FROM t WHERE a > b",
     "SELECT * FROM t WHERE c = 'x'",
     "SELECT * FROM l LEFT JOIN r ON l.k = r.k",
-    "SELECT * FROM t ORDER BY a, b",
+    "SELECT k FROM t GROUP BY k HAVING SUM(a) * 2 > 3",
+    "SELECT k FROM t GROUP BY k HAVING COUNT(*) > 'x'",
+    "SELECT k, COUNT(*) FROM t HAVING COUNT(*) > 1",
 ])
 def test_off_path_is_unsupported(sql):
     with pytest.raises(QueryModeError) as e:
@@ -98,3 +100,22 @@ def test_malformed_is_invalid_format(sql):
     with pytest.raises(QueryModeError) as e:
         S(sql)
     assert e.value.code == "INVALID_FORMAT"
+
+
+def test_post_stages_lowering():
+    d = S("SELECT * FROM t ORDER BY a, b DESC LIMIT 5")
+    assert d.orderBy == [("a", "asc"), ("b", "desc")] and d.sortColumn is None and d.limit == 5
+    d = S("SELECT DISTINCT a, b FROM t")
+    assert d.distinct == ["a", "b"] and d.projections == ["a", "b"]
+    d = S("SELECT DISTINCT * FROM t")
+    assert d.distinct == []
+    # HAVING-only aggregates are computed but not projected (compiler.ts:80-88, 159-164)
+    d = S("SELECT dept FROM data GROUP BY dept HAVING SUM(salary) > 300000")
+    assert d.projections == ["dept"] and [a.out_name() for a in d.aggregates] == ["sum_salary"]
+    assert d.having == ("cmp", "gt", "sum_salary", 300000)
+    # a SELECT alias names the HAVING column; a literal on the left flips the comparison
+    d = S("SELECT k, COUNT(*) AS n FROM t GROUP BY k HAVING 5 < COUNT(*) AND MAX(v) IS NOT NULL")
+    assert d.having == ("and", ("cmp", "gt", "n", 5), ("isnull", "max_v", True))
+    assert d.projections == ["k", "n"] and [a.out_name() for a in d.aggregates] == ["n", "max_v"]
+    d = S("SELECT k, SUM(a) FROM t GROUP BY k HAVING SUM(a) > SUM(b) OR k IN (1, NULL)")
+    assert d.having == ("or", ("cmp_col", "gt", "sum_a", "sum_b"), ("in", "k", [1, None], False))
