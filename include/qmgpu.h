@@ -235,6 +235,18 @@ int qm_result_rows(qm_result *r, uint64_t *rows);
 int qm_result_qmcb_size(qm_result *r, uint64_t *size);
 /* Writes the QMCB encoding (src/columnar.ts:523-613) into out[0..cap). */
 int qm_result_qmcb(qm_result *r, uint8_t *out, uint64_t cap, uint64_t *written);
+/* QMCB of rows [row0, row0 + nrows) of a result (clamped to the result),
+ * encoded on the device (NULL bitmaps shifted and BOOL columns packed by a
+ * kernel, data copied by DMA) into pinned host memory the library owns:
+ * *out / *size describe it until qm_qmcb_release(*out). The bytes are what
+ * qm_result_qmcb writes for the same rows; a host runtime wraps them without
+ * copying (Node: an external ArrayBuffer; Python: a memoryview), and a
+ * cursor takes one range per batch. Replaces the host-side encode of
+ * src/columnar.ts:523-613 and backs QueryExecutor.cursor (src/client.ts:1300). */
+int qm_result_qmcb_pinned(qm_result *r, uint64_t row0, uint64_t nrows, const uint8_t **out, uint64_t *size);
+/* Hands a buffer of qm_result_qmcb_pinned back (reused by later calls);
+ * valid after the result and its context are closed. */
+int qm_qmcb_release(const uint8_t *p);
 int qm_result_stats(qm_result *r, qm_stats *out);
 int qm_result_close(qm_result *r);
 

@@ -79,6 +79,9 @@ SIGNATURES = [
     ("qm_result_rows", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_qmcb_size", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qm_result_qmcb", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("qm_result_qmcb_pinned", C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_void_p),
+                                        C.POINTER(C.c_uint64)]),
+    ("qm_qmcb_release", C.c_int, [C.c_void_p]),
     ("qm_result_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("qm_result_close", C.c_int, [C.c_void_p]),
 ]

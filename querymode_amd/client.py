@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import time
+import weakref
 from dataclasses import dataclass, field, replace
 from typing import Any, Iterable
 
@@ -196,19 +197,57 @@ class _Plan:
             self.handle = None
 
 
+def pinned_qmcb(lib, rh, row0: int = 0, nrows: int = 2 ** 63) -> memoryview:
+    """QMCB of rows [row0, row0 + nrows) of a device result, encoded on the
+    device into pinned host memory (qm_result_qmcb_pinned) and wrapped without
+    a copy; the buffer goes back to the library when the last view is gone."""
+    ptr, size = C.c_void_p(), C.c_uint64()
+    N.check(lib.qm_result_qmcb_pinned(rh, row0, nrows, C.byref(ptr), C.byref(size)))
+    arr = (C.c_uint8 * size.value).from_address(ptr.value)
+    weakref.finalize(arr, lib.qm_qmcb_release, C.c_void_p(ptr.value))
+    return memoryview(arr).cast("B")
+
+
 def _read_result(lib, rh) -> tuple[memoryview, dict]:
     try:
-        size = C.c_uint64()
-        N.check(lib.qm_result_qmcb_size(rh, C.byref(size)))
-        # an uninitialised host buffer the library writes the QMCB bytes into
-        buf = np.empty(max(1, size.value), dtype=np.uint8)
-        wr = C.c_uint64()
-        N.check(lib.qm_result_qmcb(rh, buf.ctypes.data_as(C.c_void_p), size.value, C.byref(wr)))
+        data = pinned_qmcb(lib, rh)
         st = N.Stats()
         N.check(lib.qm_result_stats(rh, C.byref(st)))
-        return memoryview(buf)[: wr.value], st.as_dict()
+        return data, st.as_dict()
     finally:
         lib.qm_result_close(rh)
+
+
+class DeviceResult:
+    """A result kept in device memory; ``fetch(row0, n)`` encodes one range of
+    it as QMCB (the pull of a cursor). Closed with ``close()``."""
+
+    def __init__(self, lib, handle):
+        self._lib, self._h = lib, handle
+        n = C.c_uint64()
+        N.check(lib.qm_result_rows(handle, C.byref(n)))
+        self.rows = int(n.value)
+
+    def fetch(self, row0: int, nrows: int) -> memoryview:
+        if not self._h:
+            raise QueryModeError("INVALID_ARGUMENT", "the result is closed")
+        return pinned_qmcb(self._lib, self._h, row0, nrows)
+
+    def stats(self) -> dict:
+        st = N.Stats()
+        N.check(self._lib.qm_result_stats(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.qm_result_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
 
 
 def _batch_results(lib, outs, n: int, t0: float) -> list[QueryResult]:
@@ -315,6 +354,32 @@ class GpuExecutor:
             raise QueryModeError("TABLE_NOT_FOUND", f"Table not found: {name}")
         return t
 
+    def run_device(self, desc: QueryDescriptor) -> DeviceResult:
+        """Execute and keep the result on the device (read it with fetch)."""
+        left = self._table(desc.table)
+        right = self._table(desc.join.right.table) if desc.join is not None else None
+        plan = _Plan(self.dev, desc)
+        try:
+            rh = C.c_void_p()
+            N.check(self.dev.lib.qm_exec(plan.handle, left.handle, right.handle if right else None, C.byref(rh)))
+        finally:
+            plan.close()
+        return DeviceResult(self.dev.lib, rh)
+
+    def cursor(self, desc: QueryDescriptor, batch_size: int = 1000):
+        """QueryExecutor.cursor (src/client.ts:1300): the result stays in device
+        memory and each batch of `batch_size` rows is encoded and handed over
+        when it is pulled. Yields row lists."""
+        op = GpuQueryOperator(lambda: self.run_device(desc), batch_size)
+        try:
+            while True:
+                b = op.next()
+                if b is None:
+                    break
+                yield b
+        finally:
+            op.close()
+
     def run_qmcb(self, desc: QueryDescriptor) -> tuple[bytes, dict]:
         left = self._table(desc.table)
         right = self._table(desc.join.right.table) if desc.join is not None else None
@@ -401,7 +466,7 @@ class GpuExecutor:
         return {"table": desc.table, "operators": ops, "device": self.dev.info()}
 
     def to_operator(self, desc: QueryDescriptor) -> GpuQueryOperator:
-        return GpuQueryOperator(lambda: self.run_qmcb(desc)[0])
+        return GpuQueryOperator(lambda: self.run_device(desc))
 
     # reference-style camelCase alias
     toOperator = to_operator
@@ -529,7 +594,7 @@ class DataFrame:
         return r.rows[0] if r.rows else None
 
     def cursor(self, batch_size: int = 4096):
-        op = GpuQueryOperator(lambda: self._executor.run_qmcb(self._desc)[0], batch_size)
+        op = GpuQueryOperator(lambda: self._executor.run_device(self._desc), batch_size)
         try:
             while True:
                 b = op.next()

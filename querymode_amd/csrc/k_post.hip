@@ -297,6 +297,31 @@ __global__ void gather_u64_kernel(const uint64_t *src, const uint64_t *idx, uint
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[idx[i]];
 }
 
+// ---- QMCB encoding of a row range (src/columnar.ts:523-613) ----
+// dst[j] = the QMCB bits of rows [row0 + 8j, row0 + 8j + 8) within the range:
+// a NULL bitmap shifted to start at row0, or a one-byte-per-row BOOL column
+// packed; bits past the range are 0. `any` gets 1 when a bit is set.
+__global__ void qmcb_bits_kernel(const uint8_t *src, int is_bool, uint64_t row0, uint64_t nrows, uint8_t *dst,
+                                 uint32_t *any) {
+  const uint64_t nb = (nrows + 7) / 8;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t seen = 0;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nb; j += stride) {
+    uint32_t byte = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t k = j * 8 + b;
+      if (k >= nrows) break;
+      const uint64_t r = row0 + k;
+      const uint32_t bit = is_bool ? (gld<uint8_t>(src, r) != 0) : ((gld<uint8_t>(src, r >> 3) >> (r & 7)) & 1u);
+      byte |= bit << b;
+    }
+    dst[j] = (uint8_t)byte;
+    seen |= byte;
+  }
+  if (any && seen) atomicOr(any, 1u);
+}
+
 }  // namespace qm
 
 using namespace qm;
@@ -370,4 +395,12 @@ extern "C" hipError_t qmk_radix_pairs(const uint64_t *kin, uint64_t *kout, const
   size_t b = tmp_bytes;
   return rocprim::radix_sort_pairs(tmp, b, const_cast<uint64_t *>(kin), kout, const_cast<uint64_t *>(vin), vout,
                                    (size_t)n, 0, (unsigned)end_bit, st);
+}
+
+extern "C" hipError_t qmk_qmcb_bits(const uint8_t *src, int is_bool, uint64_t row0, uint64_t nrows, uint8_t *dst,
+                                    uint32_t *any, hipStream_t st) {
+  if (nrows == 0) return hipSuccess;
+  hipLaunchKernelGGL(qmcb_bits_kernel, dim3(post_grid((nrows + 7) / 8)), dim3(256), 0, st, src, is_bool, row0, nrows,
+                     dst, any);
+  return hipGetLastError();
 }

@@ -4459,6 +4459,130 @@ extern "C" int qm_result_qmcb(qm_result *r, uint8_t *out, uint64_t cap, uint64_t
   API_END
 }
 
+// ---- QMCB written on the device into pinned host memory ----
+namespace {
+// Pinned host buffers handed out by qm_result_qmcb_pinned, process-wide (a
+// buffer may outlive its context): live ones by address, released ones by
+// size class for reuse (no host allocator call per batch of a cursor).
+std::mutex g_pin_mu;
+std::map<const uint8_t *, size_t> g_pin_live;
+std::multimap<size_t, uint8_t *> g_pin_free;
+
+size_t pin_class(size_t bytes) {
+  size_t c = 64 << 10;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+
+uint8_t *pin_take(size_t bytes) {
+  const size_t c = pin_class(bytes);
+  {
+    std::lock_guard<std::mutex> g(g_pin_mu);
+    auto it = g_pin_free.find(c);
+    if (it != g_pin_free.end()) {
+      uint8_t *p = it->second;
+      g_pin_free.erase(it);
+      g_pin_live[p] = c;
+      return p;
+    }
+  }
+  void *p = nullptr;
+  HIPX(hipHostMalloc(&p, c, hipHostMallocDefault));
+  std::lock_guard<std::mutex> g(g_pin_mu);
+  g_pin_live[static_cast<uint8_t *>(p)] = c;
+  return static_cast<uint8_t *>(p);
+}
+}  // namespace
+
+extern "C" int qm_result_qmcb_pinned(qm_result *r, uint64_t row0, uint64_t nrows, const uint8_t **out,
+                                     uint64_t *size) {
+  API_BEGIN
+  if (!r || !out || !size) fail(QM_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  *size = 0;
+  if (r->cols.size() > 0xFFFF) fail(QM_ERR_UNSUPPORTED, "too many columns");
+  row0 = std::min<uint64_t>(row0, r->rows);
+  nrows = std::min<uint64_t>(nrows, r->rows - row0);
+  if (nrows > 0xFFFFFFFFull) fail(QM_ERR_UNSUPPORTED, "QMCB row count is 32-bit");
+  qm_ctx *ctx = r->ctx;
+  LaneHold hold(ctx);
+  const size_t nc = r->cols.size();
+  const uint64_t nb = (nrows + 7) / 8;
+  // bitmaps of the range and packed BOOL columns, on the device
+  std::vector<int64_t> bits_off(nc, -1), bool_off(nc, -1);
+  uint64_t stage = 0;
+  for (size_t i = 0; i < nc; ++i) {
+    if (r->cols[i].nulls.p) { bits_off[i] = (int64_t)stage; stage += (nb + 63) / 64 * 64; }
+    if (r->cols[i].dtype == DT_BOOL) { bool_off[i] = (int64_t)stage; stage += (nb + 63) / 64 * 64; }
+  }
+  DevBuf dstage(ctx, std::max<uint64_t>(stage, 64)), dany(ctx, nc * 4 + 64);
+  zero(ctx, dany.p, nc * 4 + 64);
+  for (size_t i = 0; i < nc; ++i) {
+    uint8_t *st = dstage.as<uint8_t>();
+    if (bits_off[i] >= 0)
+      HIPX(qmk_qmcb_bits(r->cols[i].nulls.as<uint8_t>(), 0, row0, nrows, st + bits_off[i], dany.as<uint32_t>() + i,
+                         ctx->lane().stream));
+    if (bool_off[i] >= 0)
+      HIPX(qmk_qmcb_bits(r->cols[i].data.as<uint8_t>(), 1, row0, nrows, st + bool_off[i], nullptr,
+                         ctx->lane().stream));
+  }
+  std::vector<uint32_t> any(nc, 0);
+  if (nc) d2h(ctx, any.data(), dany.p, nc);
+  // layout: the header, then per column [NULL bitmap] data (src/columnar.ts:523-613)
+  uint64_t total = 10;
+  for (const OutCol &c : r->cols) {
+    if (c.name.size() > 0xFFFF) fail(QM_ERR_UNSUPPORTED, "column name too long");
+    total += 2 + c.name.size() + 2;
+  }
+  std::vector<uint64_t> at_bits(nc), at_data(nc);
+  for (size_t i = 0; i < nc; ++i) {
+    if (any[i]) { at_bits[i] = total; total += nb; }
+    at_data[i] = total;
+    total += qmcb_data_bytes(r->cols[i].dtype, nrows);
+  }
+  uint8_t *buf = pin_take(total);
+  uint8_t *w = buf;
+  auto put32 = [&](uint32_t v) { memcpy(w, &v, 4); w += 4; };
+  auto put16 = [&](uint16_t v) { memcpy(w, &v, 2); w += 2; };
+  put32(0x42434D51u);
+  put32((uint32_t)nrows);
+  put16((uint16_t)nc);
+  for (size_t i = 0; i < nc; ++i) {
+    const OutCol &c = r->cols[i];
+    put16((uint16_t)c.name.size());
+    memcpy(w, c.name.data(), c.name.size());
+    w += c.name.size();
+    *w++ = (uint8_t)qmcb_tag(c.dtype);
+    *w++ = any[i] ? 1 : 0;
+  }
+  // the body by DMA straight into the pinned buffer
+  hipStream_t st = ctx->lane().stream;
+  for (size_t i = 0; i < nc; ++i) {
+    const OutCol &c = r->cols[i];
+    if (any[i] && nb) HIPX(hipMemcpyAsync(buf + at_bits[i], dstage.as<uint8_t>() + bits_off[i], nb, hipMemcpyDeviceToHost, st));
+    const uint64_t db = qmcb_data_bytes(c.dtype, nrows);
+    if (!db) continue;
+    const uint8_t *src = bool_off[i] >= 0 ? dstage.as<uint8_t>() + bool_off[i]
+                                          : static_cast<const uint8_t *>(c.data.p) + row0 * (uint64_t)dt_size(c.dtype);
+    HIPX(hipMemcpyAsync(buf + at_data[i], src, db, hipMemcpyDeviceToHost, st));
+  }
+  stream_sync(ctx);
+  *out = buf;
+  *size = total;
+  API_END
+}
+
+extern "C" int qm_qmcb_release(const uint8_t *p) {
+  API_BEGIN
+  if (!p) return QM_OK;
+  std::lock_guard<std::mutex> g(g_pin_mu);
+  auto it = g_pin_live.find(p);
+  if (it == g_pin_live.end()) fail(QM_ERR_INVALID_ARG, "not a QMCB buffer of this library (or released twice)");
+  g_pin_free.emplace(it->second, const_cast<uint8_t *>(p));
+  g_pin_live.erase(it);
+  API_END
+}
+
 extern "C" int qm_result_stats(qm_result *r, qm_stats *out) {
   API_BEGIN
   if (!r || !out) fail(QM_ERR_INVALID_ARG, "null argument");

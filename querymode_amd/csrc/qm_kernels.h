@@ -93,6 +93,8 @@ hipError_t qmk_order_codes(const qm::DCol *c, const uint64_t *perm, uint64_t n, 
 hipError_t qmk_distinct_marks(const qm::DCol *d_cols, int ncols, const uint64_t *perm, uint64_t n, uint8_t *keep,
                               hipStream_t st);
 hipError_t qmk_gather_u64(const uint64_t *src, const uint64_t *idx, uint64_t n, uint64_t *dst, hipStream_t st);
+hipError_t qmk_qmcb_bits(const uint8_t *src, int is_bool, uint64_t row0, uint64_t nrows, uint8_t *dst, uint32_t *any,
+                         hipStream_t st);
 size_t qmk_radix_pairs_tmp_bytes(uint64_t n);
 hipError_t qmk_radix_pairs(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout, uint64_t n,
                            int end_bit, void *tmp, size_t tmp_bytes, hipStream_t st);

@@ -13,7 +13,7 @@
  * Exports: init, shutdown, lastError, deviceInfo, tableCreate, tableDrop,
  * tableRows, planCompile, planClose, exec, batchExec, batchSubmit (a Promise of
  * the results; the batch runs while the event loop goes on), planDeviceBytes,
- * resultRows, resultQmcb, resultStats, resultClose.
+ * resultRows, resultQmcb, resultQmcbRange, resultStats, resultClose.
  */
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -564,20 +564,48 @@ static napi_value js_result_rows(napi_env env, napi_callback_info info) {
   return v;
 }
 
-/* resultQmcb(result) -> ArrayBuffer holding the QMCB bytes */
+static void release_qmcb(napi_env env, void *data, void *hint) {
+  (void)env;
+  (void)hint;
+  qm_qmcb_release((const uint8_t *)data);
+}
+
+/* QMCB of rows [row0, row0 + nrows) encoded on the device into pinned host
+ * memory, handed to JS as an external ArrayBuffer over those bytes (no copy);
+ * the buffer goes back to the library when the ArrayBuffer is collected. */
+static napi_value qmcb_range(napi_env env, handle *h, uint64_t row0, uint64_t nrows) {
+  const uint8_t *p = NULL;
+  uint64_t size = 0;
+  napi_value ab;
+  int rc = qm_result_qmcb_pinned((qm_result *)h->p, row0, nrows, &p, &size);
+  if (rc != QM_OK) return throw_rc(env, rc);
+  if (napi_create_external_arraybuffer(env, (void *)p, (size_t)size, release_qmcb, NULL, &ab) != napi_ok) {
+    qm_qmcb_release(p);
+    return throw_code(env, "QUERY_FAILED", "cannot wrap the QMCB buffer");
+  }
+  return ab;
+}
+
+/* resultQmcb(result) -> ArrayBuffer holding the QMCB bytes of the whole result */
 static napi_value js_result_qmcb(napi_env env, napi_callback_info info) {
-  napi_value argv[1], ab;
+  napi_value argv[1];
   get_args(env, info, 1, argv);
   handle *h = unwrap(env, argv[0], H_RESULT);
   if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a result");
-  uint64_t size = 0, written = 0;
-  int rc = qm_result_qmcb_size((qm_result *)h->p, &size);
-  if (rc != QM_OK) return throw_rc(env, rc);
-  void *buf;
-  CHECK_NAPI(napi_create_arraybuffer(env, (size_t)size, &buf, &ab));
-  rc = qm_result_qmcb((qm_result *)h->p, (uint8_t *)buf, size, &written);
-  if (rc != QM_OK) return throw_rc(env, rc);
-  return ab;
+  return qmcb_range(env, h, 0, UINT64_MAX);
+}
+
+/* resultQmcbRange(result, row0, nrows) -> ArrayBuffer of QMCB for those rows (a cursor's batch) */
+static napi_value js_result_qmcb_range(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  get_args(env, info, 3, argv);
+  handle *h = unwrap(env, argv[0], H_RESULT);
+  if (!h) return throw_code(env, "INVALID_ARGUMENT", "expected a result");
+  double r0 = 0, n = 0;
+  if (napi_get_value_double(env, argv[1], &r0) != napi_ok || napi_get_value_double(env, argv[2], &n) != napi_ok ||
+      r0 < 0 || n < 0)
+    return throw_code(env, "INVALID_ARGUMENT", "row0 and nrows must be non-negative numbers");
+  return qmcb_range(env, h, (uint64_t)r0, (uint64_t)n);
 }
 
 static void set_num(napi_env env, napi_value o, const char *k, double x) {
@@ -626,6 +654,7 @@ static napi_value init_module(napi_env env, napi_value exports) {
     {"planDeviceBytes", NULL, js_plan_device_bytes, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultRows", NULL, js_result_rows, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultQmcb", NULL, js_result_qmcb, NULL, NULL, NULL, napi_enumerable, NULL},
+    {"resultQmcbRange", NULL, js_result_qmcb_range, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultStats", NULL, js_result_stats, NULL, NULL, NULL, napi_enumerable, NULL},
     {"resultClose", NULL, js_result_close, NULL, NULL, NULL, napi_enumerable, NULL},
   };

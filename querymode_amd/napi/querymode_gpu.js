@@ -225,6 +225,30 @@ class GpuExecutor {
     } finally { addon.planClose(plan); }
   }
 
+  /**
+   * QueryExecutor.cursor (src/client.ts:1300): the result stays in device
+   * memory; each pull encodes the next batchSize rows as QMCB on the device
+   * into pinned memory (an external ArrayBuffer, no copy) and yields them as rows.
+   */
+  async *cursor(desc, batchSize = 1000) {
+    const left = this.tables.get(desc.table);
+    if (!left) throw qmError('TABLE_NOT_FOUND', 'table ' + desc.table + ' is not registered');
+    const right = desc.join ? this.tables.get(desc.join.right.table) : null;
+    if (desc.join && !right) throw qmError('TABLE_NOT_FOUND', 'table ' + desc.join.right.table + ' is not registered');
+    const plan = addon.planCompile(this.ctx, encodePlan(desc));
+    let res = null;
+    try {
+      res = addon.exec(plan, left, right);
+      const n = addon.resultRows(res);
+      for (let r0 = 0; r0 < n; r0 += batchSize) {
+        yield qmcbToRows(decodeQMCB(addon.resultQmcbRange(res, r0, batchSize)));
+      }
+    } finally {
+      if (res) addon.resultClose(res);
+      addon.planClose(plan);
+    }
+  }
+
   /** QueryExecutor.execute -> QueryResult-shaped object (src/types.ts:262-295). */
   execute(desc) {
     const t0 = Date.now();

@@ -48,36 +48,64 @@ def materialize_rows(batch: PipelineBatch) -> list[dict]:
 class GpuQueryOperator:
     """Terminal operator over a device result. The device work happens on the
     first pull (like AggregateOperator / TopKOperator draining upstream at their
-    first next(), src/operators.ts:1656-1707, 1819-1824)."""
+    first next(), src/operators.ts:1656-1707, 1819-1824); the result then stays
+    in device memory and every pull encodes the next `batch_size` rows as QMCB
+    on the device and hands them over (no host copy of the whole result).
+
+    `run` returns either a device result (an object with ``rows``,
+    ``fetch(row0, n)`` and ``close()``) or QMCB bytes of the whole result."""
 
     def __init__(self, run, batch_size: int = 4096):
-        self._run = run              # () -> QMCB bytes
-        self._batch_size = batch_size
+        self._run = run
+        self._batch_size = max(1, int(batch_size))
+        self._src = None
         self._batch: qmcb.ColumnarBatch | None = None
         self._pos = 0
         self._closed = False
 
     def _ensure(self) -> None:
-        if self._batch is None:
-            self._batch = qmcb.decode(self._run())
+        if self._src is None and self._batch is None:
+            r = self._run()
+            if hasattr(r, "fetch"):
+                self._src = r
+            else:
+                self._batch = qmcb.decode(r)
 
-    def next_columnar(self) -> PipelineBatch | None:
-        if self._closed:
-            return None
-        self._ensure()
+    def _slice(self) -> qmcb.ColumnarBatch | None:
+        if self._src is not None:
+            if self._pos >= self._src.rows:
+                return None
+            b = qmcb.decode(self._src.fetch(self._pos, self._batch_size))
+            self._pos += b.row_count
+            return b
         b = self._batch
         if self._pos >= b.row_count:
             return None
         lo, hi = self._pos, min(self._pos + self._batch_size, b.row_count)
         self._pos = hi
+        cols = []
+        for c in b.columns:
+            nb = None
+            if c.null_bitmap is not None:
+                nb = qmcb.pack_bits(qmcb.unpack_bits(c.null_bitmap, b.row_count)[lo:hi])
+            cols.append(qmcb.ColumnarColumn(c.name, c.dtype, c.data[lo:hi] if c.data is not None else None, hi - lo, nb))
+        return qmcb.ColumnarBatch(cols, hi - lo)
+
+    def next_columnar(self) -> PipelineBatch | None:
+        if self._closed:
+            return None
+        self._ensure()
+        b = self._slice()
+        if b is None:
+            return None
         cols = {}
         for c in b.columns:
-            vals = list(c.data[lo:hi]) if c.data is not None else [None] * (hi - lo)
+            vals = list(c.data) if c.data is not None else [None] * b.row_count
             if c.null_bitmap is not None:
-                nulls = qmcb.unpack_bits(c.null_bitmap, b.row_count)[lo:hi]
+                nulls = qmcb.unpack_bits(c.null_bitmap, b.row_count)
                 vals = [None if nulls[i] else v for i, v in enumerate(vals)]
             cols[c.name] = vals
-        return PipelineBatch(cols, hi - lo)
+        return PipelineBatch(cols, b.row_count)
 
     def next(self) -> RowBatch | None:
         b = self.next_columnar()
@@ -86,6 +114,9 @@ class GpuQueryOperator:
     def close(self) -> None:
         self._closed = True
         self._batch = None
+        if self._src is not None:
+            self._src.close()
+            self._src = None
 
 
 def drain_pipeline(op) -> list[dict]:

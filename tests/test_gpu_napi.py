@@ -46,6 +46,11 @@ out.join = qm.table('rows').join(qm.table('right'), { left: 'id', right: 'id' })
   const pending = qm.executor.executeBatch(frames.map((f) => f.desc));
   out.batch = (await pending).map((r) => r.rows);
   out.solo = frames.map((f) => f.exec().rows);
+  // cursor: the top-K result stays on the device, 7 rows per pull
+  const topDesc = qm.table('rows').sort('amount', 'desc').limit(100).select('id', 'amount').desc;
+  out.cursor = [];
+  out.cursorBatches = 0;
+  for await (const b of qm.executor.cursor(topDesc, 7)) { out.cursor.push(...b); out.cursorBatches++; }
   qm.close();
   console.log(JSON.stringify(out, (k, v) => typeof v === 'bigint' ? v.toString() : v));
 })().catch((e) => { console.error(e); process.exit(1); });
@@ -73,6 +78,7 @@ def test_js_host_end_to_end():
     key = lambda r: int(r["region"])  # noqa: E731
     assert [sorted(b, key=key) for b in out["batch"]] == [sorted(x, key=key) for x in out["solo"]]
     assert all(len(b) == 4 for b in out["batch"])
+    assert out["cursor"] == out["topk"] and out["cursorBatches"] == 15
 
 
 SCRIPT_CLOSE = r"""

@@ -22,7 +22,8 @@ NODE = shutil.which("node")
 pytestmark = pytest.mark.skipif(NODE is None or not ADDON.exists(), reason="node or the addon is not available")
 
 EXPORTS = ["init", "shutdown", "lastError", "deviceInfo", "tableCreate", "tableDrop", "tableRows", "planCompile",
-           "planClose", "exec", "batchExec", "batchSubmit", "planDeviceBytes", "resultRows", "resultQmcb", "resultStats",
+           "planClose", "exec", "batchExec", "batchSubmit", "planDeviceBytes", "resultRows", "resultQmcb", "resultQmcbRange",
+           "resultStats",
            "resultClose"]
 
 
