@@ -95,6 +95,9 @@ typedef struct qm_stats {
                                  record scatter, group count, write), HIP events on the library stream */
   uint64_t pass_records;      /* QM_STRAT_PART_RUNS: rows the first pass kept as records (selected by any
                                  query of the launch) */
+  uint32_t join_parts;        /* key-range passes of a join (1 = the whole build side in one index;
+                                 more when the index would not fit the HBM budget, or forced by the
+                                 join_parts knob) */
 } qm_stats;
 
 #define QM_STRAT_NONE        0

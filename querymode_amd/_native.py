@@ -34,7 +34,7 @@ class Stats(C.Structure):
         ("bytes_algorithmic", C.c_uint64), ("device_ms", C.c_double), ("scan_ms", C.c_double),
         ("wall_ms", C.c_double),
         ("kernel_launches", C.c_uint32), ("retries", C.c_uint32), ("strategy", C.c_uint32), ("pages_skipped", C.c_uint32),
-        ("pass_ms", C.c_double * 4), ("pass_records", C.c_uint64),
+        ("pass_ms", C.c_double * 4), ("pass_records", C.c_uint64), ("join_parts", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

@@ -1287,8 +1287,30 @@ DevBuf select_rows(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, uint64_t lim
 
 // fused: the index feeds a fused aggregate, which needs only the unique-key
 // DIRECT index when the build keys are unique.
-void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32_t *launches, DevBuf *keep,
-                bool fused) {
+// One key range of a partitioned join: the index holds the build rows with
+// lo <= key <= hi only (every probe row is looked up against it; keys outside
+// the range miss), so one pass per range covers the join once.
+struct KeyRange {
+  int64_t lo, hi;
+};
+
+void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl_in, JoinIndex &ji, uint32_t *launches, DevBuf *keep,
+                bool fused, const KeyRange *part = nullptr) {
+  Plan pl_part;
+  if (part) {  // the range as one more build-side filter
+    pl_part = pl_in;
+    Filter f;
+    f.col = pl_in.rkey;
+    f.op = OP_BETWEEN;
+    f.vkind = LIT_PAIR;
+    Lit a, b;
+    a.kind = b.kind = LIT_I64;
+    a.i = part->lo;
+    b.i = part->hi;
+    f.list = {a, b};
+    pl_part.rfilters.push_back(f);
+  }
+  const Plan &pl = part ? pl_part : pl_in;
   const qm_table *R = cs.R;
   int rk = R->find(pl.rkey);
   if (rk < 0) fail(QM_ERR_COLUMN_NOT_FOUND, "join key not found: %s", pl.rkey.c_str());
@@ -1329,12 +1351,20 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
   key.dtype = kc.dtype;
   key.side = 1;
   uint64_t dom = 0;
-  if (!ji.key_float && kc.st.has_int) dom = (uint64_t)(kc.st.imax - kc.st.imin) + 1;
+  int64_t klo = kc.st.imin;
+  if (!ji.key_float && kc.st.has_int) {
+    int64_t khi = kc.st.imax;
+    if (part) {
+      klo = std::max(klo, part->lo);
+      khi = std::min(khi, part->hi);
+    }
+    dom = khi >= klo ? (uint64_t)(khi - klo) + 1 : 1;
+  }
   if (!ji.key_float && kc.st.has_int && dom != 0 && dom <= std::max<uint64_t>(4 * nbuild, 1u << 16) &&
       dom <= (1ull << 31)) {
     ji.mode = 1;
     ji.cap = dom;
-    ji.kmin = kc.st.imin;
+    ji.kmin = klo;
   } else {
     ji.mode = 2;
     ji.cap = pow2ceil(std::max<uint64_t>(1024, 2 * nbuild));
@@ -1381,6 +1411,65 @@ void build_join(qm_ctx *ctx, ColSpace &cs, const Plan &pl, JoinIndex &ji, uint32
 // Pack the build-side columns a fused probe reads (at most JOIN_PAY_BYTES)
 // into 16 bytes per DIRECT key slot, widest first, with a flags word
 // (DScan::join_pay). Leaves sc.join_pay null when they do not fit.
+// Device bytes of the index build_join lays out for `nbuild` build rows whose
+// keys span `dom` values (DIRECT when dense enough, else open addressing). A
+// fused join over unique DIRECT keys claims one slot per key and packs the
+// build columns beside it, and builds nothing else.
+uint64_t join_index_bytes(const qm_ctx *ctx, uint64_t nbuild, uint64_t dom, bool key_float, bool fused,
+                          bool unique_keys) {
+  const bool direct = !key_float && dom != 0 && dom <= std::max<uint64_t>(4 * nbuild, 1u << 16) && dom <= (1ull << 31);
+  if (direct && fused && unique_keys && ctx->tune.join_unique && ctx->tune.join_claim)
+    return dom * 4 + (ctx->tune.join_payload ? dom * 16 : 0) + 128;
+  const uint64_t cap = direct ? dom : pow2ceil(std::max<uint64_t>(1024, 2 * nbuild));
+  const uint64_t nent = direct ? cap : cap + 1;
+  const uint64_t twords = direct ? cap : 2 * (cap + 1);
+  uint64_t b = twords * 8 + nbuild * 4 + nent * 4 + qmk_join_build_tmp_words(nent, nbuild) * 8;
+  if (direct) b += cap * 4 + (fused ? cap * 16 : 0);  // unique-row slots, packed build columns
+  return b;
+}
+
+// The key ranges of a join's passes (the reference partitions a build side
+// that exceeds its memory budget and joins partition by partition,
+// src/operators.ts:2466-2637; here the build side stays resident and the
+// index is what must fit). Empty: one pass over the whole build side. Else
+// the fewest equal ranges of the build key's span (a power of two) whose
+// indexes each fit half the HBM budget left, or `join_parts` ranges when that
+// knob is set. Integer keys only.
+std::vector<KeyRange> join_ranges(qm_ctx *ctx, const qm_table *R, const Plan &pl, bool fused) {
+  std::vector<KeyRange> out;
+  const int rk = R->find(pl.rkey);
+  if (rk < 0) return out;
+  const TCol &kc = R->cols[rk];
+  if (dt_float(kc.dtype) || !kc.st.has_int) return out;
+  const uint64_t dom = (uint64_t)kc.st.imax - (uint64_t)kc.st.imin + 1;  // 0: the whole 2^64 span
+  const bool uniq = kc.st.dup_frac == 0 && !kc.st.nulls && kc.st.distinct >= 0.9 * (double)kc.st.nonnull;
+  uint64_t P = (uint64_t)std::max(0, ctx->tune.join_parts);
+  if (P == 0) {
+    const uint64_t need = join_index_bytes(ctx, R->rows, dom, false, fused, uniq);
+    uint64_t used;
+    {
+      std::lock_guard<std::mutex> g(ctx->pool_mu);
+      used = ctx->used.load() - ctx->pooled;
+    }
+    const uint64_t room = (ctx->budget > used ? ctx->budget - used : 0) / 2;
+    if (need <= room) return out;
+    P = 2;
+    while (P < 4096 &&
+           join_index_bytes(ctx, R->rows / P + 1, dom ? (dom + P - 1) / P : (~0ull / P + 1), false, fused, uniq) > room)
+      P *= 2;
+  }
+  if (P <= 1) return out;
+  const uint64_t width = dom ? (dom + P - 1) / P : (~0ull / P + 1);
+  for (uint64_t p = 0; p < P; ++p) {
+    const uint64_t off = p * width;
+    if (dom && off >= dom) break;
+    const uint64_t last = (dom && dom - off <= width) ? dom - 1 : off + width - 1;
+    out.push_back({(int64_t)((uint64_t)kc.st.imin + off), (int64_t)((uint64_t)kc.st.imin + last)});
+    if (!dom && p + 1 == P) break;
+  }
+  return out;
+}
+
 void attach_join_payload(qm_ctx *ctx, const ColSpace &cs, const JoinIndex &ji, DScan &sc, DevBuf &jpay,
                          uint32_t *launches) {
   std::vector<int> side1;
@@ -1488,6 +1577,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.sorted1; }},
     {"join_unique", "QM_TUNE_JOIN_UNIQUE", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_unique = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
+    {"join_parts", "QM_TUNE_JOIN_PARTS", 0, 4096, [](qm_ctx::Tune &t, double v) { t.join_parts = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.join_parts; }},
     {"join_claim", "QM_TUNE_JOIN_CLAIM", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_claim = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_claim; }},
     {"join_payload", "QM_TUNE_JOIN_PAYLOAD", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_payload = v != 0; },
@@ -1924,7 +2015,8 @@ uint32_t run_partitioned(qm_ctx *ctx, const DScan &sc, const DQuery *d_q, const 
 // Run one or more aggregate queries (same table) through the fused scan and
 // finalise each into a result.
 void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std::vector<const Plan *> &plans,
-                    std::vector<qm_result *> &results, bool fused_join, JoinIndex *ji) {
+                    std::vector<qm_result *> &results, bool fused_join, JoinIndex *ji,
+                    const std::vector<KeyRange> *parts = nullptr) {
   const size_t nq = plans.size();
   std::vector<ColSpace> spaces(1);
   ColSpace &cs = spaces[0];
@@ -1962,21 +2054,10 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       q.lds_offset = (int)lds_off;
       lds_off += (uint64_t)q.lds_words;
     }
-    // launch
-    DScan sc;
-    memset(&sc, 0, sizeof sc);
-    sc.nrows = T->rows;
-    sc.ncols = (int)cs.cols.size();
-    sc.nqueries = (int)nq;
-    for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
-    if (fused_join) {
-      int ps = cs.plain(plans[0]->lkey);
-      sc.ncols = (int)cs.cols.size();
-      for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
-      attach_join(sc, *ji, ps);
-      sc.join_unique = ji->unique.as<uint32_t>();  // fused aggregate: one gather per probe row
-      if (ji->unique.p && ctx->tune.join_payload) attach_join_payload(ctx, cs, *ji, sc, jpay, &launches);
-    }
+    // launch: one pass, or one per key range of a partitioned join (every
+    // pass adds into the same group tables)
+    const bool ranged = fused_join && parts && !parts->empty();
+    if (ranged && ab[0].part) fail(QM_ERR_UNSUPPORTED, "a join in key-range passes with a partitioned group-by");
     std::vector<DQuery> hq(nq);
     for (size_t i = 0; i < nq; ++i) hq[i] = ab[i].q;
     DevBuf dq(ctx, nq * sizeof(DQuery));
@@ -1985,13 +2066,38 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
     size_t lds_bytes;
     scan_launch_shape(ctx, T->rows, lds_off, &grid, &block, &lds_bytes);
     if (lds_bytes > 160 * 1024) fail(QM_ERR_UNSUPPORTED, "fused queries exceed the workgroup LDS");
-    if (T->rows && nq == 1 && ab[0].part) {
-      launches += run_partitioned(ctx, sc, dq.as<DQuery>(), ab[0], T->rows, selected[0].as<uint64_t>());
-    } else if (T->rows) {
-      ev_record(ctx, ctx->lane().evk0);
-      HIPX(qmk_scan_agg(&sc, dq.as<DQuery>(), grid, block, lds_bytes, ctx->lane().stream));
-      ev_record(ctx, ctx->lane().evk1);
-      ++launches;
+    const size_t npass = ranged ? parts->size() : 1;
+    for (size_t pi = 0; pi < npass; ++pi) {
+      DScan sc;
+      memset(&sc, 0, sizeof sc);
+      sc.nrows = T->rows;
+      sc.ncols = (int)cs.cols.size();
+      sc.nqueries = (int)nq;
+      for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+      JoinIndex jrange;
+      DevBuf jkeep;
+      if (fused_join) {
+        const JoinIndex *jc = ji;
+        if (ranged) {
+          build_join(ctx, cs, *plans[0], jrange, &launches, &jkeep, true, &(*parts)[pi]);
+          jc = &jrange;
+          jpay.reset();  // the packed build columns of this range's index
+        }
+        int ps = cs.plain(plans[0]->lkey);
+        sc.ncols = (int)cs.cols.size();
+        for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+        attach_join(sc, *jc, ps);
+        sc.join_unique = jc->unique.as<uint32_t>();  // fused aggregate: one gather per probe row
+        if (jc->unique.p && ctx->tune.join_payload) attach_join_payload(ctx, cs, *jc, sc, jpay, &launches);
+      }
+      if (T->rows && nq == 1 && ab[0].part) {
+        launches += run_partitioned(ctx, sc, dq.as<DQuery>(), ab[0], T->rows, selected[0].as<uint64_t>());
+      } else if (T->rows) {
+        if (pi == 0) ev_record(ctx, ctx->lane().evk0);
+        HIPX(qmk_scan_agg(&sc, dq.as<DQuery>(), grid, block, lds_bytes, ctx->lane().stream));
+        if (pi + 1 == npass) ev_record(ctx, ctx->lane().evk1);
+        ++launches;
+      }
     }
     // overflow check
     bool retry = false;
@@ -2059,7 +2165,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
     double ms = tm.stop();
     float scan_ms = 0;
     if (T->rows) ev_ms(ctx, &scan_ms, ctx->lane().evk0, ctx->lane().evk1);
-    uint64_t rrows = fused_join ? ji->build_rows : 0;
+    uint64_t rrows = fused_join ? (ji ? ji->build_rows : R->rows) : 0;
     for (size_t i = 0; i < nq; ++i) {
       qm_result *res = results[i];
       res->stats.rows_scanned = T->rows + (fused_join ? R->rows : 0);
@@ -2068,6 +2174,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       res->stats.device_ms = ms;
       res->stats.scan_ms = scan_ms;
       res->stats.kernel_launches = launches;
+      res->stats.join_parts = fused_join ? (uint32_t)std::max<size_t>(1, ranged ? parts->size() : 1) : 0;
     }
     return;
   }
@@ -2107,8 +2214,6 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   bind_filters(ctx, cs, pl.filters, pl.groups, q, ls, false);
   auto proj = projection(cs, pl, pl.has_join);
   res->ctx = ctx;
-  JoinIndex ji;
-  DevBuf keep;
   DevBuf left_rows, right_rows;
   uint64_t n = 0;
   DScan sc;
@@ -2118,31 +2223,64 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   DevBuf dq(ctx, sizeof(DQuery));
   h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
   if (pl.has_join) {
-    build_join(ctx, cs, pl, ji, &launches, &keep, false);
+    // one pass, or one per key range (the pairs of each range follow the
+    // previous range's, as the reference's partitioned join emits them
+    // partition by partition, src/operators.ts:2466-2637)
+    const std::vector<KeyRange> ranges = join_ranges(ctx, R, pl, false);
+    const size_t npass = ranges.empty() ? 1 : ranges.size();
+    uint64_t want = ~0ull;
+    if (!pl.has_sort && pl.limit >= 0 && !pl.post()) want = (uint64_t)pl.limit + (uint64_t)pl.offset;
+    std::vector<DevBuf> lparts, rparts;
+    std::vector<uint64_t> nparts;
     int ps = cs.plain(pl.lkey);
     sc.ncols = (int)cs.cols.size();
     for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
-    attach_join(sc, ji, ps);
-    const uint64_t tile_rows = 256 * RPT;
-    const uint64_t ntiles = std::max<uint64_t>(1, (sc.nrows + tile_rows - 1) / tile_rows);
-    uint64_t nblocks = std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 8);
-    uint64_t tpb = (ntiles + nblocks - 1) / nblocks;
-    nblocks = (ntiles + tpb - 1) / tpb;
-    DevBuf counts(ctx, nblocks * 8), offs(ctx, nblocks * 8), stmp(ctx, qmk_exscan_tmp_words(nblocks) * 8 + 64);
-    HIPX(qmk_join_count(&sc, dq.as<DQuery>(), nblocks, tpb, counts.as<uint64_t>(), ctx->lane().stream));
-    uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nblocks);
-    HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
-                        ctx->lane().stream));
-    uint64_t total = 0;
-    d2h(ctx, &total, total_d, 1);
-    n = total;
-    if (!pl.has_sort && pl.limit >= 0 && !pl.post()) n = std::min<uint64_t>(n, (uint64_t)pl.limit + (uint64_t)pl.offset);
-    left_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
-    right_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
-    if (n)
-      HIPX(qmk_join_write(&sc, dq.as<DQuery>(), nblocks, tpb, offs.as<uint64_t>(), n, left_rows.as<uint64_t>(),
-                          right_rows.as<uint64_t>(), ctx->lane().stream));
-    launches += 5;
+    for (size_t pi = 0; pi < npass && n < want; ++pi) {
+      JoinIndex jr;
+      DevBuf kr;
+      build_join(ctx, cs, pl, jr, &launches, &kr, false, ranges.empty() ? nullptr : &ranges[pi]);
+      DScan sp = sc;
+      attach_join(sp, jr, ps);
+      const uint64_t tile_rows = 256 * RPT;
+      const uint64_t ntiles = std::max<uint64_t>(1, (sp.nrows + tile_rows - 1) / tile_rows);
+      uint64_t nblocks = std::min<uint64_t>(ntiles, (uint64_t)ctx->num_cus * 8);
+      uint64_t tpb = (ntiles + nblocks - 1) / nblocks;
+      nblocks = (ntiles + tpb - 1) / tpb;
+      DevBuf counts(ctx, nblocks * 8), offs(ctx, nblocks * 8), stmp(ctx, qmk_exscan_tmp_words(nblocks) * 8 + 64);
+      HIPX(qmk_join_count(&sp, dq.as<DQuery>(), nblocks, tpb, counts.as<uint64_t>(), ctx->lane().stream));
+      uint64_t *total_d = stmp.as<uint64_t>() + qmk_exscan_tmp_words(nblocks);
+      HIPX(qmk_exscan_u64(counts.as<uint64_t>(), nblocks, offs.as<uint64_t>(), total_d, stmp.as<uint64_t>(),
+                          ctx->lane().stream));
+      uint64_t total = 0;
+      d2h(ctx, &total, total_d, 1);
+      const uint64_t take = std::min<uint64_t>(total, want - n);
+      DevBuf lr(ctx, std::max<uint64_t>(take, 1) * 8), rr(ctx, std::max<uint64_t>(take, 1) * 8);
+      if (take)
+        HIPX(qmk_join_write(&sp, dq.as<DQuery>(), nblocks, tpb, offs.as<uint64_t>(), take, lr.as<uint64_t>(),
+                            rr.as<uint64_t>(), ctx->lane().stream));
+      launches += 5;
+      n += take;
+      lparts.push_back(std::move(lr));
+      rparts.push_back(std::move(rr));
+      nparts.push_back(take);
+    }
+    if (lparts.size() == 1) {
+      left_rows = std::move(lparts[0]);
+      right_rows = std::move(rparts[0]);
+    } else {
+      left_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
+      right_rows = DevBuf(ctx, std::max<uint64_t>(n, 1) * 8);
+      uint64_t at = 0;
+      for (size_t pi = 0; pi < lparts.size(); ++pi) {
+        if (!nparts[pi]) continue;
+        HIPX(hipMemcpyAsync(left_rows.as<uint64_t>() + at, lparts[pi].p, nparts[pi] * 8, hipMemcpyDeviceToDevice,
+                            ctx->lane().stream));
+        HIPX(hipMemcpyAsync(right_rows.as<uint64_t>() + at, rparts[pi].p, nparts[pi] * 8, hipMemcpyDeviceToDevice,
+                            ctx->lane().stream));
+        at += nparts[pi];
+      }
+    }
+    res->stats.join_parts = (uint32_t)npass;
     res->stats.rows_scanned = T->rows + R->rows;
   } else {
     sc.ncols = (int)cs.cols.size();
@@ -3815,18 +3953,29 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       cs.R = right;
       cs.rkey = pl.rkey;
       cs.plan = &pl;
-      uint32_t l = 0;
-      Timer tb(ctx);
-      build_join(ctx, cs, pl, ji, &l, &keep, true);
-      double bms = tb.stop();
-      try {
-        run_aggregates(ctx, left, right, plans, results, true, &ji);
-      } catch (...) {
-        delete results[0];
-        throw;
+      const std::vector<KeyRange> ranges = join_ranges(ctx, right, pl, true);
+      if (ranges.empty()) {
+        uint32_t l = 0;
+        Timer tb(ctx);
+        build_join(ctx, cs, pl, ji, &l, &keep, true);
+        double bms = tb.stop();
+        try {
+          run_aggregates(ctx, left, right, plans, results, true, &ji);
+        } catch (...) {
+          delete results[0];
+          throw;
+        }
+        results[0]->stats.device_ms += bms;
+        results[0]->stats.kernel_launches += l;
+      } else {
+        // the index in key-range passes (each builds its range's index, then probes)
+        try {
+          run_aggregates(ctx, left, right, plans, results, true, nullptr, &ranges);
+        } catch (...) {
+          delete results[0];
+          throw;
+        }
       }
-      results[0]->stats.device_ms += bms;
-      results[0]->stats.kernel_launches += l;
     } else {
       // pages the filter cannot match at either end are not read (zone maps)
       qm_table view;
@@ -4005,8 +4154,19 @@ extern "C" int qm_plan_device_bytes(qm_plan *p, qm_table *left, qm_table *right,
   uint64_t b = 0;
   const uint64_t n = left->rows;
   if (pl.has_join) {
-    // join index: open addressing at twice the build rows (16-byte slots) + row lists
-    b += right->rows * 2 * 16 + right->rows * 4 + 4096;
+    // the join index (of one key-range pass when the budget needs several)
+    const int rk = right->find(pl.rkey);
+    bool kf = true, uniq = false;
+    uint64_t dom = 0;
+    if (rk >= 0 && !dt_float(right->cols[rk].dtype) && right->cols[rk].st.has_int) {
+      kf = false;
+      dom = (uint64_t)right->cols[rk].st.imax - (uint64_t)right->cols[rk].st.imin + 1;
+      const ColStats &ks = right->cols[rk].st;
+      uniq = ks.dup_frac == 0 && !ks.nulls && ks.distinct >= 0.9 * (double)ks.nonnull;
+    }
+    const uint64_t P = std::max<uint64_t>(1, join_ranges(ctx, right, pl, !pl.aggs.empty()).size());
+    const uint64_t pdom = kf ? 0 : (dom ? (dom + P - 1) / P : (~0ull / P + 1));
+    b += join_index_bytes(ctx, right->rows / P + 1, pdom, kf, !pl.aggs.empty(), uniq) + 4096;
   }
   if (!pl.aggs.empty()) {
     BItem it;

@@ -50,8 +50,10 @@ def test_status_codes_match_header():
 
 
 def test_struct_layouts_match_header():
-    # qm_stats: 4 x u64, 3 x f64, 4 x u32, 4 x f64, u64; qm_column_desc: ptr, i32, ptr, ptr
-    assert C.sizeof(N.Stats) == 4 * 8 + 3 * 8 + 4 * 4 + 4 * 8 + 8 and N.Stats.pass_ms.offset == 72
+    # qm_stats: 4 x u64, 3 x f64, 4 x u32, 4 x f64, u64, u32 (+ 4 bytes of tail padding);
+    # qm_column_desc: ptr, i32, ptr, ptr
+    assert C.sizeof(N.Stats) == 4 * 8 + 3 * 8 + 4 * 4 + 4 * 8 + 8 + 8 and N.Stats.pass_ms.offset == 72
+    assert N.Stats.join_parts.offset == 112
     assert N.ColumnDesc.data.offset == 16 and C.sizeof(N.ColumnDesc) == 32
 
 
