@@ -34,9 +34,77 @@ __global__ void gather_cls_kernel(const uint8_t *cls, const uint64_t *idx, uint6
   for (; i < n; i += stride) out[i] = cls[idx[i]];
 }
 
+// ---- the same sort in key-range passes (when its temporaries exceed the budget) ----
+__device__ __forceinline__ bool sort_key_of(const DCol &c, uint64_t i, int desc, uint64_t *key) {
+  bool null = load1_null(c, i);
+  uint64_t code;
+  if (dt_is_float(c.dtype)) {
+    const double v = load1_f(c, i);
+    if (v != v) null = true;
+    code = code_f64_bits(f64_bits(v));
+  } else {
+    code = code_i64(load1_i(c, i));
+  }
+  *key = desc ? ~code : code;
+  return !null;
+}
+
+// rows per 2^16 bins of the top key bits (non-NULL, non-NaN rows)
+__global__ void sort_hist_kernel(DCol c, uint64_t n, int desc, uint32_t *hist) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t k;
+    if (sort_key_of(c, i, desc, &k)) atomicAdd(&hist[k >> 48], 1u);
+  }
+}
+
+// keep[i]: the row's key lies in [lo, hi] (NULL / NaN rows: `nulls` selects them instead)
+__global__ void sort_range_kernel(DCol c, uint64_t n, int desc, uint64_t lo, uint64_t hi, int nulls, uint8_t *keep) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint64_t k;
+    const bool ok = sort_key_of(c, i, desc, &k);
+    keep[i] = nulls ? !ok : (ok && k >= lo && k <= hi);
+  }
+}
+
+__global__ void sort_gather_keys_kernel(DCol c, const uint64_t *ids, uint64_t m, int desc, uint64_t *keys) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    uint64_t k;
+    (void)sort_key_of(c, ids[i], desc, &k);
+    keys[i] = k;
+  }
+}
+
 }  // namespace qm
 
 using namespace qm;
+
+static unsigned sgrid(uint64_t n) {
+  const uint64_t g = (n + 255) / 256;
+  return (unsigned)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+extern "C" hipError_t qmk_sort_hist(const DCol *c, uint64_t n, int desc, uint32_t *hist, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(sort_hist_kernel, dim3(sgrid(n)), dim3(256), 0, st, *c, n, desc, hist);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_sort_range(const DCol *c, uint64_t n, int desc, uint64_t lo, uint64_t hi, int nulls,
+                                     uint8_t *keep, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(sort_range_kernel, dim3(sgrid(n)), dim3(256), 0, st, *c, n, desc, lo, hi, nulls, keep);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t qmk_sort_gather_keys(const DCol *c, const uint64_t *ids, uint64_t m, int desc, uint64_t *keys,
+                                           hipStream_t st) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(sort_gather_keys_kernel, dim3(sgrid(m)), dim3(256), 0, st, *c, ids, m, desc, keys);
+  return hipGetLastError();
+}
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 

@@ -968,6 +968,92 @@ void permute_result(qm_result *res, const uint64_t *d_rows, uint64_t n) {
 
 // rows of a sorted/limited order over one column `c` of length n:
 // returns device row ids (ascending order positions) for [offset, offset+limit)
+DevBuf select_kept(qm_ctx *ctx, const uint8_t *keep, uint64_t n, uint64_t *n_out);
+
+// Rows per pass of a full ORDER BY whose one-pass temporaries would not fit
+// half the HBM budget left (0: one pass), or n / sort_parts when that knob is
+// above 1 (1: always one pass). A pass holds its rows' ids and keys twice plus
+// the radix sort's scratch.
+uint64_t sort_pass_rows(qm_ctx *ctx, uint64_t n, size_t one_pass_bytes) {
+  if (n < 2 || ctx->tune.sort_parts == 1) return 0;
+  if (ctx->tune.sort_parts > 1) return (n + ctx->tune.sort_parts - 1) / ctx->tune.sort_parts;
+  uint64_t used;
+  {
+    std::lock_guard<std::mutex> g(ctx->pool_mu);
+    used = ctx->used.load() - ctx->pooled;
+  }
+  const uint64_t room = (ctx->budget > used ? ctx->budget - used : 0) / 2;
+  if (one_pass_bytes + n * 8 <= room) return 0;
+  const uint64_t fixed = 2 * n + (1u << 20);  // the pass mask (pool-rounded), the key histogram
+  // ids, keys twice, the sorted ids and the radix scratch: 48 bytes a row,
+  // up to 64 once the pool rounds each buffer up to a power of two
+  return room > fixed ? std::max<uint64_t>(1, (room - fixed) / 64) : 1;
+}
+
+// The stable sort of `order_rows` in passes over ranges of the sort key (the
+// reference sorts runs that fit its budget and merges them from spill files,
+// src/operators.ts:2811-3010; here the column stays resident and each pass
+// takes the rows of one key range, so no merge is needed): a histogram of the
+// top 16 key bits cuts the key space into ranges of at most `per_pass` rows
+// (a single hotter bin is one range); each pass selects its rows in row order,
+// sorts them stably by key, and its place in the order is the rows before it;
+// NULL and NaN rows come last in row order. Only the passes overlapping
+// [start, start + take) are sorted and copied into `rows`.
+void sort_in_passes(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, uint64_t start, uint64_t take,
+                    uint64_t per_pass, DevBuf &rows, uint32_t *launches) {
+  const int d = desc ? 1 : 0;
+  DevBuf hist(ctx, 65536 * 4);
+  zero(ctx, hist.p, 65536 * 4);
+  HIPX(qmk_sort_hist(&c, n, d, hist.as<uint32_t>(), ctx->lane().stream));
+  std::vector<uint32_t> h(65536);
+  d2h(ctx, h.data(), hist.p, 65536);
+  std::vector<std::pair<uint64_t, uint64_t>> ranges;  // inclusive key bounds
+  uint64_t acc = 0;
+  uint32_t b0 = 0;
+  for (uint32_t b = 0; b < 65536; ++b) {
+    if (acc && acc + h[b] > per_pass) {
+      ranges.push_back({(uint64_t)b0 << 48, ((uint64_t)b << 48) - 1});
+      b0 = b;
+      acc = 0;
+    }
+    acc += h[b];
+  }
+  ranges.push_back({(uint64_t)b0 << 48, ~0ull});
+  DevBuf keep(ctx, n);
+  uint64_t at = 0;
+  const uint64_t end = start + take;
+  for (size_t pi = 0; pi <= ranges.size() && at < end; ++pi) {
+    const bool nulls = pi == ranges.size();
+    HIPX(qmk_sort_range(&c, n, d, nulls ? 0 : ranges[pi].first, nulls ? 0 : ranges[pi].second, nulls ? 1 : 0,
+                        keep.as<uint8_t>(), ctx->lane().stream));
+    uint64_t m = 0;
+    DevBuf ids = select_kept(ctx, keep.as<uint8_t>(), n, &m);
+    *launches += 4;
+    if (at + m <= start) {  // wholly before the slice
+      at += m;
+      continue;
+    }
+    const uint64_t *src = ids.as<uint64_t>();
+    DevBuf k0, k1, sorted, tmp;
+    if (!nulls && m > 1) {
+      k0 = DevBuf(ctx, m * 8);
+      k1 = DevBuf(ctx, m * 8);
+      sorted = DevBuf(ctx, m * 8);
+      tmp = DevBuf(ctx, std::max<size_t>(qmk_radix_pairs_tmp_bytes(m), 256));
+      HIPX(qmk_sort_gather_keys(&c, ids.as<uint64_t>(), m, d, k0.as<uint64_t>(), ctx->lane().stream));
+      HIPX(qmk_radix_pairs(k0.as<uint64_t>(), k1.as<uint64_t>(), ids.as<uint64_t>(), sorted.as<uint64_t>(), m, 64,
+                           tmp.p, tmp.n, ctx->lane().stream));
+      src = sorted.as<uint64_t>();
+      *launches += 3;
+    }
+    const uint64_t lo = std::max(start, at), hi = std::min(end, at + m);
+    if (hi > lo)
+      HIPX(hipMemcpyAsync(rows.as<uint64_t>() + (lo - start), src + (lo - at), (hi - lo) * 8, hipMemcpyDeviceToDevice,
+                          ctx->lane().stream));
+    at += m;
+  }
+}
+
 DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t limit, int64_t offset, uint64_t *out_n,
                   const DQuery *d_noq, uint32_t *launches) {
   uint64_t start = (uint64_t)offset;
@@ -1019,6 +1105,11 @@ DevBuf order_rows(qm_ctx *ctx, const DCol &c, uint64_t n, bool desc, int64_t lim
   }
   // full stable sort, then slice
   size_t tb = qmk_sort_tmp_bytes(n);
+  const uint64_t per_pass = sort_pass_rows(ctx, n, tb);
+  if (per_pass) {
+    sort_in_passes(ctx, c, n, desc, start, take, per_pass, rows, launches);
+    return rows;
+  }
   DevBuf tmp(ctx, tb);
   DevBuf all(ctx, n * 8);
   HIPX(qmk_sort_keys(&c, n, desc ? 1 : 0, all.as<uint64_t>(), tmp.p, tmp.n, ctx->lane().stream));
@@ -1579,6 +1670,8 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.join_unique; }},
     {"join_parts", "QM_TUNE_JOIN_PARTS", 0, 4096, [](qm_ctx::Tune &t, double v) { t.join_parts = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.join_parts; }},
+    {"sort_parts", "QM_TUNE_SORT_PARTS", 0, 4096, [](qm_ctx::Tune &t, double v) { t.sort_parts = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.sort_parts; }},
     {"join_claim", "QM_TUNE_JOIN_CLAIM", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_claim = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.join_claim; }},
     {"join_payload", "QM_TUNE_JOIN_PAYLOAD", 0, 1, [](qm_ctx::Tune &t, double v) { t.join_payload = v != 0; },

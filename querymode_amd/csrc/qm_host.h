@@ -194,6 +194,7 @@ struct qm_ctx {
     int sorted1 = 1;           // a single sorted-key query takes the lean single-query pass
     int join_unique = 1;       // DIRECT join indexes of unique build keys hold the build row itself
     int join_parts = 0;        // key-range passes of a join (0 = as the HBM budget needs, else forced)
+    int sort_parts = 0;        // passes of a full ORDER BY (0 = as the HBM budget needs, 1 = one, else ~this many)
     int join_claim = 1;        // ... built by one claiming pass over the build rows (fused aggregates)
     int join_payload = 1;      // ... and the fused probe gathers the build columns packed per key slot
     int zone_maps = 1;         // single aggregate queries skip the pages a range filter cannot match

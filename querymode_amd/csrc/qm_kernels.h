@@ -139,6 +139,11 @@ uint32_t qmk_topk_cap(uint32_t k);
 hipError_t qmk_sort_keys(const qm::DCol *c, uint64_t n, int desc, uint64_t *rows_out, void *tmp,
                          size_t tmp_bytes, hipStream_t st);
 size_t qmk_sort_tmp_bytes(uint64_t n);
+hipError_t qmk_sort_hist(const qm::DCol *c, uint64_t n, int desc, uint32_t *hist, hipStream_t st);
+hipError_t qmk_sort_range(const qm::DCol *c, uint64_t n, int desc, uint64_t lo, uint64_t hi, int nulls, uint8_t *keep,
+                          hipStream_t st);
+hipError_t qmk_sort_gather_keys(const qm::DCol *c, const uint64_t *ids, uint64_t m, int desc, uint64_t *keys,
+                                hipStream_t st);
 
 // join build
 // sel: optional ascending build-row ids (rows passing the right side's filters);
