@@ -144,18 +144,9 @@ class Device:
                 nulls[c.name] = qmcb.unpack_bits(c.null_bitmap, b.row_count)
         return self.register(name, cols, nulls)
 
-    def register_parquet(self, name: str, path: str) -> "GpuTable":
-        """Host-side Parquet decode (pyarrow), then one upload."""
-        import pyarrow.parquet as pq
-        tbl = pq.read_table(path)
-        cols, nulls = {}, {}
-        for cname in tbl.column_names:
-            col = tbl.column(cname).combine_chunks()
-            mask = np.asarray(col.is_null(), dtype=bool) if col.null_count else None
-            arr = col.fill_null(0).to_numpy(zero_copy_only=False) if col.null_count else col.to_numpy()
-            cols[cname] = np.asarray(arr)
-            if mask is not None:
-                nulls[cname] = mask
+    def register_parquet(self, name: str, path: str, columns: list[str] | None = None) -> "GpuTable":
+        """Host-side Parquet decode (decode_parquet), then one upload."""
+        cols, nulls = decode_parquet(path, columns)
         return self.register(name, cols, nulls)
 
     def close(self) -> None:
@@ -170,6 +161,32 @@ class Device:
 
     def __exit__(self, *a):
         self.close()
+
+
+_PQ_TYPES = {"double": np.float64, "float": np.float32, "int64": np.int64, "int32": np.int32, "int16": np.int16,
+             "int8": np.int8, "bool": np.bool_, "uint8": np.int16, "uint16": np.int32, "uint32": np.int64}
+
+
+def decode_parquet(path: str, columns: list[str] | None = None) -> tuple[dict, dict]:
+    """Parquet pages -> typed host columns and NULL masks (the host decode of
+    src/parquet-decode.ts:280-350, by pyarrow): numeric and boolean columns;
+    unsigned widths widen to the next signed width; NULL slots hold 0.
+    Other physical types raise UNSUPPORTED."""
+    import pyarrow.parquet as pq
+    tbl = pq.read_table(path, columns=columns)
+    cols, nulls = {}, {}
+    for cname in tbl.column_names:
+        col = tbl.column(cname).combine_chunks()
+        t = str(col.type)
+        if t not in _PQ_TYPES:
+            raise QueryModeError("UNSUPPORTED", f"column {cname}: Parquet type {t} is not on the device path")
+        mask = np.asarray(col.is_null(), dtype=bool) if col.null_count else None
+        arr = col.fill_null(False if t == "bool" else 0).to_numpy(zero_copy_only=False) if col.null_count \
+            else col.to_numpy(zero_copy_only=False)
+        cols[cname] = np.ascontiguousarray(np.asarray(arr).astype(_PQ_TYPES[t], copy=False))
+        if mask is not None:
+            nulls[cname] = mask
+    return cols, nulls
 
 
 class GpuTable:

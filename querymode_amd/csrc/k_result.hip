@@ -320,6 +320,7 @@ __global__ void __launch_bounds__(256) select_count_kernel(DScan sc, const DQuer
   for (uint64_t t = t0; t < t0 + tiles_per_block; ++t) {
     uint64_t row0 = t * tile_rows + (uint64_t)threadIdx.x * RPT;
     if (row0 >= sc.nrows) break;
+    if (page_skipped(sc, t * tile_rows)) continue;
     uint64_t left = sc.nrows - row0;
     uint32_t valid = left >= RPT ? 0xFFu : ((1u << left) - 1);
     cnt += __builtin_popcount(eval_query8(q, sc.cols, row0, valid));
@@ -342,6 +343,7 @@ __global__ void __launch_bounds__(256) select_write_kernel(DScan sc, const DQuer
   for (uint64_t t = t0; t < t0 + tiles_per_block; ++t) {
     if (t * tile_rows >= sc.nrows) break;   // uniform across the block
     if (out_base >= out_cap) break;          // limit reached
+    if (page_skipped(sc, t * tile_rows)) continue;  // uniform: no row of a skipped page is written
     uint64_t row0 = t * tile_rows + (uint64_t)threadIdx.x * RPT;
     uint32_t m = 0;
     if (row0 < sc.nrows) {

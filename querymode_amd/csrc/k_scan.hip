@@ -505,6 +505,7 @@ __global__ void __launch_bounds__(512) scan_agg_kernel(DScan sc_arg, const DQuer
   uint64_t nmatch = 0;  // joined rows (fused probe), added to the counter once
 
   for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    if (page_skipped(sc, tile * tile_rows)) continue;  // uniform across the workgroup
     const uint64_t row0 = tile * tile_rows + (uint64_t)threadIdx.x * RPT;
     uint32_t valid = 0;
     if (row0 < sc.nrows) {

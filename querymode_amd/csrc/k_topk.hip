@@ -98,7 +98,9 @@ __global__ void __launch_bounds__(TK_BLOCK) topk_scan_kernel(DScan sc, const DQu
       uint64_t left = sc.nrows - row0;
       valid = left >= RPT ? 0xFFu : ((1u << left) - 1);
     }
-    uint32_t m = (q.npreds > 0) ? eval_query8(q, sc.cols, row0, valid) : valid;
+    // a page the zone maps rule out holds no candidate: nothing of it is read
+    const bool skip = page_skipped(sc, tile * TK_TILE);
+    uint32_t m = skip ? 0u : ((q.npreds > 0) ? eval_query8(q, sc.cols, row0, valid) : valid);
     if (m) {
       TopKey thr = s_thr;
       if (ga != ~0ull) {

@@ -2167,6 +2167,7 @@ void run_aggregates(qm_ctx *ctx, const qm_table *T, const qm_table *R, const std
       sc.ncols = (int)cs.cols.size();
       sc.nqueries = (int)nq;
       for (size_t i = 0; i < cs.cols.size(); ++i) sc.cols[i] = cs.cols[i];
+      if (!fused_join && !(nq == 1 && ab[0].part)) sc.page_skip = T->page_skip;  // zone maps of a single query
       JoinIndex jrange;
       DevBuf jkeep;
       if (fused_join) {
@@ -2313,6 +2314,7 @@ void run_rows_query(qm_ctx *ctx, const qm_table *T, const qm_table *R, const Pla
   memset(&sc, 0, sizeof sc);
   sc.nrows = T->rows;
   sc.nqueries = 1;
+  if (!pl.has_join) sc.page_skip = T->page_skip;  // zone maps: pages no row of which can be selected
   DevBuf dq(ctx, sizeof(DQuery));
   h2d(ctx, dq.p, reinterpret_cast<const uint8_t *>(&q), sizeof q);
   if (pl.has_join) {
@@ -3961,14 +3963,23 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
   }
 }
 
-// Zone maps: the pages [p0, p1] that an AND of range filters can match at
-// either end of the table. Returns true (and a view of those rows in `v`)
-// when pages are skipped; the view shares the table's device columns
-// (offsets of whole pages keep every column aligned; a qm_table releases
-// nothing by itself). Joins read their whole probe table.
-bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint32_t *skipped) {
+// Zone maps (src/decode.ts:9-100 canSkipPage): a page can hold a selected row
+// only if every top-level AND predicate's code range meets the page's
+// [min, max] codes and, with OR groups, some group's predicates all do (a
+// predicate without a code range, such as IN, IS NULL or a negation, meets
+// every page). Pages that cannot match at either end of the table are cut
+// off: `v` is a view of the remaining whole pages sharing the table's device
+// columns. Pages that cannot match inside that span are marked in a device
+// bitmap (`v.page_skip`) that the scan, selection and top-K kernels skip tile
+// by tile. Returns true when `v` is to be used; `skipped` counts the pages cut
+// off and `interior` the marked ones. Joins read their whole probe table.
+bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint32_t *skipped,
+               uint32_t *interior = nullptr) {
   *skipped = 0;
-  if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || pl.has_join || !pl.computed.empty() || pl.filters.empty()) return false;
+  if (interior) *interior = 0;
+  if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || pl.has_join || !pl.computed.empty() ||
+      (pl.filters.empty() && pl.groups.empty()))
+    return false;
   ColSpace cs;
   cs.L = T;
   cs.plan = &pl;
@@ -3980,26 +3991,53 @@ bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint
   } catch (const Error &) {
     return false;  // the query itself reports the error
   }
-  if (!q.and_only || q.ngroups) return false;
   const uint64_t pages = (T->rows + ZONE_ROWS - 1) / ZONE_ROWS;
-  std::vector<uint8_t> can(pages, 1);
+  const int ng = q.ngroups;
+  // per group (0 = the AND, g = OR group g): can a page hold a row passing it?
+  std::vector<std::vector<uint8_t>> meet(ng + 1, std::vector<uint8_t>(pages, 1));
   bool any = false;
   for (int i = 0; i < q.npreds; ++i) {
+    const DPred &pr = q.preds[i];
     uint64_t lo = 0, hi = 0;
     bool never = false;
-    if (!pred_code_range(q.preds[i], lo, hi, never) || never) continue;
-    const TCol *c = cs.tcols[q.slot_col[q.preds[i].slot]];
+    if (!pred_code_range(pr, lo, hi, never)) continue;
+    const int g = pr.group;
+    if (g > ng) continue;
+    if (never) {
+      std::fill(meet[g].begin(), meet[g].end(), (uint8_t)0);
+      any = true;
+      continue;
+    }
+    const TCol *c = cs.tcols[pr.col];
     if (!c || c->zmin.size() != pages) continue;
     any = true;
     for (uint64_t p = 0; p < pages; ++p)
-      if (c->zmax[p] < lo || c->zmin[p] > hi || c->zmin[p] > c->zmax[p]) can[p] = 0;
+      if (c->zmax[p] < lo || c->zmin[p] > hi || c->zmin[p] > c->zmax[p]) meet[g][p] = 0;
   }
   if (!any) return false;
+  std::vector<uint8_t> can(pages);
+  for (uint64_t p = 0; p < pages; ++p) {
+    bool ok = meet[0][p] != 0;
+    if (ok && ng) {
+      bool some = false;
+      for (int g = 1; g <= ng && !some; ++g) some = meet[g][p] != 0;
+      ok = some;
+    }
+    can[p] = ok ? 1 : 0;
+  }
   uint64_t p0 = 0, p1 = pages - 1;
   while (p0 < pages && !can[p0]) ++p0;
   if (p0 == pages) p0 = p1 = 0;  // nothing can match: one page keeps the launch shapes
   else while (!can[p1]) --p1;
-  if (p0 == 0 && p1 == pages - 1) return false;
+  std::vector<uint32_t> bits((p1 - p0 + 1 + 31) / 32, 0);
+  uint32_t inner = 0;
+  if (interior && can[p0])
+    for (uint64_t p = p0; p <= p1; ++p)
+      if (!can[p]) {
+        bits[(p - p0) >> 5] |= 1u << ((p - p0) & 31);
+        ++inner;
+      }
+  if (p0 == 0 && p1 == pages - 1 && !inner) return false;
   const uint64_t r0 = p0 * ZONE_ROWS, r1 = std::min(T->rows, (p1 + 1) * ZONE_ROWS);
   v.ctx = T->ctx;
   v.name = T->name;
@@ -4015,7 +4053,13 @@ bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint
     c.st = t.st;
     v.cols.push_back(std::move(c));
   }
+  if (inner) {
+    v.page_skip_buf = std::make_shared<DevBuf>(ctx, bits.size() * 4 + 64);
+    h2d(ctx, v.page_skip_buf->p, bits.data(), bits.size());
+    v.page_skip = v.page_skip_buf->as<uint32_t>();
+  }
   *skipped = (uint32_t)(pages - (p1 - p0 + 1));
+  if (interior) *interior = inner;
   return true;
 }
 
@@ -4072,25 +4116,29 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
     } else {
       // pages the filter cannot match at either end are not read (zone maps)
       qm_table view;
-      uint32_t skipped = 0;
-      qm_table *T = zone_view(ctx, left, pl, view, &skipped) ? &view : left;
+      uint32_t skipped = 0, interior = 0;
+      qm_table *T = zone_view(ctx, left, pl, view, &skipped, &interior) ? &view : left;
       // a key column stored in order: segmented reduction over its runs
       BItem it;
       it.idx = 0;
       it.plan = &pl;
       batch_classify(ctx, T, pl, it);
+      bool inner_read = false;  // the path ran reads the pages marked inside the span
       try {
         if (it.kind == BK_SORTED || (it.kind == BK_SHARED && ctx->tune.shared_lean)) {
           std::vector<BItem *> items{&it};
           run_batch(ctx, T, items, results, nullptr);
+          inner_read = true;
         } else {
           run_aggregates(ctx, T, nullptr, plans, results, false, nullptr);
+          const uint32_t st = results[0]->stats.strategy;
+          inner_read = st == QM_STRAT_PART_HASH || st == QM_STRAT_PART_DIRECT;
         }
       } catch (...) {
         delete results[0];
         throw;
       }
-      results[0]->stats.pages_skipped = skipped;
+      results[0]->stats.pages_skipped = skipped + (inner_read ? 0 : interior);
     }
     res.reset(results[0]);
   } else {
@@ -4098,10 +4146,10 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
     // row outputs over a view: the selected rows, their order (row id ties)
     // and the gathered values are all taken within the same view
     qm_table view;
-    uint32_t skipped = 0;
-    qm_table *T = zone_view(ctx, left, pl, view, &skipped) ? &view : left;
+    uint32_t skipped = 0, interior = 0;
+    qm_table *T = zone_view(ctx, left, pl, view, &skipped, &interior) ? &view : left;
     run_rows_query(ctx, T, right, pl, res.get());
-    res->stats.pages_skipped = skipped;
+    res->stats.pages_skipped = skipped + interior;
   }
   stream_sync(ctx);
   res->stats.wall_ms =

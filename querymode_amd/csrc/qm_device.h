@@ -186,6 +186,9 @@ struct DScan {
   const uint32_t *join_pay;    // nullptr: no payload
   int8_t pay_off[MAX_COLS];
   int8_t pay_null[MAX_COLS];
+  // zone maps: bit p set = the ZONE_ROWS-row page p (counted from row 0 of the
+  // scan) holds no row the query can select; nullptr: every page is read
+  const uint32_t *page_skip;
 };
 static constexpr int JOIN_PAY_BYTES = 12;
 

@@ -282,6 +282,9 @@ struct qm_table {
   std::string name;
   uint64_t rows = 0;
   std::atomic<int> prepared{0};  // prepared batches whose graphs read this table's columns
+  // a zone-map view: pages of the view its query skips (device bitmap), or nullptr
+  const uint32_t *page_skip = nullptr;
+  std::shared_ptr<qm::DevBuf> page_skip_buf;
   std::vector<qm::TCol> cols;
   int find(const std::string &n) const {
     for (size_t i = 0; i < cols.size(); ++i) if (cols[i].name == n) return (int)i;
