@@ -63,13 +63,13 @@ __device__ __forceinline__ uint64_t glb_find(const DQuery &q, uint64_t stored) {
       if (old == 0) {
         // new group: bound the fill so probe sequences stay short
         uint64_t n = g_atomic_add(q.selected + 1, 1);
-        if (n + 1 > (q.gcap >> 1) + (q.gcap >> 2)) q.status[0] = 1;
+        if (n + 1 > (q.gcap >> 1) + (q.gcap >> 2)) *gptr(q.status) = 1;
         return slot;
       }
       if (old == stored) return slot;
     }
   }
-  q.status[0] = 1;
+  *gptr(q.status) = 1;
   return ~0ull;
 }
 
@@ -91,7 +91,7 @@ __device__ __forceinline__ uint64_t glb_find_ins(const DQuery &q, uint64_t store
       if (old == stored) return slot;
     }
   }
-  q.status[0] = 1;
+  *gptr(q.status) = 1;
   return ~0ull;
 }
 

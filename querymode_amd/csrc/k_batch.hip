@@ -178,17 +178,17 @@ __device__ __forceinline__ void acc_flush(const Q &q, uint64_t kc, const AccT<W>
           g_add128(w, a.lo[s], a.hi[s]);
           if (st.fixed && f) g_atomic_or(w + 2, f);
         } else {
-          w[0] = a.lo[s];
-          w[1] = a.hi[s];
-          if (st.fixed) w[2] = f;
+          gptr(w)[0] = a.lo[s];
+          gptr(w)[1] = a.hi[s];
+          if (st.fixed) gptr(w)[2] = f;
         }
       }
     } else if (st.kind == SK_MAX_CODE) {
       if (atomic) { if (a.lo[s]) g_atomic_max(w, a.lo[s]); }
-      else w[0] = a.lo[s];
+      else *gptr(w) = a.lo[s];
     } else {
       if (atomic) { if (a.lo[s]) g_atomic_add(w, a.lo[s]); }
-      else w[0] = a.lo[s];
+      else *gptr(w) = a.lo[s];
     }
   }
 }
@@ -769,7 +769,7 @@ __global__ void __launch_bounds__(B_BLOCK) batch_qpf_kernel(BScan bs, const BQue
           if (s < NADD) g_atomic_add(w, acc[k][s]);
           else if (acc[k][s]) g_atomic_max(w, acc[k][s]);
         } else {
-          w[0] = acc[k][s];
+          *gptr(w) = acc[k][s];
         }
         acc[k][s] = 0;
       }

@@ -7,22 +7,33 @@
 
 namespace qm {
 
+// The same pointer as a global one, for loads, stores and atomics on result
+// columns, group tables and counters (device global memory): a flat access
+// would also count against every LDS wait of the kernel. Only for memory
+// that is never LDS.
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+// device-scope relaxed atomics on global memory
 __device__ __forceinline__ uint64_t g_atomic_add(uint64_t *p, uint64_t v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_add(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void g_atomic_max(uint64_t *p, uint64_t v) {
-  __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_max(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void g_atomic_or(uint64_t *p, uint64_t v) {
-  __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_or(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void g_atomic_or(uint32_t *p, uint32_t v) {
+  __hip_atomic_fetch_or(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t g_atomic_cas(uint64_t *p, uint64_t expected, uint64_t desired) {
-  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+  __hip_atomic_compare_exchange_strong(gptr(p), &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
   return expected;  // old value
 }
 __device__ __forceinline__ uint64_t g_load(const uint64_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(gptr(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 // 128-bit two's-complement add as two 64-bit atomics: the carry out of the
 // low word is computed from the value that atomic saw, so the total is exact

@@ -49,6 +49,7 @@ static constexpr int PR_WAVES = PR_BLOCK / 64;
 static constexpr int PR_RBLOCK = 1024;
 static constexpr int PR_RWAVES = PR_RBLOCK / 64;
 static constexpr int PR_RIPT = 4;  // records per thread in the write pass (cap <= PR_RIPT * PR_RBLOCK)
+static constexpr int PR_GPT = 4;   // groups a thread holds while the write pass stores the outputs
 
 __device__ __forceinline__ bool pr_null(const uint8_t *nb, uint64_t row) {
   return nb != nullptr && ((gld<uint8_t>(nb, row >> 3) >> (row & 7)) & 1);
@@ -397,23 +398,25 @@ __device__ __forceinline__ uint32_t pr_start(const uint32_t *bins, uint32_t x) {
   return (x & 1) ? (w >> 16) : (w & 0xFFFFu);
 }
 
-// one output value of one group (the values finalize_slot derives: a group
-// with no counted value is NULL; an all-NaN MIN / MAX keeps the identity),
-// formed as a bit pattern and stored with the output's width
-__device__ __forceinline__ void pr_emit(const ROut &ro, void *dst, uint32_t *nbm, uint64_t gg, int64_t key,
-                                        uint64_t v, uint64_t cnt) {
-  uint64_t bits = 0;
-  bool isnull = false;
-  int width = 8;
+// log2 of the byte width an output is stored with
+__device__ __forceinline__ int pr_out_lw(const ROut &ro) {
   const int32_t dt = ro.out_dtype;
+  const int iw = dt == DT_I64 ? 3 : (dt == DT_I32 ? 2 : (dt == DT_I16 ? 1 : 0));
+  if (ro.kind == OK_KEY) return iw;
+  if (ro.kind == OK_MIN || ro.kind == OK_MAX) return ro.src_is_float ? (dt == DT_F32 ? 2 : 3) : iw;
+  return 3;
+}
+
+// one output value of one group as a bit pattern (the values finalize_slot
+// derives: a group with no counted value is NULL; an all-NaN MIN / MAX keeps
+// the identity); the output kind is uniform
+__device__ __forceinline__ uint64_t pr_out_bits(const ROut &ro, int64_t key, uint64_t v, uint64_t cnt, bool &isnull) {
+  isnull = false;
   switch (ro.kind) {
-    case OK_KEY: bits = (uint64_t)key; width = (dt == DT_I64) ? 8 : (dt == DT_I32 ? 4 : (dt == DT_I16 ? 2 : 1)); break;
-    case OK_COUNT: bits = cnt; break;
-    case OK_SUM_I64: isnull = cnt == 0; bits = isnull ? 0 : v; break;
-    case OK_AVG_I64:
-      isnull = cnt == 0;
-      bits = f64_bits(isnull ? 0.0 : (double)(int64_t)v / (double)cnt);
-      break;
+    case OK_KEY: return (uint64_t)key;
+    case OK_COUNT: return cnt;
+    case OK_SUM_I64: isnull = cnt == 0; return isnull ? 0 : v;
+    case OK_AVG_I64: isnull = cnt == 0; return f64_bits(isnull ? 0.0 : (double)(int64_t)v / (double)cnt);
     default: {  // OK_MIN / OK_MAX
       const bool mn = ro.kind == OK_MIN;
       isnull = cnt == 0;
@@ -421,24 +424,51 @@ __device__ __forceinline__ void pr_emit(const ROut &ro, void *dst, uint32_t *nbm
         const uint64_t d = isnull ? 0ull
                                   : (v == 0 ? (mn ? 0x7FF0000000000000ull : 0xFFF0000000000000ull)
                                             : decode_f64_code(mn ? ~v : v));
-        if (dt == DT_F32) { bits = f32_bits((float)bits_f64(d)); width = 4; }
-        else bits = d;
-      } else {
-        bits = isnull ? 0ull : (uint64_t)decode_i64(mn ? ~v : v);
-        width = (dt == DT_I64) ? 8 : (dt == DT_I32 ? 4 : (dt == DT_I16 ? 2 : 1));
+        return ro.out_dtype == DT_F32 ? (uint64_t)f32_bits((float)bits_f64(d)) : d;
       }
-      break;
+      return isnull ? 0ull : (uint64_t)decode_i64(mn ? ~v : v);
     }
   }
-  uint8_t *p = static_cast<uint8_t *>(dst) + gg * (uint64_t)width;
-  if (width == 8) *reinterpret_cast<uint64_t *>(p) = bits;
-  else if (width == 4) *reinterpret_cast<uint32_t *>(p) = (uint32_t)bits;
-  else if (width == 2) *reinterpret_cast<uint16_t *>(p) = (uint16_t)bits;
-  else *p = (uint8_t)bits;
-  if (isnull && nbm) atomicOr(&nbm[gg >> 5], 1u << (gg & 31));
 }
 
-// contribution of one record to non-COUNT(*) state 1 + s2 (raw record words in w)
+// store the low 2^lw bytes of bits as row `row` of a column (lw uniform)
+__device__ __forceinline__ void pr_store(void *dst, uint64_t row, int lw, uint64_t bits) {
+  uint8_t *p = static_cast<uint8_t *>(dst) + (row << lw);
+  if (lw == 3) *gptr(reinterpret_cast<uint64_t *>(p)) = bits;
+  else if (lw == 2) *gptr(reinterpret_cast<uint32_t *>(p)) = (uint32_t)bits;
+  else if (lw == 1) *gptr(reinterpret_cast<uint16_t *>(p)) = (uint16_t)bits;
+  else *gptr(p) = (uint8_t)bits;
+}
+
+// one record's words (rw uniform): one vector load when the record is 2, 4,
+// 6 or 8 words and the buffer is 32-byte aligned, else a load per word
+__device__ __forceinline__ void pr_load_rec(const uint32_t *rec, uint64_t idx, uint32_t rw, bool aligned,
+                                            uint32_t (&w)[PR_MAX_RW]) {
+  const uint32_t *r = rec + idx * (uint64_t)rw;
+#pragma unroll
+  for (int k = 0; k < PR_MAX_RW; ++k) w[k] = 0;
+  if (aligned && rw == 4) {
+    const uint4 x = *reinterpret_cast<const uint4 *>(r);
+    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+  } else if (aligned && rw == 8) {
+    const uint4 x = reinterpret_cast<const uint4 *>(r)[0], y = reinterpret_cast<const uint4 *>(r)[1];
+    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+    w[4] = y.x; w[5] = y.y; w[6] = y.z; w[7] = y.w;
+  } else if (aligned && rw == 2) {
+    const uint2 x = *reinterpret_cast<const uint2 *>(r);
+    w[0] = x.x; w[1] = x.y;
+  } else if (aligned && rw == 6) {
+    const uint2 x = reinterpret_cast<const uint2 *>(r)[0], y = reinterpret_cast<const uint2 *>(r)[1],
+                z = reinterpret_cast<const uint2 *>(r)[2];
+    w[0] = x.x; w[1] = x.y; w[2] = y.x; w[3] = y.y; w[4] = z.x; w[5] = z.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < PR_MAX_RW; ++k)
+      if ((uint32_t)k < rw) w[k] = r[k];
+  }
+}
+
+// contribution of one record (raw words w) to non-COUNT(*) state 1 + s2
 __device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const uint32_t (&w)[PR_MAX_RW]) {
   const DState &st = ps.st[1 + s2];
   const int vj = ps.st_v[1 + s2];
@@ -469,9 +499,7 @@ __device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const u
 
 // A workgroup takes a contiguous range of final partitions, so the rows it
 // writes for one query are consecutive across its partitions (each result
-// column is written as long runs, not as short pieces from many workgroups);
-// the next partition's records are loaded while the current one's queries
-// are answered.
+// column is written as long runs, not as short pieces from many workgroups).
 template <int NSUM, int NMAX>
 __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   constexpr int NSA = NSUM > 0 ? NSUM : 1, NMA = NMAX > 0 ? NMAX : 1;
@@ -479,7 +507,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   __shared__ uint32_t s_sw[PR_RWAVES + 1];
   __shared__ uint64_t s_sw64[PR_RWAVES];
   __shared__ uint32_t s_qe[R_MAX_QUERIES];
-  __shared__ uint64_t s_qrow[R_MAX_QUERIES];  // first result row of (query, current partition)
+  __shared__ uint64_t s_qrow[R_MAX_QUERIES];  // first result row of (query, loaded partition)
   __shared__ uint32_t s_qcnt[R_MAX_QUERIES], s_qbase[R_MAX_QUERIES];  // groups of (query, partition), flat start
   __shared__ uint8_t s_gk[R_MAX_QUERIES * 64];  // keys (partition lanes) of each query's groups, ascending
   __shared__ uint8_t s_gq[R_MAX_QUERIES * 64];  // query of each flat group
@@ -493,6 +521,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   const uint32_t nbw = nbin / 2 + 1;              // packed u16 entries 0 .. nbin
   const uint32_t cap = ps.cap;
   const uint32_t rw = (uint32_t)ps.rw;
+  const bool raligned = (reinterpret_cast<uintptr_t>(ps.rec2) & 31u) == 0;
   uint32_t *s_bin = reinterpret_cast<uint32_t *>(s_dyn);
   uint64_t *s_add = reinterpret_cast<uint64_t *>(s_dyn + ((nbw * 4 + 15) & ~15u));
   uint64_t *s_max = s_add + (uint64_t)NSUM * cap;
@@ -513,29 +542,24 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   const uint64_t lt = (1ull << lane) - 1ull;
   const uint32_t kbin = (uint32_t)lane * NBK;
 
-  // partition P's size and (thread t < nq) the first result row of (query t, P)
-  uint32_t n = 0;
-  uint64_t qrow = 0, pbase = 0;
-  auto load_part = [&](uint32_t P) {
-    pbase = ps.start2[P];
+  // ---- load: partition P's records -> (key, bucket) bin and rank within it
+  // (LDS counters), state contributions (registers); the first result row of
+  // (query t, P) for t < nq
+  uint32_t nP = 0;
+  uint32_t bn[PR_RIPT], rk[PR_RIPT];
+  uint64_t ca[PR_RIPT][NSA], cm[PR_RIPT][NMA];
+  auto stage_load = [&](uint32_t P) {
+    const uint64_t pbase = ps.start2[P];
     const uint64_t capr = ps.start2[P + 1] - pbase;
     const uint32_t c2 = ps.cur2[P];
-    n = (uint32_t)(c2 < capr ? c2 : capr);
+    nP = (uint32_t)(c2 < capr ? c2 : capr);
     if (t < ps.nq) {
       const uint64_t *gq = ps.gcount + (uint64_t)t * ps.np;
-      qrow = ps.row_off[t] + (gq[P] - gq[0]);
+      s_qrow[t] = ps.row_off[t] + (gq[P] - gq[0]);
     }
-  };
-  for (uint32_t P = P0; P < P1; ++P) {
-    load_part(P);
     for (uint32_t i = t; i < nbw; i += PR_RBLOCK) s_bin[i] = 0;
-    if (t < ps.nq) s_qrow[t] = qrow;
-    const uint32_t nP = n;
     if (nP > cap && t == 0) atomicOr(ps.status, 2u);  // the host sizes partitions from the key histogram
     __syncthreads();
-    // 1. records -> (key, bucket) bin and rank within it; state contributions
-    uint32_t bn[PR_RIPT], rk[PR_RIPT];
-    uint64_t ca[PR_RIPT][NSA], cm[PR_RIPT][NMA];
 #pragma unroll
     for (int i = 0; i < PR_RIPT; ++i) {
       const uint32_t j = (uint32_t)t + (uint32_t)i * PR_RBLOCK;
@@ -547,9 +571,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       for (int k = 0; k < NMA; ++k) cm[i][k] = 0;
       if (j >= nP || j >= cap) continue;
       uint32_t wr[PR_MAX_RW];
-      const uint32_t *r = ps.rec2 + (pbase + j) * (uint64_t)rw;
-#pragma unroll
-      for (int k = 0; k < PR_MAX_RW; ++k) wr[k] = (uint32_t)k < rw ? r[k] : 0u;
+      pr_load_rec(ps.rec2, pbase + j, rw, raligned, wr);
 #pragma unroll
       for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr);
 #pragma unroll
@@ -560,7 +582,13 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       rk[i] = (atomicAdd(&s_bin[bn[i] >> 1], 1u << sh) >> sh) & 0xFFFFu;
     }
     __syncthreads();
-    // 2. exclusive scan of the bins (packed pairs; thread t owns a word range)
+  };
+
+  // ---- sort: the loaded partition's contributions in (key, bucket) order,
+  // prefix sums of the additive states, and its non-empty (query, key) groups
+  uint32_t ngroups = 0;
+  auto stage_sort = [&]() {
+    // exclusive scan of the bins (packed pairs; thread t owns a word range)
     {
       const uint32_t wpt = (nbw + PR_RBLOCK - 1) / PR_RBLOCK;
       const uint32_t w0 = (uint32_t)t * wpt < nbw ? (uint32_t)t * wpt : nbw;
@@ -579,7 +607,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       }
     }
     __syncthreads();
-    // 3. scatter the contributions into (key, bucket) order
+    // scatter the contributions into (key, bucket) order
 #pragma unroll
     for (int i = 0; i < PR_RIPT; ++i) {
       if (rk[i] == ~0u) continue;
@@ -590,7 +618,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       for (int j = 0; j < NMAX; ++j) s_max[(uint64_t)j * cap + slot] = cm[i][j];
     }
     __syncthreads();
-    // 4. inclusive prefix sums of the additive states over the sorted order
+    // inclusive prefix sums of the additive states over the sorted order
     const uint32_t nn = nP < cap ? nP : cap;
 #pragma unroll
     for (int j = 0; j < NSUM; ++j) {
@@ -604,7 +632,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       for (uint32_t i = i0; i < i1; ++i) { run += a[i]; a[i] = run; }
       __syncthreads();
     }
-    // 5. the non-empty (query, key) groups: lanes = keys, one query per wave
+    // the non-empty (query, key) groups: lanes = keys, one query per wave
     // step, lists each query's keys in ascending order ...
     for (int q = wave; q < ps.nq; q += PR_RWAVES) {
       const uint32_t qe = s_qe[q];
@@ -616,8 +644,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       if (lane == 0) s_qcnt[q] = (uint32_t)__builtin_popcountll(m);
     }
     __syncthreads();
-    // ... the flat index of each query's first group, and each group's query ...
-    uint32_t ngroups;
+    // ... the flat index of each query's first group, and each group's query
     {
       const uint32_t c = t < ps.nq ? s_qcnt[t] : 0u;
       const uint32_t b0 = pr_block_exscan_n<PR_RWAVES>(c, s_sw, &ngroups);
@@ -629,60 +656,89 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       if ((uint32_t)lane < c) s_gq[s_qbase[q] + lane] = (uint8_t)q;
     }
     __syncthreads();
-    // ... then lanes = groups: consecutive lanes take consecutive keys of one
-    // query, so each result column is written in runs
-    for (uint32_t g = t; g < ngroups; g += PR_RBLOCK) {
-      const uint32_t q = s_gq[g];
-      const uint32_t j = g - s_qbase[q];
-      const uint32_t k = s_gk[q * 64 + j];
-      const uint32_t qe = s_qe[q];
-      const uint32_t a = pr_start(s_bin, k * NBK + (qe & 0xFFFFu));
-      const uint32_t b = pr_start(s_bin, k * NBK + (qe >> 16));
-      const uint64_t gg = s_qrow[q] + j;
-      const int64_t key = ps.kmin + (int64_t)(((uint64_t)P << ps.s) + (uint64_t)k);
-      uint64_t sv[1 + NSUM + NMAX];  // state values: COUNT(*), sums, max codes
-      sv[0] = b - a;
+  };
+
+  // ---- groups -> registers: lanes = groups, consecutive lanes take
+  // consecutive keys of one query, so each result column is written in runs
+  uint32_t ga[PR_GPT], gb[PR_GPT], gk[PR_GPT];  // sorted range [ga, gb) (gb = 0: no group), key lane
+  uint64_t grow[PR_GPT];                         // result row
+  auto decode = [&](uint32_t g0) {
 #pragma unroll
-      for (int jj = 0; jj < NSUM; ++jj) {
-        const uint64_t *ad = s_add + (uint64_t)jj * cap;
-        sv[1 + jj] = ad[b - 1] - (a ? ad[a - 1] : 0ull);
-      }
-      if constexpr (NMAX > 0) {
-        uint64_t x0[NMA], x1[NMA];
-#pragma unroll
-        for (int jj = 0; jj < NMA; ++jj) { x0[jj] = 0; x1[jj] = 0; }
-        uint32_t i = a;
-        for (; i + 1 < b; i += 2) {
-#pragma unroll
-          for (int jj = 0; jj < NMAX; ++jj) {
-            const uint64_t *md = s_max + (uint64_t)jj * cap;
-            x0[jj] = u64max(x0[jj], md[i]);
-            x1[jj] = u64max(x1[jj], md[i + 1]);
-          }
-        }
-#pragma unroll
-        for (int jj = 0; jj < NMAX; ++jj) {
-          if (i < b) x0[jj] = u64max(x0[jj], s_max[(uint64_t)jj * cap + i]);
-          sv[1 + NSUM + jj] = u64max(x0[jj], x1[jj]);
-        }
-      }
-#pragma unroll 1
-      for (int o = 0; o < nouts; ++o) {
-        const uint64_t rb = rfl64(s_out[o][0]);
-        ROut ro;
-        memcpy(&ro, &rb, sizeof ro);
-        void *ob = reinterpret_cast<void *>(rfl64(s_out[o][1]));
-        uint32_t *onb = reinterpret_cast<uint32_t *>(rfl64(s_out[o][2]));
-        uint64_t v = 0, c = 0;
-#pragma unroll
-        for (int jj = 0; jj < 1 + NSUM + NMAX; ++jj) {
-          if (ro.state == jj) v = sv[jj];
-          if (ro.cnt_state == jj) c = sv[jj];
-        }
-        pr_emit(ro, ob, onb, gg, key, v, c);
+    for (int i = 0; i < PR_GPT; ++i) {
+      const uint32_t g = g0 + (uint32_t)t + (uint32_t)i * PR_RBLOCK;
+      ga[i] = 0;
+      gb[i] = 0;
+      gk[i] = 0;
+      grow[i] = 0;
+      if (g < ngroups) {
+        const uint32_t q = s_gq[g];
+        const uint32_t jq = g - s_qbase[q];
+        const uint32_t k = s_gk[q * 64 + jq];
+        const uint32_t qe = s_qe[q];
+        ga[i] = pr_start(s_bin, k * NBK + (qe & 0xFFFFu));
+        gb[i] = pr_start(s_bin, k * NBK + (qe >> 16));  // > ga: the group is non-empty
+        gk[i] = k;
+        grow[i] = s_qrow[q] + jq;
       }
     }
-    __syncthreads();  // LDS (bins, sorted states, first rows) is reused by the next partition
+  };
+  // state st over the sorted positions [a, b) of a non-empty group (st uniform; out of range: 0)
+  auto stval = [&](int st, uint32_t a, uint32_t b) -> uint64_t {
+    if (st == 0) return (uint64_t)(b - a);
+    if (st >= 1 && st <= NSUM) {
+      const uint64_t *ad = s_add + (uint64_t)(st - 1) * cap;
+      return ad[b - 1] - (a ? ad[a - 1] : 0ull);
+    }
+    if (NMAX > 0 && st > NSUM && st <= NSUM + NMAX) {
+      const uint64_t *md = s_max + (uint64_t)(st - 1 - NSUM) * cap;
+      uint64_t x0 = 0, x1 = 0;
+      uint32_t i = a;
+      for (; i + 1 < b; i += 2) {
+        x0 = u64max(x0, md[i]);
+        x1 = u64max(x1, md[i + 1]);
+      }
+      if (i < b) x0 = u64max(x0, md[i]);
+      return u64max(x0, x1);
+    }
+    return 0;
+  };
+  // ---- write: the groups in registers (partition P), one output at a time,
+  // so an output's kind and state are uniform and only that state is read
+  auto write = [&](uint32_t P) {
+#pragma unroll 1
+    for (int o = 0; o < nouts; ++o) {
+      const uint64_t rb = rfl64(s_out[o][0]);
+      ROut ro;
+      memcpy(&ro, &rb, sizeof ro);
+      void *ob = reinterpret_cast<void *>(rfl64(s_out[o][1]));
+      uint32_t *onb = reinterpret_cast<uint32_t *>(rfl64(s_out[o][2]));
+      const int lw = pr_out_lw(ro);
+#pragma unroll
+      for (int i = 0; i < PR_GPT; ++i) {
+        if (gb[i] == 0) continue;
+        uint64_t v = 0, c = 0;
+        int64_t key = 0;
+        if (ro.kind == OK_KEY) key = ps.kmin + (int64_t)(((uint64_t)P << ps.s) + (uint64_t)gk[i]);
+        else {
+          c = stval(ro.cnt_state, ga[i], gb[i]);
+          if (ro.kind != OK_COUNT) v = stval(ro.state, ga[i], gb[i]);
+        }
+        bool isnull;
+        const uint64_t bits = pr_out_bits(ro, key, v, c, isnull);
+        pr_store(ob, grow[i], lw, bits);
+        if (isnull && onb) g_atomic_or(&onb[grow[i] >> 5], 1u << (grow[i] & 31));
+      }
+    }
+  };
+
+  for (uint32_t P = P0; P < P1; ++P) {
+    stage_load(P);
+    stage_sort();
+    for (uint32_t g0 = 0; g0 < ngroups; g0 += PR_GPT * PR_RBLOCK) {
+      decode(g0);
+      write(P);
+    }
+    __syncthreads();  // LDS (bins, sorted states, group lists, first rows) is reused by the next partition
   }
 }
 

@@ -25,7 +25,7 @@
 namespace qm {
 
 // ---- one query over the thread's 8 rows ----
-__device__ void agg_rows(const DScan &sc, const DQuery &q, const DCol *cols, uint64_t row0,
+__device__ __forceinline__ void agg_rows(const DScan &sc, const DQuery &q, const DCol *cols, uint64_t row0,
                          uint32_t mask, lds_u64 *lds) {
   const int W = q.entry_words;
   // Resolve each selected row to an entry: bit 63 set = global index.
@@ -126,7 +126,7 @@ __device__ __forceinline__ uint64_t lds_words_total(const DQuery *qs, int nq) {
 }
 
 // Merge a workgroup's LDS table into the global table.
-__device__ void flush_lds(const DQuery &q, lds_u64 *lds) {
+__device__ __forceinline__ void flush_lds(const DQuery &q, lds_u64 *lds) {
   const int W = q.entry_words;
   if (q.lmode == LM_DIRECT) {
     const uint64_t nslots = (uint64_t)q.lds_entries;
@@ -176,7 +176,7 @@ __device__ void flush_lds(const DQuery &q, lds_u64 *lds) {
 // One joined row (probe row prow, build row brow) into the query's group
 // table: key and state values through single-row gathers (build columns are
 // the side-1 columns).
-__device__ void join_apply(const DScan &sc, const DQuery &q, uint64_t prow, uint32_t brow, lds_u64 *lds) {
+__device__ __forceinline__ void join_apply(const DScan &sc, const DQuery &q, uint64_t prow, uint32_t brow, lds_u64 *lds) {
   // One joined row: key and state values through single-row gathers.
   const int W = q.entry_words;
   uint64_t ent = ~0ull;

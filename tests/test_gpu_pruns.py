@@ -123,3 +123,22 @@ def test_unsorted_tiny_tables(qm):
         _say(f"unsorted tiny table {n}")
         cols = _table(n, 50, seed=n)
         _check(qm, f"t{n}", cols, _mix(f"t{n}", 8, seed=n, min_sel=0.05))
+
+
+def test_unsorted_output_kinds(qm):
+    """AVG, MIN of a float32 column (stored as f32), MAX and SUM of an int16
+    column (MAX stored as int16), with NULLs in both: every output kind of the
+    write pass against the oracle."""
+    n = 700_001
+    _say("unsorted output kinds")
+    cols = _table(n, 30_000, seed=29)
+    rng = np.random.default_rng(31)
+    cols["f3"] = rng.normal(0, 50, n).astype(np.float32)
+    cols["s16"] = rng.integers(-30_000, 30_000, n).astype(np.int16)
+    nulls = {"f3": rng.random(n) < 0.07, "s16": rng.random(n) < 0.04}
+    aggs = [AggregateOp("avg", "v1"), AggregateOp("min", "f3"), AggregateOp("max", "s16"), AggregateOp("count", "*"),
+            AggregateOp("sum", "s16")]
+    descs = [QueryDescriptor("o", filters=[FilterOp("v2", "gte", lo), FilterOp("v2", "lt", lo + w)], groupBy=["k2"],
+                             aggregates=aggs) for lo, w in ((0.0, 0.001), (0.2, 0.3), (0.0, 1.0), (0.6, 0.05))]
+    batch = _check(qm, "o", cols, descs, nulls)
+    print([r.stats["strategy"] for r in batch], flush=True)
