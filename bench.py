@@ -49,13 +49,29 @@ def copy_calibration(nbytes: int) -> None:
     hip.hipFree(dst)
 
 
-def _traffic(kname: str, rows: int, fname: str):
+JOIN_KERNEL = "scan_agg_kernel<true> (fused join probe)"
+
+
+def _traffic_rec(kname: str, rows: int, fname: str):
     tfile = ROOT / "profiles" / fname
     if tfile.exists():
         rec = json.loads(tfile.read_text())
         if rec.get("kernel") == kname and rec.get("rows") == rows:
-            return rec.get("hbm_bytes_per_launch")
+            return rec
     return None
+
+
+def _traffic(kname: str, rows: int, fname: str):
+    """HBM bytes per launch of the kernel from the committed FETCH_SIZE / WRITE_SIZE summary (None if the
+    summary is of another kernel or table size)."""
+    rec = _traffic_rec(kname, rows, fname)
+    return rec.get("hbm_bytes_per_launch") if rec else None
+
+
+def _traffic_build(kname: str, rows: int, fname: str):
+    """The build the committed traffic summary was measured on."""
+    rec = _traffic_rec(kname, rows, fname)
+    return rec.get("build") if rec else None
 
 
 def _exec_stats(lib, N, plan, table):
@@ -123,7 +139,8 @@ def bench_zipf(args) -> dict:
                    "strategy": strategy, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": _traffic(kname, rows, "traffic_filter_groupby_zipf.json")},
+                     "traffic": _traffic(kname, rows, "traffic_filter_groupby_zipf.json"),
+                     "traffic_build": _traffic_build(kname, rows, "traffic_filter_groupby_zipf.json")},
         "kernel": {"name": kname, "avg_ms": scan_ms, "algorithmic_bytes": alg_bytes,
                    "query_device_ms": float(np.mean([s.device_ms for s in stats]))},
         "cpu_baseline": cpu,
@@ -338,7 +355,7 @@ def bench_batch(args) -> dict:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "frac_of_measured_read": (achieved / read_ceiling) if read_ceiling else None,
-                     "traffic": _traffic(kname, rows, tfile)},
+                     "traffic": _traffic(kname, rows, tfile), "traffic_build": _traffic_build(kname, rows, tfile)},
         "kernel": {"name": kname, "avg_ms": scan_ms, "algorithmic_bytes": alg_bytes, "records": records},
         "kernels": kernels,
         "read_ceiling_gbs": read_ceiling,
@@ -499,8 +516,10 @@ def bench_join(args) -> dict:
                    "probe_rows": nprobe, "build_rows": nbuild, "selected": int(stats[-1].rows_selected),
                    "strategy": N.STRATEGIES.get(stats[-1].strategy), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": alg / (scan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": alg / (scan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
-        "kernel": {"name": "scan_agg_kernel<true> (fused join probe)", "avg_ms": scan_ms, "algorithmic_bytes": alg,
+                     "frac": alg / (scan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": _traffic(JOIN_KERNEL, rows, "traffic_join_probe.json"),
+                     "traffic_build": _traffic_build(JOIN_KERNEL, rows, "traffic_join_probe.json")},
+        "kernel": {"name": JOIN_KERNEL, "avg_ms": scan_ms, "algorithmic_bytes": alg,
                    "query_device_ms": float(np.mean([s.device_ms for s in stats]))},
         "cpu_baseline": cpu,
         "setup": {"generate_s": gen_s, "upload_s": upload_s},
