@@ -227,9 +227,10 @@ int qm_measure_read_bw(qm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbps)
  * reference counterpart (the reference has no device planner). Knobs:
  * scatter_rpb, part_load, hot, qp_min, runs_min_q, runs_min_len,
  * runs_chunks_per_cu, runs_win, runs_qblock, runs_lds_pad, shared_lean, shared_waves, sorted1,
- * join_unique, zone_maps, topk_bpc;
- * values are clamped to each
- * knob's range; an unknown name is QM_ERR_INVALID_ARG. */
+ * join_unique, join_parts, sort_parts, join_claim, join_payload, zone_maps, topk_bpc,
+ * pr_max_s, pr_b2, pr_scatter_bpc;
+ * values are clamped to each knob's range; an unknown name is QM_ERR_INVALID_ARG.
+ * No knob changes a result: only how the planner lays the work out. */
 int qm_tune_set(qm_ctx *ctx, const char *name, double value);
 int qm_tune_get(qm_ctx *ctx, const char *name, double *value);
 

@@ -1678,6 +1678,12 @@ const KnobDef kKnobs[] = {
      [](const qm_ctx::Tune &t) { return (double)t.join_payload; }},
     {"zone_maps", "QM_TUNE_ZONE_MAPS", 0, 1, [](qm_ctx::Tune &t, double v) { t.zone_maps = v != 0; },
      [](const qm_ctx::Tune &t) { return (double)t.zone_maps; }},
+    {"pr_max_s", "QM_TUNE_PR_MAX_S", 1, PR_MAX_S, [](qm_ctx::Tune &t, double v) { t.pr_max_s = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.pr_max_s; }},
+    {"pr_b2", "QM_TUNE_PR_B2", 0, 10, [](qm_ctx::Tune &t, double v) { t.pr_b2 = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.pr_b2; }},
+    {"pr_scatter_bpc", "QM_TUNE_PR_SCATTER_BPC", 1, 8, [](qm_ctx::Tune &t, double v) { t.pr_scatter_bpc = (int)v; },
+     [](const qm_ctx::Tune &t) { return (double)t.pr_scatter_bpc; }},
     {"topk_bpc", "QM_TUNE_TOPK_BPC", 1, 16, [](qm_ctx::Tune &t, double v) { t.topk_bpc = (int)v; },
      [](const qm_ctx::Tune &t) { return (double)t.topk_bpc; }},
     {"runs_lds_pad", "QM_TUNE_RUNS_LDS_PAD", 0, 96 * 1024, [](qm_ctx::Tune &t, double v) { t.runs_lds_pad = (int)v; },
@@ -3158,25 +3164,32 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
 
 // Partition geometry for a launch of `nbk` buckets and `nsn` non-COUNT(*)
 // states, from the key histogram: final partitions of 2^s keys (the largest
-// s whose fullest partition fits the write pass's LDS), level-1 partitions of
-// 2^(sh1 - s) final ones. Cached on the key column; nullptr when no s fits.
+// s, at most the pr_max_s knob, whose fullest partition fits the write pass's
+// LDS), level-1 partitions of 2^(sh1 - s) final ones (sh1 - s from the pr_b2
+// knob when set). Cached on the key column per knob setting; nullptr when no
+// s fits.
 const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uint32_t nbk, int nsum, int nmax,
                               uint32_t rw) {
   TCol *kt = const_cast<TCol *>(kt_c);  // the cache, guarded by ctx->cache_mu
   std::lock_guard<std::mutex> cg(ctx->cache_mu);
   const uint32_t nsn = (uint32_t)(nsum + nmax);
+  const int max_s = std::min(PR_MAX_S, std::max(1, ctx->tune.pr_max_s));
+  const int want_b2 = ctx->tune.pr_b2;
   for (auto &g : kt->prgeo)
-    if (g.nbk == nbk && g.nsn == nsn && g.rw == rw) return g.s >= 0 ? &g : nullptr;
+    if (g.nbk == nbk && g.nsn == nsn && g.rw == rw && g.max_s == max_s && g.want_b2 == want_b2)
+      return g.s >= 0 ? &g : nullptr;
   TCol::PRGeo g;
   g.nbk = nbk;
   g.nsn = nsn;
   g.rw = rw;
+  g.max_s = max_s;
+  g.want_b2 = want_b2;
   g.tile = qmk_pr_tile((int)rw);
   const std::vector<uint32_t> &h = kt->khist;
   const int kg = kt->khist_g;
   const uint64_t D = (uint64_t)kt->st.imax - (uint64_t)kt->st.imin + 1;
   std::vector<uint64_t> caps;
-  for (int s = PR_MAX_S; s >= kg && g.tile; --s) {
+  for (int s = max_s; s >= kg && g.tile; --s) {
     const uint32_t cap = qmk_pr_max_cap(nbk << s, nsum, nmax);
     if (!cap) continue;
     const uint64_t per = 1ull << (s - kg);
@@ -3191,7 +3204,7 @@ const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uin
     int lg = 0;
     while ((1ull << lg) < np) ++lg;
     const int b2max = std::min(10, 16 - s);  // 2^b2 <= PR_MAX_P digits in pass 2
-    int b2 = std::min((lg + 1) / 2, b2max);
+    int b2 = std::min(want_b2 > 0 ? want_b2 : (lg + 1) / 2, b2max);
     while (((D + (1ull << (s + b2)) - 1) >> (s + b2)) > (uint64_t)PR_MAX_P && b2 < b2max) ++b2;
     const uint64_t np1 = (D + (1ull << (s + b2)) - 1) >> (s + b2);
     if (np1 > (uint64_t)PR_MAX_P || np >= (1ull << 31)) continue;
@@ -3345,7 +3358,7 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
   // persistent scatter grids: the workgroups that fit the device at once
   const uint64_t tile = g->tile;
   const uint64_t ntiles = (rows + tile - 1) / tile;
-  const uint64_t pgrid = (uint64_t)ctx->num_cus * 2;
+  const uint64_t pgrid = (uint64_t)ctx->num_cus * (uint64_t)std::max(1, ctx->tune.pr_scatter_bpc);
   hipEvent_t ev[6];
   for (auto &e : ev) HIPX(hipEventCreate(&e));
   struct EvFree { hipEvent_t *e; ~EvFree() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); } } evfree{ev};

@@ -199,6 +199,9 @@ struct qm_ctx {
     int join_payload = 1;      // ... and the fused probe gathers the build columns packed per key slot
     int zone_maps = 1;         // single aggregate queries skip the pages a range filter cannot match
     int topk_bpc = 2;          // top-K scan workgroups per CU (each keeps k candidates)
+    int pr_max_s = 6;          // partitioned runs: at most 2^pr_max_s keys per final partition
+    int pr_b2 = 0;             // ... key bits split by the second scatter pass (0 = from the partition count)
+    int pr_scatter_bpc = 2;    // ... scatter workgroups per CU
   } tune;
 
   void *alloc(size_t bytes);
@@ -266,6 +269,7 @@ struct TCol {
   // (buckets, states) of the launch (device copies of the region starts)
   struct PRGeo {
     uint32_t nbk = 0, nsn = 0, rw = 0;   // what the geometry was derived for
+    int max_s = 0, want_b2 = 0;          // ... and under which knobs
     int s = -1, sh1 = 0;
     uint32_t np1 = 0, np = 0, cap = 0, tile = 0;
     uint64_t total = 0;                  // records the regions hold

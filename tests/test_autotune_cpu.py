@@ -34,3 +34,22 @@ def test_cache_roundtrip(tmp_path):
     data = load_cache(f)
     assert data["gfx950|rows=10|n=2|abc"] == {"runs_chunks_per_cu": 32}
     assert json.loads(f.read_text())["gfx950|rows=20|n=2|def"] == {"runs_win": 2048}
+
+
+def test_space_follows_the_strategies():
+    from querymode_amd.autotune import BATCH_SPACES, QUERY_SPACES, space_for
+    sp = space_for(["part_runs", "shared_interval", "part_runs"], BATCH_SPACES)
+    assert list(sp) == ["pr_scatter_bpc", "pr_b2", "pr_max_s", "shared_waves"]
+    assert space_for(["sorted_runs"], BATCH_SPACES) == BATCH_SPACES["sorted_runs"]
+    assert space_for(["direct_lds"], BATCH_SPACES) == {}        # a fused scan has no knob here
+    assert "topk_bpc" in space_for(["topk"], QUERY_SPACES)
+    # every value of every space lies in the knob's range (qmgpu.h clamps; the tuner must not rely on it)
+    ranges = {"pr_scatter_bpc": (1, 8), "pr_b2": (0, 10), "pr_max_s": (1, 6), "shared_waves": (1, 64),
+              "runs_chunks_per_cu": (1, 1024), "runs_qblock": (0, 64), "runs_win": (1024, 2048),
+              "runs_min_len": (0, 1e12), "scatter_rpb": (0, 64), "part_load": (0.1, 0.9), "hot": (0, 1),
+              "topk_bpc": (1, 16)}
+    for spaces in (BATCH_SPACES, QUERY_SPACES):
+        for sp in spaces.values():
+            for k, vals in sp.items():
+                lo, hi = ranges[k]
+                assert all(lo <= v <= hi for v in vals), k

@@ -99,3 +99,56 @@ def test_topk_blocks_keep_results(qm):
             assert got == want, bpc
     finally:
         dev.tune_set("topk_bpc", old)
+
+
+PR_SETTINGS = [
+    {"pr_max_s": 5}, {"pr_max_s": 4}, {"pr_b2": 7}, {"pr_b2": 10}, {"pr_max_s": 5, "pr_b2": 9},
+    {"pr_scatter_bpc": 1}, {"pr_scatter_bpc": 3}, {"shared_waves": 1}, {"shared_waves": 16},
+]
+
+
+def test_partitioned_runs_knobs_keep_results(qm):
+    """The batched path over a key in any row order under every partition
+    shape the tuner may pick (keys per final partition, the split between the
+    two scatter passes, scatter workgroups per CU) and the shared-interval
+    pass's wave rounds: the same bytes as the default, which equal the oracle."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    from ._compare import assert_groups_equal
+
+    n = 3_000_000
+    _say(f"partitioned-runs knobs: {n} rows, 64 queries, {len(PR_SETTINGS)} settings")
+    cols = datagen.batch_table(n=n, seed=9)
+    cols["k2"] = datagen.uniform_int(9, 72, 0, n, 0, 60_000, np.int64)
+    qm.register_table("pk", cols)
+    descs = [QueryDescriptor("pk", filters=d.filters, groupBy=d.groupBy, aggregates=d.aggregates)
+             for d in datagen.batch_queries(64, seed=9, min_sel=1e-3)]
+    dev = qm.device
+    names = sorted({k for s in PR_SETTINGS for k in s})
+    defaults = {k: dev.tune_get(k) for k in names}
+    ex = qm.executor
+    base = ex.execute_batch(descs)
+    k2 = [i for i, d in enumerate(descs) if d.groupBy == ["k2"]]
+    assert all(base[i].stats["strategy"] == "part_runs" for i in k2)
+    ot = O.Table(cols)
+    for i in k2[:4] + [i for i in range(len(descs)) if i not in k2][:2]:
+        assert_groups_equal(base[i].rows, O.run(descs[i], {"pk": ot}), descs[i].groupBy, ctx=f"default q{i}")
+    want = _bytes(base)
+    try:
+        for setting in PR_SETTINGS:
+            _say(f"  setting {setting}")
+            for k, v in setting.items():
+                dev.tune_set(k, v)
+            got = ex.execute_batch(descs)
+            for i, (g, w) in enumerate(zip(got, base)):
+                if g.stats["strategy"] == w.stats["strategy"]:
+                    assert bytes(g.columnarData) == want[i], (setting, i)
+                else:  # the setting left no partition shape for this launch: another path, same groups
+                    assert_groups_equal(g.rows, w.rows, descs[i].groupBy, ctx=f"{setting} q{i}")
+            for k in setting:
+                dev.tune_set(k, defaults[k])
+    finally:
+        for k, v in defaults.items():
+            dev.tune_set(k, v)
