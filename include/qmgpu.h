@@ -90,7 +90,9 @@ typedef struct qm_stats {
   uint32_t kernel_launches;
   uint32_t retries;           /* hash-table regrowths */
   uint32_t strategy;          /* group-by strategy picked by the tuner (QM_STRAT_*) */
-  uint32_t pages_skipped;     /* zone-map pages (65,536 rows) not read (QueryResult.pagesSkipped) */
+  uint32_t pages_skipped;     /* zone-map pages (65,536 rows) not read (QueryResult.pagesSkipped); in a batch,
+                                 the pages no fused query can match, which its shared-interval and
+                                 partitioned-runs passes skip */
   double   pass_ms[4];        /* device time of the strategy's passes (QM_STRAT_PART_RUNS: row scatter,
                                  record scatter, group count, write), HIP events on the library stream */
   uint64_t pass_records;      /* QM_STRAT_PART_RUNS: rows the first pass kept as records (selected by any

@@ -313,11 +313,12 @@ __device__ __forceinline__ uint32_t eval_query8(const DQuery &q, const DCol *col
 }
 
 // ---- zone maps: a tile starting at `row` lies in a page the query skips ----
-__device__ __forceinline__ bool page_skipped(const DScan &sc, uint64_t row) {
-  if (!sc.page_skip) return false;
+__device__ __forceinline__ bool page_bit(const uint32_t *bits, uint64_t row) {
+  if (!bits) return false;
   const uint64_t p = row / ZONE_ROWS;
-  return (sc.page_skip[p >> 5] >> (p & 31)) & 1u;
+  return (gld<uint32_t>(bits, (p >> 5) * 4) >> (p & 31)) & 1u;
 }
+__device__ __forceinline__ bool page_skipped(const DScan &sc, uint64_t row) { return page_bit(sc.page_skip, row); }
 
 // ---- join probe: returns the build-row range of a probe key ----
 __device__ __forceinline__ bool join_lookup(const DScan &sc, int64_t key, uint32_t *start,

@@ -236,6 +236,7 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
       if (it >= srcn) continue;
       if constexpr (SRC == 0) {
         const uint64_t row = src0 + it;
+        if (page_bit(ps.page_skip, row)) continue;  // no query can match a row of this page
         DCol kc;
         kc.data = ps.key; kc.nulls = nullptr; kc.dtype = ps.key_dt; kc.side = 0;
         const uint64_t code = (uint64_t)(load1_i(kc, row) - ps.kmin);

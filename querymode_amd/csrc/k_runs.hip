@@ -715,7 +715,7 @@ struct SRow {
 
 template <int NSV>
 __device__ __forceinline__ void sload(const SScan &ss, uint64_t row, uint64_t r1, SRow<NSV> &w) {
-  w.ok = row < r1;
+  w.ok = row < r1 && !page_bit(ss.page_skip, row);  // a page no query can match is not read
   w.kr = 0;
 #pragma unroll
   for (int j = 0; j < NSV; ++j) { w.vr[j] = 0; w.vn[j] = 0; }
@@ -749,7 +749,7 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
   sload<NSV>(ss, row, r1, cur);
   for (; row < r1; row += S_BLOCK) {
     sload<NSV>(ss, row + S_BLOCK, r1, nxt);
-    if (!(ss.vnull[0] && rdec_null(cur.vn[0], row))) {
+    if (cur.ok && !(ss.vnull[0] && rdec_null(cur.vn[0], row))) {
       const uint64_t x = rdec_code(cur.vr[0], ss.v_dt[0], row);
       uint32_t f = 0;
       for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)

@@ -3240,7 +3240,7 @@ const TCol::PRGeo *pr_geometry(qm_ctx *ctx, const TCol *kt_c, uint64_t rows, uin
 // One launch through the partitioned-runs passes; false (nothing run) when
 // its records or partitions do not fit the kernels.
 bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &rp, std::vector<qm_result *> &out,
-               float &scan_ms, double pass_ms[4]) {
+               float &scan_ms, double pass_ms[4], const uint32_t *page_skip) {
   const size_t n = L.sorted.size();
   const std::vector<uint64_t> &b = rp.bounds;
   const uint32_t nb = (uint32_t)b.size();
@@ -3354,6 +3354,7 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
   ps.bucket_rows = brows.as<uint64_t>();
   ps.status = status.as<uint32_t>();
   ps.gcount = gcount.as<uint64_t>();
+  ps.page_skip = page_skip;  // the row scatter makes no records of these pages' rows
 
   // persistent scatter grids: the workgroups that fit the device at once
   const uint64_t tile = g->tile;
@@ -3466,10 +3467,11 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
 // One shared-interval group through the lean pass: bounds padded for the
 // branch-free search, the group's distinct value columns (the range column
 // first), one workgroup range of rows per block.
-void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms) {
+void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms, const uint32_t *page_skip) {
   SScan ss;
   memset(&ss, 0, sizeof ss);
   ss.nrows = T->rows;
+  ss.page_skip = page_skip;
   std::vector<const TCol *> vc{g.col};
   for (const TCol *t : g.st_t)
     if (t && std::find(vc.begin(), vc.end(), t) == vc.end()) vc.push_back(t);
@@ -3521,7 +3523,7 @@ void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms) 
 // BK_RUNS items the run-sorted kernels cannot take are left to the caller:
 // their idx go to `rejected` (their out[] entries stay null).
 void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std::vector<qm_result *> &out,
-               std::vector<uint32_t> *rejected) {
+               std::vector<uint32_t> *rejected, const uint32_t *page_skip = nullptr) {
   Timer tm(ctx);
   // shared groups
   std::vector<std::unique_ptr<BGroup>> groups;
@@ -3685,7 +3687,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     for (auto &L : pl) {
       RunsPlan rp;
       double pms[4] = {0, 0, 0, 0};
-      if (runs_plan(ctx, T, L, rp, true) && run_pruns(ctx, T, L, rp, out, scan_ms_total, pms)) {
+      if (runs_plan(ctx, T, L, rp, true) && run_pruns(ctx, T, L, rp, out, scan_ms_total, pms, page_skip)) {
         for (BItem *it : L.sorted) runs_done.push_back(it);
         ++nlaunches;
       } else {
@@ -3735,7 +3737,7 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
     h2d(ctx, g->dbounds.p, g->bounds.data(), g->bounds.size());
   }
   for (auto &g : groups)
-    if (g->lean && T->rows) run_shared_lean(ctx, T, *g, scan_ms_total);
+    if (g->lean && T->rows) run_shared_lean(ctx, T, *g, scan_ms_total, page_skip);
   const uint64_t ntiles = std::max<uint64_t>(1, (T->rows + 511) / 512);
   std::vector<DevBuf> qbufs;
   for (auto &L : launches) {
@@ -3986,10 +3988,12 @@ void run_batch(qm_ctx *ctx, const qm_table *T, std::vector<BItem *> &items, std:
 // bitmap (`v.page_skip`) that the scan, selection and top-K kernels skip tile
 // by tile. Returns true when `v` is to be used; `skipped` counts the pages cut
 // off and `interior` the marked ones. Joins read their whole probe table.
-bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint32_t *skipped,
-               uint32_t *interior = nullptr) {
-  *skipped = 0;
-  if (interior) *interior = 0;
+// Per ZONE_ROWS-row page of T: can a row of the page pass the plan's filters
+// (the top-level AND and, if any, one OR group), from the columns' zone maps
+// (src/decode.ts:9-100 canSkipPage)? false when the maps cannot tell (zone
+// maps off, a small table, a join, computed columns, no filter, or a filter the
+// plan reports as an error).
+bool zone_can(qm_ctx *ctx, const qm_table *T, const Plan &pl, std::vector<uint8_t> &can) {
   if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || pl.has_join || !pl.computed.empty() ||
       (pl.filters.empty() && pl.groups.empty()))
     return false;
@@ -4028,7 +4032,7 @@ bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint
       if (c->zmax[p] < lo || c->zmin[p] > hi || c->zmin[p] > c->zmax[p]) meet[g][p] = 0;
   }
   if (!any) return false;
-  std::vector<uint8_t> can(pages);
+  can.assign(pages, 0);
   for (uint64_t p = 0; p < pages; ++p) {
     bool ok = meet[0][p] != 0;
     if (ok && ng) {
@@ -4038,6 +4042,19 @@ bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint
     }
     can[p] = ok ? 1 : 0;
   }
+  return true;
+}
+
+// A view of T without the pages at either end that the plan's filters cannot
+// match, and (interior != nullptr) a bitmap of the pages inside that span they
+// cannot match either; false (no view) when nothing can be skipped.
+bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint32_t *skipped,
+               uint32_t *interior = nullptr) {
+  *skipped = 0;
+  if (interior) *interior = 0;
+  std::vector<uint8_t> can;
+  if (!zone_can(ctx, T, pl, can)) return false;
+  const uint64_t pages = can.size();
   uint64_t p0 = 0, p1 = pages - 1;
   while (p0 < pages && !can[p0]) ++p0;
   if (p0 == pages) p0 = p1 = 0;  // nothing can match: one page keeps the launch shapes
@@ -4074,6 +4091,33 @@ bool zone_view(qm_ctx *ctx, const qm_table *T, const Plan &pl, qm_table &v, uint
   *skipped = (uint32_t)(pages - (p1 - p0 + 1));
   if (interior) *interior = inner;
   return true;
+}
+
+// Pages of T no plan of a batch can match: a bitmap over the whole table (bit
+// p = page p) for the batch's shared table-reading passes, in `buf`; nullptr
+// when every page may hold a row some plan passes (or the maps cannot tell).
+const uint32_t *batch_page_skip(qm_ctx *ctx, const qm_table *T, qm_plan *const *plans, uint32_t n,
+                                std::shared_ptr<DevBuf> &buf, uint32_t *nskip) {
+  *nskip = 0;
+  if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || n == 0) return nullptr;
+  std::vector<uint8_t> any, can;
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!plans[i] || !zone_can(ctx, T, plans[i]->plan, can)) return nullptr;
+    if (any.empty()) any.assign(can.size(), 0);
+    for (size_t p = 0; p < can.size(); ++p) any[p] |= can[p];
+  }
+  std::vector<uint32_t> bits((any.size() + 31) / 32, 0);
+  uint32_t cnt = 0;
+  for (size_t p = 0; p < any.size(); ++p)
+    if (!any[p]) {
+      bits[p >> 5] |= 1u << (p & 31);
+      ++cnt;
+    }
+  if (!cnt) return nullptr;
+  buf = std::make_shared<DevBuf>(ctx, bits.size() * 4 + 64);
+  h2d(ctx, buf->p, bits.data(), bits.size());
+  *nskip = cnt;
+  return buf->as<uint32_t>();
 }
 
 }  // namespace
@@ -4140,8 +4184,8 @@ extern "C" int qm_exec(qm_plan *p, qm_table *left, qm_table *right, qm_result **
       try {
         if (it.kind == BK_SORTED || (it.kind == BK_SHARED && ctx->tune.shared_lean)) {
           std::vector<BItem *> items{&it};
-          run_batch(ctx, T, items, results, nullptr);
-          inner_read = true;
+          run_batch(ctx, T, items, results, nullptr, T->page_skip);
+          inner_read = it.kind == BK_SORTED;  // the shared-interval pass skips the marked pages
         } else {
           run_aggregates(ctx, T, nullptr, plans, results, false, nullptr);
           const uint32_t st = results[0]->stats.strategy;
@@ -4421,9 +4465,23 @@ void batch_exec_body(qm_ctx *ctx, qm_plan *const *plans, uint32_t nplans, qm_tab
   try {
     if (!fused.empty()) {
       std::vector<BItem *> items;
-      for (auto &it : fused) items.push_back(it.get());
+      std::vector<qm_plan *> fplans;
+      for (auto &it : fused) {
+        items.push_back(it.get());
+        fplans.push_back(plans[it->idx]);
+      }
+      // pages no fused plan can match: the partitioned-runs row scatter and
+      // the shared-interval pass do not read them (zone maps)
+      std::shared_ptr<DevBuf> skipbuf;
+      uint32_t nskip = 0;
+      const uint32_t *skip = batch_page_skip(ctx, table, fplans.data(), (uint32_t)fplans.size(), skipbuf, &nskip);
       std::vector<uint32_t> rejected;
-      run_batch(ctx, table, items, all, &rejected);
+      run_batch(ctx, table, items, all, &rejected, skip);
+      for (auto &it : fused) {
+        qm_result *r = all[it->idx];
+        if (r && (r->stats.strategy == QM_STRAT_PART_RUNS || r->stats.strategy == QM_STRAT_SHARED_INTERVAL))
+          r->stats.pages_skipped = nskip;
+      }
       agg_idx.insert(agg_idx.end(), rejected.begin(), rejected.end());
       std::sort(agg_idx.begin(), agg_idx.end());
     }

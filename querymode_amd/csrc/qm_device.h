@@ -366,6 +366,7 @@ struct SScan {
   int32_t _p0;
   const uint64_t *bounds;
   uint64_t *table;                   // (nbounds + 1) x dom x entry_words, zeroed
+  const uint32_t *page_skip;         // bit p: no query can match a row of ZONE_ROWS-row page p (nullptr: none)
 };
 
 // ---- one query grouped by a sorted key (lean): a wave owns the runs that
@@ -455,6 +456,7 @@ struct PRScan {
   uint32_t *out_nbase[R_MAX_OUTS];
   const uint64_t *row_off;       // [nq] first arena row of each query
   uint32_t *status;              // [0] a partition exceeded cap
+  const uint32_t *page_skip;     // bit p: no query can match a row of ZONE_ROWS-row page p (nullptr: none)
 };
 
 }  // namespace qm

@@ -133,3 +133,49 @@ def test_parquet_upload(qm, tmp_path):
     d2 = QueryDescriptor("pq", filters=[FilterOp("cat", "eq", 3)], projections=["id", "amount"],
                          sortColumn="amount", sortDirection="desc", limit=20)
     assert_rows_equal(qm.executor.execute(d2).rows, O.run(d2, {"pq": ot}), ctx="parquet top-k")
+
+
+def test_batch_skips_pages_no_query_can_match(qm):
+    """A batch whose range filters all fall on a clustered column: the pages
+    no query of the batch can match are not read by the shared-interval pass
+    (GROUP BY a small key) nor by the partitioned-runs row scatter (GROUP BY a
+    key in any row order). Results equal the same batch with zone maps off,
+    byte for byte, and the oracle; pagesSkipped counts those pages."""
+    from querymode_amd import datagen
+
+    n = 3_000_001
+    cols = datagen.batch_table(n=n, seed=81)
+    cols["v2"] = np.sort(cols["v2"])                       # clustered: pages are disjoint value ranges
+    cols["k2"] = datagen.uniform_int(81, 72, 0, n, 0, 60_000, np.int64)
+    qm.register_table("zb", cols)
+    ranges = [(0.20, 0.26), (0.21, 0.22), (0.60, 0.63), (0.605, 0.61), (0.90, 0.9005)]
+    aggs = [AggregateOp("sum", "v1"), AggregateOp("count", "*"), AggregateOp("min", "v1"), AggregateOp("max", "v2")]
+    descs = [QueryDescriptor("zb", filters=[FilterOp("v2", "gte", lo), FilterOp("v2", "lt", hi)], groupBy=[key],
+                             aggregates=aggs) for lo, hi in ranges for key in ("k1", "k2")]
+    v = cols["v2"]
+    pages = (n + Z - 1) // Z
+    can = np.zeros(pages, bool)
+    for p in range(pages):
+        a = v[p * Z:(p + 1) * Z]
+        can[p] = any(a.max() >= lo and a.min() < hi for lo, hi in ranges)
+    want_skip = int((~can).sum())
+    assert want_skip >= 30
+    ex, dev = qm.executor, qm.device
+    try:
+        dev.tune_set("zone_maps", 1)
+        on = ex.execute_batch(descs)
+        dev.tune_set("zone_maps", 0)
+        off = ex.execute_batch(descs)
+    finally:
+        dev.tune_set("zone_maps", 1)
+    assert {r.stats["strategy"] for r in on} == {"shared_interval", "part_runs"}, [r.stats["strategy"] for r in on]
+    ot = O.Table(cols)
+    for i, (d, a, b) in enumerate(zip(descs, on, off)):
+        assert bytes(a.columnarData) == bytes(b.columnarData), i
+        assert a.pagesSkipped == want_skip and b.pagesSkipped == 0, (i, a.pagesSkipped)
+        if i < 4:
+            assert_groups_equal(a.rows, O.run(d, {"zb": ot}), d.groupBy, ctx=f"zone batch q{i}")
+    # one query with no filter: every page may hold its rows, nothing is skipped
+    more = ex.execute_batch(descs[:2] + [QueryDescriptor("zb", groupBy=["k1"], aggregates=aggs)])
+    assert all(r.pagesSkipped == 0 for r in more)
+    assert bytes(more[0].columnarData) == bytes(on[0].columnarData)
