@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("QM_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_scan.hip", "k_part.hip", "k_result.hip", "k_topk.hip", "k_join.hip", "k_stats.hip", "k_sort.hip",
-               "k_pruns.hip", "k_post.hip",
+               "k_pruns.hip", "k_post.hip", "k_radix.hip",
                "k_batch.hip", "k_runs.hip", "qm_plan.cpp", "qm_api.cpp"]
 
 HIP_FLAGS = [

@@ -4,7 +4,6 @@
 // these stages are selections (ascending row ids) and permutations that the
 // host applies with the gather kernels.
 #include <cmath>
-#include <rocprim/device/device_radix_sort.hpp>
 #include "k_common.hip.h"
 #include "qm_kernels.h"
 
@@ -380,21 +379,12 @@ extern "C" hipError_t qmk_gather_u64(const uint64_t *src, const uint64_t *idx, u
   return hipGetLastError();
 }
 
-extern "C" size_t qmk_radix_pairs_tmp_bytes(uint64_t n) {
-  size_t b = 0;
-  if (rocprim::radix_sort_pairs(nullptr, b, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                (uint64_t *)nullptr, (size_t)n, 0, 64) != hipSuccess)
-    return 0;
-  return b;
-}
+extern "C" size_t qmk_radix_pairs_tmp_bytes(uint64_t n) { return qmk_radix_tmp_bytes(n, 8); }
 
-// Stable radix sort of (key, value) pairs on bits [0, end_bit) of the keys.
+// Stable radix sort of (key, value) pairs on bits [0, end_bit) of the keys (k_radix.hip).
 extern "C" hipError_t qmk_radix_pairs(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout,
                                       uint64_t n, int end_bit, void *tmp, size_t tmp_bytes, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  size_t b = tmp_bytes;
-  return rocprim::radix_sort_pairs(tmp, b, const_cast<uint64_t *>(kin), kout, const_cast<uint64_t *>(vin), vout,
-                                   (size_t)n, 0, (unsigned)end_bit, st);
+  return qmk_radix_sort_u64(kin, kout, vin, vout, n, 0, end_bit, tmp, tmp_bytes, st);
 }
 
 extern "C" hipError_t qmk_qmcb_bits(const uint8_t *src, int is_bool, uint64_t row0, uint64_t nrows, uint8_t *dst,

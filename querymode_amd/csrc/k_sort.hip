@@ -3,7 +3,6 @@
 // (rocPRIM), NULL/NaN last, ties in input order — the order V8's stable
 // Array.prototype.sort gives with rowComparator (src/types.ts:26-38).
 #include <cstring>
-#include <rocprim/device/device_radix_sort.hpp>
 #include "k_common.hip.h"
 #include "qm_kernels.h"
 
@@ -108,17 +107,14 @@ extern "C" hipError_t qmk_sort_gather_keys(const DCol *c, const uint64_t *ids, u
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-static size_t rocprim_bytes(uint64_t n) {
-  size_t a = 0, b = 0;
-  rocprim::radix_sort_pairs(nullptr, a, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                            (uint64_t *)nullptr, (size_t)n, 0, 64);
-  rocprim::radix_sort_pairs(nullptr, b, (uint8_t *)nullptr, (uint8_t *)nullptr, (uint64_t *)nullptr,
-                            (uint64_t *)nullptr, (size_t)n, 0, 1);
+// scratch of the two radix sorts (k_radix.hip): 64-bit order codes, then 1-bit classes
+static size_t radix_bytes(uint64_t n) {
+  const size_t a = qmk_radix_tmp_bytes(n, 8), b = qmk_radix_tmp_bytes(n, 1);
   return a > b ? a : b;
 }
 
 extern "C" size_t qmk_sort_tmp_bytes(uint64_t n) {
-  return 4 * align256(n * 8) + 2 * align256(n) + align256(rocprim_bytes(n));
+  return 4 * align256(n * 8) + 2 * align256(n) + align256(radix_bytes(n));
 }
 
 extern "C" hipError_t qmk_sort_keys(const DCol *c, uint64_t n, int desc, uint64_t *rows_out, void *tmp,
@@ -132,14 +128,14 @@ extern "C" hipError_t qmk_sort_keys(const DCol *c, uint64_t n, int desc, uint64_
   uint64_t *v1 = (uint64_t *)p; p += align256(n * 8);
   uint8_t *c0 = (uint8_t *)p; p += align256(n);
   uint8_t *c1 = (uint8_t *)p; p += align256(n);
-  size_t rb = rocprim_bytes(n);
+  const size_t rb = radix_bytes(n);
   void *rtmp = p;
   unsigned grid = (unsigned)((n + 255) / 256 > 65536 ? 65536 : (n + 255) / 256);
   hipLaunchKernelGGL(sort_keys_kernel, dim3(grid), dim3(256), 0, st, *c, n, desc, k0, v0, c0);
-  hipError_t e = rocprim::radix_sort_pairs(rtmp, rb, k0, k1, v0, v1, (size_t)n, 0, 64, st);
+  hipError_t e = qmk_radix_sort_u64(k0, k1, v0, v1, n, 0, 64, rtmp, rb, st);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(gather_cls_kernel, dim3(grid), dim3(256), 0, st, c0, v1, n, c1);
-  e = rocprim::radix_sort_pairs(rtmp, rb, c1, c0, v1, rows_out, (size_t)n, 0, 1, st);
+  e = qmk_radix_sort_u8(c1, c0, v1, rows_out, n, 0, 1, rtmp, rb, st);
   if (e != hipSuccess) return e;
   return hipGetLastError();
 }

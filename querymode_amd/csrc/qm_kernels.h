@@ -96,6 +96,13 @@ hipError_t qmk_gather_u64(const uint64_t *src, const uint64_t *idx, uint64_t n, 
 hipError_t qmk_qmcb_bits(const uint8_t *src, int is_bool, uint64_t row0, uint64_t nrows, uint8_t *dst, uint32_t *any,
                          hipStream_t st);
 size_t qmk_radix_pairs_tmp_bytes(uint64_t n);
+// stable LSD radix sort of (key, 64-bit value) pairs on key bits [begin_bit, end_bit) (k_radix.hip);
+// kin != kout, vin != vout; tmp of qmk_radix_tmp_bytes(n, key bytes)
+size_t qmk_radix_tmp_bytes(uint64_t n, int key_bytes);
+hipError_t qmk_radix_sort_u64(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout, uint64_t n,
+                              int begin_bit, int end_bit, void *tmp, size_t tmp_bytes, hipStream_t st);
+hipError_t qmk_radix_sort_u8(const uint8_t *kin, uint8_t *kout, const uint64_t *vin, uint64_t *vout, uint64_t n,
+                             int begin_bit, int end_bit, void *tmp, size_t tmp_bytes, hipStream_t st);
 hipError_t qmk_radix_pairs(const uint64_t *kin, uint64_t *kout, const uint64_t *vin, uint64_t *vout, uint64_t n,
                            int end_bit, void *tmp, size_t tmp_bytes, hipStream_t st);
 // Raises a kernel's dynamic-LDS limit to at least `bytes` (the attribute is
