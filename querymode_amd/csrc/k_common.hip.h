@@ -313,6 +313,26 @@ __device__ __forceinline__ uint32_t eval_query8(const DQuery &q, const DCol *col
 }
 
 // ---- zone maps: a tile starting at `row` lies in a page the query skips ----
+// Interval of an order code x among sorted bounds (#{bounds <= x}) through a
+// slot table: codes [lo + (k << shift), lo + ((k + 1) << shift)) lie in
+// buckets ts[k] .. ts[k + 1] (ts[k] = the bounds below the slot's first code),
+// so only the few bounds inside x's slot are searched (usually 0 to 2 steps)
+// instead of all of them; a bound search over scattered LDS words is what the
+// per-row bucket of the batched passes otherwise costs.
+__device__ __forceinline__ uint32_t bucket_slot(const uint64_t *sb, const uint16_t *ts, uint64_t lo, int shift,
+                                                uint32_t slots, uint32_t nb, uint64_t x) {
+  if (nb == 0 || x < lo) return 0;
+  const uint64_t k = (x - lo) >> shift;
+  if (k >= slots) return nb;
+  uint32_t f = ts[k], h = ts[k + 1];
+  while (f < h) {  // first bound above x in [f, h)
+    const uint32_t m = (f + h) >> 1;
+    if (sb[m] <= x) f = m + 1;
+    else h = m;
+  }
+  return f;
+}
+
 __device__ __forceinline__ bool page_bit(const uint32_t *bits, uint64_t row) {
   if (!bits) return false;
   const uint64_t p = row / ZONE_ROWS;

@@ -178,7 +178,8 @@ __device__ __forceinline__ uint64_t pr_block_exscan64_n(uint64_t v, uint64_t *sw
 // passes 1 and 2: scatter by a key digit through an LDS-staged counting sort
 // ---------------------------------------------------------------------------
 template <int SRC, int RW>
-__global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
+// (at most 128 VGPRs: two workgroups per CU, which their LDS allows, need four waves per SIMD)
+__global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
   constexpr int IPT = PR_STAGE_WORDS / RW / PR_BLOCK;
   constexpr uint32_t TILE = (uint32_t)IPT * PR_BLOCK;
   constexpr int NVX = (RW - 1) < PR_MAX_V ? (RW - 1) : PR_MAX_V;  // value columns a record of RW words can hold
@@ -188,6 +189,7 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
   __shared__ uint32_t s_lim[PR_MAX_P];
   __shared__ uint64_t s_gofs[PR_MAX_P];
   __shared__ uint64_t s_bounds[SRC == 0 ? R_MAX_BUCKETS : 1];
+  __shared__ uint16_t s_bslot[SRC == 0 ? PR_BSLOTS + 1 : 1];
   __shared__ uint32_t s_brow[SRC == 0 ? R_MAX_BUCKETS : 1];
   __shared__ uint32_t s_sw[PR_WAVES + 1];
   const int t = threadIdx.x;
@@ -197,6 +199,8 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
   if constexpr (SRC == 0) {
     const int nbp = (1 << ps.bsteps) - 1;
     for (int i = t; i < nbp; i += PR_BLOCK) s_bounds[i] = ps.bounds[i];
+    if (ps.bslot)
+      for (uint32_t i = t; i <= ps.bslots; i += PR_BLOCK) s_bslot[i] = ps.bslot[i];
     for (int i = t; i < R_MAX_BUCKETS; i += PR_BLOCK) s_brow[i] = 0;
   }
   const uint64_t nunits = SRC == 0 ? (ps.nrows + TILE - 1) / TILE : (uint64_t)ps.chunk0[ps.np1];
@@ -226,44 +230,73 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_scatter_kernel(PRScan ps) {
     uint32_t kb[IPT], dg[IPT], rk[IPT], nbits[IPT];
     uint64_t v[IPT][NVX > 0 ? NVX : 1];
     uint32_t w[IPT][RW];
+    if constexpr (SRC == 0) {
+      // every load of the thread's rows first (the rows are independent), then
+      // the buckets: no load waits for another row's bucket search
+      uint64_t kraw[IPT], praw[IPT];
+      uint32_t live = 0, pnul = 0;
 #pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-      const uint64_t it = (uint64_t)t + (uint64_t)i * PR_BLOCK;
-      rk[i] = ~0u;
-      dg[i] = 0;
-      kb[i] = 0;
-      nbits[i] = 0;
-      if (it >= srcn) continue;
-      if constexpr (SRC == 0) {
+      for (int i = 0; i < IPT; ++i) {
+        const uint64_t it = (uint64_t)t + (uint64_t)i * PR_BLOCK;
+        rk[i] = ~0u;
+        dg[i] = 0;
+        kb[i] = 0;
+        nbits[i] = 0;
+        kraw[i] = 0;
+        praw[i] = 0;
+#pragma unroll
+        for (int j = 0; j < NVX; ++j) v[i][j] = 0;
+        if (it >= srcn) continue;
         const uint64_t row = src0 + it;
         if (page_bit(ps.page_skip, row)) continue;  // no query can match a row of this page
+        live |= 1u << i;
         DCol kc;
         kc.data = ps.key; kc.nulls = nullptr; kc.dtype = ps.key_dt; kc.side = 0;
-        const uint64_t code = (uint64_t)(load1_i(kc, row) - ps.kmin);
-        uint32_t e;
-        if (pr_null(ps.pnull, row)) e = nb + 1;
-        else e = pr_bucket(s_bounds, ps.bsteps, nb, pr_code(pr_bits(ps.pcol, ps.p_dt, row), ps.p_dt));
-        if (!((ps.bucket_on[e >> 5] >> (e & 31)) & 1u)) continue;
-        const uint64_t d = code >> ps.sh1;
-        if (d >= ndig) { atomicOr(ps.status, 1u); continue; }
-        dg[i] = (uint32_t)d;
-        kb[i] = ((uint32_t)code & kmask1) | (e << 16);
+        kraw[i] = (uint64_t)load1_i(kc, row);
+        praw[i] = pr_bits(ps.pcol, ps.p_dt, row);
+        if (pr_null(ps.pnull, row)) pnul |= 1u << i;
 #pragma unroll
         for (int j = 0; j < NVX; ++j) {
-          v[i][j] = 0;
           if (j < ps.nv) {
             v[i][j] = pr_bits(ps.vcol[j], ps.v_dt[j], row);
             if (pr_null(ps.vnull[j], row)) nbits[i] |= 1u << j;
           }
         }
+      }
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        if (!((live >> i) & 1u)) continue;
+        const uint64_t code = kraw[i] - (uint64_t)ps.kmin;
+        uint32_t e;
+        if ((pnul >> i) & 1u) e = nb + 1;
+        else {
+          const uint64_t x = pr_code(praw[i], ps.p_dt);
+          e = ps.bslot ? bucket_slot(s_bounds, s_bslot, ps.bslot_lo, ps.bslot_shift, ps.bslots, nb, x)
+                       : pr_bucket(s_bounds, ps.bsteps, nb, x);
+        }
+        if (!((ps.bucket_on[e >> 5] >> (e & 31)) & 1u)) continue;
+        const uint64_t d = code >> ps.sh1;
+        if (d >= ndig) { atomicOr(ps.status, 1u); continue; }
+        dg[i] = (uint32_t)d;
+        kb[i] = ((uint32_t)code & kmask1) | (e << 16);
         atomicAdd(&s_brow[e], 1u);
-      } else {
+        rk[i] = atomicAdd(&s_hist[dg[i]], 1u);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        const uint64_t it = (uint64_t)t + (uint64_t)i * PR_BLOCK;
+        rk[i] = ~0u;
+        dg[i] = 0;
+        kb[i] = 0;
+        nbits[i] = 0;
+        if (it >= srcn) continue;
         const uint32_t *r = ps.rec1 + (src0 + it) * RW;
 #pragma unroll
         for (int k = 0; k < RW; ++k) w[i][k] = r[k];
         dg[i] = (w[i][0] & 0xFFFFu) >> ps.s;
+        rk[i] = atomicAdd(&s_hist[dg[i]], 1u);
       }
-      rk[i] = atomicAdd(&s_hist[dg[i]], 1u);
     }
     __syncthreads();
     // digit starts in the tile, and one reservation per digit in its region

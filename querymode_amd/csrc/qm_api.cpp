@@ -3162,6 +3162,25 @@ void run_runs(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan &
 // ---- partitioned-runs launches: queries grouped by one integer key stored in
 // any row order (k_pruns.hip) ----
 
+// Slot table of sorted, distinct bounds for bucket_slot(): at most PR_BSLOTS
+// equal slots of [b.front(), b.back()]; ts[k] = the bounds below slot k's first
+// code, ts[slots] = all of them. Returns the slot count (0: no bounds).
+uint32_t bucket_slots(const std::vector<uint64_t> &b, std::vector<uint16_t> &ts, uint64_t &lo, int32_t &shift) {
+  ts.clear();
+  lo = 0;
+  shift = 0;
+  if (b.empty() || b.size() > 0xFFFF) return 0;
+  lo = b.front();
+  const uint64_t span = b.back() - b.front();
+  while ((span >> shift) >= PR_BSLOTS) ++shift;
+  const uint32_t slots = (uint32_t)(span >> shift) + 1;
+  ts.resize(slots + 1);
+  for (uint32_t k = 0; k < slots; ++k)
+    ts[k] = (uint16_t)(std::lower_bound(b.begin(), b.end(), lo + ((uint64_t)k << shift)) - b.begin());
+  ts[slots] = (uint16_t)b.size();
+  return slots;
+}
+
 // Partition geometry for a launch of `nbk` buckets and `nsn` non-COUNT(*)
 // states, from the key histogram: final partitions of 2^s keys (the largest
 // s, at most the pr_max_s knob, whose fullest partition fits the write pass's
@@ -3336,6 +3355,14 @@ bool run_pruns(qm_ctx *ctx, const qm_table *T, const BLaunch &L, const RunsPlan 
   h2d(ctx, dbounds.p, padded.data(), padded.size());
   h2d(ctx, dqe.p, qe.data(), n);
   ps.bounds = dbounds.as<uint64_t>();
+  // the row scatter's bucket lookup (bucket_slot)
+  std::vector<uint16_t> bslot;
+  ps.bslots = bucket_slots(b, bslot, ps.bslot_lo, ps.bslot_shift);
+  DevBuf dbslot(ctx, bslot.size() * 2 + 64);
+  if (ps.bslots) {
+    h2d(ctx, dbslot.p, bslot.data(), bslot.size());
+    ps.bslot = dbslot.as<uint16_t>();
+  }
   ps.qe = dqe.as<uint32_t>();
   const uint64_t recb = g->total * (uint64_t)rw * 4 + 64;
   DevBuf rec1(ctx, recb), rec2(ctx, recb);
@@ -3504,6 +3531,13 @@ void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms, 
   ss.bsteps = steps;
   ss.bounds = db.as<uint64_t>();
   ss.table = g.table.as<uint64_t>();
+  std::vector<uint16_t> bslot;
+  ss.bslots = bucket_slots(g.bounds, bslot, ss.bslot_lo, ss.bslot_shift);
+  DevBuf dbslot(ctx, bslot.size() * 2 + 64);
+  if (ss.bslots) {
+    h2d(ctx, dbslot.p, bslot.data(), bslot.size());
+    ss.bslot = dbslot.as<uint16_t>();
+  }
   const int bpc = qmk_shared_blocks_per_cu(&ss);
   uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)std::max(1, bpc) * (uint64_t)ctx->tune.shared_waves;
   uint64_t rpb = (T->rows + grid - 1) / grid;

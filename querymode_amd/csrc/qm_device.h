@@ -367,6 +367,10 @@ struct SScan {
   const uint64_t *bounds;
   uint64_t *table;                   // (nbounds + 1) x dom x entry_words, zeroed
   const uint32_t *page_skip;         // bit p: no query can match a row of ZONE_ROWS-row page p (nullptr: none)
+  const uint16_t *bslot;             // interval lookup (bucket_slot), as PRScan::bslot; nullptr: search
+  uint64_t bslot_lo;
+  int32_t bslot_shift;
+  uint32_t bslots;
 };
 
 // ---- one query grouped by a sorted key (lean): a wave owns the runs that
@@ -457,6 +461,13 @@ struct PRScan {
   const uint64_t *row_off;       // [nq] first arena row of each query
   uint32_t *status;              // [0] a partition exceeded cap
   const uint32_t *page_skip;     // bit p: no query can match a row of ZONE_ROWS-row page p (nullptr: none)
+  // bucket lookup (bucket_slot): codes [bslot_lo + (k << bslot_shift), + (1 << bslot_shift)) hold
+  // buckets bslot[k] .. bslot[k + 1] (bslot[k] = bounds below the slot's first code); nullptr: search
+  const uint16_t *bslot;
+  uint64_t bslot_lo;
+  int32_t bslot_shift;
+  uint32_t bslots;               // slots (<= PR_BSLOTS)
 };
+static constexpr uint32_t PR_BSLOTS = 1024;
 
 }  // namespace qm
