@@ -190,6 +190,7 @@ __global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
   __shared__ uint64_t s_gofs[PR_MAX_P];
   __shared__ uint64_t s_bounds[SRC == 0 ? R_MAX_BUCKETS : 1];
   __shared__ uint16_t s_bslot[SRC == 0 ? PR_BSLOTS + 1 : 1];
+  __shared__ uint32_t s_chunk0[SRC == 1 ? PR_MAX_P + 1 : 1];  // first unit of each level-1 region
   __shared__ uint32_t s_brow[SRC == 0 ? R_MAX_BUCKETS : 1];
   __shared__ uint32_t s_sw[PR_WAVES + 1];
   const int t = threadIdx.x;
@@ -203,6 +204,8 @@ __global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
       for (uint32_t i = t; i <= ps.bslots; i += PR_BLOCK) s_bslot[i] = ps.bslot[i];
     for (int i = t; i < R_MAX_BUCKETS; i += PR_BLOCK) s_brow[i] = 0;
   }
+  if constexpr (SRC == 1)
+    for (uint32_t i = t; i <= ps.np1; i += PR_BLOCK) s_chunk0[i] = ps.chunk0[i];
   const uint64_t nunits = SRC == 0 ? (ps.nrows + TILE - 1) / TILE : (uint64_t)ps.chunk0[ps.np1];
   const uint32_t kmask1 = (1u << ps.sh1) - 1u;
   for (uint64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
@@ -215,13 +218,13 @@ __global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
       src0 = u * TILE;
       srcn = ps.nrows - src0 < TILE ? ps.nrows - src0 : TILE;
     } else {
-      uint32_t lo = 0, hi = ps.np1;  // last p1 with chunk0[p1] <= u
+      uint32_t lo = 0, hi = ps.np1;  // last p1 with chunk0[p1] <= u (in LDS)
       while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (ps.chunk0[mid] <= u) lo = mid; else hi = mid;
+        if (s_chunk0[mid] <= u) lo = mid; else hi = mid;
       }
       p1 = lo;
-      const uint64_t ci = u - ps.chunk0[p1];
+      const uint64_t ci = u - s_chunk0[p1];
       const uint64_t cap1 = ps.start1[p1 + 1] - ps.start1[p1];
       const uint64_t n1 = ps.cur1[p1] < cap1 ? ps.cur1[p1] : cap1;
       src0 = ps.start1[p1] + ci * TILE;
@@ -292,8 +295,16 @@ __global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
         nbits[i] = 0;
         if (it >= srcn) continue;
         const uint32_t *r = ps.rec1 + (src0 + it) * RW;
+        if constexpr (RW % 4 == 0) {  // records of 4 or 8 words start 16-byte aligned
 #pragma unroll
-        for (int k = 0; k < RW; ++k) w[i][k] = r[k];
+          for (int k = 0; k < RW; k += 4) {
+            const uint4 x = *reinterpret_cast<const uint4 *>(r + k);
+            w[i][k] = x.x; w[i][k + 1] = x.y; w[i][k + 2] = x.z; w[i][k + 3] = x.w;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < RW; ++k) w[i][k] = r[k];
+        }
         dg[i] = (w[i][0] & 0xFFFFu) >> ps.s;
         rk[i] = atomicAdd(&s_hist[dg[i]], 1u);
       }
