@@ -317,8 +317,7 @@ __device__ __forceinline__ uint32_t eval_query8(const DQuery &q, const DCol *col
 // slot table: codes [lo + (k << shift), lo + ((k + 1) << shift)) lie in
 // buckets ts[k] .. ts[k + 1] (ts[k] = the bounds below the slot's first code),
 // so only the few bounds inside x's slot are searched (usually 0 to 2 steps)
-// instead of all of them; a bound search over scattered LDS words is what the
-// per-row bucket of the batched passes otherwise costs.
+// instead of all of them (the partitioned-runs row scatter).
 __device__ __forceinline__ uint32_t bucket_slot(const uint64_t *sb, const uint16_t *ts, uint64_t lo, int shift,
                                                 uint32_t slots, uint32_t nb, uint64_t x) {
   if (nb == 0 || x < lo) return 0;

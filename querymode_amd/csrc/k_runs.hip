@@ -738,11 +738,8 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
   lds_u64 *tab = (lds_u64 *)(sb + nbp + 1);               // (interval, key) entries
   const uint32_t W = (uint32_t)ss.entry_words;
   const uint32_t nent = (uint32_t)(ss.nbounds + 1) * ss.dom;
-  uint16_t *ts = reinterpret_cast<uint16_t *>(s_dyn + ((uint64_t)nbp + 1 + (uint64_t)nent * W) * 8);  // slot table
   for (int i = threadIdx.x; i < nbp; i += S_BLOCK) sb[i] = ss.bounds[i];
   for (uint32_t i = threadIdx.x; i < nent * W; i += S_BLOCK) tab[i] = 0;
-  if (ss.bslot)
-    for (uint32_t i = threadIdx.x; i <= ss.bslots; i += S_BLOCK) ts[i] = ss.bslot[i];
   __syncthreads();
   const uint64_t r0 = (uint64_t)blockIdx.x * ss.rows_per_block;
   const uint64_t r1 = (r0 + ss.rows_per_block < ss.nrows) ? r0 + ss.rows_per_block : ss.nrows;
@@ -754,15 +751,10 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
     sload<NSV>(ss, row + S_BLOCK, r1, nxt);
     if (cur.ok && !(ss.vnull[0] && rdec_null(cur.vn[0], row))) {
       const uint64_t x = rdec_code(cur.vr[0], ss.v_dt[0], row);
-      uint32_t e;
-      if (ss.bslot) {
-        e = bucket_slot(sb, ts, ss.bslot_lo, ss.bslot_shift, ss.bslots, nb, x);
-      } else {
-        uint32_t f = 0;
-        for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)
-          if (sb[f + step - 1] <= x) f += step;
-        e = f < nb ? f : nb;
-      }
+      uint32_t f = 0;
+      for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)
+        if (sb[f + step - 1] <= x) f += step;
+      const uint32_t e = f < nb ? f : nb;
       uint32_t d = 0;
       if (ss.key) {
         const int64_t kv = rdec_i(cur.kr, ss.key_dt, row) - ss.kmin;
@@ -1077,8 +1069,7 @@ extern "C" hipError_t qmk_runs_place(const RScan *rs, int win, hipStream_t st) {
 
 extern "C" size_t qmk_shared_lds_bytes(const SScan *ss) {
   const uint64_t nbp = (1ull << ss->bsteps) - 1;
-  const uint64_t slot_words = ss->bslot ? ((uint64_t)ss->bslots + 1 + 3) / 4 : 0;  // u16 slots, in 8-byte words
-  return (size_t)(nbp + 1 + (uint64_t)(ss->nbounds + 1) * ss->dom * (uint64_t)ss->entry_words + slot_words) * 8;
+  return (size_t)(nbp + 1 + (uint64_t)(ss->nbounds + 1) * ss->dom * (uint64_t)ss->entry_words) * 8;
 }
 
 extern "C" int qmk_shared_blocks_per_cu(const SScan *ss) {

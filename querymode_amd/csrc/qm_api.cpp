@@ -3531,13 +3531,6 @@ void run_shared_lean(qm_ctx *ctx, const qm_table *T, BGroup &g, float &scan_ms, 
   ss.bsteps = steps;
   ss.bounds = db.as<uint64_t>();
   ss.table = g.table.as<uint64_t>();
-  std::vector<uint16_t> bslot;
-  ss.bslots = bucket_slots(g.bounds, bslot, ss.bslot_lo, ss.bslot_shift);
-  DevBuf dbslot(ctx, bslot.size() * 2 + 64);
-  if (ss.bslots) {
-    h2d(ctx, dbslot.p, bslot.data(), bslot.size());
-    ss.bslot = dbslot.as<uint16_t>();
-  }
   const int bpc = qmk_shared_blocks_per_cu(&ss);
   uint64_t grid = (uint64_t)ctx->num_cus * (uint64_t)std::max(1, bpc) * (uint64_t)ctx->tune.shared_waves;
   uint64_t rpb = (T->rows + grid - 1) / grid;

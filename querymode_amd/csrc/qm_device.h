@@ -367,10 +367,6 @@ struct SScan {
   const uint64_t *bounds;
   uint64_t *table;                   // (nbounds + 1) x dom x entry_words, zeroed
   const uint32_t *page_skip;         // bit p: no query can match a row of ZONE_ROWS-row page p (nullptr: none)
-  const uint16_t *bslot;             // interval lookup (bucket_slot), as PRScan::bslot; nullptr: search
-  uint64_t bslot_lo;
-  int32_t bslot_shift;
-  uint32_t bslots;
 };
 
 // ---- one query grouped by a sorted key (lean): a wave owns the runs that
