@@ -4128,10 +4128,19 @@ const uint32_t *batch_page_skip(qm_ctx *ctx, const qm_table *T, qm_plan *const *
   *nskip = 0;
   if (!ctx->tune.zone_maps || T->rows <= ZONE_ROWS || n == 0) return nullptr;
   std::vector<uint8_t> any, can;
+  size_t open = 0;  // pages no plan seen so far can match
   for (uint32_t i = 0; i < n; ++i) {
     if (!plans[i] || !zone_can(ctx, T, plans[i]->plan, can)) return nullptr;
-    if (any.empty()) any.assign(can.size(), 0);
-    for (size_t p = 0; p < can.size(); ++p) any[p] |= can[p];
+    if (any.empty()) {
+      any.assign(can.size(), 0);
+      open = can.size();
+    }
+    for (size_t p = 0; p < can.size(); ++p)
+      if (can[p] && !any[p]) {
+        any[p] = 1;
+        --open;
+      }
+    if (open == 0) return nullptr;  // every page may hold a row some plan passes
   }
   std::vector<uint32_t> bits((any.size() + 31) / 32, 0);
   uint32_t cnt = 0;
