@@ -452,76 +452,57 @@ __device__ __forceinline__ int pr_out_lw(const ROut &ro) {
   return 3;
 }
 
-// one output value of one group as a bit pattern (the values finalize_slot
-// derives: a group with no counted value is NULL; an all-NaN MIN / MAX keeps
-// the identity); the output kind is uniform
-__device__ __forceinline__ uint64_t pr_out_bits(const ROut &ro, int64_t key, uint64_t v, uint64_t cnt, bool &isnull) {
-  isnull = false;
-  switch (ro.kind) {
-    case OK_KEY: return (uint64_t)key;
-    case OK_COUNT: return cnt;
-    case OK_SUM_I64: isnull = cnt == 0; return isnull ? 0 : v;
-    case OK_AVG_I64: isnull = cnt == 0; return f64_bits(isnull ? 0.0 : (double)(int64_t)v / (double)cnt);
-    default: {  // OK_MIN / OK_MAX
-      const bool mn = ro.kind == OK_MIN;
-      isnull = cnt == 0;
-      if (ro.src_is_float) {
-        const uint64_t d = isnull ? 0ull
-                                  : (v == 0 ? (mn ? 0x7FF0000000000000ull : 0xFFF0000000000000ull)
-                                            : decode_f64_code(mn ? ~v : v));
-        return ro.out_dtype == DT_F32 ? (uint64_t)f32_bits((float)bits_f64(d)) : d;
-      }
-      return isnull ? 0ull : (uint64_t)decode_i64(mn ? ~v : v);
-    }
-  }
-}
-
-// store the low 2^lw bytes of bits as row `row` of a column (lw uniform)
-__device__ __forceinline__ void pr_store(void *dst, uint64_t row, int lw, uint64_t bits) {
-  uint8_t *p = static_cast<uint8_t *>(dst) + (row << lw);
-  if (lw == 3) *gptr(reinterpret_cast<uint64_t *>(p)) = bits;
-  else if (lw == 2) *gptr(reinterpret_cast<uint32_t *>(p)) = (uint32_t)bits;
-  else if (lw == 1) *gptr(reinterpret_cast<uint16_t *>(p)) = (uint16_t)bits;
-  else *gptr(p) = (uint8_t)bits;
-}
-
-// one record's words (rw uniform): one vector load when the record is 2, 4,
-// 6 or 8 words and the buffer is 32-byte aligned, else a load per word
+// one record's words (rw uniform; NW = the array length, rw <= NW): one
+// vector load when the record is 2, 4, 6 or 8 words and the buffer is
+// 32-byte aligned, else a load per word
+template <int NW>
 __device__ __forceinline__ void pr_load_rec(const uint32_t *rec, uint64_t idx, uint32_t rw, bool aligned,
-                                            uint32_t (&w)[PR_MAX_RW]) {
+                                            uint32_t (&w)[NW]) {
   const uint32_t *r = rec + idx * (uint64_t)rw;
 #pragma unroll
-  for (int k = 0; k < PR_MAX_RW; ++k) w[k] = 0;
-  if (aligned && rw == 4) {
-    const uint4 x = *reinterpret_cast<const uint4 *>(r);
-    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
-  } else if (aligned && rw == 8) {
-    const uint4 x = reinterpret_cast<const uint4 *>(r)[0], y = reinterpret_cast<const uint4 *>(r)[1];
-    w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
-    w[4] = y.x; w[5] = y.y; w[6] = y.z; w[7] = y.w;
-  } else if (aligned && rw == 2) {
-    const uint2 x = *reinterpret_cast<const uint2 *>(r);
-    w[0] = x.x; w[1] = x.y;
-  } else if (aligned && rw == 6) {
-    const uint2 x = reinterpret_cast<const uint2 *>(r)[0], y = reinterpret_cast<const uint2 *>(r)[1],
-                z = reinterpret_cast<const uint2 *>(r)[2];
-    w[0] = x.x; w[1] = x.y; w[2] = y.x; w[3] = y.y; w[4] = z.x; w[5] = z.y;
+  for (int k = 0; k < NW; ++k) w[k] = 0;
+  if constexpr (NW == 4) {
+    if (aligned) {
+      const uint4 x = *reinterpret_cast<const uint4 *>(r);
+      w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+      return;
+    }
   } else {
-#pragma unroll
-    for (int k = 0; k < PR_MAX_RW; ++k)
-      if ((uint32_t)k < rw) w[k] = r[k];
+    if (aligned && rw == 4) {
+      const uint4 x = *reinterpret_cast<const uint4 *>(r);
+      w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+      return;
+    } else if (aligned && rw == 8) {
+      const uint4 x = reinterpret_cast<const uint4 *>(r)[0], y = reinterpret_cast<const uint4 *>(r)[1];
+      w[0] = x.x; w[1] = x.y; w[2] = x.z; w[3] = x.w;
+      w[4] = y.x; w[5] = y.y; w[6] = y.z; w[7] = y.w;
+      return;
+    } else if (aligned && rw == 2) {
+      const uint2 x = *reinterpret_cast<const uint2 *>(r);
+      w[0] = x.x; w[1] = x.y;
+      return;
+    } else if (aligned && rw == 6) {
+      const uint2 x = reinterpret_cast<const uint2 *>(r)[0], y = reinterpret_cast<const uint2 *>(r)[1],
+                  z = reinterpret_cast<const uint2 *>(r)[2];
+      w[0] = x.x; w[1] = x.y; w[2] = y.x; w[3] = y.y; w[4] = z.x; w[5] = z.y;
+      return;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < NW; ++k)
+    if ((uint32_t)k < rw) w[k] = r[k];
 }
 
 // contribution of one record (raw words w) to non-COUNT(*) state 1 + s2
-__device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const uint32_t (&w)[PR_MAX_RW]) {
+template <int NW>
+__device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const uint32_t (&w)[NW]) {
   const DState &st = ps.st[1 + s2];
   const int vj = ps.st_v[1 + s2];
   if (vj < 0) return st.kind == SK_COUNT ? 1ull : 0ull;  // COUNT(*)
   uint32_t nbits = 0, lo = 0, hi = 0;
   // words by runtime index, selected over the unrolled record (no dynamic register indexing)
 #pragma unroll
-  for (int k = 0; k < PR_MAX_RW; ++k) {
+  for (int k = 0; k < NW; ++k) {
     nbits = k == ps.null_word ? w[k] : nbits;
     lo = k == ps.v_word[vj] ? w[k] : lo;
     hi = k == ps.v_word[vj] + 1 ? w[k] : hi;
@@ -545,8 +526,11 @@ __device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const u
 // A workgroup takes a contiguous range of final partitions, so the rows it
 // writes for one query are consecutive across its partitions (each result
 // column is written as long runs, not as short pieces from many workgroups).
-template <int NSUM, int NMAX>
+// RW: the record width in words when fixed at compile time (4), else 0 (any
+// width up to PR_MAX_RW, read from the launch).
+template <int NSUM, int NMAX, int RW>
 __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
+  constexpr int NW = RW > 0 ? RW : PR_MAX_RW;
   constexpr int NSA = NSUM > 0 ? NSUM : 1, NMA = NMAX > 0 ? NMAX : 1;
   extern __shared__ __align__(16) uint8_t s_dyn[];
   __shared__ uint32_t s_sw[PR_RWAVES + 1];
@@ -565,7 +549,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   const uint32_t nbin = (NBK << ps.s);
   const uint32_t nbw = nbin / 2 + 1;              // packed u16 entries 0 .. nbin
   const uint32_t cap = ps.cap;
-  const uint32_t rw = (uint32_t)ps.rw;
+  const uint32_t rw = RW > 0 ? (uint32_t)RW : (uint32_t)ps.rw;
   const bool raligned = (reinterpret_cast<uintptr_t>(ps.rec2) & 31u) == 0;
   uint32_t *s_bin = reinterpret_cast<uint32_t *>(s_dyn);
   uint64_t *s_add = reinterpret_cast<uint64_t *>(s_dyn + ((nbw * 4 + 15) & ~15u));
@@ -615,7 +599,7 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
 #pragma unroll
       for (int k = 0; k < NMA; ++k) cm[i][k] = 0;
       if (j >= nP || j >= cap) continue;
-      uint32_t wr[PR_MAX_RW];
+      uint32_t wr[NW];
       pr_load_rec(ps.rec2, pbase + j, rw, raligned, wr);
 #pragma unroll
       for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr);
@@ -727,64 +711,152 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
       }
     }
   };
-  // state st over the sorted positions [a, b) of a non-empty group (st uniform; out of range: 0)
-  auto stval = [&](int st, uint32_t a, uint32_t b) -> uint64_t {
-    if (st == 0) return (uint64_t)(b - a);
+  // The held groups are finished in pairs: per output, each pair dispatches
+  // the output's kind, states and store width once (uniform), and only one
+  // pair's values are live at a time.
+  constexpr int PW = 2;
+  // count state st (COUNT(*) = 0 or an additive COUNT(col)) of held groups
+  // h .. h + PW - 1 (no group: 0)
+  auto cntw = [&](int st, int h, uint64_t (&r)[PW]) {
     if (st >= 1 && st <= NSUM) {
       const uint64_t *ad = s_add + (uint64_t)(st - 1) * cap;
-      return ad[b - 1] - (a ? ad[a - 1] : 0ull);
+#pragma unroll
+      for (int i = 0; i < PW; ++i)
+        r[i] = gb[h + i] ? ad[gb[h + i] - 1] - (ga[h + i] ? ad[ga[h + i] - 1] : 0ull) : 0ull;
+    } else {
+#pragma unroll
+      for (int i = 0; i < PW; ++i) r[i] = st == 0 ? (uint64_t)(gb[h + i] - ga[h + i]) : 0ull;
     }
+  };
+  // value state st (additive, or an order code: the max over the group's
+  // sorted positions [ga, gb)) of held groups h .. h + PW - 1
+  auto valw = [&](int st, int h, uint64_t (&r)[PW]) {
     if (NMAX > 0 && st > NSUM && st <= NSUM + NMAX) {
       const uint64_t *md = s_max + (uint64_t)(st - 1 - NSUM) * cap;
-      uint64_t x0 = 0, x1 = 0;
-      uint32_t i = a;
-      for (; i + 1 < b; i += 2) {
-        x0 = u64max(x0, md[i]);
-        x1 = u64max(x1, md[i + 1]);
+#pragma unroll
+      for (int i = 0; i < PW; ++i) {
+        uint64_t x0 = 0, x1 = 0;
+        uint32_t k = ga[h + i];
+        for (; k + 1 < gb[h + i]; k += 2) {
+          x0 = u64max(x0, md[k]);
+          x1 = u64max(x1, md[k + 1]);
+        }
+        if (k < gb[h + i]) x0 = u64max(x0, md[k]);
+        r[i] = u64max(x0, x1);
       }
-      if (i < b) x0 = u64max(x0, md[i]);
-      return u64max(x0, x1);
+    } else {
+      cntw(st, h, r);
     }
-    return 0;
   };
   // ---- write: the groups in registers (partition P), one output at a time,
-  // so an output's kind and state are uniform and only that state is read
+  // so an output's kind and state are uniform and only that state is read.
+  // Values as finalize_slot derives them: a group with no counted value is
+  // NULL; an all-NaN MIN / MAX keeps the identity.
   auto write = [&](uint32_t P) {
 #pragma unroll 1
     for (int o = 0; o < nouts; ++o) {
       const uint64_t rb = rfl64(s_out[o][0]);
       ROut ro;
       memcpy(&ro, &rb, sizeof ro);
-      void *ob = reinterpret_cast<void *>(rfl64(s_out[o][1]));
+      uint8_t *ob = reinterpret_cast<uint8_t *>(rfl64(s_out[o][1]));
       uint32_t *onb = reinterpret_cast<uint32_t *>(rfl64(s_out[o][2]));
       const int lw = pr_out_lw(ro);
 #pragma unroll
-      for (int i = 0; i < PR_GPT; ++i) {
-        if (gb[i] == 0) continue;
-        uint64_t v = 0, c = 0;
-        int64_t key = 0;
-        if (ro.kind == OK_KEY) key = ps.kmin + (int64_t)(((uint64_t)P << ps.s) + (uint64_t)gk[i]);
-        else {
-          c = stval(ro.cnt_state, ga[i], gb[i]);
-          if (ro.kind != OK_COUNT) v = stval(ro.state, ga[i], gb[i]);
+      for (int h = 0; h < PR_GPT; h += PW) {
+        uint64_t val[PW];
+        uint32_t nm = 0;  // bit i: held group h + i is NULL
+        if (ro.kind == OK_KEY) {
+          const int64_t k0 = ps.kmin + (int64_t)((uint64_t)P << ps.s);
+#pragma unroll
+          for (int i = 0; i < PW; ++i) val[i] = (uint64_t)(k0 + (int64_t)gk[h + i]);
+        } else if (ro.kind == OK_COUNT) {
+          cntw(ro.cnt_state, h, val);
+        } else {
+          {
+            uint64_t c[PW];
+            cntw(ro.cnt_state, h, c);
+#pragma unroll
+            for (int i = 0; i < PW; ++i) nm |= c[i] == 0 ? 1u << i : 0u;
+          }
+          valw(ro.state, h, val);
+          if (ro.kind == OK_SUM_I64) {
+#pragma unroll
+            for (int i = 0; i < PW; ++i) val[i] = (nm >> i) & 1u ? 0ull : val[i];
+          } else if (ro.kind == OK_AVG_I64) {
+            uint64_t c[PW];
+            cntw(ro.cnt_state, h, c);
+#pragma unroll
+            for (int i = 0; i < PW; ++i)
+              val[i] = f64_bits((nm >> i) & 1u ? 0.0 : (double)(int64_t)val[i] / (double)c[i]);
+          } else {  // OK_MIN / OK_MAX (order codes; MIN keeps ~code)
+            const bool mn = ro.kind == OK_MIN;
+            if (ro.src_is_float) {
+              const bool f32 = ro.out_dtype == DT_F32;
+              const uint64_t ident = mn ? 0x7FF0000000000000ull : 0xFFF0000000000000ull;
+#pragma unroll
+              for (int i = 0; i < PW; ++i) {
+                const uint64_t d = (nm >> i) & 1u ? 0ull
+                                                   : (val[i] == 0 ? ident : decode_f64_code(mn ? ~val[i] : val[i]));
+                val[i] = f32 ? (uint64_t)f32_bits((float)bits_f64(d)) : d;
+              }
+            } else {
+#pragma unroll
+              for (int i = 0; i < PW; ++i)
+                val[i] = (nm >> i) & 1u ? 0ull : (uint64_t)decode_i64(mn ? ~val[i] : val[i]);
+            }
+          }
         }
-        bool isnull;
-        const uint64_t bits = pr_out_bits(ro, key, v, c, isnull);
-        pr_store(ob, grow[i], lw, bits);
-        if (isnull && onb) g_atomic_or(&onb[grow[i] >> 5], 1u << (grow[i] & 31));
+        if (lw == 3) {
+#pragma unroll
+          for (int i = 0; i < PW; ++i)
+            if (gb[h + i]) *gptr(reinterpret_cast<uint64_t *>(ob + (grow[h + i] << 3))) = val[i];
+        } else if (lw == 2) {
+#pragma unroll
+          for (int i = 0; i < PW; ++i)
+            if (gb[h + i]) *gptr(reinterpret_cast<uint32_t *>(ob + (grow[h + i] << 2))) = (uint32_t)val[i];
+        } else if (lw == 1) {
+#pragma unroll
+          for (int i = 0; i < PW; ++i)
+            if (gb[h + i]) *gptr(reinterpret_cast<uint16_t *>(ob + (grow[h + i] << 1))) = (uint16_t)val[i];
+        } else {
+#pragma unroll
+          for (int i = 0; i < PW; ++i)
+            if (gb[h + i]) *gptr(ob + grow[h + i]) = (uint8_t)val[i];
+        }
+        if (onb && nm) {
+#pragma unroll
+          for (int i = 0; i < PW; ++i)
+            if (gb[h + i] && ((nm >> i) & 1u)) g_atomic_or(&onb[grow[h + i] >> 5], 1u << (grow[h + i] & 31));
+        }
       }
     }
   };
 
+  // The next partition's records are touched while this one's groups are
+  // written (one word per 128-byte line, a stretch without barriers), so its
+  // load finds them in L2; the words are folded into `sink`, which only
+  // decides a store that cannot change anything (OR of 0).
+  uint32_t sink = 0;
   for (uint32_t P = P0; P < P1; ++P) {
     stage_load(P);
     stage_sort();
+    uint32_t pf = 0;
+    if (P + 1 < P1) {
+      const uint64_t nbase = ps.start2[P + 1];
+      const uint64_t ncap = ps.start2[P + 2] - nbase;
+      const uint32_t nc = ps.cur2[P + 1];
+      const uint64_t nwords = (uint64_t)(nc < ncap ? nc : ncap) * rw;
+      const uint64_t w = (uint64_t)t * 32;
+      if (w < nwords) pf = gld<uint32_t>(ps.rec2, (nbase * rw + w) * 4);
+    }
     for (uint32_t g0 = 0; g0 < ngroups; g0 += PR_GPT * PR_RBLOCK) {
       decode(g0);
       write(P);
     }
+    sink += pf;
     __syncthreads();  // LDS (bins, sorted states, group lists, first rows) is reused by the next partition
   }
+  if (sink == 0x9E3779B9u) g_atomic_or(ps.status, 0u);
 }
 
 }  // namespace qm
@@ -807,8 +879,9 @@ const void *pr_scatter_fn(int rw) {
   }
 }
 
-const void *pr_runs_fn(int nsum, int nmax) {
-#define PR_CASE(a, b) if (nsum == a && nmax == b) return (const void *)pr_runs_kernel<a, b>;
+const void *pr_runs_fn(int nsum, int nmax, int rw) {
+#define PR_CASE(a, b) \
+  if (nsum == a && nmax == b) return rw == 4 ? (const void *)pr_runs_kernel<a, b, 4> : (const void *)pr_runs_kernel<a, b, 0>;
   PR_CASE(0, 0) PR_CASE(1, 0) PR_CASE(2, 0) PR_CASE(3, 0) PR_CASE(4, 0)
   PR_CASE(0, 1) PR_CASE(1, 1) PR_CASE(2, 1) PR_CASE(3, 1) PR_CASE(4, 1)
   PR_CASE(0, 2) PR_CASE(1, 2) PR_CASE(2, 2) PR_CASE(3, 2)
@@ -823,7 +896,7 @@ extern "C" uint32_t qmk_pr_tile(int rw) {
   return rw >= 2 && rw <= PR_MAX_RW ? (uint32_t)(PR_STAGE_WORDS / rw / PR_BLOCK) * PR_BLOCK : 0u;
 }
 
-extern "C" int qmk_pr_supported(int nsum, int nmax) { return pr_runs_fn(nsum, nmax) != nullptr; }
+extern "C" int qmk_pr_supported(int nsum, int nmax) { return pr_runs_fn(nsum, nmax, 0) != nullptr; }
 
 // dynamic LDS of the write pass for a partition capacity `cap`
 extern "C" size_t qmk_pr_runs_lds(uint32_t nbin, int nsum, int nmax, uint32_t cap) {
@@ -859,7 +932,7 @@ extern "C" hipError_t qmk_pr_count(const PRScan *ps, hipStream_t st) {
 
 extern "C" hipError_t qmk_pr_runs(const PRScan *ps, hipStream_t st) {
   const int nmax = ps->nstates - 1 - ps->nsum;
-  const void *fn = pr_runs_fn(ps->nsum, nmax);
+  const void *fn = pr_runs_fn(ps->nsum, nmax, ps->rw);
   if (!fn) return hipErrorInvalidValue;
   if (ps->np == 0) return hipSuccess;
   const uint32_t nbin = (uint32_t)(ps->nbounds + 2) << ps->s;
