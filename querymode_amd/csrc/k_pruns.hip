@@ -832,31 +832,15 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
     }
   };
 
-  // The next partition's records are touched while this one's groups are
-  // written (one word per 128-byte line, a stretch without barriers), so its
-  // load finds them in L2; the words are folded into `sink`, which only
-  // decides a store that cannot change anything (OR of 0).
-  uint32_t sink = 0;
   for (uint32_t P = P0; P < P1; ++P) {
     stage_load(P);
     stage_sort();
-    uint32_t pf = 0;
-    if (P + 1 < P1) {
-      const uint64_t nbase = ps.start2[P + 1];
-      const uint64_t ncap = ps.start2[P + 2] - nbase;
-      const uint32_t nc = ps.cur2[P + 1];
-      const uint64_t nwords = (uint64_t)(nc < ncap ? nc : ncap) * rw;
-      const uint64_t w = (uint64_t)t * 32;
-      if (w < nwords) pf = gld<uint32_t>(ps.rec2, (nbase * rw + w) * 4);
-    }
     for (uint32_t g0 = 0; g0 < ngroups; g0 += PR_GPT * PR_RBLOCK) {
       decode(g0);
       write(P);
     }
-    sink += pf;
     __syncthreads();  // LDS (bins, sorted states, group lists, first rows) is reused by the next partition
   }
-  if (sink == 0x9E3779B9u) g_atomic_or(ps.status, 0u);
 }
 
 }  // namespace qm
