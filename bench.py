@@ -254,7 +254,7 @@ def bench_batch(args) -> dict:
     if strat == "part_runs":
         # the k2 queries' launch: row scatter (reads k2, v1, v2 once), record
         # scatter, group count, write pass (records in, result columns out)
-        names = ["pr_scatter_kernel<0, 4>", "pr_scatter_kernel<1, 4>", "pr_count_kernel", "pr_runs_kernel<1, 2>"]
+        names = ["pr_scatter_kernel<0, 4>", "pr_scatter_kernel<1, 4>", "pr_count_kernel", "pr_runs_kernel<1, 2, 4>"]
         pm = [float(np.mean([st[k2[0]].pass_ms[j] for st in stats])) for j in range(4)]
         records = int(last[k2[0]].pass_records)
         k2_groups = int(sum(last[i].groups for i in k2))
