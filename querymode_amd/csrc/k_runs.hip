@@ -700,10 +700,11 @@ __global__ void __launch_bounds__(1024) runs_offsets_kernel(const uint32_t *__re
 
 // ---------------------------------------------------------------------------
 // Lean shared-interval pass (SScan). Each workgroup owns a contiguous row
-// range; a thread takes every R-th row of it (one row per lane, coalesced),
-// with the next row's raw words in flight while the current row is applied:
-// interval of the range value (binary search over the bounds in LDS), then
-// each state's contribution into the (interval, key) entry with LDS atomics.
+// range; a thread takes RPL rows of it per step (each a row per lane,
+// coalesced), with the next step's raw words in flight while the current
+// rows are applied: interval of the range value (binary search over the
+// bounds in LDS, the rows' searches interleaved), then each state's
+// contribution into the (interval, key) entry with LDS atomics.
 // Rows whose range value is NULL match no query of the group.
 template <int NSV>
 struct SRow {
@@ -730,7 +731,10 @@ __device__ __forceinline__ void sload(const SScan &ss, uint64_t row, uint64_t r1
   }
 }
 
-template <int NSV>
+// RPL rows per thread per step: 2 where the bound search is deep (many
+// ranges; the entry table in LDS then bounds the occupancy anyway), else 1
+// (the registers of a second row would cost waves)
+template <int NSV, int RPL>
 __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
   extern __shared__ __align__(16) uint8_t s_dyn[];
   const int nbp = (1 << ss.bsteps) - 1;
@@ -744,28 +748,53 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
   const uint64_t r0 = (uint64_t)blockIdx.x * ss.rows_per_block;
   const uint64_t r1 = (r0 + ss.rows_per_block < ss.nrows) ? r0 + ss.rows_per_block : ss.nrows;
   const uint32_t nb = (uint32_t)ss.nbounds;
-  SRow<NSV> cur, nxt;
+  // a thread takes RPL rows per step (S_BLOCK apart, so each load is
+  // coalesced): their bound searches interleave, and each state's kind is
+  // dispatched once for all of them
+  constexpr uint64_t STEP = (uint64_t)S_BLOCK * RPL;
+  SRow<NSV> cur[RPL], nxt[RPL];
   uint64_t row = r0 + threadIdx.x;
-  sload<NSV>(ss, row, r1, cur);
-  for (; row < r1; row += S_BLOCK) {
-    sload<NSV>(ss, row + S_BLOCK, r1, nxt);
-    if (cur.ok && !(ss.vnull[0] && rdec_null(cur.vn[0], row))) {
-      const uint64_t x = rdec_code(cur.vr[0], ss.v_dt[0], row);
-      uint32_t f = 0;
-      for (int step = (nbp + 1) >> 1; step > 0; step >>= 1)
-        if (sb[f + step - 1] <= x) f += step;
-      const uint32_t e = f < nb ? f : nb;
+#pragma unroll
+  for (int j = 0; j < RPL; ++j) sload<NSV>(ss, row + (uint64_t)j * S_BLOCK, r1, cur[j]);
+  for (; row < r1; row += STEP) {
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) sload<NSV>(ss, row + STEP + (uint64_t)j * S_BLOCK, r1, nxt[j]);
+    bool ok[RPL];
+    uint64_t x[RPL];
+    uint32_t f[RPL];
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+      const uint64_t rj = row + (uint64_t)j * S_BLOCK;
+      ok[j] = cur[j].ok && !(ss.vnull[0] && rdec_null(cur[j].vn[0], rj));
+      x[j] = ok[j] ? rdec_code(cur[j].vr[0], ss.v_dt[0], rj) : 0ull;
+      f[j] = 0;
+    }
+    for (int step = (nbp + 1) >> 1; step > 0; step >>= 1) {
+#pragma unroll
+      for (int j = 0; j < RPL; ++j)
+        if (sb[f[j] + step - 1] <= x[j]) f[j] += step;
+    }
+    lds_u64 *ent[RPL];
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) {
+      const uint64_t rj = row + (uint64_t)j * S_BLOCK;
+      const uint32_t e = f[j] < nb ? f[j] : nb;
       uint32_t d = 0;
-      if (ss.key) {
-        const int64_t kv = rdec_i(cur.kr, ss.key_dt, row) - ss.kmin;
+      if (ss.key && ok[j]) {
+        const int64_t kv = rdec_i(cur[j].kr, ss.key_dt, rj) - ss.kmin;
         d = (kv >= 0 && kv < (int64_t)ss.dom) ? (uint32_t)kv : 0u;  // the key's statistics bound it
       }
-      lds_u64 *ent = tab + ((uint64_t)e * ss.dom + d) * W;
+      ent[j] = tab + ((uint64_t)e * ss.dom + d) * W;
+    }
 #pragma unroll
-      for (int si = 0; si < B_MAX_SHARED_STATES; ++si) {
-        if (si >= ss.nstates) break;
-        const DState &st = ss.st[si];
-        const int vi = ss.st_v[si];
+    for (int si = 0; si < B_MAX_SHARED_STATES; ++si) {
+      if (si >= ss.nstates) break;
+      const DState &st = ss.st[si];
+      const int vi = ss.st_v[si];
+#pragma unroll
+      for (int j = 0; j < RPL; ++j) {
+        if (!ok[j]) continue;
+        const uint64_t rj = row + (uint64_t)j * S_BLOCK;
         Contrib k;
         k.a = 1; k.b = k.c = 0; k.skip = false;
         if (vi >= 0) {
@@ -773,19 +802,20 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
           uint32_t rn = 0;
           int32_t dt = 0;
 #pragma unroll
-          for (int j = 0; j < NSV; ++j)
-            if (vi == j) { raw = cur.vr[j]; rn = cur.vn[j]; dt = ss.v_dt[j]; }
-          if (ss.vnull[vi] && rdec_null(rn, row)) continue;
+          for (int v = 0; v < NSV; ++v)
+            if (vi == v) { raw = cur[j].vr[v]; rn = cur[j].vn[v]; dt = ss.v_dt[v]; }
+          if (ss.vnull[vi] && rdec_null(rn, rj)) continue;
           if (st.kind != SK_COUNT) {
             if (dt_is_float(dt)) k = contrib_of(st, true, 0, rdec_f(raw, dt));
-            else k = contrib_of(st, false, rdec_i(raw, dt, row), 0.0);
+            else k = contrib_of(st, false, rdec_i(raw, dt, rj), 0.0);
             if (k.skip) continue;
           }
         }
-        apply_lds(st, ent, k);
+        apply_lds(st, ent[j], k);
       }
     }
-    cur = nxt;
+#pragma unroll
+    for (int j = 0; j < RPL; ++j) cur[j] = nxt[j];
   }
   __syncthreads();
   // merge into the global table: one entry per thread, its states in order
@@ -808,12 +838,19 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
   }
 }
 
-static const void *shared_fn(int nv) {
-  switch (nv) {
-    case 1: return (const void *)shared_kernel<1>;
-    case 2: return (const void *)shared_kernel<2>;
-    case 3: return (const void *)shared_kernel<3>;
-    case 4: return (const void *)shared_kernel<4>;
+// rows per thread per step for a launch (deep bound searches: 2)
+static int shared_rpl(const SScan *ss) { return ss->bsteps >= 4 ? 2 : 1; }
+
+static const void *shared_fn(int nv, int rpl) {
+  switch (nv * 2 + (rpl == 2 ? 1 : 0)) {
+    case 2: return (const void *)shared_kernel<1, 1>;
+    case 3: return (const void *)shared_kernel<1, 2>;
+    case 4: return (const void *)shared_kernel<2, 1>;
+    case 5: return (const void *)shared_kernel<2, 2>;
+    case 6: return (const void *)shared_kernel<3, 1>;
+    case 7: return (const void *)shared_kernel<3, 2>;
+    case 8: return (const void *)shared_kernel<4, 1>;
+    case 9: return (const void *)shared_kernel<4, 2>;
   }
   return nullptr;
 }
@@ -1073,7 +1110,7 @@ extern "C" size_t qmk_shared_lds_bytes(const SScan *ss) {
 }
 
 extern "C" int qmk_shared_blocks_per_cu(const SScan *ss) {
-  const void *fn = shared_fn(ss->nv);
+  const void *fn = shared_fn(ss->nv, shared_rpl(ss));
   if (!fn) return 0;
   const size_t lds = qmk_shared_lds_bytes(ss);
   if (lds > 64 * 1024) (void)qmk_func_lds(fn, (size_t)lds);
@@ -1083,7 +1120,7 @@ extern "C" int qmk_shared_blocks_per_cu(const SScan *ss) {
 }
 
 extern "C" hipError_t qmk_shared_scan(const SScan *ss, uint32_t grid, hipStream_t st) {
-  const void *fn = shared_fn(ss->nv);
+  const void *fn = shared_fn(ss->nv, shared_rpl(ss));
   if (!fn || grid == 0 || ss->nstates < 1 || ss->nstates > B_MAX_SHARED_STATES || ss->dom < 1 ||
       ss->bsteps < 0 || ss->bsteps > 10 || ((1 << ss->bsteps) - 1) < ss->nbounds ||
       (uint64_t)grid * ss->rows_per_block < ss->nrows)

@@ -323,10 +323,16 @@ void d2h_rows(qm_ctx *c, void *dst, const void *src, size_t width, size_t pitch,
     ++k->pos;
     return;
   }
-  if (pitch == width)
+  DevBuf rows_dev;
+  if (pitch == width) {
     HIPX(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->lane().stream));
-  else
-    HIPX(hipMemcpy2DAsync(dst, width, src, pitch, width, rows, hipMemcpyDeviceToHost, c->lane().stream));
+  } else {
+    // the rows are gathered on the device first: a strided copy into pageable
+    // host memory goes row by row (≈ 7 µs per row)
+    rows_dev = DevBuf(c, bytes);
+    HIPX(hipMemcpy2DAsync(rows_dev.p, width, src, pitch, width, rows, hipMemcpyDeviceToDevice, c->lane().stream));
+    HIPX(hipMemcpyAsync(dst, rows_dev.p, bytes, hipMemcpyDeviceToHost, c->lane().stream));
+  }
   HIPX(hipStreamSynchronize(c->lane().stream));
   if (k && k->mode == LANE_RECORD) {
     const uint8_t *b = static_cast<const uint8_t *>(dst);
