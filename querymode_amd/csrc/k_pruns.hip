@@ -523,6 +523,19 @@ __device__ __forceinline__ uint64_t pr_contrib(const PRScan &ps, int s2, const u
   return st.is_min ? ~code : code;
 }
 
+// An order-code state over an integer column of at most 32 bits keeps 32-bit
+// codes in LDS: (uint32_t)v ^ 2^31 (MIN: its complement), the low word of the
+// 64-bit code with the top bit flipped; 0 (no value) stays 0. A real code can
+// be 0 too (MIN of INT32_MAX, MAX of INT32_MIN), but a group's max is only
+// read when it counted a value, and then the real codes' max is >= every 0.
+__device__ __forceinline__ bool pr_narrow(const PRScan &ps, int st) {
+  const int vj = ps.st_v[st];
+  if (vj < 0) return false;
+  const int32_t dt = ps.v_dt[vj];
+  return !dt_is_float(dt) && dt != DT_I64;
+}
+__device__ __forceinline__ uint32_t pr_code32(uint64_t c) { return c ? ((uint32_t)c ^ 0x80000000u) : 0u; }
+
 // A workgroup takes a contiguous range of final partitions, so the rows it
 // writes for one query are consecutive across its partitions (each result
 // column is written as long runs, not as short pieces from many workgroups).
@@ -570,6 +583,9 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   const bool kon = (uint32_t)lane <= km;
   const uint64_t lt = (1ull << lane) - 1ull;
   const uint32_t kbin = (uint32_t)lane * NBK;
+  uint32_t narm = 0;  // bit j: order-code state NSUM + 1 + j keeps 32-bit codes
+#pragma unroll
+  for (int j = 0; j < NMAX; ++j) narm |= pr_narrow(ps, NSUM + 1 + j) ? 1u << j : 0u;
 
   // ---- load: partition P's records -> (key, bucket) bin and rank within it
   // (LDS counters), state contributions (registers); the first result row of
@@ -644,7 +660,10 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
 #pragma unroll
       for (int j = 0; j < NSUM; ++j) s_add[(uint64_t)j * cap + slot] = ca[i][j];
 #pragma unroll
-      for (int j = 0; j < NMAX; ++j) s_max[(uint64_t)j * cap + slot] = cm[i][j];
+      for (int j = 0; j < NMAX; ++j) {
+        if ((narm >> j) & 1u) reinterpret_cast<uint32_t *>(s_max + (uint64_t)j * cap)[slot] = pr_code32(cm[i][j]);
+        else s_max[(uint64_t)j * cap + slot] = cm[i][j];
+      }
     }
     __syncthreads();
     // inclusive prefix sums of the additive states over the sorted order
@@ -731,7 +750,20 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
   // value state st (additive, or an order code: the max over the group's
   // sorted positions [ga, gb)) of held groups h .. h + PW - 1
   auto valw = [&](int st, int h, uint64_t (&r)[PW]) {
-    if (NMAX > 0 && st > NSUM && st <= NSUM + NMAX) {
+    if (NMAX > 0 && st > NSUM && st <= NSUM + NMAX && ((narm >> (st - 1 - NSUM)) & 1u)) {
+      const uint32_t *md = reinterpret_cast<const uint32_t *>(s_max + (uint64_t)(st - 1 - NSUM) * cap);
+#pragma unroll
+      for (int i = 0; i < PW; ++i) {
+        uint32_t x0 = 0, x1 = 0;
+        uint32_t k = ga[h + i];
+        for (; k + 1 < gb[h + i]; k += 2) {
+          x0 = x0 > md[k] ? x0 : md[k];
+          x1 = x1 > md[k + 1] ? x1 : md[k + 1];
+        }
+        if (k < gb[h + i]) x0 = x0 > md[k] ? x0 : md[k];
+        r[i] = x0 > x1 ? x0 : x1;
+      }
+    } else if (NMAX > 0 && st > NSUM && st <= NSUM + NMAX) {
       const uint64_t *md = s_max + (uint64_t)(st - 1 - NSUM) * cap;
 #pragma unroll
       for (int i = 0; i < PW; ++i) {
@@ -798,6 +830,12 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
                 const uint64_t d = (nm >> i) & 1u ? 0ull
                                                    : (val[i] == 0 ? ident : decode_f64_code(mn ? ~val[i] : val[i]));
                 val[i] = f32 ? (uint64_t)f32_bits((float)bits_f64(d)) : d;
+              }
+            } else if (ro.state > NSUM && ro.state <= NSUM + NMAX && ((narm >> (ro.state - 1 - NSUM)) & 1u)) {
+#pragma unroll
+              for (int i = 0; i < PW; ++i) {
+                const uint32_t c32 = (uint32_t)val[i];
+                val[i] = (nm >> i) & 1u ? 0ull : (uint64_t)(int64_t)(int32_t)((mn ? ~c32 : c32) ^ 0x80000000u);
               }
             } else {
 #pragma unroll

@@ -142,3 +142,45 @@ def test_unsorted_output_kinds(qm):
                              aggregates=aggs) for lo, w in ((0.0, 0.001), (0.2, 0.3), (0.0, 1.0), (0.6, 0.05))]
     batch = _check(qm, "o", cols, descs, nulls)
     print([r.stats["strategy"] for r in batch], flush=True)
+
+
+def test_unsorted_narrow_minmax_extremes(qm):
+    """MIN / MAX of int32 and int8 columns keep 32-bit order codes in the write
+    pass. Groups whose values are all INT32_MAX (MIN's code is 0), all
+    INT32_MIN (MAX's code is 0), all NULL, or mixed with NULLs; int8 at -128 /
+    127; record widths 2, 3 and 4 words."""
+    n = 240_000
+    _say("unsorted narrow MIN / MAX extremes")
+    cols = _table(n, 24_000, seed=37)
+    k = cols["k2"]
+    rng = np.random.default_rng(41)
+    v1 = rng.integers(-(2 ** 31), 2 ** 31, n, dtype=np.int64)
+    v1[k % 5 == 0] = 2 ** 31 - 1
+    v1[k % 5 == 1] = -(2 ** 31)
+    v1[(k % 5 == 2) & (rng.random(n) < 0.5)] = 2 ** 31 - 1
+    v1[(k % 5 == 3) & (rng.random(n) < 0.5)] = -(2 ** 31)
+    cols["v1"] = v1.astype(np.int32)
+    i8 = rng.integers(-128, 128, n).astype(np.int8)
+    i8[k % 7 == 0] = 127
+    i8[k % 7 == 1] = -128
+    cols["i8"] = i8
+    ranges = ((0.0, 1.0), (0.1, 0.4), (0.5, 0.02), (0.0, 0.3))
+
+    def qs(name, aggs):
+        return [QueryDescriptor(name, filters=[FilterOp("v2", "gte", lo), FilterOp("v2", "lt", lo + w)],
+                                groupBy=["k2"], aggregates=aggs) for lo, w in ranges]
+
+    # no NULLs: 4-word records (key / bucket word, v1, v2)
+    a = qs("x", [AggregateOp("max", "v1"), AggregateOp("count", "*"), AggregateOp("min", "v1"),
+                 AggregateOp("max", "v2")])
+    b = _check(qm, "x", cols, a)
+    assert all(r.stats["strategy"] == "part_runs" for r in b), [r.stats["strategy"] for r in b]
+    # int8 only: 2-word records
+    c = qs("x", [AggregateOp("min", "i8"), AggregateOp("max", "i8"), AggregateOp("count", "*")])
+    b = _check(qm, "x", cols, c)
+    assert all(r.stats["strategy"] == "part_runs" for r in b), [r.stats["strategy"] for r in b]
+    # NULLs in v1, whole groups NULL (keys % 5 == 4): a NULL-bit word, NULL-guarded outputs
+    nulls = {"v1": (k % 5 == 4) | (rng.random(n) < 0.1)}
+    d = qs("y", [AggregateOp("min", "v1"), AggregateOp("max", "v1"), AggregateOp("count", "*")])
+    b = _check(qm, "y", cols, d, nulls)
+    assert all(r.stats["strategy"] == "part_runs" for r in b), [r.stats["strategy"] for r in b]
