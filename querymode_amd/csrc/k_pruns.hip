@@ -410,10 +410,22 @@ __global__ void __launch_bounds__(PR_BLOCK) pr_count_kernel(PRScan ps) {
   for (int i = t; i < 64 * BW; i += PR_BLOCK) s_bm[i] = 0;
   __syncthreads();
   const uint32_t km = (1u << ps.s) - 1u;
-  for (uint64_t j = t; j < n; j += PR_BLOCK) {
-    const uint32_t kb = ps.rec2[(base + j) * (uint64_t)ps.rw];
-    const uint32_t e = kb >> 16;
-    atomicOr(&s_bm[(kb & km) * BW + (e >> 5)], 1u << (e & 31));
+  // a thread's record words are all loaded before any of them is applied, so
+  // its loads are in flight together (a partition is at most a few rounds)
+  constexpr int CPT = 8;
+  for (uint64_t j0 = t; j0 < n; j0 += (uint64_t)CPT * PR_BLOCK) {
+    uint32_t kb[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const uint64_t j = j0 + (uint64_t)i * PR_BLOCK;
+      kb[i] = j < n ? gld<uint32_t>(ps.rec2, (base + j) * (uint64_t)ps.rw * 4) : 0u;
+    }
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      if (j0 + (uint64_t)i * PR_BLOCK >= n) break;
+      const uint32_t e = kb[i] >> 16;
+      atomicOr(&s_bm[(kb[i] & km) * BW + (e >> 5)], 1u << (e & 31));
+    }
   }
   __syncthreads();
   const bool kon = (uint32_t)lane <= km;
@@ -605,6 +617,14 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
     for (uint32_t i = t; i < nbw; i += PR_RBLOCK) s_bin[i] = 0;
     if (nP > cap && t == 0) atomicOr(ps.status, 2u);  // the host sizes partitions from the key histogram
     __syncthreads();
+    // every record of the thread is loaded before any is applied, so the
+    // loads are in flight together
+    uint32_t wr[PR_RIPT][NW];
+#pragma unroll
+    for (int i = 0; i < PR_RIPT; ++i) {
+      const uint32_t j = (uint32_t)t + (uint32_t)i * PR_RBLOCK;
+      if (j < nP && j < cap) pr_load_rec(ps.rec2, pbase + j, rw, raligned, wr[i]);
+    }
 #pragma unroll
     for (int i = 0; i < PR_RIPT; ++i) {
       const uint32_t j = (uint32_t)t + (uint32_t)i * PR_RBLOCK;
@@ -615,13 +635,11 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
 #pragma unroll
       for (int k = 0; k < NMA; ++k) cm[i][k] = 0;
       if (j >= nP || j >= cap) continue;
-      uint32_t wr[NW];
-      pr_load_rec(ps.rec2, pbase + j, rw, raligned, wr);
 #pragma unroll
-      for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr);
+      for (int s2 = 0; s2 < NSUM; ++s2) ca[i][s2] = pr_contrib(ps, s2, wr[i]);
 #pragma unroll
-      for (int s2 = 0; s2 < NMAX; ++s2) cm[i][s2] = pr_contrib(ps, NSUM + s2, wr);
-      const uint32_t kb = wr[0];
+      for (int s2 = 0; s2 < NMAX; ++s2) cm[i][s2] = pr_contrib(ps, NSUM + s2, wr[i]);
+      const uint32_t kb = wr[i][0];
       bn[i] = (kb & km) * NBK + (kb >> 16);
       const uint32_t sh = (bn[i] & 1u) * 16u;
       rk[i] = (atomicAdd(&s_bin[bn[i] >> 1], 1u << sh) >> sh) & 0xFFFFu;
