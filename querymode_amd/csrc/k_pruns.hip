@@ -286,13 +286,11 @@ __global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
         rk[i] = atomicAdd(&s_hist[dg[i]], 1u);
       }
     } else {
+      // every record of the thread is loaded before any is counted, so the
+      // loads are in flight together
 #pragma unroll
       for (int i = 0; i < IPT; ++i) {
         const uint64_t it = (uint64_t)t + (uint64_t)i * PR_BLOCK;
-        rk[i] = ~0u;
-        dg[i] = 0;
-        kb[i] = 0;
-        nbits[i] = 0;
         if (it >= srcn) continue;
         const uint32_t *r = ps.rec1 + (src0 + it) * RW;
         if constexpr (RW % 4 == 0) {  // records of 4 or 8 words start 16-byte aligned
@@ -305,6 +303,15 @@ __global__ void __launch_bounds__(PR_BLOCK, 4) pr_scatter_kernel(PRScan ps) {
 #pragma unroll
           for (int k = 0; k < RW; ++k) w[i][k] = r[k];
         }
+      }
+#pragma unroll
+      for (int i = 0; i < IPT; ++i) {
+        const uint64_t it = (uint64_t)t + (uint64_t)i * PR_BLOCK;
+        rk[i] = ~0u;
+        dg[i] = 0;
+        kb[i] = 0;
+        nbits[i] = 0;
+        if (it >= srcn) continue;
         dg[i] = (w[i][0] & 0xFFFFu) >> ps.s;
         rk[i] = atomicAdd(&s_hist[dg[i]], 1u);
       }
@@ -610,10 +617,6 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
     const uint64_t capr = ps.start2[P + 1] - pbase;
     const uint32_t c2 = ps.cur2[P];
     nP = (uint32_t)(c2 < capr ? c2 : capr);
-    if (t < ps.nq) {
-      const uint64_t *gq = ps.gcount + (uint64_t)t * ps.np;
-      s_qrow[t] = ps.row_off[t] + (gq[P] - gq[0]);
-    }
     for (uint32_t i = t; i < nbw; i += PR_RBLOCK) s_bin[i] = 0;
     if (nP > cap && t == 0) atomicOr(ps.status, 2u);  // the host sizes partitions from the key histogram
     __syncthreads();
@@ -624,6 +627,13 @@ __global__ void __launch_bounds__(PR_RBLOCK) pr_runs_kernel(PRScan ps) {
     for (int i = 0; i < PR_RIPT; ++i) {
       const uint32_t j = (uint32_t)t + (uint32_t)i * PR_RBLOCK;
       if (j < nP && j < cap) pr_load_rec(ps.rec2, pbase + j, rw, raligned, wr[i]);
+    }
+    // the first result row of (query t, P), read while the records load (the
+    // previous partition's groups are all written: it is read after the
+    // barriers of the sort)
+    if (t < ps.nq) {
+      const uint64_t *gq = ps.gcount + (uint64_t)t * ps.np;
+      s_qrow[t] = ps.row_off[t] + (gq[P] - gq[0]);
     }
 #pragma unroll
     for (int i = 0; i < PR_RIPT; ++i) {
