@@ -795,6 +795,14 @@ __global__ void __launch_bounds__(S_BLOCK) shared_kernel(SScan ss) {
       for (int j = 0; j < RPL; ++j) {
         if (!ok[j]) continue;
         const uint64_t rj = row + (uint64_t)j * S_BLOCK;
+        if (si == 0 && st.kind == SK_COUNT && vi < 0) {
+          // COUNT(*) (always state 0): a 32-bit add into the low half of its
+          // entry word — a workgroup counts fewer than 2^32 rows
+          // (qmk_shared_scan), so the high half stays 0
+          __hip_atomic_fetch_add(reinterpret_cast<__attribute__((address_space(3))) uint32_t *>(ent[j] + st.word), 1u,
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          continue;
+        }
         Contrib k;
         k.a = 1; k.b = k.c = 0; k.skip = false;
         if (vi >= 0) {
@@ -1123,7 +1131,7 @@ extern "C" hipError_t qmk_shared_scan(const SScan *ss, uint32_t grid, hipStream_
   const void *fn = shared_fn(ss->nv, shared_rpl(ss));
   if (!fn || grid == 0 || ss->nstates < 1 || ss->nstates > B_MAX_SHARED_STATES || ss->dom < 1 ||
       ss->bsteps < 0 || ss->bsteps > 10 || ((1 << ss->bsteps) - 1) < ss->nbounds ||
-      (uint64_t)grid * ss->rows_per_block < ss->nrows)
+      (uint64_t)grid * ss->rows_per_block < ss->nrows || ss->rows_per_block >= (1ull << 32))  // 32-bit COUNT(*) in LDS
     return hipErrorInvalidValue;
   for (int i = 0; i < ss->nstates; ++i)
     if (ss->st_v[i] >= ss->nv) return hipErrorInvalidValue;
